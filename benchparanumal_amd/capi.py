@@ -1,0 +1,286 @@
+"""ctypes binding of libbpx.so (include/bpx.h)."""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+HEADER = os.path.join(ROOT, "include", "bpx.h")
+
+BPX_OK = 0
+STATUS = {
+    0: "BPX_OK", -1: "BPX_ERR_INVALID_ARGUMENT", -2: "BPX_ERR_UNSUPPORTED", -3: "BPX_ERR_HIP",
+    -4: "BPX_ERR_OUT_OF_MEMORY", -5: "BPX_ERR_NEGATIVE_JACOBIAN", -6: "BPX_ERR_OGS",
+    -7: "BPX_ERR_BREAKDOWN", -8: "BPX_ERR_NO_DEVICE",
+}
+
+_lib = None
+_P = C.c_void_p
+_I = C.c_int
+_D = C.c_double
+_F = C.c_float
+_I64P = C.POINTER(C.c_int64)
+
+
+class BpxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def library_path():
+    return os.path.join(HERE, "lib", "libbpx.so")
+
+
+_SIGS = {
+    "bpx_last_error": (C.c_char_p, []),
+    "bpx_version": (_I, []),
+    "bpx_device_count": (_I, [C.POINTER(_I)]),
+    "bpx_random_normal": (_I, [C.c_uint64, C.c_int64, C.c_int64, _P]),
+    "bpx_platform_create": (_I, [_I, C.POINTER(_P)]),
+    "bpx_platform_destroy": (_I, [_P]),
+    "bpx_platform_set_stream": (_I, [_P, _P]),
+    "bpx_platform_stream": (_P, [_P]),
+    "bpx_platform_finish": (_I, [_P]),
+    "bpx_rhs_hex": (_I, [_P, _I, _I, _I, _I] + [_P] * 6 + [_D, _P]),
+    "bpx_ogs_gather": (_I, [_P, _I, _I] + [_P] * 5),
+    "bpx_ogs_gather_f32": (_I, [_P, _I, _I] + [_P] * 5),
+    "bpx_ogs_scatter": (_I, [_P, _I, _I] + [_P] * 5),
+    "bpx_ogs_gather_scatter": (_I, [_P, _I, _I] + [_P] * 4),
+    "bpx_linalg_set": (_I, [_P, _I, _D, _P]),
+    "bpx_linalg_axpy": (_I, [_P, _I, _D, _P, _D, _P]),
+    "bpx_linalg_inner_prod": (_I, [_P, _I, _P, _P, C.POINTER(_D)]),
+    "bpx_linalg_norm2": (_I, [_P, _I, _P, C.POINTER(_D)]),
+    "bpx_cg_update": (_I, [_P, _I, _P, _P, _D, _P, _P, C.POINTER(_D)]),
+    "bpx_mesh_create": (_I, [_P, C.POINTER(_P)]),
+    "bpx_mesh_destroy": (_I, [_P]),
+    "bpx_mesh_info": (_I, [_P, _I64P, _I]),
+    "bpx_mesh_array": (_I, [_P, C.c_char_p, _P, _I64P, C.POINTER(_I)]),
+    "bpx_ogs_setup": (_I, [_I, _P, C.POINTER(_P)]),
+    "bpx_ogs_setup_masked": (_I, [_P, _I, C.POINTER(_P)]),
+    "bpx_ogs_destroy": (_I, [_P]),
+    "bpx_ogs_info": (_I, [_P, _I64P, _I]),
+    "bpx_ogs_array": (_I, [_P, C.c_char_p, _P, _I64P, C.POINTER(_I)]),
+    "bpx_ogs_gather_dev": (_I, [_P, _P, _I, _P, _P]),
+    "bpx_ogs_scatter_dev": (_I, [_P, _P, _I, _P, _P]),
+    "bpx_bp_create": (_I, [_P, _P, _I, _D, C.POINTER(_P)]),
+    "bpx_bp_destroy": (_I, [_P]),
+    "bpx_bp_info": (_I, [_P, _I64P, _I]),
+    "bpx_bp_operator": (_I, [_P, _P, _P]),
+    "bpx_bp_ax_local": (_I, [_P, _P, _P]),
+    "bpx_bp_operator_f32": (_I, [_P, _P, _P]),
+    "bpx_bp_rhs_forcing": (_I, [_P, _P]),
+    "bpx_bp_solve": (_I, [_P, _P, _P, _D, _I, _I, C.POINTER(_I), _P]),
+    "bpx_bp_solve_mixed": (_I, [_P, _P, _P, _D, _D, _I, _I, _P]),
+}
+# Ax kernels: (platform, N, Nelements, elementList, G2L, <pointers...>, [lambda], q, Aq)
+for _name in ("bpx_bp5_ax_hex", "bpx_bp5_ax_affine_hex", "bpx_bp6_ax_hex", "bpx_bp6_ax_affine_hex"):
+    _SIGS[_name] = (_I, [_P, _I, _I, _P, _P] + [_P] * 7 + [_D, _P, _P])
+for _name in ("bpx_bp1_ax_hex", "bpx_bp1_ax_affine_hex", "bpx_bp2_ax_hex", "bpx_bp2_ax_affine_hex"):
+    _SIGS[_name] = (_I, [_P, _I, _I, _P, _P] + [_P] * 3 + [_P, _P])
+for _name in ("bpx_bp3_ax_hex", "bpx_bp3_ax_affine_hex", "bpx_bp4_ax_hex", "bpx_bp4_ax_affine_hex"):
+    _SIGS[_name] = (_I, [_P, _I, _I, _P, _P] + [_P] * 7 + [_D, _P, _P])
+_SIGS["bpx_bp5_ax_hex_f32"] = (_I, [_P, _I, _I, _P, _P] + [_P] * 7 + [_F, _P, _P])
+_SIGS["bpx_bp3_ax_hex_f32"] = (_I, [_P, _I, _I, _P, _P] + [_P] * 7 + [_F, _P, _P])
+
+
+class _MeshOpts(C.Structure):
+    _fields_ = [("N", _I), ("nx", _I), ("ny", _I), ("nz", _I), ("affine", _I), ("cubature", _I),
+                ("lo", _D * 3), ("dim", _D * 3), ("perturb", _D), ("seed", C.c_uint64)]
+
+
+def lib():
+    """Load libbpx.so; raises when it has not been built (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = library_path()
+    if not os.path.exists(path):
+        raise ImportError(f"libbpx.so not built at {path}; run __graft_entry__.build()")
+    L = C.CDLL(path)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def check(code):
+    if code != BPX_OK:
+        raise BpxError(code, lib().bpx_last_error().decode())
+    return code
+
+
+def ptr(t):
+    """Device (or host numpy) pointer of a tensor/array; None -> NULL."""
+    if t is None:
+        return None
+    if isinstance(t, np.ndarray):
+        return t.ctypes.data
+    return t.data_ptr()
+
+
+def random_normal(seed, n, counter0=0):
+    """Seeded N(0,1) vector (SplitMix64 + Box-Muller), computed by libbpx."""
+    out = np.zeros(int(n))
+    check(lib().bpx_random_normal(seed, counter0, int(n), out.ctypes.data if n else None))
+    return out
+
+
+def device_count():
+    n = _I()
+    check(lib().bpx_device_count(C.byref(n)))
+    return n.value
+
+
+class Platform:
+    """platform_t: one device and the stream every call is queued on. By
+    default the stream is torch's current stream, so torch allocations and
+    bpx kernels are ordered."""
+
+    def __init__(self, device=0, torch_stream=True):
+        h = _P()
+        check(lib().bpx_platform_create(device, C.byref(h)))
+        self.h = h
+        self.device = device
+        if torch_stream:
+            import torch
+
+            s = torch.cuda.current_stream(device).cuda_stream
+            check(lib().bpx_platform_set_stream(self.h, s))
+
+    def finish(self):
+        check(lib().bpx_platform_finish(self.h))
+
+    def close(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.bpx_platform_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def _read_array(fn, h, name):
+    n = C.c_int64()
+    dt = _I()
+    check(fn(h, name.encode(), None, C.byref(n), C.byref(dt)))
+    dtype = {0: np.float64, 1: np.int32, 2: np.int64}[dt.value]
+    out = np.zeros(n.value, dtype=dtype)
+    check(fn(h, name.encode(), out.ctypes.data if n.value else None, C.byref(n), C.byref(dt)))
+    return out
+
+
+class Mesh:
+    """mesh_t for the hex box (host data)."""
+
+    def __init__(self, N, nx, ny=None, nz=None, affine=False, cubature=False, lo=(-1.0, -1.0, -1.0),
+                 dim=(2.0, 2.0, 2.0), perturb=0.0, seed=0):
+        o = _MeshOpts()
+        o.N, o.nx = N, nx
+        o.ny = nx if ny is None else ny
+        o.nz = nx if nz is None else nz
+        o.affine, o.cubature = int(affine), int(cubature)
+        for c in range(3):
+            o.lo[c], o.dim[c] = lo[c], dim[c]
+        o.perturb, o.seed = perturb, seed
+        h = _P()
+        check(lib().bpx_mesh_create(C.byref(o), C.byref(h)))
+        self.h = h
+        buf = (C.c_int64 * 8)()
+        n = lib().bpx_mesh_info(self.h, buf, 8)
+        keys = ["N", "Nq", "Np", "cubNq", "cubNp", "Nelements"]
+        self.info = {k: int(buf[i]) for i, k in enumerate(keys[:n])}
+        self.__dict__.update(self.info)
+
+    def array(self, name):
+        return _read_array(lib().bpx_mesh_array, self.h, name)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.bpx_mesh_destroy(self.h)
+            self.h = None
+
+
+class Ogs:
+    """ogs::ogs_t T maps built on the host."""
+
+    def __init__(self, ids=None, mesh=None, nfields=1):
+        h = _P()
+        if mesh is not None:
+            check(lib().bpx_ogs_setup_masked(mesh.h, nfields, C.byref(h)))
+        else:
+            ids = np.ascontiguousarray(ids, dtype=np.int64)
+            check(lib().bpx_ogs_setup(len(ids), ids.ctypes.data, C.byref(h)))
+        self.h = h
+        buf = (C.c_int64 * 4)()
+        lib().bpx_ogs_info(self.h, buf, 4)
+        self.N, self.Ngather, self.nnzT, self.NrowBlocksT = (int(v) for v in buf)
+
+    def array(self, name):
+        return _read_array(lib().bpx_ogs_array, self.h, name)
+
+    def gather(self, platform, v, gv, K=1):
+        check(lib().bpx_ogs_gather_dev(platform.h, self.h, K, ptr(v), ptr(gv)))
+
+    def scatter(self, platform, gv, v, K=1):
+        check(lib().bpx_ogs_scatter_dev(platform.h, self.h, K, ptr(gv), ptr(v)))
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.bpx_ogs_destroy(self.h)
+            self.h = None
+
+
+class BP:
+    """bpN_t: masked ogs, operator data on the device, Operator and CG."""
+
+    def __init__(self, platform, mesh, bp, lam=1.0):
+        h = _P()
+        check(lib().bpx_bp_create(platform.h, mesh.h, bp, lam, C.byref(h)))
+        self.h = h
+        self.platform = platform
+        self.mesh = mesh
+        buf = (C.c_int64 * 6)()
+        lib().bpx_bp_info(self.h, buf, 6)
+        self.bp, self.Nfields, self.Ngather, self.NLocal, self.Nelements, self.N = (int(v) for v in buf)
+
+    def operator(self, q, Aq):
+        check(lib().bpx_bp_operator(self.h, ptr(q), ptr(Aq)))
+
+    def ax_local(self, q, AqL):
+        check(lib().bpx_bp_ax_local(self.h, ptr(q), ptr(AqL)))
+
+    def operator_f32(self, q, Aq):
+        check(lib().bpx_bp_operator_f32(self.h, ptr(q), ptr(Aq)))
+
+    def rhs_forcing(self, b):
+        check(lib().bpx_bp_rhs_forcing(self.h, ptr(b)))
+
+    def solve(self, x, r, tol=0.0, maxit=100, verbose=False, history=True):
+        its = _I()
+        hist = np.zeros(maxit + 1) if history else None
+        check(lib().bpx_bp_solve(self.h, ptr(x), ptr(r), tol, maxit, int(verbose), C.byref(its),
+                                 ptr(hist)))
+        return its.value, (hist[: its.value + 1] if history else None)
+
+    def solve_mixed(self, x, b, tol=1e-10, inner_tol=1e-5, max_outer=50, max_inner=5000):
+        stats = np.zeros(3)
+        check(lib().bpx_bp_solve_mixed(self.h, ptr(x), ptr(b), tol, inner_tol, max_outer, max_inner,
+                                       ptr(stats)))
+        return {"outer": int(stats[0]), "inner": int(stats[1]), "rel_res": float(stats[2])}
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.bpx_bp_destroy(self.h)
+            self.h = None
+
+
+def header_symbols():
+    """Names of the functions include/bpx.h declares."""
+    import re
+
+    src = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(bpx_[a-z0-9_]+)\s*\(", src)))

@@ -1,0 +1,13 @@
+// Per-degree Ax launchers. Each ax_bp*.hip translation unit is compiled
+// once per Nq (-DBPX_NQ) and instantiates its launcher for that Nq;
+// ax_dispatch.cpp maps the run-time degree onto them.
+#pragma once
+#include "bpx_internal.hpp"
+
+namespace bpx {
+
+template <typename T, int NQ, int NF> int launch_bp5_nq(hipStream_t s, const AxArgs<T>& a);
+template <typename T, int NQ, int NF> int launch_bp1_nq(hipStream_t s, const AxArgs<T>& a);
+template <typename T, int NQ, int NF> int launch_bp3_nq(hipStream_t s, const AxArgs<T>& a);
+
+}  // namespace bpx

@@ -1,0 +1,280 @@
+// BP1/BP2 (mass) and BP3/BP4 (over-integrated stiffness + lambda*mass) hex
+// Ax, affine and non-affine.
+//
+// Semantics:
+//  BP1  okl/bp1AxHex3D.okl:61-325        Aq = I^T (cubwJ .* (I q)), I = cubInterp
+//  BP1A okl/bp1AxAffineHex3D.okl:61-191  Aq = J (MM x MM x MM) q
+//  BP3  okl/bp3AxHex3D.okl:369-712       Aq = I^T (cubD^T G cubD + lambda cubwJ) I q
+//  BP3A okl/bp3AxAffineHex3D.okl:668-924 same in the modal basis: I = invV,
+//       cubD = modal derivative, per-element G and J (no weights)
+//  BP2/BP4: three fields, local layout [element][field][node].
+//
+// Design (gfx950): one element per workgroup, an MQ x MQ sheet of lanes
+// (MQ = points per direction of the quadrature grid: p+2 Gauss points, or
+// Nq for the affine modal/mass forms). Lane (a,b) keeps its pencil along the
+// third direction in registers. The tensor interpolation runs direction by
+// direction: the pencil direction in registers, the other two slab by slab
+// through two LDS slabs. BP3 then runs the slab-swept gradient/weak
+// divergence on the quadrature grid (the same scheme as the BP5 kernel),
+// and the projection back mirrors the interpolation. Only MQ x (MQ+1)
+// slabs live in LDS, so p = 15 fits.
+#include <hip/hip_runtime.h>
+
+#include "ax_common.hpp"
+
+#ifndef BPX_NQ
+#error "compile with -DBPX_NQ=<Nq>"
+#endif
+
+namespace bpx {
+namespace {
+
+enum { MODE_BP1 = 0, MODE_BP1A = 1, MODE_BP3 = 2, MODE_BP3A = 3 };
+
+template <int MQ, int NF>
+constexpr int cub_waves() {
+  return NF == 1 ? (MQ <= 9 ? 2 : 1) : 1;
+}
+
+template <typename T, int NQ, int MQ, int NF, int MODE, int WAVES>
+__global__ __launch_bounds__(MQ* MQ) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_cub_ax(
+    int Nelements, const int* __restrict__ elementList, const int* __restrict__ G2L,
+    const T* __restrict__ W, const T* __restrict__ G, const T* __restrict__ Dq,
+    const T* __restrict__ I, T lambda, const T* __restrict__ q, T* __restrict__ Aq) {
+  constexpr int NP = NQ * NQ * NQ;
+  constexpr int MP = MQ * MQ * MQ;
+  constexpr int LD = MQ + 1;
+  constexpr bool STIFF = (MODE == MODE_BP3 || MODE == MODE_BP3A);
+  constexpr bool PROJECT = (MODE != MODE_BP1A);
+
+  __shared__ T s_I[MQ][NQ + 1];
+  __shared__ T s_D[STIFF ? MQ : 1][LD];
+  __shared__ T s_a[NF][MQ][LD];
+  __shared__ T s_b[NF][MQ][LD];
+  __shared__ T s_c[STIFF ? NF : 1][MQ][LD];
+
+  const int t = threadIdx.x;
+  const int a = t % MQ;
+  const int b = t / MQ;
+  const int e = elementList[blockIdx.x];
+  const bool in_nodes = (a < NQ) && (b < NQ);
+
+  for (int n = t; n < MQ * NQ; n += MQ * MQ) {
+    const int g = n / NQ, c = n % NQ;
+    // BP1A applies MM^T as okl/bp1AxAffineHex3D.okl does (s_MM[m][i])
+    s_I[g][c] = (MODE == MODE_BP1A) ? I[c * NQ + g] : I[g * NQ + c];
+  }
+  if (STIFF)
+    for (int n = t; n < MQ * MQ; n += MQ * MQ) s_D[n / MQ][n % MQ] = Dq[n];
+
+  T u[NF][MQ];
+  {
+    T r[NF][NQ];
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int c = 0; c < NQ; ++c) {
+        const int id = in_nodes ? G2L[(size_t)e * NP * NF + f * NP + c * NQ * NQ + b * NQ + a] : -1;
+        r[f][c] = (id >= 0) ? q[id] : T(0);
+      }
+    __syncthreads();
+    // interpolate along c in registers
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int g = 0; g < MQ; ++g) {
+        T v = 0;
+#pragma unroll
+        for (int c = 0; c < NQ; ++c) v += s_I[g][c] * r[f][c];
+        u[f][g] = v;
+      }
+  }
+
+  // interpolate along b, then a, slab by slab
+#pragma unroll
+  for (int g = 0; g < MQ; ++g) {
+#pragma unroll
+    for (int f = 0; f < NF; ++f) asm volatile("" : "+v"(u[f][g]));
+    if (in_nodes)
+#pragma unroll
+      for (int f = 0; f < NF; ++f) s_a[f][b][a] = u[f][g];
+    __syncthreads();
+    asm volatile("" ::: "memory");
+    if (a < NQ)
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        T v = 0;
+#pragma unroll
+        for (int m = 0; m < NQ; ++m) v += s_I[b][m] * s_a[f][m][a];
+        s_b[f][b][a] = v;
+      }
+    __syncthreads();
+    asm volatile("" ::: "memory");
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      T v = 0;
+#pragma unroll
+      for (int m = 0; m < NQ; ++m) v += s_I[a][m] * s_b[f][b][m];
+      u[f][g] = v;
+    }
+  }
+
+  if (MODE == MODE_BP1) {
+    const T* Wp = W + (size_t)e * MP + b * MQ + a;
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int g = 0; g < MQ; ++g) u[f][g] *= Wp[g * MQ * MQ];
+  } else if (MODE == MODE_BP1A) {
+    const T J = W[e];
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int g = 0; g < NQ; ++g)
+        Aq[(size_t)e * NP * NF + f * NP + g * NQ * NQ + b * NQ + a] = J * u[f][g];
+    return;
+  } else {
+    // gradient, G, weak divergence on the quadrature grid, g-slab by g-slab
+    T res[NF][MQ];
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int g = 0; g < MQ; ++g) res[f][g] = 0;
+    T e0 = 0, e1 = 0, e2 = 0, e3 = 0, e4 = 0, e5 = 0, eW = 0;
+    if (MODE == MODE_BP3A) {
+      const T* ge = G + (size_t)6 * e;
+      e0 = ge[0]; e1 = ge[1]; e2 = ge[2]; e3 = ge[3]; e4 = ge[4]; e5 = ge[5];
+      eW = W[e];
+    }
+#pragma unroll
+    for (int g = 0; g < MQ; ++g) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f)
+#pragma unroll
+        for (int m = 0; m < MQ; ++m) asm volatile("" : "+v"(res[f][m]));
+      T G00, G01, G02, G11, G12, G22, WW;
+      if (MODE == MODE_BP3A) {
+        G00 = e0; G01 = e1; G02 = e2; G11 = e3; G12 = e4; G22 = e5; WW = eW;
+      } else {
+        const size_t cb = (size_t)e * MP + g * MQ * MQ + b * MQ + a;
+        const T* gp = G + 6 * cb;
+        G00 = gp[0]; G01 = gp[1]; G02 = gp[2]; G11 = gp[3]; G12 = gp[4]; G22 = gp[5];
+        WW = W[cb];
+      }
+#pragma unroll
+      for (int f = 0; f < NF; ++f) s_a[f][b][a] = u[f][g];
+      __syncthreads();
+      asm volatile("" ::: "memory");
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        T ur = 0, us = 0, ut = 0;
+#pragma unroll
+        for (int m = 0; m < MQ; ++m) {
+          ur += s_D[a][m] * s_a[f][b][m];
+          us += s_D[b][m] * s_a[f][m][a];
+          ut += s_D[g][m] * u[f][m];
+        }
+        const T Gr = G00 * ur + G01 * us + G02 * ut;
+        const T Gs = G01 * ur + G11 * us + G12 * ut;
+        const T Gt = G02 * ur + G12 * us + G22 * ut;
+        s_b[f][b][a] = Gr;
+        s_c[f][b][a] = Gs;
+        res[f][g] += lambda * WW * u[f][g];
+#pragma unroll
+        for (int m = 0; m < MQ; ++m) res[f][m] += s_D[g][m] * Gt;
+      }
+      __syncthreads();
+      asm volatile("" ::: "memory");
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        T acc = 0;
+#pragma unroll
+        for (int m = 0; m < MQ; ++m) {
+          acc += s_D[m][a] * s_b[f][b][m];
+          acc += s_D[m][b] * s_c[f][m][a];
+        }
+        res[f][g] += acc;
+      }
+    }
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int g = 0; g < MQ; ++g) u[f][g] = res[f][g];
+  }
+
+  if (PROJECT) {
+    // project back along a, then b, slab by slab
+#pragma unroll
+    for (int g = 0; g < MQ; ++g) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) asm volatile("" : "+v"(u[f][g]));
+      __syncthreads();
+#pragma unroll
+      for (int f = 0; f < NF; ++f) s_a[f][b][a] = u[f][g];
+      __syncthreads();
+      asm volatile("" ::: "memory");
+      if (a < NQ)
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+          T v = 0;
+#pragma unroll
+          for (int m = 0; m < MQ; ++m) v += s_I[m][a] * s_a[f][b][m];
+          s_b[f][b][a] = v;
+        }
+      __syncthreads();
+      asm volatile("" ::: "memory");
+      if (in_nodes)
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+          T v = 0;
+#pragma unroll
+          for (int m = 0; m < MQ; ++m) v += s_I[m][b] * s_b[f][m][a];
+          u[f][g] = v;
+        }
+    }
+    // project back along c in registers
+    if (in_nodes) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f)
+#pragma unroll
+        for (int c = 0; c < NQ; ++c) {
+          T v = 0;
+#pragma unroll
+          for (int g = 0; g < MQ; ++g) v += s_I[g][c] * u[f][g];
+          Aq[(size_t)e * NP * NF + f * NP + c * NQ * NQ + b * NQ + a] = v;
+        }
+    }
+  }
+}
+
+template <typename T, int NQ, int MQ, int NF, int MODE>
+int launch_mode(hipStream_t s, const AxArgs<T>& x, const T* W, const T* G, const T* Dq, const T* I) {
+  constexpr int WV = cub_waves<MQ, NF>();
+  hipLaunchKernelGGL((k_cub_ax<T, NQ, MQ, NF, MODE, WV>), dim3(x.Nelements), dim3(MQ * MQ), 0, s,
+                     x.Nelements, x.elementList, x.GlobalToLocal, W, G, Dq, I, x.lambda, x.q, x.Aq);
+  BPX_HIP(hipGetLastError());
+  return BPX_OK;
+}
+
+}  // namespace
+
+template <typename T, int NQ, int NF>
+int launch_bp1_nq(hipStream_t s, const AxArgs<T>& a) {
+  if (a.affine) return launch_mode<T, NQ, NQ, NF, MODE_BP1A>(s, a, a.wJ, nullptr, nullptr, a.MM);
+  return launch_mode<T, NQ, NQ + 1, NF, MODE_BP1>(s, a, a.wJ, nullptr, nullptr, a.I);
+}
+
+template <typename T, int NQ, int NF>
+int launch_bp3_nq(hipStream_t s, const AxArgs<T>& a) {
+  if (a.affine) return launch_mode<T, NQ, NQ, NF, MODE_BP3A>(s, a, a.wJ, a.ggeo, a.D, a.invV);
+  return launch_mode<T, NQ, NQ + 1, NF, MODE_BP3>(s, a, a.wJ, a.ggeo, a.D, a.I);
+}
+
+template int launch_bp1_nq<double, BPX_NQ, 1>(hipStream_t, const AxArgs<double>&);
+template int launch_bp1_nq<double, BPX_NQ, 3>(hipStream_t, const AxArgs<double>&);
+template int launch_bp1_nq<float, BPX_NQ, 1>(hipStream_t, const AxArgs<float>&);
+template int launch_bp3_nq<double, BPX_NQ, 1>(hipStream_t, const AxArgs<double>&);
+template int launch_bp3_nq<double, BPX_NQ, 3>(hipStream_t, const AxArgs<double>&);
+template int launch_bp3_nq<float, BPX_NQ, 1>(hipStream_t, const AxArgs<float>&);
+
+}  // namespace bpx

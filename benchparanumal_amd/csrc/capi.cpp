@@ -1,0 +1,555 @@
+// extern "C" entry points of include/bpx.h. No C++ exception crosses the
+// ABI: every entry point catches and returns an error code.
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "host/bpx.hpp"
+#include "host/rng.hpp"
+
+using namespace bpx;
+
+struct bpx_platform_s {
+  platform_t* p = nullptr;
+};
+struct bpx_mesh_s {
+  mesh_t m;
+};
+struct bpx_ogs_s {
+  ogs::ogs_t o;
+  std::vector<dlong> G2L;
+  std::vector<hlong> ids;
+};
+struct bpx_bp_s {
+  bpx_platform_s* plat = nullptr;
+  bpx_mesh_s* mesh = nullptr;
+  bp_t bp;
+  cg* solver = nullptr;
+  ~bpx_bp_s() { delete solver; }
+};
+
+#define BPX_GUARD_BEGIN try {
+#define BPX_GUARD_END                                                        \
+  }                                                                          \
+  catch (const std::bad_alloc&) {                                            \
+    return set_error(BPX_ERR_OUT_OF_MEMORY, "host allocation failed");       \
+  }                                                                          \
+  catch (const std::exception& ex) {                                         \
+    return set_error(BPX_ERR_INVALID_ARGUMENT, ex.what());                   \
+  }                                                                          \
+  catch (...) {                                                              \
+    return set_error(BPX_ERR_INVALID_ARGUMENT, "unknown C++ exception");     \
+  }
+
+static int need(const void* p, const char* what) {
+  return p ? BPX_OK : set_error(BPX_ERR_INVALID_ARGUMENT, std::string(what) + " is NULL");
+}
+
+static hipStream_t S(bpx_platform p) { return p->p->stream; }
+
+template <typename T>
+static int copy_out(const std::vector<T>& v, void* dst, int64_t* count, int* dtype, int code) {
+  if (count) *count = (int64_t)v.size();
+  if (dtype) *dtype = code;
+  if (dst && !v.empty()) std::memcpy(dst, v.data(), v.size() * sizeof(T));
+  return BPX_OK;
+}
+
+extern "C" {
+
+const char* bpx_last_error(void) { return last_error(); }
+int bpx_version(void) { return 100; }
+
+int bpx_device_count(int* count) {
+  if (int st = need(count, "count")) return st;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  *count = (e == hipSuccess) ? n : 0;
+  return BPX_OK;
+}
+
+int bpx_random_normal(uint64_t seed, int64_t counter0, int64_t n, double* out) {
+  if (n < 0) return set_error(BPX_ERR_INVALID_ARGUMENT, "random: n < 0");
+  if (n > 0)
+    if (int st = need(out, "out")) return st;
+#pragma omp parallel for
+  for (int64_t i = 0; i < n; ++i) out[i] = rng::normal(seed, (uint64_t)(counter0 + i));
+  return BPX_OK;
+}
+
+// ------------------------------------------------------------ platform --
+int bpx_platform_create(int device, bpx_platform* out) {
+  BPX_GUARD_BEGIN
+  if (int st = need(out, "out")) return st;
+  platform_t* p = nullptr;
+  if (int st = platform_t::Create(device, &p)) return st;
+  *out = new bpx_platform_s{p};
+  return BPX_OK;
+  BPX_GUARD_END
+}
+
+int bpx_platform_destroy(bpx_platform p) {
+  if (!p) return BPX_OK;
+  delete p->p;
+  delete p;
+  return BPX_OK;
+}
+
+int bpx_platform_set_stream(bpx_platform p, void* hip_stream) {
+  if (int st = need(p, "platform")) return st;
+  return p->p->setStream(static_cast<hipStream_t>(hip_stream));
+}
+
+void* bpx_platform_stream(bpx_platform p) { return p ? (void*)p->p->stream : nullptr; }
+
+int bpx_platform_finish(bpx_platform p) {
+  if (int st = need(p, "platform")) return st;
+  return p->p->finish();
+}
+
+// ---------------------------------------------------------- Ax kernels --
+#define AX_COMMON(T)           \
+  AxArgs<T> a;                 \
+  a.N = N;                     \
+  a.Nelements = Nelements;     \
+  a.elementList = elementList; \
+  a.GlobalToLocal = GlobalToLocal; \
+  a.q = q;                     \
+  a.Aq = Aq;
+
+static int bp5_entry(bpx_platform p, int N, int Nelements, const int* elementList, const int* GlobalToLocal,
+                     const double* wJ, const double* ggeo, const double* gllw, const double* D, double lambda,
+                     const double* q, double* Aq, int affine, int Nfields) {
+  if (int st = need(p, "platform")) return st;
+  AX_COMMON(double)
+  a.Nfields = Nfields;
+  a.affine = affine;
+  a.wJ = wJ;
+  a.ggeo = ggeo;
+  a.gllw = gllw;
+  a.D = D;
+  a.lambda = lambda;
+  return launch_bp5_ax<double>(S(p), a);
+}
+
+int bpx_bp5_ax_hex(bpx_platform p, int N, int Nelements, const int* elementList, const int* GlobalToLocal,
+                   const double* wJ, const double* vgeo, const double* ggeo, const double* gllw, const double* D,
+                   const double* S_, const double* MM, double lambda, const double* q, double* Aq) {
+  (void)vgeo; (void)S_; (void)MM;
+  return bp5_entry(p, N, Nelements, elementList, GlobalToLocal, wJ, ggeo, gllw, D, lambda, q, Aq, 0, 1);
+}
+int bpx_bp5_ax_affine_hex(bpx_platform p, int N, int Nelements, const int* elementList, const int* GlobalToLocal,
+                          const double* wJ, const double* vgeo, const double* ggeo, const double* gllw,
+                          const double* D, const double* S_, const double* MM, double lambda, const double* q,
+                          double* Aq) {
+  (void)vgeo; (void)S_; (void)MM;
+  return bp5_entry(p, N, Nelements, elementList, GlobalToLocal, wJ, ggeo, gllw, D, lambda, q, Aq, 1, 1);
+}
+int bpx_bp6_ax_hex(bpx_platform p, int N, int Nelements, const int* elementList, const int* GlobalToLocal,
+                   const double* wJ, const double* vgeo, const double* ggeo, const double* gllw, const double* D,
+                   const double* S_, const double* MM, double lambda, const double* q, double* Aq) {
+  (void)vgeo; (void)S_; (void)MM;
+  return bp5_entry(p, N, Nelements, elementList, GlobalToLocal, wJ, ggeo, gllw, D, lambda, q, Aq, 0, 3);
+}
+int bpx_bp6_ax_affine_hex(bpx_platform p, int N, int Nelements, const int* elementList, const int* GlobalToLocal,
+                          const double* wJ, const double* vgeo, const double* ggeo, const double* gllw,
+                          const double* D, const double* S_, const double* MM, double lambda, const double* q,
+                          double* Aq) {
+  (void)vgeo; (void)S_; (void)MM;
+  return bp5_entry(p, N, Nelements, elementList, GlobalToLocal, wJ, ggeo, gllw, D, lambda, q, Aq, 1, 3);
+}
+
+static int bp1_entry(bpx_platform p, int N, int Nelements, const int* elementList, const int* GlobalToLocal,
+                     const double* cubwJ, const double* I, const double* MM, const double* q, double* Aq,
+                     int affine, int Nfields) {
+  if (int st = need(p, "platform")) return st;
+  AX_COMMON(double)
+  a.Nfields = Nfields;
+  a.affine = affine;
+  a.wJ = cubwJ;
+  a.I = I;
+  a.MM = MM;
+  return launch_bp1_ax<double>(S(p), a);
+}
+int bpx_bp1_ax_hex(bpx_platform p, int N, int Nelements, const int* elementList, const int* GlobalToLocal,
+                   const double* cubwJ, const double* I, const double* MM, const double* q, double* Aq) {
+  return bp1_entry(p, N, Nelements, elementList, GlobalToLocal, cubwJ, I, MM, q, Aq, 0, 1);
+}
+int bpx_bp1_ax_affine_hex(bpx_platform p, int N, int Nelements, const int* elementList, const int* GlobalToLocal,
+                          const double* cubwJ, const double* I, const double* MM, const double* q, double* Aq) {
+  return bp1_entry(p, N, Nelements, elementList, GlobalToLocal, cubwJ, I, MM, q, Aq, 1, 1);
+}
+int bpx_bp2_ax_hex(bpx_platform p, int N, int Nelements, const int* elementList, const int* GlobalToLocal,
+                   const double* cubwJ, const double* I, const double* MM, const double* q, double* Aq) {
+  return bp1_entry(p, N, Nelements, elementList, GlobalToLocal, cubwJ, I, MM, q, Aq, 0, 3);
+}
+int bpx_bp2_ax_affine_hex(bpx_platform p, int N, int Nelements, const int* elementList, const int* GlobalToLocal,
+                          const double* cubwJ, const double* I, const double* MM, const double* q, double* Aq) {
+  return bp1_entry(p, N, Nelements, elementList, GlobalToLocal, cubwJ, I, MM, q, Aq, 1, 3);
+}
+
+static int bp3_entry(bpx_platform p, int N, int Nelements, const int* elementList, const int* GlobalToLocal,
+                     const double* W, const double* G, const double* D, const double* I, const double* invV,
+                     double lambda, const double* q, double* Aq, int affine, int Nfields) {
+  if (int st = need(p, "platform")) return st;
+  AX_COMMON(double)
+  a.Nfields = Nfields;
+  a.affine = affine;
+  a.wJ = W;
+  a.ggeo = G;
+  a.D = D;
+  a.I = I;
+  a.invV = invV;
+  a.lambda = lambda;
+  return launch_bp3_ax<double>(S(p), a);
+}
+int bpx_bp3_ax_hex(bpx_platform p, int N, int Nelements, const int* elementList, const int* GlobalToLocal,
+                   const double* cubwJ, const double* cubggeo, const double* D, const double* I,
+                   const double* invV, const double* S_, const double* MM, double lambda, const double* q,
+                   double* Aq) {
+  (void)S_; (void)MM;
+  return bp3_entry(p, N, Nelements, elementList, GlobalToLocal, cubwJ, cubggeo, D, I, invV, lambda, q, Aq, 0, 1);
+}
+int bpx_bp3_ax_affine_hex(bpx_platform p, int N, int Nelements, const int* elementList, const int* GlobalToLocal,
+                          const double* wJ, const double* ggeo, const double* D, const double* I,
+                          const double* invV, const double* S_, const double* MM, double lambda, const double* q,
+                          double* Aq) {
+  (void)S_; (void)MM;
+  return bp3_entry(p, N, Nelements, elementList, GlobalToLocal, wJ, ggeo, D, I, invV, lambda, q, Aq, 1, 1);
+}
+int bpx_bp4_ax_hex(bpx_platform p, int N, int Nelements, const int* elementList, const int* GlobalToLocal,
+                   const double* cubwJ, const double* cubggeo, const double* D, const double* I,
+                   const double* invV, const double* S_, const double* MM, double lambda, const double* q,
+                   double* Aq) {
+  (void)S_; (void)MM;
+  return bp3_entry(p, N, Nelements, elementList, GlobalToLocal, cubwJ, cubggeo, D, I, invV, lambda, q, Aq, 0, 3);
+}
+int bpx_bp4_ax_affine_hex(bpx_platform p, int N, int Nelements, const int* elementList, const int* GlobalToLocal,
+                          const double* wJ, const double* ggeo, const double* D, const double* I,
+                          const double* invV, const double* S_, const double* MM, double lambda, const double* q,
+                          double* Aq) {
+  (void)S_; (void)MM;
+  return bp3_entry(p, N, Nelements, elementList, GlobalToLocal, wJ, ggeo, D, I, invV, lambda, q, Aq, 1, 3);
+}
+
+int bpx_bp5_ax_hex_f32(bpx_platform p, int N, int Nelements, const int* elementList, const int* GlobalToLocal,
+                       const float* wJ, const float* vgeo, const float* ggeo, const float* gllw, const float* D,
+                       const float* S_, const float* MM, float lambda, const float* q, float* Aq) {
+  (void)vgeo; (void)S_; (void)MM;
+  if (int st = need(p, "platform")) return st;
+  AX_COMMON(float)
+  a.wJ = wJ;
+  a.ggeo = ggeo;
+  a.gllw = gllw;
+  a.D = D;
+  a.lambda = lambda;
+  return launch_bp5_ax<float>(S(p), a);
+}
+
+int bpx_bp3_ax_hex_f32(bpx_platform p, int N, int Nelements, const int* elementList, const int* GlobalToLocal,
+                       const float* cubwJ, const float* cubggeo, const float* D, const float* I,
+                       const float* invV, const float* S_, const float* MM, float lambda, const float* q,
+                       float* Aq) {
+  (void)S_; (void)MM;
+  if (int st = need(p, "platform")) return st;
+  AX_COMMON(float)
+  a.wJ = cubwJ;
+  a.ggeo = cubggeo;
+  a.D = D;
+  a.I = I;
+  a.invV = invV;
+  a.lambda = lambda;
+  return launch_bp3_ax<float>(S(p), a);
+}
+
+int bpx_rhs_hex(bpx_platform p, int N, int Nfields, int affine, int Nelements, const double* wJ,
+                const double* gllw, const double* MM, const double* x, const double* y, const double* z,
+                double lambda, double* rhs) {
+  (void)MM;
+  if (int st = need(p, "platform")) return st;
+  return launch_rhs(S(p), N, Nfields, affine, Nelements, wJ, gllw, x, y, z, lambda, rhs);
+}
+
+// ------------------------------------------------------------ ogs kernels --
+int bpx_ogs_gather(bpx_platform p, int Nblocks, int K, const int* blockStarts, const int* gatherStarts,
+                   const int* gatherIds, const double* q, double* gatherq) {
+  if (int st = need(p, "platform")) return st;
+  return launch_ogs_gather<double>(S(p), Nblocks, K, blockStarts, gatherStarts, gatherIds, q, gatherq);
+}
+int bpx_ogs_gather_f32(bpx_platform p, int Nblocks, int K, const int* blockStarts, const int* gatherStarts,
+                       const int* gatherIds, const float* q, float* gatherq) {
+  if (int st = need(p, "platform")) return st;
+  return launch_ogs_gather<float>(S(p), Nblocks, K, blockStarts, gatherStarts, gatherIds, q, gatherq);
+}
+int bpx_ogs_scatter(bpx_platform p, int Nblocks, int K, const int* blockStarts, const int* scatterStarts,
+                    const int* scatterIds, const double* gatherq, double* q) {
+  if (int st = need(p, "platform")) return st;
+  return launch_ogs_scatter<double>(S(p), Nblocks, K, blockStarts, scatterStarts, scatterIds, gatherq, q);
+}
+int bpx_ogs_gather_scatter(bpx_platform p, int Nblocks, int K, const int* blockStarts, const int* gatherStarts,
+                           const int* gatherIds, double* q) {
+  if (int st = need(p, "platform")) return st;
+  return launch_ogs_gather_scatter<double>(S(p), Nblocks, K, blockStarts, gatherStarts, gatherIds, q);
+}
+
+// -------------------------------------------------------------- linAlg --
+int bpx_linalg_set(bpx_platform p, int N, double alpha, double* x) {
+  if (int st = need(p, "platform")) return st;
+  return launch_set<double>(S(p), N, alpha, x);
+}
+int bpx_linalg_axpy(bpx_platform p, int N, double alpha, const double* x, double beta, double* y) {
+  if (int st = need(p, "platform")) return st;
+  return launch_axpy<double>(S(p), N, alpha, x, beta, y);
+}
+int bpx_linalg_inner_prod(bpx_platform p, int N, const double* x, const double* y, double* result) {
+  if (int st = need(p, "platform")) return st;
+  if (int st = need(result, "result")) return st;
+  linAlg_t la(*p->p);
+  return la.innerProd(N, x, y, result);
+}
+int bpx_linalg_norm2(bpx_platform p, int N, const double* x, double* result) {
+  if (int st = need(p, "platform")) return st;
+  if (int st = need(result, "result")) return st;
+  linAlg_t la(*p->p);
+  return la.norm2(N, x, result);
+}
+int bpx_cg_update(bpx_platform p, int N, const double* Ap, const double* o_p, double alpha, double* r, double* x,
+                  double* rdotr) {
+  if (int st = need(p, "platform")) return st;
+  if (int st = need(rdotr, "rdotr")) return st;
+  // one CG state slot run through the fused update (iteration bookkeeping ignored)
+  deviceBuffer<CgState> stt;
+  if (int st = stt.alloc(1)) return st;
+  CgState h{};
+  h.alpha = alpha;
+  h.maxit = 1 << 30;
+  h.tol2 = -1.0;
+  BPX_HIP(hipMemcpyAsync(stt.ptr(), &h, sizeof(h), hipMemcpyHostToDevice, S(p)));
+  if (int st = launch_cg_update_xr<double>(S(p), N, o_p, Ap, x, r, p->p->reduce_scratch, stt.ptr(), nullptr,
+                                           nullptr))
+    return st;
+  BPX_HIP(hipMemcpyAsync(&h, stt.ptr(), sizeof(h), hipMemcpyDeviceToHost, S(p)));
+  if (int st = p->p->finish()) return st;
+  *rdotr = h.rdotr;
+  return BPX_OK;
+}
+
+// ---------------------------------------------------------------- mesh --
+int bpx_mesh_create(const bpx_mesh_opts* o, bpx_mesh* out) {
+  BPX_GUARD_BEGIN
+  if (int st = need(o, "opts")) return st;
+  if (int st = need(out, "out")) return st;
+  MeshOpts mo;
+  mo.N = o->N;
+  mo.nx = o->nx;
+  mo.ny = o->ny;
+  mo.nz = o->nz;
+  mo.affine = o->affine;
+  mo.cubature = o->cubature;
+  for (int c = 0; c < 3; ++c) {
+    mo.lo[c] = o->lo[c];
+    mo.dim[c] = o->dim[c];
+  }
+  mo.perturb = o->perturb;
+  mo.seed = o->seed;
+  bpx_mesh_s* m = new bpx_mesh_s;
+  if (int st = m->m.Setup(mo)) {
+    delete m;
+    return st;
+  }
+  *out = m;
+  return BPX_OK;
+  BPX_GUARD_END
+}
+
+int bpx_mesh_destroy(bpx_mesh m) {
+  delete m;
+  return BPX_OK;
+}
+
+int bpx_mesh_info(bpx_mesh m, int64_t* out, int capacity) {
+  if (int st = need(m, "mesh")) return st;
+  const int64_t v[] = {m->m.N, m->m.Nq, m->m.Np, m->m.cubNq, m->m.cubNp, m->m.Nelements};
+  const int n = (int)(sizeof(v) / sizeof(v[0]));
+  for (int i = 0; i < n && i < capacity; ++i) out[i] = v[i];
+  return n;
+}
+
+int bpx_mesh_array(bpx_mesh m, const char* name, void* dst, int64_t* count, int* dtype) {
+  BPX_GUARD_BEGIN
+  if (int st = need(m, "mesh")) return st;
+  if (int st = need(name, "name")) return st;
+  const mesh_t& M = m->m;
+  const std::string s(name);
+#define F64(f) if (s == #f) return copy_out(M.f, dst, count, dtype, 0);
+  F64(gllz) F64(gllw) F64(D) F64(MM) F64(cubr) F64(cubw) F64(cubInterp) F64(cubD) F64(invV)
+  F64(x) F64(y) F64(z) F64(wJ) F64(ggeo) F64(cubwJ) F64(cubggeo) F64(EX) F64(EY) F64(EZ)
+#undef F64
+  if (s == "globalIds") return copy_out(M.globalIds, dst, count, dtype, 2);
+  return set_error(BPX_ERR_INVALID_ARGUMENT, "mesh: unknown array " + s);
+  BPX_GUARD_END
+}
+
+// ----------------------------------------------------------------- ogs --
+int bpx_ogs_setup(int N, const int64_t* ids, bpx_ogs* out) {
+  BPX_GUARD_BEGIN
+  if (int st = need(out, "out")) return st;
+  if (N > 0)
+    if (int st = need(ids, "ids")) return st;
+  bpx_ogs_s* o = new bpx_ogs_s;
+  o->ids.assign(ids, ids + N);
+  if (int st = o->o.Setup(N, o->ids.data())) {
+    delete o;
+    return st;
+  }
+  o->G2L.resize(N);
+  o->o.SetupGlobalToLocalMapping(o->G2L.data());
+  *out = o;
+  return BPX_OK;
+  BPX_GUARD_END
+}
+
+int bpx_ogs_setup_masked(bpx_mesh m, int Nfields, bpx_ogs* out) {
+  BPX_GUARD_BEGIN
+  if (int st = need(m, "mesh")) return st;
+  if (int st = need(out, "out")) return st;
+  if (Nfields < 1) return set_error(BPX_ERR_INVALID_ARGUMENT, "ogs: Nfields < 1");
+  bpx_ogs_s* o = new bpx_ogs_s;
+  m->m.MaskedGlobalIds(Nfields, o->ids);
+  if (int st = o->o.Setup((dlong)o->ids.size(), o->ids.data())) {
+    delete o;
+    return st;
+  }
+  o->G2L.resize(o->ids.size());
+  o->o.SetupGlobalToLocalMapping(o->G2L.data());
+  *out = o;
+  return BPX_OK;
+  BPX_GUARD_END
+}
+
+int bpx_ogs_destroy(bpx_ogs o) {
+  delete o;
+  return BPX_OK;
+}
+
+int bpx_ogs_info(bpx_ogs o, int64_t* out, int capacity) {
+  if (int st = need(o, "ogs")) return st;
+  const int64_t v[] = {o->o.N, o->o.Ngather, o->o.nnzT, o->o.NrowBlocksT};
+  const int n = 4;
+  for (int i = 0; i < n && i < capacity; ++i) out[i] = v[i];
+  return n;
+}
+
+int bpx_ogs_array(bpx_ogs o, const char* name, void* dst, int64_t* count, int* dtype) {
+  BPX_GUARD_BEGIN
+  if (int st = need(o, "ogs")) return st;
+  if (int st = need(name, "name")) return st;
+  const std::string s(name);
+  if (s == "rowStartsT") return copy_out(o->o.rowStartsT, dst, count, dtype, 1);
+  if (s == "colIdsT") return copy_out(o->o.colIdsT, dst, count, dtype, 1);
+  if (s == "blockRowStartsT") return copy_out(o->o.blockRowStartsT, dst, count, dtype, 1);
+  if (s == "GlobalToLocal") return copy_out(o->G2L, dst, count, dtype, 1);
+  if (s == "maskedIds") return copy_out(o->ids, dst, count, dtype, 2);
+  return set_error(BPX_ERR_INVALID_ARGUMENT, "ogs: unknown array " + s);
+  BPX_GUARD_END
+}
+
+int bpx_ogs_gather_dev(bpx_platform p, bpx_ogs o, int K, const double* v, double* gv) {
+  if (int st = need(p, "platform")) return st;
+  if (int st = need(o, "ogs")) return st;
+  if (!o->o.o_rowStartsT.ptr() && o->o.Ngather > 0)
+    if (int st = o->o.Upload(*p->p)) return st;
+  return o->o.Gather<double>(*p->p, gv, v, K);
+}
+
+int bpx_ogs_scatter_dev(bpx_platform p, bpx_ogs o, int K, const double* gv, double* v) {
+  if (int st = need(p, "platform")) return st;
+  if (int st = need(o, "ogs")) return st;
+  if (!o->o.o_rowStartsT.ptr() && o->o.Ngather > 0)
+    if (int st = o->o.Upload(*p->p)) return st;
+  return o->o.Scatter<double>(*p->p, v, gv, K);
+}
+
+// ------------------------------------------------------------------ BP --
+int bpx_bp_create(bpx_platform p, bpx_mesh m, int bp, double lambda, bpx_bp* out) {
+  BPX_GUARD_BEGIN
+  if (int st = need(p, "platform")) return st;
+  if (int st = need(m, "mesh")) return st;
+  if (int st = need(out, "out")) return st;
+  bpx_bp_s* b = new bpx_bp_s;
+  b->plat = p;
+  b->mesh = m;
+  if (int st = b->bp.Setup(*p->p, m->m, bp, lambda)) {
+    delete b;
+    return st;
+  }
+  b->solver = new cg(*p->p, b->bp.ogs.Ngather, 0);
+  if (int st = b->solver->status()) {
+    delete b;
+    return st;
+  }
+  *out = b;
+  return BPX_OK;
+  BPX_GUARD_END
+}
+
+int bpx_bp_destroy(bpx_bp b) {
+  delete b;
+  return BPX_OK;
+}
+
+int bpx_bp_info(bpx_bp b, int64_t* out, int capacity) {
+  if (int st = need(b, "bp")) return st;
+  const int64_t v[] = {b->bp.bp, b->bp.Nfields, b->bp.ogs.Ngather, b->bp.NLocal, b->bp.mesh->Nelements,
+                       b->bp.mesh->N};
+  const int n = 6;
+  for (int i = 0; i < n && i < capacity; ++i) out[i] = v[i];
+  return n;
+}
+
+int bpx_bp_operator(bpx_bp b, const double* o_q, double* o_Aq) {
+  if (int st = need(b, "bp")) return st;
+  return b->bp.Operator(o_q, o_Aq);
+}
+
+int bpx_bp_ax_local(bpx_bp b, const double* o_q, double* o_AqL) {
+  if (int st = need(b, "bp")) return st;
+  return b->bp.AxLocal(o_q, o_AqL);
+}
+
+int bpx_bp_operator_f32(bpx_bp b, const float* o_q, float* o_Aq) {
+  if (int st = need(b, "bp")) return st;
+  return b->bp.Operator32(o_q, o_Aq);
+}
+
+int bpx_bp_rhs_forcing(bpx_bp b, double* o_b) {
+  if (int st = need(b, "bp")) return st;
+  return b->bp.Forcing(o_b);
+}
+
+int bpx_bp_solve(bpx_bp b, double* o_x, double* o_r, double tol, int maxit, int verbose, int* iterations,
+                 double* rnorm_hist) {
+  BPX_GUARD_BEGIN
+  if (int st = need(b, "bp")) return st;
+  int its = 0;
+  const int st = b->solver->Solve(b->bp, o_x, o_r, tol, maxit, verbose, &its);
+  if (iterations) *iterations = its;
+  if (rnorm_hist) {
+    std::vector<double> rr, al;
+    if (int s2 = b->solver->impl.History(rr, al)) return s2;
+    for (int i = 0; i <= maxit; ++i) rnorm_hist[i] = (i < (int)rr.size()) ? std::sqrt(rr[i]) : 0.0;
+  }
+  return st;
+  BPX_GUARD_END
+}
+
+int bpx_bp_solve_mixed(bpx_bp b, double* o_x, const double* o_b, double tol, double inner_tol, int max_outer,
+                       int max_inner, double* stats) {
+  BPX_GUARD_BEGIN
+  if (int st = need(b, "bp")) return st;
+  return b->bp.SolveMixed(o_x, o_b, tol, inner_tol, max_outer, max_inner, stats);
+  BPX_GUARD_END
+}
+
+}  // extern "C"

@@ -1,0 +1,223 @@
+// C++ host layer of the BK/BP hot path. Class and method names follow the
+// reference interfaces the path sits behind (platform_t, mesh_t,
+// ogs::ogs_t, linAlg_t, solver_t, linearSolver_t/cg, bpN_t) so host code
+// written against benchParanumal reads the same; the device work goes
+// through the HIP launchers of bpx_internal.hpp.
+#pragma once
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../bpx_internal.hpp"
+
+namespace bpx {
+
+typedef double dfloat;
+
+// ------------------------------------------------------------ device ----
+// Owning device allocation (the reference's deviceMemory<T> from
+// platform_t::malloc, include/memory.hpp:288-460, without offsets).
+template <typename T>
+class deviceBuffer {
+ public:
+  deviceBuffer() = default;
+  ~deviceBuffer() { release(); }
+  deviceBuffer(const deviceBuffer&) = delete;
+  deviceBuffer& operator=(const deviceBuffer&) = delete;
+  int alloc(size_t n) {
+    release();
+    if (n == 0) return BPX_OK;
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, n * sizeof(T));
+    if (e != hipSuccess) return hip_error(e, "hipMalloc");
+    ptr_ = static_cast<T*>(p);
+    n_ = n;
+    return BPX_OK;
+  }
+  // synchronous copy: the host source may be a temporary
+  int upload(const T* h, size_t n, hipStream_t s) {
+    (void)s;
+    if (int st = alloc(n)) return st;
+    if (n == 0) return BPX_OK;
+    BPX_HIP(hipMemcpy(ptr_, h, n * sizeof(T), hipMemcpyHostToDevice));
+    return BPX_OK;
+  }
+  int upload(const std::vector<T>& v, hipStream_t s) { return upload(v.data(), v.size(), s); }
+  void release() {
+    if (ptr_) (void)hipFree(ptr_);
+    ptr_ = nullptr;
+    n_ = 0;
+  }
+  T* ptr() const { return ptr_; }
+  size_t length() const { return n_; }
+
+ private:
+  T* ptr_ = nullptr;
+  size_t n_ = 0;
+};
+
+// ---------------------------------------------------------- platform ----
+// platform_t (include/platform.hpp:61-185): one device, one stream, the
+// linAlg scratch (reduction partials) and a pinned host scalar.
+class platform_t {
+ public:
+  ~platform_t();
+  static int Create(int device, platform_t** out);
+  int finish();
+  int setStream(hipStream_t s);
+
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  void* reduce_scratch = nullptr;  // kReduceBlocks partials + ticket
+  double* d_scalar = nullptr;      // device result slot
+  double* h_scalar = nullptr;      // pinned host copy
+};
+
+// ------------------------------------------------------------ linAlg ----
+// linAlg_t (include/linAlg.hpp:38-115; libs/core/linAlg.cpp:38-89).
+class linAlg_t {
+ public:
+  explicit linAlg_t(platform_t& p) : platform(p) {}
+  int set(dlong N, dfloat alpha, dfloat* o_x);
+  int axpy(dlong N, dfloat alpha, const dfloat* o_x, dfloat beta, dfloat* o_y);
+  int innerProd(dlong N, const dfloat* o_x, const dfloat* o_y, dfloat* result);  // synchronises
+  int norm2(dlong N, const dfloat* o_x, dfloat* result);                           // synchronises
+  platform_t& platform;
+};
+
+// -------------------------------------------------------------- mesh ----
+struct MeshOpts {
+  int N = 1, nx = 1, ny = 1, nz = 1;
+  int affine = 0, cubature = 0;
+  double lo[3] = {-1, -1, -1}, dim[3] = {2, 2, 2};
+  double perturb = 0.0;
+  uint64_t seed = 0;
+};
+
+// mesh_t for the hex box (libs/mesh/meshSetup.cpp:31-85), host data.
+class mesh_t {
+ public:
+  int Setup(const MeshOpts& o);
+  // mesh_t::MaskedGatherScatterSetup ids (meshMaskedGatherScatterSetup.cpp:31-77)
+  void MaskedGlobalIds(int Nfields, std::vector<hlong>& ids) const;
+
+  MeshOpts opts;
+  int N = 0, Nq = 0, Np = 0, cubNq = 0, cubNp = 0;
+  dlong Nelements = 0;
+  std::vector<double> gllz, gllw, D, MM, cubr, cubw, cubInterp, cubD, invV;
+  std::vector<double> EX, EY, EZ, x, y, z;
+  std::vector<double> wJ, ggeo, cubwJ, cubggeo;
+  std::vector<hlong> globalIds;
+};
+
+// ---------------------------------------------------------------- ogs ---
+namespace ogs {
+// ogs::ogs_t (include/ogs.hpp:216-341) on one rank: Signed, unique, the
+// T (transpose) maps only; the N maps pick a random member per group in
+// the reference (ogsSetup.cpp:361-382) and are off the path.
+class ogs_t {
+ public:
+  int Setup(dlong N, const hlong* ids);
+  void SetupGlobalToLocalMapping(dlong* GlobalToLocal) const;
+  int Upload(platform_t& p);  // device copies of the T maps
+  template <typename T>
+  int Gather(platform_t& p, T* o_gv, const T* o_v, int K) const;  // Add, Trans
+  template <typename T>
+  int Scatter(platform_t& p, T* o_v, const T* o_gv, int K) const;  // NoTrans
+  int GatherScatter(platform_t& p, dfloat* o_v, int K) const;
+
+  dlong N = 0, Ngather = 0, nnzT = 0, NrowBlocksT = 0;
+  std::vector<dlong> rowStartsT, colIdsT, blockRowStartsT;
+  deviceBuffer<dlong> o_rowStartsT, o_colIdsT, o_blockRowStartsT;
+};
+}  // namespace ogs
+
+// ------------------------------------------------------ solver / CG -----
+// solver_t (include/solver.hpp:34-50)
+class solver_t {
+ public:
+  virtual ~solver_t() = default;
+  virtual int Operator(const dfloat* o_q, dfloat* o_Aq) = 0;
+};
+
+// linearSolver_t (include/linearSolver.hpp:37-81)
+class linearSolver_t {
+ public:
+  virtual ~linearSolver_t() = default;
+  virtual int Solve(solver_t& solver, dfloat* o_x, dfloat* o_r, dfloat tol, int MAXIT, int verbose,
+                    int* iterations) = 0;
+};
+
+// cg (libs/core/linearSolverCG.cpp:33-156) with device-resident scalars.
+// T is the vector/operator precision; reductions accumulate in FP64.
+template <typename T>
+class cg_t {
+ public:
+  int Setup(platform_t& p, dlong N);
+  // apply(o_q, o_Aq) is the operator in precision T.
+  template <class OP>
+  int Solve(OP&& apply, T* o_x, T* o_r, double tol, int MAXIT, int* iterations);
+  // squared residual norms r.r of the last Solve (entry 0 = initial)
+  int History(std::vector<double>& rr, std::vector<double>& alpha);
+
+  platform_t* platform = nullptr;
+  dlong N = 0;
+  deviceBuffer<T> o_p, o_Ap;
+  deviceBuffer<CgState> o_state;
+  deviceBuffer<double> o_hist_rr, o_hist_alpha;
+  int hist_len = 0;
+  int last_maxit = 0;
+};
+
+class cg : public linearSolver_t {
+ public:
+  cg(platform_t& p, dlong N, dlong Nhalo);
+  int status() const { return status_; }
+  int Solve(solver_t& solver, dfloat* o_x, dfloat* o_r, dfloat tol, int MAXIT, int verbose,
+            int* iterations) override;
+  cg_t<double> impl;
+
+ private:
+  int status_ = BPX_OK;
+};
+
+// ------------------------------------------------------------------ BP --
+// bpN_t (BP/BP5/bp5.hpp:43-74) for N = 1..6 on the hex box.
+class bp_t : public solver_t {
+ public:
+  int Setup(platform_t& p, mesh_t& m, int bp, dfloat lambda);
+  int Operator(const dfloat* o_q, dfloat* o_Aq) override;
+  int AxLocal(const dfloat* o_q, dfloat* o_AqL);
+  int Operator32(const float* o_q, float* o_Aq);
+  int Forcing(dfloat* o_b);
+  int SolveMixed(dfloat* o_x, const dfloat* o_b, double tol, double inner_tol, int max_outer,
+                 int max_inner, double* stats);
+
+  platform_t* platform = nullptr;
+  mesh_t* mesh = nullptr;
+  int bp = 5, Nfields = 1;
+  dfloat lambda = 1.0;
+  ogs::ogs_t ogs;
+  dlong NLocal = 0;
+  std::vector<dlong> GlobalToLocal;
+  deviceBuffer<dlong> o_GlobalToLocal, o_elementList;
+  deviceBuffer<double> o_wJ, o_ggeo, o_gllw, o_D, o_I, o_MM, o_invV;
+  deviceBuffer<double> o_AqL;
+  // FP32 copies (created on first use)
+  int Prepare32();
+  bool have32 = false;
+  deviceBuffer<float> o_wJ32, o_ggeo32, o_gllw32, o_D32, o_I32, o_MM32, o_invV32, o_AqL32;
+  // nodal coordinates for the forcing (created on first use)
+  deviceBuffer<double> o_x, o_y, o_z, o_gllwRhs;
+
+ private:
+  template <typename T>
+  int ax(const T* q, T* AqL, const T* wJ, const T* ggeo, const T* gllw, const T* D, const T* I,
+         const T* MM, const T* invV);
+};
+
+}  // namespace bpx

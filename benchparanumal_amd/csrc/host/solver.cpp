@@ -1,0 +1,298 @@
+// platform_t, linAlg_t, cg and bp_t.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "bpx.hpp"
+#include "cg_impl.hpp"
+
+namespace bpx {
+
+// ------------------------------------------------------------- errors ---
+static thread_local std::string g_last_error;
+
+int set_error(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+const char* last_error() { return g_last_error.c_str(); }
+
+int hip_error(hipError_t e, const char* what) {
+  const int code = (e == hipErrorOutOfMemory) ? BPX_ERR_OUT_OF_MEMORY
+                   : (e == hipErrorNoDevice || e == hipErrorInvalidDevice) ? BPX_ERR_NO_DEVICE
+                                                                          : BPX_ERR_HIP;
+  return set_error(code, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// ----------------------------------------------------------- platform ---
+platform_t::~platform_t() {
+  if (device >= 0) (void)hipSetDevice(device);
+  if (reduce_scratch) (void)hipFree(reduce_scratch);
+  if (d_scalar) (void)hipFree(d_scalar);
+  if (h_scalar) (void)hipHostFree(h_scalar);
+  if (own_stream && stream) (void)hipStreamDestroy(stream);
+}
+
+int platform_t::Create(int dev, platform_t** out) {
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count == 0) return set_error(BPX_ERR_NO_DEVICE, "platform: no HIP device visible");
+  if (dev < 0 || dev >= count) return set_error(BPX_ERR_INVALID_ARGUMENT, "platform: device index out of range");
+  platform_t* p = new platform_t;
+  p->device = dev;
+  int st = BPX_OK;
+  auto fail = [&](int s) {
+    delete p;
+    return s;
+  };
+  if ((e = hipSetDevice(dev)) != hipSuccess) return fail(hip_error(e, "hipSetDevice"));
+  if ((e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking)) != hipSuccess)
+    return fail(hip_error(e, "hipStreamCreate"));
+  p->own_stream = true;
+  if ((e = hipMalloc(&p->reduce_scratch, reduce_scratch_bytes())) != hipSuccess)
+    return fail(hip_error(e, "hipMalloc(scratch)"));
+  if ((e = hipMemset(p->reduce_scratch, 0, reduce_scratch_bytes())) != hipSuccess)
+    return fail(hip_error(e, "hipMemset(scratch)"));
+  if ((e = hipMalloc(&p->d_scalar, sizeof(double))) != hipSuccess) return fail(hip_error(e, "hipMalloc"));
+  if ((e = hipHostMalloc(&p->h_scalar, sizeof(double), hipHostMallocDefault)) != hipSuccess)
+    return fail(hip_error(e, "hipHostMalloc"));
+  (void)st;
+  *out = p;
+  return BPX_OK;
+}
+
+int platform_t::finish() {
+  BPX_HIP(hipStreamSynchronize(stream));
+  return BPX_OK;
+}
+
+int platform_t::setStream(hipStream_t s) {
+  if (int st = finish()) return st;
+  if (own_stream && stream) (void)hipStreamDestroy(stream);
+  own_stream = false;
+  stream = s;
+  return BPX_OK;
+}
+
+// ------------------------------------------------------------- linAlg ---
+int linAlg_t::set(dlong N, dfloat alpha, dfloat* o_x) { return launch_set<double>(platform.stream, N, alpha, o_x); }
+
+int linAlg_t::axpy(dlong N, dfloat alpha, const dfloat* o_x, dfloat beta, dfloat* o_y) {
+  return launch_axpy<double>(platform.stream, N, alpha, o_x, beta, o_y);
+}
+
+int linAlg_t::innerProd(dlong N, const dfloat* o_x, const dfloat* o_y, dfloat* result) {
+  if (int st = launch_dot<double>(platform.stream, N, o_x, o_y, platform.reduce_scratch, platform.d_scalar))
+    return st;
+  BPX_HIP(hipMemcpyAsync(platform.h_scalar, platform.d_scalar, sizeof(double), hipMemcpyDeviceToHost,
+                         platform.stream));
+  if (int st = platform.finish()) return st;
+  *result = *platform.h_scalar;
+  return BPX_OK;
+}
+
+int linAlg_t::norm2(dlong N, const dfloat* o_x, dfloat* result) {
+  double d;
+  if (int st = innerProd(N, o_x, o_x, &d)) return st;
+  *result = std::sqrt(d);
+  return BPX_OK;
+}
+
+// ----------------------------------------------------------------- cg ---
+template <typename T>
+int cg_t<T>::Setup(platform_t& p, dlong N_) {
+  platform = &p;
+  N = N_;
+  if (int st = o_p.alloc(N)) return st;
+  if (int st = o_Ap.alloc(N)) return st;
+  if (int st = o_state.alloc(1)) return st;
+  if (N > 0) {
+    BPX_HIP(hipMemsetAsync(o_p.ptr(), 0, N * sizeof(T), p.stream));
+    BPX_HIP(hipMemsetAsync(o_Ap.ptr(), 0, N * sizeof(T), p.stream));
+  }
+  return BPX_OK;
+}
+
+template <typename T>
+int cg_t<T>::History(std::vector<double>& rr, std::vector<double>& alpha) {
+  CgState hst;
+  BPX_HIP(hipMemcpy(&hst, o_state.ptr(), sizeof(CgState), hipMemcpyDeviceToHost));
+  rr.assign(hst.iter + 1, 0.0);
+  alpha.assign(hst.iter, 0.0);
+  if (hist_len == 0) return BPX_OK;
+  BPX_HIP(hipMemcpy(rr.data(), o_hist_rr.ptr(), rr.size() * sizeof(double), hipMemcpyDeviceToHost));
+  if (hst.iter > 0)
+    BPX_HIP(hipMemcpy(alpha.data(), o_hist_alpha.ptr(), alpha.size() * sizeof(double), hipMemcpyDeviceToHost));
+  return BPX_OK;
+}
+
+template class cg_t<double>;
+template class cg_t<float>;
+
+cg::cg(platform_t& p, dlong N, dlong Nhalo) { status_ = impl.Setup(p, N + Nhalo); }
+
+int cg::Solve(solver_t& solver, dfloat* o_x, dfloat* o_r, dfloat tol, int MAXIT, int verbose, int* iterations) {
+  if (status_) return status_;
+  int its = 0;
+  const int st = impl.Solve([&](const double* q, double* Aq) { return solver.Operator(q, Aq); }, o_x, o_r, tol,
+                            MAXIT, &its);
+  if (iterations) *iterations = its;
+  if (verbose) {
+    std::vector<double> rr, al;
+    if (impl.History(rr, al) == BPX_OK) {
+      printf("CG: initial res norm %12.12f \n", std::sqrt(rr[0]));
+      for (int it = 1; it < (int)rr.size(); ++it)
+        printf("CG: it %d, r norm %12.12le, alpha = %le \n", it, std::sqrt(rr[it]), al[it - 1]);
+    }
+  }
+  return st;
+}
+
+// ----------------------------------------------------------------- BP ---
+int bp_t::Setup(platform_t& p, mesh_t& m, int bp_, dfloat lambda_) {
+  if (bp_ < 1 || bp_ > 6) return set_error(BPX_ERR_INVALID_ARGUMENT, "bp: bp must be in 1..6");
+  const bool needCub = bp_ <= 4;
+  if (needCub && m.cubD.empty())
+    return set_error(BPX_ERR_INVALID_ARGUMENT, "bp: BP1-BP4 need a mesh built with cubature=1");
+  platform = &p;
+  mesh = &m;
+  bp = bp_;
+  lambda = lambda_;
+  Nfields = (bp == 2 || bp == 4 || bp == 6) ? 3 : 1;
+  BPX_HIP(hipSetDevice(p.device));
+  hipStream_t s = p.stream;
+
+  // masked ogs + GlobalToLocal (BP/BP5/src/bp5Setup.cpp:43-52)
+  std::vector<hlong> ids;
+  m.MaskedGlobalIds(Nfields, ids);
+  NLocal = (dlong)ids.size();
+  if (int st = ogs.Setup(NLocal, ids.data())) return st;
+  GlobalToLocal.resize(NLocal);
+  ogs.SetupGlobalToLocalMapping(GlobalToLocal.data());
+  if (int st = ogs.Upload(p)) return st;
+  if (int st = o_GlobalToLocal.upload(GlobalToLocal, s)) return st;
+  std::vector<dlong> list(m.Nelements);
+  for (dlong e = 0; e < m.Nelements; ++e) list[e] = e;
+  if (int st = o_elementList.upload(list, s)) return st;
+  if (int st = o_AqL.alloc(NLocal)) return st;
+
+  // operator data (BP/BPn/src/bpnOperator.cpp argument lists)
+  const int affine = m.opts.affine;
+  if (bp == 5 || bp == 6) {
+    if (int st = o_wJ.upload(m.wJ, s)) return st;
+    if (int st = o_ggeo.upload(m.ggeo, s)) return st;
+    if (int st = o_gllw.upload(m.gllw, s)) return st;
+    if (int st = o_D.upload(m.D, s)) return st;
+  } else if (bp == 1 || bp == 2) {
+    if (int st = o_wJ.upload(affine ? m.wJ : m.cubwJ, s)) return st;
+    if (int st = o_I.upload(m.cubInterp, s)) return st;
+    if (int st = o_MM.upload(m.MM, s)) return st;
+  } else {
+    if (int st = o_wJ.upload(affine ? m.wJ : m.cubwJ, s)) return st;
+    if (int st = o_ggeo.upload(affine ? m.ggeo : m.cubggeo, s)) return st;
+    if (int st = o_D.upload(m.cubD, s)) return st;
+    if (int st = o_I.upload(m.cubInterp, s)) return st;
+    if (affine)
+      if (int st = o_invV.upload(m.invV, s)) return st;
+  }
+  return p.finish();
+}
+
+template <typename T>
+int bp_t::ax(const T* q, T* AqL, const T* wJ, const T* ggeo, const T* gllw, const T* D, const T* I, const T* MM,
+             const T* invV) {
+  AxArgs<T> a;
+  a.N = mesh->N;
+  a.Nelements = mesh->Nelements;
+  a.Nfields = Nfields;
+  a.affine = mesh->opts.affine;
+  a.elementList = o_elementList.ptr();
+  a.GlobalToLocal = o_GlobalToLocal.ptr();
+  a.wJ = wJ;
+  a.ggeo = ggeo;
+  a.gllw = gllw;
+  a.D = D;
+  a.I = I;
+  a.MM = MM;
+  a.invV = invV;
+  a.lambda = (T)lambda;
+  a.q = q;
+  a.Aq = AqL;
+  if (bp == 5 || bp == 6) return launch_bp5_ax<T>(platform->stream, a);
+  if (bp == 1 || bp == 2) return launch_bp1_ax<T>(platform->stream, a);
+  return launch_bp3_ax<T>(platform->stream, a);
+}
+
+int bp_t::AxLocal(const dfloat* o_q, dfloat* o_AqL) {
+  return ax<double>(o_q, o_AqL, o_wJ.ptr(), o_ggeo.ptr(), o_gllw.ptr(), o_D.ptr(), o_I.ptr(), o_MM.ptr(),
+                    o_invV.ptr());
+}
+
+// bpN_t::Operator (BP/BP5/src/bp5Operator.cpp:29-85). On one GPU the halo
+// exchanges move nothing, so the two element halves become one launch.
+int bp_t::Operator(const dfloat* o_q, dfloat* o_Aq) {
+  if (int st = AxLocal(o_q, o_AqL.ptr())) return st;
+  return ogs.Gather<double>(*platform, o_Aq, o_AqL.ptr(), 1);
+}
+
+int bp_t::Prepare32() {
+  if (have32) return BPX_OK;
+  hipStream_t s = platform->stream;
+  auto cvt = [&](deviceBuffer<float>& dst, const std::vector<double>& src) {
+    std::vector<float> f(src.begin(), src.end());
+    return dst.upload(f, s);
+  };
+  const mesh_t& m = *mesh;
+  const int affine = m.opts.affine;
+  int st = BPX_OK;
+  if (bp == 5 || bp == 6) {
+    if ((st = cvt(o_wJ32, m.wJ)) || (st = cvt(o_ggeo32, m.ggeo)) || (st = cvt(o_gllw32, m.gllw)) ||
+        (st = cvt(o_D32, m.D)))
+      return st;
+  } else if (bp == 1 || bp == 2) {
+    if ((st = cvt(o_wJ32, affine ? m.wJ : m.cubwJ)) || (st = cvt(o_I32, m.cubInterp)) || (st = cvt(o_MM32, m.MM)))
+      return st;
+  } else {
+    if ((st = cvt(o_wJ32, affine ? m.wJ : m.cubwJ)) || (st = cvt(o_ggeo32, affine ? m.ggeo : m.cubggeo)) ||
+        (st = cvt(o_D32, m.cubD)) || (st = cvt(o_I32, m.cubInterp)))
+      return st;
+    if (affine && (st = cvt(o_invV32, m.invV))) return st;
+  }
+  if ((st = o_AqL32.alloc(NLocal))) return st;
+  if ((st = platform->finish())) return st;
+  have32 = true;
+  return BPX_OK;
+}
+
+int bp_t::Operator32(const float* o_q, float* o_Aq) {
+  if (Nfields != 1) return set_error(BPX_ERR_UNSUPPORTED, "bp: the FP32 operator covers one field (BP1/3/5)");
+  if (int st = Prepare32()) return st;
+  if (int st = ax<float>(o_q, o_AqL32.ptr(), o_wJ32.ptr(), o_ggeo32.ptr(), o_gllw32.ptr(), o_D32.ptr(),
+                         o_I32.ptr(), o_MM32.ptr(), o_invV32.ptr()))
+    return st;
+  return ogs.Gather<float>(*platform, o_Aq, o_AqL32.ptr(), 1);
+}
+
+// b = gather(forcing) (BP/BP5/src/bp5Run.cpp:51-64); BP1/BP2 force with lambda = 0.
+int bp_t::Forcing(dfloat* o_b) {
+  hipStream_t s = platform->stream;
+  const mesh_t& m = *mesh;
+  if (!o_x.ptr()) {
+    if (int st = o_x.upload(m.x, s)) return st;
+    if (int st = o_y.upload(m.y, s)) return st;
+    if (int st = o_z.upload(m.z, s)) return st;
+    if (int st = o_gllwRhs.upload(m.gllw, s)) return st;
+  }
+  // the forcing uses the GLL-weighted wJ of the mesh (mesh.o_wJ)
+  deviceBuffer<double> wJ;
+  if (int st = wJ.upload(m.wJ, s)) return st;
+  const double lam = (bp <= 2) ? 0.0 : lambda;
+  if (int st = launch_rhs(s, m.N, Nfields, m.opts.affine, m.Nelements, wJ.ptr(), o_gllwRhs.ptr(), o_x.ptr(),
+                          o_y.ptr(), o_z.ptr(), lam, o_AqL.ptr()))
+    return st;
+  if (int st = ogs.Gather<double>(*platform, o_b, o_AqL.ptr(), 1)) return st;
+  return platform->finish();
+}
+
+}  // namespace bpx

@@ -1,0 +1,265 @@
+/*
+ * bpx.h — C ABI of the MI355X (gfx950) BK/BP hex hot path.
+ *
+ * Drop-in boundary for paranumal/benchparanumal's OCCA path. Two levels:
+ *
+ *  1. Kernel level: one entry point per reference kernel, with the
+ *     reference kernel's positional arguments (device pointers, counts,
+ *     scalars), asynchronous on the platform's stream. They replace
+ *     platform_t::buildKernel + occa::kernel launches.
+ *  2. Object level: platform / mesh / ogs / BP solver handles whose calls
+ *     mirror mesh_t, ogs::ogs_t, linAlg_t, linearSolver_t (cg) and bpN_t.
+ *
+ * Conventions (mirroring include/types.h:32-69 and include/memory.hpp):
+ *  - dfloat is double (suffix-free entry points) or float (_f32);
+ *    dlong is int32; hlong (global ids) is int64.
+ *  - Every pointer named o_* or documented "device" is device memory owned
+ *    by the caller; the library never frees caller memory and never
+ *    allocates inside a kernel-level call.
+ *  - Every call returns BPX_OK (0) or a negative bpx_status; the message of
+ *    the last failure on the calling thread is bpx_last_error(). Nothing
+ *    throws across the ABI (the reference's LIBP_ABORT, libs/core/
+ *    exception.cpp:75-79, becomes an error code).
+ *  - Calls are asynchronous on the platform stream unless they return a
+ *    host scalar (then they synchronise that stream, as linAlg_t::norm2 does
+ *    at libs/core/linAlg.cpp:61-62).
+ */
+#ifndef BPX_H
+#define BPX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  BPX_OK = 0,
+  BPX_ERR_INVALID_ARGUMENT = -1,
+  BPX_ERR_UNSUPPORTED = -2,
+  BPX_ERR_HIP = -3,
+  BPX_ERR_OUT_OF_MEMORY = -4,
+  BPX_ERR_NEGATIVE_JACOBIAN = -5, /* libs/mesh/meshGeometricFactorsHex3D.cpp:69-70,118-119 */
+  BPX_ERR_OGS = -6,               /* libs/ogs/ogsOperator.cpp:487-488, ogsSetup.cpp:827-830 */
+  BPX_ERR_BREAKDOWN = -7,         /* non-finite p.Ap or r.r in CG */
+  BPX_ERR_NO_DEVICE = -8
+} bpx_status;
+
+typedef struct bpx_platform_s* bpx_platform; /* replaces platform_t (include/platform.hpp:61-185) */
+typedef struct bpx_mesh_s* bpx_mesh;         /* replaces mesh_t (include/mesh.hpp) */
+typedef struct bpx_ogs_s* bpx_ogs;           /* replaces ogs::ogs_t (include/ogs.hpp:216-341) */
+typedef struct bpx_bp_s* bpx_bp;             /* replaces bpN_t (BP/BP5/bp5.hpp:43-74) */
+
+const char* bpx_last_error(void);
+int bpx_version(void);
+/* 1 when this build has HIP device code for gfx950 and a device is visible. */
+int bpx_device_count(int* count);
+
+/* Synthetic inputs (no reference counterpart): out[i] is N(0,1) draw number
+ * counter0+i of the SplitMix64 + Box-Muller stream `seed` (host memory). */
+int bpx_random_normal(uint64_t seed, int64_t counter0, int64_t n, double* out);
+
+/* ------------------------------------------------------------ platform --
+ * platform_t(settings) (include/platform.hpp:72-82): picks the device and
+ * owns the stream, linAlg scratch and pinned scalars. */
+int bpx_platform_create(int device, bpx_platform* out);
+int bpx_platform_destroy(bpx_platform p);
+/* Run on a caller stream (hipStream_t) instead of the platform's own. */
+int bpx_platform_set_stream(bpx_platform p, void* hip_stream);
+void* bpx_platform_stream(bpx_platform p);
+/* platform_t::finish (include/platform.hpp:179-181) */
+int bpx_platform_finish(bpx_platform p);
+
+/* ------------------------------------------------ kernel level: Ax ------
+ * Same positional arguments as the OKL kernels. N is the polynomial
+ * degree (1..15), the reference's compile-time p_N. Unused reference
+ * arguments (vgeo, S, ...) are accepted and ignored, as in the OKL. */
+
+/* okl/bp5AxHex3D.okl:415-427 */
+int bpx_bp5_ax_hex(bpx_platform p, int N, int Nelements, const int* elementList,
+                   const int* GlobalToLocal, const double* wJ, const double* vgeo,
+                   const double* ggeo, const double* gllw, const double* D, const double* S,
+                   const double* MM, double lambda, const double* q, double* Aq);
+/* okl/bp5AxAffineHex3D.okl:419-431 */
+int bpx_bp5_ax_affine_hex(bpx_platform p, int N, int Nelements, const int* elementList,
+                          const int* GlobalToLocal, const double* wJ, const double* vgeo,
+                          const double* ggeo, const double* gllw, const double* D,
+                          const double* S, const double* MM, double lambda, const double* q,
+                          double* Aq);
+/* okl/bp6AxHex3D.okl:555-567 (3 fields, local layout [e][f][n]) */
+int bpx_bp6_ax_hex(bpx_platform p, int N, int Nelements, const int* elementList,
+                   const int* GlobalToLocal, const double* wJ, const double* vgeo,
+                   const double* ggeo, const double* gllw, const double* D, const double* S,
+                   const double* MM, double lambda, const double* q, double* Aq);
+/* okl/bp6AxAffineHex3D.okl */
+int bpx_bp6_ax_affine_hex(bpx_platform p, int N, int Nelements, const int* elementList,
+                          const int* GlobalToLocal, const double* wJ, const double* vgeo,
+                          const double* ggeo, const double* gllw, const double* D,
+                          const double* S, const double* MM, double lambda, const double* q,
+                          double* Aq);
+/* okl/bp1AxHex3D.okl:61-68 (I = cubInterp [cubNq x Nq]) */
+int bpx_bp1_ax_hex(bpx_platform p, int N, int Nelements, const int* elementList,
+                   const int* GlobalToLocal, const double* cubwJ, const double* I,
+                   const double* MM, const double* q, double* Aq);
+/* okl/bp1AxAffineHex3D.okl:59-66 (cubwJ = per-element J) */
+int bpx_bp1_ax_affine_hex(bpx_platform p, int N, int Nelements, const int* elementList,
+                          const int* GlobalToLocal, const double* cubwJ, const double* I,
+                          const double* MM, const double* q, double* Aq);
+/* okl/bp2AxHex3D.okl (3 fields of BP1) */
+int bpx_bp2_ax_hex(bpx_platform p, int N, int Nelements, const int* elementList,
+                   const int* GlobalToLocal, const double* cubwJ, const double* I,
+                   const double* MM, const double* q, double* Aq);
+int bpx_bp2_ax_affine_hex(bpx_platform p, int N, int Nelements, const int* elementList,
+                          const int* GlobalToLocal, const double* cubwJ, const double* I,
+                          const double* MM, const double* q, double* Aq);
+/* okl/bp3AxHex3D.okl:369-381 (D = cubD [cubNq x cubNq], I = cubInterp) */
+int bpx_bp3_ax_hex(bpx_platform p, int N, int Nelements, const int* elementList,
+                   const int* GlobalToLocal, const double* cubwJ, const double* cubggeo,
+                   const double* D, const double* I, const double* invV, const double* S,
+                   const double* MM, double lambda, const double* q, double* Aq);
+/* okl/bp3AxAffineHex3D.okl:668-680 (D = modal cubD [Nq x Nq], invV) */
+int bpx_bp3_ax_affine_hex(bpx_platform p, int N, int Nelements, const int* elementList,
+                          const int* GlobalToLocal, const double* wJ, const double* ggeo,
+                          const double* D, const double* I, const double* invV,
+                          const double* S, const double* MM, double lambda, const double* q,
+                          double* Aq);
+/* okl/bp4AxHex3D.okl (3 fields of BP3) */
+int bpx_bp4_ax_hex(bpx_platform p, int N, int Nelements, const int* elementList,
+                   const int* GlobalToLocal, const double* cubwJ, const double* cubggeo,
+                   const double* D, const double* I, const double* invV, const double* S,
+                   const double* MM, double lambda, const double* q, double* Aq);
+int bpx_bp4_ax_affine_hex(bpx_platform p, int N, int Nelements, const int* elementList,
+                          const int* GlobalToLocal, const double* wJ, const double* ggeo,
+                          const double* D, const double* I, const double* invV,
+                          const double* S, const double* MM, double lambda, const double* q,
+                          double* Aq);
+
+/* FP32 operator (no reference counterpart: the reference builds its
+ * kernels with dfloat=double only, include/types.h:32-40). Same arguments
+ * with float data; geometric factors and D/I matrices rounded to FP32. */
+int bpx_bp5_ax_hex_f32(bpx_platform p, int N, int Nelements, const int* elementList,
+                       const int* GlobalToLocal, const float* wJ, const float* vgeo,
+                       const float* ggeo, const float* gllw, const float* D, const float* S,
+                       const float* MM, float lambda, const float* q, float* Aq);
+int bpx_bp3_ax_hex_f32(bpx_platform p, int N, int Nelements, const int* elementList,
+                       const int* GlobalToLocal, const float* cubwJ, const float* cubggeo,
+                       const float* D, const float* I, const float* invV, const float* S,
+                       const float* MM, float lambda, const float* q, float* Aq);
+
+/* okl/rhsHex3D.okl:44-85 / :87-128. Nfields 1 or 3. */
+int bpx_rhs_hex(bpx_platform p, int N, int Nfields, int affine, int Nelements,
+                const double* wJ, const double* gllw, const double* MM, const double* x,
+                const double* y, const double* z, double lambda, double* rhs);
+
+/* ---------------------------------------------- kernel level: ogs -------
+ * libs/ogs/okl/ogsKernels.okl:86-92 (gather, Add) and :127-133 (scatter).
+ * K interleaved vectors (entry k of node n at k + n*K). blockStarts are
+ * the <=512-nnz row blocks of libs/ogs/ogsOperator.cpp:474-546. */
+int bpx_ogs_gather(bpx_platform p, int Nblocks, int K, const int* blockStarts,
+                   const int* gatherStarts, const int* gatherIds, const double* q,
+                   double* gatherq);
+int bpx_ogs_gather_f32(bpx_platform p, int Nblocks, int K, const int* blockStarts,
+                       const int* gatherStarts, const int* gatherIds, const float* q,
+                       float* gatherq);
+int bpx_ogs_scatter(bpx_platform p, int Nblocks, int K, const int* blockStarts,
+                    const int* scatterStarts, const int* scatterIds, const double* gatherq,
+                    double* q);
+/* libs/ogs/okl/ogsKernels.okl:32-81 (gather then scatter back in place) */
+int bpx_ogs_gather_scatter(bpx_platform p, int Nblocks, int K, const int* blockStarts,
+                           const int* gatherStarts, const int* gatherIds, double* q);
+
+/* ------------------------------------------- kernel level: linAlg -------
+ * libs/core/linAlg.cpp:38-89 (set, axpy, norm2, innerProd). The
+ * reductions are deterministic: the same inputs give the same bits on
+ * every call. */
+int bpx_linalg_set(bpx_platform p, int N, double alpha, double* x);
+int bpx_linalg_axpy(bpx_platform p, int N, double alpha, const double* x, double beta,
+                    double* y);
+/* Host-scalar results synchronise the stream (libs/core/linAlg.cpp:61-62). */
+int bpx_linalg_inner_prod(bpx_platform p, int N, const double* x, const double* y,
+                          double* result);
+int bpx_linalg_norm2(bpx_platform p, int N, const double* x, double* result);
+/* cg::UpdateCG (libs/core/linearSolverCG.cpp:131-156): r -= alpha*Ap,
+ * x += alpha*p, returns r.r. */
+int bpx_cg_update(bpx_platform p, int N, const double* Ap, const double* o_p, double alpha,
+                  double* r, double* x, double* rdotr);
+
+/* ---------------------------------------------------------------- mesh --
+ * mesh_t(platform, settings, comm) -> Setup for a hex box
+ * (libs/mesh/meshSetup.cpp:31-85, meshSetupBoxHex3D.cpp:31-131).
+ * lo/dim: box corner and size (reference: lo=-1, dim=2). perturb: seeded
+ * interior-vertex displacement as a fraction of h (0 = the reference's
+ * undistorted box). cubature: also build the Gauss (p+2) factors
+ * (meshCubatureSetupHex3D.cpp). Host-side setup; copies to the device on
+ * first use. */
+typedef struct {
+  int N, nx, ny, nz;
+  int affine;
+  int cubature;
+  double lo[3], dim[3];
+  double perturb;
+  uint64_t seed;
+} bpx_mesh_opts;
+
+int bpx_mesh_create(const bpx_mesh_opts* opts, bpx_mesh* out);
+int bpx_mesh_destroy(bpx_mesh m);
+/* counts: N Nq Np cubNq cubNp Nelements (int64 each) */
+int bpx_mesh_info(bpx_mesh m, int64_t* out, int capacity);
+/* Copy a host array of the mesh (names: gllz gllw D MM cubr cubw cubInterp
+ * cubD invV x y z wJ ggeo cubwJ cubggeo EX EY EZ globalIds) into dst.
+ * *count gets the element count; dtype 0 = f64, 2 = i64. dst may be NULL
+ * to query the size. */
+int bpx_mesh_array(bpx_mesh m, const char* name, void* dst, int64_t* count, int* dtype);
+
+/* ----------------------------------------------------------------- ogs --
+ * ogs::ogs_t::Setup (include/ogs.hpp:221-232, libs/ogs/ogsSetup.cpp:87-290)
+ * on one rank, Signed, unique: ids[N] (0 = masked), then the T maps and
+ * GlobalToLocal (ogsSetup.cpp:833-849). Host-side; no device needed. */
+int bpx_ogs_setup(int N, const int64_t* ids, bpx_ogs* out);
+/* mesh_t::MaskedGatherScatterSetup(Nfields) (meshMaskedGatherScatterSetup.
+ * cpp:31-77): masked ids of the box, then ogs_t::Setup. */
+int bpx_ogs_setup_masked(bpx_mesh m, int Nfields, bpx_ogs* out);
+int bpx_ogs_destroy(bpx_ogs o);
+/* counts: N Ngather nnzT NrowBlocksT (int64 each) */
+int bpx_ogs_info(bpx_ogs o, int64_t* out, int capacity);
+/* names: rowStartsT colIdsT blockRowStartsT GlobalToLocal maskedIds */
+int bpx_ogs_array(bpx_ogs o, const char* name, void* dst, int64_t* count, int* dtype);
+/* ogs_t::Gather(o_gv, o_v, K, Add, Trans) and Scatter(o_v, o_gv, K, NoTrans)
+ * (libs/ogs/ogs.cpp:205-301, :394-492) on device vectors. */
+int bpx_ogs_gather_dev(bpx_platform p, bpx_ogs o, int K, const double* v, double* gv);
+int bpx_ogs_scatter_dev(bpx_platform p, bpx_ogs o, int K, const double* gv, double* v);
+
+/* ------------------------------------------------------------------ BP --
+ * bpN_t(platform, settings, mesh, lambda) (BP/BP5/src/bp5Setup.cpp:29-95)
+ * for bp in 1..6: masked ogs, GlobalToLocal, operator data on the device. */
+int bpx_bp_create(bpx_platform p, bpx_mesh m, int bp, double lambda, bpx_bp* out);
+int bpx_bp_destroy(bpx_bp b);
+/* counts: bp Nfields Ngather NLocal Nelements N (int64 each) */
+int bpx_bp_info(bpx_bp b, int64_t* out, int capacity);
+/* bpN_t::Operator(o_q, o_Aq) (BP/BP5/src/bp5Operator.cpp:29-85): Ax over the
+ * local elements, then the T gather. Device vectors of length Ngather. */
+int bpx_bp_operator(bpx_bp b, const double* o_q, double* o_Aq);
+/* Ax alone into the local vector (BK/BK5/src/bk5Run.cpp:59-71). */
+int bpx_bp_ax_local(bpx_bp b, const double* o_q, double* o_AqL);
+/* FP32 operator on the same mesh (factors rounded to float). */
+int bpx_bp_operator_f32(bpx_bp b, const float* o_q, float* o_Aq);
+/* b = gather(forcing) as bpN_t::Run does (BP/BP5/src/bp5Run.cpp:51-64). */
+int bpx_bp_rhs_forcing(bpx_bp b, double* o_b);
+/* cg::Solve (libs/core/linearSolverCG.cpp:62-129) with device-resident
+ * scalars: no host wait inside an iteration. o_r holds b on entry and the
+ * residual on exit; o_x the initial guess / solution. rnorm_hist (host,
+ * may be NULL) gets maxit+1 residual norms (entry 0 = initial). Returns
+ * BPX_OK and *iterations. */
+int bpx_bp_solve(bpx_bp b, double* o_x, double* o_r, double tol, int maxit, int verbose,
+                 int* iterations, double* rnorm_hist);
+/* FP32 inner CG inside an FP64 defect-correction loop until
+ * ||b - A x|| / ||b|| < tol (FP64 residual, norm and update on the GPU).
+ * inner_tol: relative reduction per inner solve; stats (host, may be NULL):
+ * [outer iterations, total inner iterations, final relative residual]. */
+int bpx_bp_solve_mixed(bpx_bp b, double* o_x, const double* o_b, double tol, double inner_tol,
+                       int max_outer, int max_inner, double* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BPX_H */

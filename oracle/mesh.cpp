@@ -482,6 +482,14 @@ int blockRows(dlong Nrows, const std::vector<dlong>& rowStarts, dlong& Nblocks,
   sizes[0] = rowStarts[Nrows];
   dlong maxSize = sizes[0];
   while (maxSize > gatherNodesPerBlock) {
+    // The reference sizes these buffers for log2(Nrows) passes; unbalanced
+    // row sizes can need one more, which writes past them there. Grow them
+    // so the bisection completes as intended.
+    if ((dlong)newS.size() < 2 * Nblocks + 1) {
+      oldS.resize(2 * Nblocks + 1);
+      newS.resize(2 * Nblocks + 1);
+      sizes.resize(2 * Nblocks);
+    }
     newS[2 * Nblocks] = Nrows;
     for (dlong n = 0; n < Nblocks; ++n) {
       const dlong start = oldS[n], end = oldS[n + 1];

@@ -1,0 +1,134 @@
+"""CPU-side checks of libbpx.so: the C ABI loads and exports every symbol
+include/bpx.h declares, and the host setup (closed-form basis, box mesh,
+geometric factors, ogs T maps) agrees with the oracle's restatement of the
+reference. No GPU is needed: these calls touch host memory only."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import brute
+import oracle_lib
+from benchparanumal_amd import capi
+
+
+def test_library_exports_every_header_symbol():
+    L = capi.lib()
+    names = capi.header_symbols()
+    assert len(names) > 40
+    for n in names:
+        assert hasattr(L, n), n
+    # the binding declares a signature for each of them
+    assert sorted(capi._SIGS) == names
+
+
+def test_platform_without_device_is_an_error():
+    if capi.device_count() > 0:
+        pytest.skip("a device is visible")
+    with pytest.raises(capi.BpxError) as ei:
+        capi.Platform(0, torch_stream=False)
+    assert ei.value.code == -8  # BPX_ERR_NO_DEVICE
+
+
+@pytest.mark.parametrize("kw,code", [
+    (dict(N=0, nx=2), -1), (dict(N=16, nx=2), -1), (dict(N=3, nx=0), -1),
+    (dict(N=3, nx=2, perturb=0.6), -1), (dict(N=3, nx=2, dim=(2.0, -1.0, 2.0)), -1),
+])
+def test_mesh_rejects_bad_options(kw, code):
+    with pytest.raises(capi.BpxError) as ei:
+        capi.Mesh(**kw)
+    assert ei.value.code == code
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 5, 7, 9, 12, 15])
+def test_basis_matches_oracle(N):
+    m = capi.Mesh(N=N, nx=1, cubature=True)
+    P = oracle_lib.Problem(bp=3, N=N, nx=1)
+    np.testing.assert_allclose(m.array("gllz"), P.array("gllz"), atol=2e-15)
+    np.testing.assert_allclose(m.array("gllw"), P.array("gllw"), rtol=1e-12)
+    D = m.array("D")
+    assert np.abs(D - P.array("D")).max() <= 1e-12 * np.abs(D).max()
+    np.testing.assert_allclose(m.array("MM"), P.array("MM"), atol=1e-13)
+    np.testing.assert_allclose(m.array("cubr"), P.array("cubr"), atol=2e-15)
+    np.testing.assert_allclose(m.array("cubw"), P.array("cubw"), rtol=1e-12)
+    np.testing.assert_allclose(m.array("cubInterp"), P.array("cubInterp"), atol=1e-12)
+    cD = m.array("cubD")
+    assert np.abs(cD - P.array("cubD")).max() <= 1e-12 * np.abs(cD).max()
+
+
+@pytest.mark.parametrize("N", [1, 3, 7, 11])
+def test_affine_modal_operators_match_oracle(N):
+    m = capi.Mesh(N=N, nx=1, cubature=True, affine=True)
+    P = oracle_lib.Problem(bp=3, N=N, nx=1, affine=True)
+    iv = m.array("invV")
+    assert np.abs(iv - P.array("invV")).max() <= 1e-11 * np.abs(iv).max()
+    cD = m.array("cubD")
+    assert np.abs(cD - P.array("cubD")).max() <= 1e-11 * max(1.0, np.abs(cD).max())
+
+
+@pytest.mark.parametrize("affine,perturb", [(False, 0.0), (False, 0.1), (True, 0.0)])
+def test_geometry_matches_oracle(affine, perturb):
+    kw = dict(N=5, nx=3, ny=2, nz=4, affine=affine, perturb=perturb, seed=11)
+    m = capi.Mesh(cubature=True, **kw)
+    P = oracle_lib.Problem(bp=3, lam=1.0, **kw)
+    for name in ("EX", "EY", "EZ", "x", "y", "z"):
+        np.testing.assert_allclose(m.array(name), P.array(name), atol=1e-15)
+    for name in ("wJ", "ggeo") + (() if affine else ("cubwJ", "cubggeo")):
+        a, b = m.array(name), P.array(name)
+        assert a.shape == b.shape
+        assert np.abs(a - b).max() <= 2e-13 * np.abs(b).max(), name
+
+
+@pytest.mark.parametrize("N,nx,ny,nz,nf", [(1, 2, 2, 2, 1), (4, 3, 2, 2, 1), (7, 3, 3, 3, 1),
+                                           (2, 2, 3, 2, 3), (3, 1, 1, 1, 1), (15, 2, 1, 1, 1)])
+def test_ogs_maps_bit_exact_vs_oracle(N, nx, ny, nz, nf):
+    m = capi.Mesh(N=N, nx=nx, ny=ny, nz=nz)
+    o = capi.Ogs(mesh=m, nfields=nf)
+    P = oracle_lib.Problem(bp=5 if nf == 1 else 6, N=N, nx=nx, ny=ny, nz=nz)
+    assert o.Ngather == P.Ngather and o.nnzT == P.nnzT and o.NrowBlocksT == P.NrowBlocksT
+    for name in ("rowStartsT", "colIdsT", "blockRowStartsT", "GlobalToLocal"):
+        np.testing.assert_array_equal(o.array(name), P.array(name), err_msg=name)
+
+
+def test_ogs_general_ids_vs_bruteforce():
+    rng = np.random.default_rng(7)
+    for n, span in [(1, 3), (50, 10), (5000, 700), (9000, 30)]:
+        ids = rng.integers(-span, span + 1, size=n)
+        o = capi.Ogs(ids=ids)
+        rs, ci, g2l = brute.first_appearance_maps(np.abs(ids))
+        np.testing.assert_array_equal(o.array("rowStartsT"), rs)
+        np.testing.assert_array_equal(o.array("colIdsT"), ci)
+        np.testing.assert_array_equal(o.array("GlobalToLocal"), g2l)
+        np.testing.assert_array_equal(o.array("blockRowStartsT"), oracle_lib.block_rows(rs) if len(rs) > 1 else [])
+
+
+def test_ogs_empty_and_all_masked():
+    o = capi.Ogs(ids=np.zeros(10, dtype=np.int64))
+    assert o.Ngather == 0 and o.nnzT == 0 and o.NrowBlocksT == 0
+    np.testing.assert_array_equal(o.array("GlobalToLocal"), -np.ones(10))
+
+
+def test_ogs_rejects_multiplicity_above_512():
+    with pytest.raises(capi.BpxError) as ei:
+        capi.Ogs(ids=np.full(513, 5, dtype=np.int64))
+    assert ei.value.code == -6
+    o = capi.Ogs(ids=np.full(512, 5, dtype=np.int64))
+    assert o.Ngather == 1
+
+
+def test_block_rows_match_oracle_on_random_multiplicities():
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        sizes = rng.integers(1, rng.integers(2, 513), size=rng.integers(1, 3000))
+        ids = np.repeat(np.arange(1, len(sizes) + 1), sizes)
+        rng.shuffle(ids)
+        o = capi.Ogs(ids=ids)
+        rs = o.array("rowStartsT")
+        np.testing.assert_array_equal(o.array("blockRowStartsT"), oracle_lib.block_rows(rs))
+
+
+def test_negative_jacobian_is_reported():
+    # a perturbation of 0.49 h at p=1 still keeps J > 0; fold an element by
+    # using an absurd box instead: zero thickness is rejected as invalid input
+    with pytest.raises(capi.BpxError):
+        capi.Mesh(N=2, nx=2, dim=(2.0, 0.0, 2.0))
