@@ -72,7 +72,7 @@ _SIGS = {
     "bpx_bp_operator_f32": (_I, [_P, _P, _P]),
     "bpx_bp_rhs_forcing": (_I, [_P, _P]),
     "bpx_bp_solve": (_I, [_P, _P, _P, _D, _I, _I, C.POINTER(_I), _P]),
-    "bpx_bp_solve_mixed": (_I, [_P, _P, _P, _D, _D, _I, _I, _P]),
+    "bpx_bp_solve_mixed": (_I, [_P, _P, _P, _D, _I, _I, _P]),
 }
 # Ax kernels: (platform, N, Nelements, elementList, G2L, <pointers...>, [lambda], q, Aq)
 for _name in ("bpx_bp5_ax_hex", "bpx_bp5_ax_affine_hex", "bpx_bp6_ax_hex", "bpx_bp6_ax_affine_hex"):
@@ -266,11 +266,11 @@ class BP:
                                  ptr(hist)))
         return its.value, (hist[: its.value + 1] if history else None)
 
-    def solve_mixed(self, x, b, tol=1e-10, inner_tol=1e-5, max_outer=50, max_inner=5000):
+    def solve_mixed(self, x, b, tol=1e-10, interval=32, maxit=100000):
+        """FP32 CG with FP64 defect corrections every `interval` iterations."""
         stats = np.zeros(3)
-        check(lib().bpx_bp_solve_mixed(self.h, ptr(x), ptr(b), tol, inner_tol, max_outer, max_inner,
-                                       ptr(stats)))
-        return {"outer": int(stats[0]), "inner": int(stats[1]), "rel_res": float(stats[2])}
+        check(lib().bpx_bp_solve_mixed(self.h, ptr(x), ptr(b), tol, interval, maxit, ptr(stats)))
+        return {"updates": int(stats[0]), "iterations": int(stats[1]), "rel_res": float(stats[2])}
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:

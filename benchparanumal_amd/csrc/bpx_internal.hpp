@@ -46,10 +46,16 @@ struct AxArgs {
   T lambda = 0;
   const T* q = nullptr;
   T* Aq = nullptr;
+  // k right-hand sides (BP5 only): q is [Ndofs][K] and Aq [NLocal][K], so
+  // entry r of node n sits at r + n*K, the ogs interleaving
+  int K = 1;
 };
 
 // Launchers (defined in the .hip files). Return BPX_OK or an error code.
 template <typename T> int launch_bp5_ax(hipStream_t s, const AxArgs<T>& a);
+// a.K right-hand sides in one launch; chunk = right-hand sides held in
+// registers per sweep (1, 2 or 4, dividing K; 0 picks the default)
+template <typename T> int launch_bp5_ax_krhs(hipStream_t s, const AxArgs<T>& a, int chunk);
 template <typename T> int launch_bp1_ax(hipStream_t s, const AxArgs<T>& a);
 template <typename T> int launch_bp3_ax(hipStream_t s, const AxArgs<T>& a);
 
@@ -68,12 +74,17 @@ template <typename T>
 int launch_ogs_gather_scatter(hipStream_t s, int Nblocks, int K, const int* blockStarts,
                               const int* rowStarts, const int* colIds, T* q);
 
-// linAlg. Reductions write their result to a device scalar *out (after a
-// two-level deterministic tree); `scratch` holds kReduceBlocks partials
-// plus a ticket counter (see reduce_scratch_bytes).
-constexpr int kReduceThreads = 256;
-constexpr int kReduceBlocks = 1024;
-size_t reduce_scratch_bytes();
+// gather with a fused p.Ap: one FP64 partial per workgroup, in part[0..n)
+// with n = ogs_gather_partials(Nblocks)
+int ogs_gather_partials(int Nblocks);
+template <typename T>
+int launch_ogs_gather_dot(hipStream_t s, int Nblocks, const int* blockStarts, const int* rowStarts,
+                          const int* colIds, const T* q, T* gq, const T* p, double* part);
+
+// linAlg. Reductions run on a fixed grid of kRedBlocks workgroups and write
+// kRedBlocks FP64 partials; launch_fold sums partials into a device scalar.
+constexpr int kRedThreads = 1024;
+constexpr int kRedBlocks = 256;
 template <typename T> int launch_set(hipStream_t s, dlong N, T alpha, T* x);
 template <typename T> int launch_axpy(hipStream_t s, dlong N, T alpha, const T* x, T beta, T* y);
 // y = alpha*x + beta*y in FP64 with x given in FP32 (defect-correction update)
@@ -81,35 +92,45 @@ int launch_axpy_f32_to_f64(hipStream_t s, dlong N, double alpha, const float* x,
                            double* y);
 int launch_copy_f64_to_f32(hipStream_t s, dlong N, double scale, const double* x, float* y);
 template <typename T>
-int launch_dot(hipStream_t s, dlong N, const T* x, const T* y, void* scratch, double* out);
+int launch_dot_partials(hipStream_t s, dlong N, const T* x, const T* y, double* part);
+int launch_fold(hipStream_t s, const double* part, int n, double* out);
+// x.y into *out (device); part holds kRedBlocks doubles
+template <typename T>
+int launch_dot(hipStream_t s, dlong N, const T* x, const T* y, double* part, double* out);
+// cg::UpdateCG (r -= alpha Ap, x += alpha p) with r.r into *out (device)
+int launch_update_cg(hipStream_t s, dlong N, double alpha, const double* p, const double* Ap, double* x,
+                     double* r, double* part, double* out);
 
 // Device-resident CG state (one per solver).
 struct CgState {
-  double rdotr;       // current r.r
-  double rdotr_prev;  // previous r.r (for beta)
-  double alpha;
-  double pAp;
-  double tol2;        // stop when rdotr <= tol2
-  int iter;           // iterations completed
-  int done;           // 1 once rdotr <= tol2 (or breakdown)
-  int breakdown;      // 1 when p.Ap or r.r stopped being finite/positive
+  double tol2;    // stop when r.r <= tol2
+  int iter;       // iterations completed (valid once done, or after finish)
+  int done;       // 1 once r.r <= tol2, the iteration cap is reached, or breakdown
+  int breakdown;  // 1 when p.Ap or r.r stopped being finite/positive
   int maxit;
 };
 
-// CG kernels, templated on the vector type (double: plain CG; float: the
-// inner solve of the mixed-precision path). hist_rr/hist_alpha: device
-// arrays of length maxit+1 / maxit (may be null).
+// CG steps, templated on the vector type (double: plain CG; float: the
+// inner solve of the mixed-precision path). Iteration k is passed by the
+// host. hist_rr[k] = r.r before iteration k (length maxit+1), hist_alpha[k]
+// = alpha of iteration k. part_rr holds kRedBlocks partials of r.r.
 template <typename T>
-int launch_cg_init(hipStream_t s, dlong N, const T* Ap, T* r, void* scratch, CgState* st,
-                   double tol, int maxit, double* hist_rr);
+int launch_cg_init(hipStream_t s, dlong N, const T* Ap, T* r, double* part_rr);
 template <typename T>
-int launch_cg_update_p(hipStream_t s, dlong N, const T* r, T* p, const CgState* st);
+int launch_cg_update_p(hipStream_t s, dlong N, int k, const T* r, T* p, const double* part_rr, int nrr,
+                       CgState* st, double* hist_rr, double tol);
 template <typename T>
-int launch_cg_pAp(hipStream_t s, dlong N, const T* p, const T* Ap, void* scratch, CgState* st);
-template <typename T>
-int launch_cg_update_xr(hipStream_t s, dlong N, const T* p, const T* Ap, T* x, T* r,
-                        void* scratch, CgState* st, double* hist_rr, double* hist_alpha);
+int launch_cg_update_xr(hipStream_t s, dlong N, int k, const T* p, const T* Ap, T* x, T* r,
+                        const double* part_pAp, int npAp, double* part_rr, CgState* st, const double* hist_rr,
+                        double* hist_alpha);
+int launch_cg_finish(hipStream_t s, int kend, int maxit, const double* part_rr, int nrr, CgState* st,
+                     double* hist_rr);
 
-// Geometric factors on the device from nodal coordinates
-// (libs/mesh/meshGeometricFactorsHex3D.cpp:84-139 semantics).
+// Mixed precision reliable update: x64 += c32, c32 = 0; then from Ax (FP64)
+// r32 = float(b - Ax) with kRedBlocks partials of (b-Ax).(b-Ax) in
+// part_true and of r32.r32 in part_rr32.
+int launch_accumulate_f32(hipStream_t s, dlong N, float* c32, double* x64);
+int launch_true_residual(hipStream_t s, dlong N, const double* b, const double* Ax, float* r32, double* part_true,
+                         double* part_rr32);
+
 }  // namespace bpx

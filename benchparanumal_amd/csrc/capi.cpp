@@ -318,20 +318,11 @@ int bpx_cg_update(bpx_platform p, int N, const double* Ap, const double* o_p, do
                   double* rdotr) {
   if (int st = need(p, "platform")) return st;
   if (int st = need(rdotr, "rdotr")) return st;
-  // one CG state slot run through the fused update (iteration bookkeeping ignored)
-  deviceBuffer<CgState> stt;
-  if (int st = stt.alloc(1)) return st;
-  CgState h{};
-  h.alpha = alpha;
-  h.maxit = 1 << 30;
-  h.tol2 = -1.0;
-  BPX_HIP(hipMemcpyAsync(stt.ptr(), &h, sizeof(h), hipMemcpyHostToDevice, S(p)));
-  if (int st = launch_cg_update_xr<double>(S(p), N, o_p, Ap, x, r, p->p->reduce_scratch, stt.ptr(), nullptr,
-                                           nullptr))
-    return st;
-  BPX_HIP(hipMemcpyAsync(&h, stt.ptr(), sizeof(h), hipMemcpyDeviceToHost, S(p)));
-  if (int st = p->p->finish()) return st;
-  *rdotr = h.rdotr;
+  platform_t& P = *p->p;
+  if (int st = launch_update_cg(P.stream, N, alpha, o_p, Ap, x, r, P.partials, P.d_scalar)) return st;
+  BPX_HIP(hipMemcpyAsync(P.h_scalar, P.d_scalar, sizeof(double), hipMemcpyDeviceToHost, P.stream));
+  if (int st = P.finish()) return st;
+  *rdotr = *P.h_scalar;
   return BPX_OK;
 }
 
@@ -484,7 +475,7 @@ int bpx_bp_create(bpx_platform p, bpx_mesh m, int bp, double lambda, bpx_bp* out
     delete b;
     return st;
   }
-  b->solver = new cg(*p->p, b->bp.ogs.Ngather, 0);
+  b->solver = new cg(*p->p, b->bp.ogs.Ngather, 0, b->bp.GatherPartials());
   if (int st = b->solver->status()) {
     delete b;
     return st;
@@ -544,11 +535,11 @@ int bpx_bp_solve(bpx_bp b, double* o_x, double* o_r, double tol, int maxit, int 
   BPX_GUARD_END
 }
 
-int bpx_bp_solve_mixed(bpx_bp b, double* o_x, const double* o_b, double tol, double inner_tol, int max_outer,
-                       int max_inner, double* stats) {
+int bpx_bp_solve_mixed(bpx_bp b, double* o_x, const double* o_b, double tol, int interval, int maxit,
+                       double* stats) {
   BPX_GUARD_BEGIN
   if (int st = need(b, "bp")) return st;
-  return b->bp.SolveMixed(o_x, o_b, tol, inner_tol, max_outer, max_inner, stats);
+  return b->bp.SolveMixed(o_x, o_b, tol, interval, maxit, stats);
   BPX_GUARD_END
 }
 

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <cstdint>
+#include <functional>
 #include <memory>
 #include <string>
 #include <vector>
@@ -72,9 +73,9 @@ class platform_t {
   int device = 0;
   hipStream_t stream = nullptr;
   bool own_stream = false;
-  void* reduce_scratch = nullptr;  // kReduceBlocks partials + ticket
-  double* d_scalar = nullptr;      // device result slot
-  double* h_scalar = nullptr;      // pinned host copy
+  double* partials = nullptr;  // kRedBlocks reduction partials
+  double* d_scalar = nullptr;  // device result slot
+  double* h_scalar = nullptr;  // pinned host copy
 };
 
 // ------------------------------------------------------------ linAlg ----
@@ -142,6 +143,12 @@ class solver_t {
  public:
   virtual ~solver_t() = default;
   virtual int Operator(const dfloat* o_q, dfloat* o_Aq) = 0;
+  // Optional: Operator fused with the FP64 partials of q.Aq (used by cg).
+  virtual bool HasOperatorDot() const { return false; }
+  virtual int OperatorDot(const dfloat* o_q, dfloat* o_Aq, double* part, int* npart) {
+    (void)o_q; (void)o_Aq; (void)part; (void)npart;
+    return set_error(BPX_ERR_UNSUPPORTED, "solver: no fused operator + dot");
+  }
 };
 
 // linearSolver_t (include/linearSolver.hpp:37-81)
@@ -157,10 +164,16 @@ class linearSolver_t {
 template <typename T>
 class cg_t {
  public:
-  int Setup(platform_t& p, dlong N);
-  // apply(o_q, o_Aq) is the operator in precision T.
-  template <class OP>
-  int Solve(OP&& apply, T* o_x, T* o_r, double tol, int MAXIT, int* iterations);
+  // apply(o_q, o_Aq): the operator in precision T.
+  typedef std::function<int(const T*, T*)> Apply;
+  // apply_dot(o_q, o_Aq, part, &npart): the operator fused with the p.Ap
+  // partial sums (npart FP64 partials written to part); may be empty.
+  typedef std::function<int(const T*, T*, double*, int*)> ApplyDot;
+
+  // npart_max: most partials apply_dot writes (at least kRedBlocks is kept)
+  int Setup(platform_t& p, dlong N, int npart_max = 0);
+  int Solve(const Apply& apply, const ApplyDot& apply_dot, T* o_x, T* o_r, double tol, int MAXIT,
+            int* iterations);
   // squared residual norms r.r of the last Solve (entry 0 = initial)
   int History(std::vector<double>& rr, std::vector<double>& alpha);
 
@@ -168,14 +181,14 @@ class cg_t {
   dlong N = 0;
   deviceBuffer<T> o_p, o_Ap;
   deviceBuffer<CgState> o_state;
+  deviceBuffer<double> o_part_rr, o_part_pAp;
   deviceBuffer<double> o_hist_rr, o_hist_alpha;
   int hist_len = 0;
-  int last_maxit = 0;
 };
 
 class cg : public linearSolver_t {
  public:
-  cg(platform_t& p, dlong N, dlong Nhalo);
+  cg(platform_t& p, dlong N, dlong Nhalo, int npart_max = 0);
   int status() const { return status_; }
   int Solve(solver_t& solver, dfloat* o_x, dfloat* o_r, dfloat tol, int MAXIT, int verbose,
             int* iterations) override;
@@ -191,11 +204,26 @@ class bp_t : public solver_t {
  public:
   int Setup(platform_t& p, mesh_t& m, int bp, dfloat lambda);
   int Operator(const dfloat* o_q, dfloat* o_Aq) override;
+  // Operator with the gather fused with the q.Aq partial sums
+  bool HasOperatorDot() const override { return true; }
+  int OperatorDot(const dfloat* o_q, dfloat* o_Aq, double* part, int* npart) override;
+  int OperatorDot32(const float* o_q, float* o_Aq, double* part, int* npart);
+  int GatherPartials() const { return ogs_gather_partials(ogs.NrowBlocksT); }
   int AxLocal(const dfloat* o_q, dfloat* o_AqL);
   int Operator32(const float* o_q, float* o_Aq);
   int Forcing(dfloat* o_b);
-  int SolveMixed(dfloat* o_x, const dfloat* o_b, double tol, double inner_tol, int max_outer,
-                 int max_inner, double* stats);
+  // FP32 CG with FP64 reliable updates every `interval` iterations until
+  // ||b - A x|| / ||b|| < tol; stats = {updates, FP32 iterations, rel. residual}
+  int SolveMixed(dfloat* o_x, const dfloat* o_b, double tol, int interval, int maxit, double* stats);
+
+  // work buffers of SolveMixed, kept between solves
+  struct MixedWork {
+    dlong N = -1;
+    int maxit = 0;
+    deviceBuffer<double> Ax64, part_true, part_rr, part_pAp, hist_rr, hist_alpha, norm;
+    deviceBuffer<float> r32, p32, Ap32, c32;
+    deviceBuffer<CgState> state;
+  } mixed;
 
   platform_t* platform = nullptr;
   mesh_t* mesh = nullptr;

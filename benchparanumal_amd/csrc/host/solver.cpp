@@ -3,8 +3,9 @@
 #include <cstdio>
 #include <cstring>
 
+#include <algorithm>
+
 #include "bpx.hpp"
-#include "cg_impl.hpp"
 
 namespace bpx {
 
@@ -28,7 +29,7 @@ int hip_error(hipError_t e, const char* what) {
 // ----------------------------------------------------------- platform ---
 platform_t::~platform_t() {
   if (device >= 0) (void)hipSetDevice(device);
-  if (reduce_scratch) (void)hipFree(reduce_scratch);
+  if (partials) (void)hipFree(partials);
   if (d_scalar) (void)hipFree(d_scalar);
   if (h_scalar) (void)hipHostFree(h_scalar);
   if (own_stream && stream) (void)hipStreamDestroy(stream);
@@ -50,10 +51,8 @@ int platform_t::Create(int dev, platform_t** out) {
   if ((e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking)) != hipSuccess)
     return fail(hip_error(e, "hipStreamCreate"));
   p->own_stream = true;
-  if ((e = hipMalloc(&p->reduce_scratch, reduce_scratch_bytes())) != hipSuccess)
-    return fail(hip_error(e, "hipMalloc(scratch)"));
-  if ((e = hipMemset(p->reduce_scratch, 0, reduce_scratch_bytes())) != hipSuccess)
-    return fail(hip_error(e, "hipMemset(scratch)"));
+  if ((e = hipMalloc(&p->partials, kRedBlocks * sizeof(double))) != hipSuccess)
+    return fail(hip_error(e, "hipMalloc(partials)"));
   if ((e = hipMalloc(&p->d_scalar, sizeof(double))) != hipSuccess) return fail(hip_error(e, "hipMalloc"));
   if ((e = hipHostMalloc(&p->h_scalar, sizeof(double), hipHostMallocDefault)) != hipSuccess)
     return fail(hip_error(e, "hipHostMalloc"));
@@ -83,7 +82,7 @@ int linAlg_t::axpy(dlong N, dfloat alpha, const dfloat* o_x, dfloat beta, dfloat
 }
 
 int linAlg_t::innerProd(dlong N, const dfloat* o_x, const dfloat* o_y, dfloat* result) {
-  if (int st = launch_dot<double>(platform.stream, N, o_x, o_y, platform.reduce_scratch, platform.d_scalar))
+  if (int st = launch_dot<double>(platform.stream, N, o_x, o_y, platform.partials, platform.d_scalar))
     return st;
   BPX_HIP(hipMemcpyAsync(platform.h_scalar, platform.d_scalar, sizeof(double), hipMemcpyDeviceToHost,
                          platform.stream));
@@ -101,16 +100,75 @@ int linAlg_t::norm2(dlong N, const dfloat* o_x, dfloat* result) {
 
 // ----------------------------------------------------------------- cg ---
 template <typename T>
-int cg_t<T>::Setup(platform_t& p, dlong N_) {
+int cg_t<T>::Setup(platform_t& p, dlong N_, int npart_max) {
   platform = &p;
   N = N_;
   if (int st = o_p.alloc(N)) return st;
   if (int st = o_Ap.alloc(N)) return st;
   if (int st = o_state.alloc(1)) return st;
+  if (int st = o_part_rr.alloc(kRedBlocks)) return st;
+  if (int st = o_part_pAp.alloc(std::max(npart_max, kRedBlocks))) return st;
   if (N > 0) {
     BPX_HIP(hipMemsetAsync(o_p.ptr(), 0, N * sizeof(T), p.stream));
     BPX_HIP(hipMemsetAsync(o_Ap.ptr(), 0, N * sizeof(T), p.stream));
   }
+  return BPX_OK;
+}
+
+// cg::Solve (libs/core/linearSolverCG.cpp:62-129). The host queues
+// iterations without waiting; with tol > 0 it reads the device stop flag
+// after every chunk of 16 iterations, so a converged solve stops early.
+template <typename T>
+int cg_t<T>::Solve(const Apply& apply, const ApplyDot& apply_dot, T* o_x, T* o_r, double tol, int MAXIT,
+                   int* iterations) {
+  if (MAXIT < 0) return set_error(BPX_ERR_INVALID_ARGUMENT, "cg: MAXIT < 0");
+  hipStream_t s = platform->stream;
+  if (hist_len < MAXIT + 1) {
+    if (int st = o_hist_rr.alloc(MAXIT + 1)) return st;
+    if (int st = o_hist_alpha.alloc(MAXIT + 1)) return st;
+    hist_len = MAXIT + 1;
+  }
+  CgState* st_d = o_state.ptr();
+  double* prr = o_part_rr.ptr();
+  double* ppAp = o_part_pAp.ptr();
+  // r <- b - A x (linearSolverCG.cpp:79-82)
+  if (int st = apply(o_x, o_Ap.ptr())) return st;
+  if (int st = launch_cg_init<T>(s, N, o_Ap.ptr(), o_r, prr)) return st;
+  CgState hst;
+  if (MAXIT == 0) {  // only the initial residual and the stopping test
+    if (int st = launch_cg_update_p<T>(s, N, 0, o_r, o_p.ptr(), prr, kRedBlocks, st_d, o_hist_rr.ptr(), tol))
+      return st;
+  }
+  const int chunk = (tol > 0) ? 16 : std::max(MAXIT, 1);
+  int queued = 0;
+  while (queued < MAXIT) {
+    const int n = std::min(chunk, MAXIT - queued);
+    for (int k = queued; k < queued + n; ++k) {
+      if (int st = launch_cg_update_p<T>(s, N, k, o_r, o_p.ptr(), prr, kRedBlocks, st_d, o_hist_rr.ptr(), tol))
+        return st;
+      int npart = kRedBlocks;
+      if (apply_dot) {
+        if (int st = apply_dot(o_p.ptr(), o_Ap.ptr(), ppAp, &npart)) return st;
+      } else {
+        if (int st = apply(o_p.ptr(), o_Ap.ptr())) return st;
+        if (int st = launch_dot_partials<T>(s, N, o_p.ptr(), o_Ap.ptr(), ppAp)) return st;
+      }
+      if (int st = launch_cg_update_xr<T>(s, N, k, o_p.ptr(), o_Ap.ptr(), o_x, o_r, ppAp, npart, prr, st_d,
+                                          o_hist_rr.ptr(), o_hist_alpha.ptr()))
+        return st;
+    }
+    queued += n;
+    if (int st = launch_cg_finish(s, queued, MAXIT, prr, kRedBlocks, st_d, o_hist_rr.ptr())) return st;
+    if (queued < MAXIT) {
+      BPX_HIP(hipMemcpyAsync(&hst, st_d, sizeof(CgState), hipMemcpyDeviceToHost, s));
+      BPX_HIP(hipStreamSynchronize(s));
+      if (hst.done) break;
+    }
+  }
+  BPX_HIP(hipMemcpyAsync(&hst, st_d, sizeof(CgState), hipMemcpyDeviceToHost, s));
+  BPX_HIP(hipStreamSynchronize(s));
+  if (iterations) *iterations = (MAXIT == 0) ? 0 : hst.iter;
+  if (MAXIT > 0 && hst.breakdown) return set_error(BPX_ERR_BREAKDOWN, "cg: breakdown (non-finite or non-positive p.Ap / r.r)");
   return BPX_OK;
 }
 
@@ -130,13 +188,16 @@ int cg_t<T>::History(std::vector<double>& rr, std::vector<double>& alpha) {
 template class cg_t<double>;
 template class cg_t<float>;
 
-cg::cg(platform_t& p, dlong N, dlong Nhalo) { status_ = impl.Setup(p, N + Nhalo); }
+cg::cg(platform_t& p, dlong N, dlong Nhalo, int npart_max) { status_ = impl.Setup(p, N + Nhalo, npart_max); }
 
 int cg::Solve(solver_t& solver, dfloat* o_x, dfloat* o_r, dfloat tol, int MAXIT, int verbose, int* iterations) {
   if (status_) return status_;
   int its = 0;
-  const int st = impl.Solve([&](const double* q, double* Aq) { return solver.Operator(q, Aq); }, o_x, o_r, tol,
-                            MAXIT, &its);
+  cg_t<double>::ApplyDot fused;
+  if (solver.HasOperatorDot())
+    fused = [&](const double* q, double* Aq, double* part, int* n) { return solver.OperatorDot(q, Aq, part, n); };
+  const int st = impl.Solve([&](const double* q, double* Aq) { return solver.Operator(q, Aq); }, fused, o_x, o_r,
+                            tol, MAXIT, &its);
   if (iterations) *iterations = its;
   if (verbose) {
     std::vector<double> rr, al;
@@ -234,6 +295,24 @@ int bp_t::AxLocal(const dfloat* o_q, dfloat* o_AqL) {
 int bp_t::Operator(const dfloat* o_q, dfloat* o_Aq) {
   if (int st = AxLocal(o_q, o_AqL.ptr())) return st;
   return ogs.Gather<double>(*platform, o_Aq, o_AqL.ptr(), 1);
+}
+
+int bp_t::OperatorDot(const dfloat* o_q, dfloat* o_Aq, double* part, int* npart) {
+  if (int st = AxLocal(o_q, o_AqL.ptr())) return st;
+  *npart = GatherPartials();
+  return launch_ogs_gather_dot<double>(platform->stream, ogs.NrowBlocksT, ogs.o_blockRowStartsT.ptr(),
+                                       ogs.o_rowStartsT.ptr(), ogs.o_colIdsT.ptr(), o_AqL.ptr(), o_Aq, o_q, part);
+}
+
+int bp_t::OperatorDot32(const float* o_q, float* o_Aq, double* part, int* npart) {
+  if (Nfields != 1) return set_error(BPX_ERR_UNSUPPORTED, "bp: the FP32 operator covers one field (BP1/3/5)");
+  if (int st = Prepare32()) return st;
+  if (int st = ax<float>(o_q, o_AqL32.ptr(), o_wJ32.ptr(), o_ggeo32.ptr(), o_gllw32.ptr(), o_D32.ptr(),
+                         o_I32.ptr(), o_MM32.ptr(), o_invV32.ptr()))
+    return st;
+  *npart = GatherPartials();
+  return launch_ogs_gather_dot<float>(platform->stream, ogs.NrowBlocksT, ogs.o_blockRowStartsT.ptr(),
+                                      ogs.o_rowStartsT.ptr(), ogs.o_colIdsT.ptr(), o_AqL32.ptr(), o_Aq, o_q, part);
 }
 
 int bp_t::Prepare32() {

@@ -5,15 +5,17 @@
 // cg::UpdateCG of libs/core/linearSolverCG.cpp:62-156 with
 // okl/linearSolverUpdateCG.okl (r -= alpha*Ap fused with r.r).
 //
-// Design (gfx950): every reduction is one launch. Each workgroup folds a
-// grid-strided slice (four independent accumulators per lane to keep four
-// loads in flight), reduces across the wave with xor shuffles and across
-// waves through LDS in a fixed order, stores its partial, and takes a
-// ticket; the last workgroup sums the partials in index order and runs the
-// scalar epilogue (alpha, beta, stopping test) on the device. The grid
-// size depends only on N, so results are bit-reproducible. CG scalars live
-// in a device CgState, so an iteration never waits on the host (the
-// reference waits 8 times per iteration, linearSolverCG.cpp:146 et al.).
+// Design (gfx950, 8 XCDs with private L2s): a reduction never synchronises
+// workgroups inside a kernel. A streaming kernel with a fixed grid of
+// kRedBlocks workgroups writes one FP64 partial per workgroup; the next
+// kernel of the sequence (which needs the scalar anyway) folds the partials
+// at its start, every workgroup in the same fixed order. Kernel boundaries
+// provide the coherence, so there are no device-scope fences or atomics,
+// and with a grid that depends only on kRedBlocks the results are
+// bit-reproducible. The CG scalars (alpha, beta, stopping test, history)
+// are computed on the device; the host only queues kernels, passing the
+// iteration index as an argument, so an iteration never waits on the host
+// (the reference waits 8 times per iteration, linearSolverCG.cpp:146 et al.).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -23,95 +25,51 @@
 namespace bpx {
 namespace {
 
-constexpr int BS = kReduceThreads;
-constexpr int kWaves = BS / 64;
-
-struct ReduceScratch {
-  double partial[kReduceBlocks];
-  unsigned int ticket;
-};
-
-inline int reduce_blocks(dlong N) {
-  long long nb = ((long long)N + BS * 4 - 1) / (BS * 4);
-  if (nb < 1) nb = 1;
-  if (nb > kReduceBlocks) nb = kReduceBlocks;
-  return (int)nb;
-}
+constexpr int RT = kRedThreads;
+constexpr int RB = kRedBlocks;
+constexpr int ST = 256;  // threads of the plain streaming kernels
 
 inline int stream_blocks(dlong N) {
-  long long nb = ((long long)N + BS - 1) / BS;
+  long long nb = ((long long)N + ST - 1) / ST;
   if (nb < 1) nb = 1;
-  if (nb > 8192) nb = 8192;
+  if (nb > 16384) nb = 16384;
   return (int)nb;
 }
 
-// fixed-order block sum; the total is valid in thread 0
-__device__ inline double block_sum(double v, double* s_w) {
+// Fixed-order sum over the workgroup; every thread gets the total.
+__device__ inline double block_allsum(double v, double* s_w) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int nw = (blockDim.x + 63) >> 6;
+  __syncthreads();  // s_w may still be read from a previous call
   if (lane == 0) s_w[w] = v;
   __syncthreads();
-  double tot = 0;
-  if (threadIdx.x == 0)
-#pragma unroll
-    for (int i = 0; i < kWaves; ++i) tot += s_w[i];
-  return tot;
-}
-
-// Publish this block's partial; returns true in every thread of the last
-// block, after which all partials are visible.
-__device__ inline bool finish_partials(ReduceScratch* rs, double blocksum, bool* s_last) {
   if (threadIdx.x == 0) {
-    rs->partial[blockIdx.x] = blocksum;
-    __threadfence();
-    const unsigned t = atomicAdd(&rs->ticket, 1u);
-    *s_last = (t == gridDim.x - 1);
+    double t = 0;
+    for (int i = 0; i < nw; ++i) t += s_w[i];
+    s_w[32] = t;
   }
   __syncthreads();
-  if (!*s_last) return false;
-  __threadfence();
-  return true;
+  return s_w[32];
 }
 
-// Sum of the partials in index order (valid in thread 0); resets the ticket.
-__device__ inline double total_of_partials(ReduceScratch* rs, double* s_w) {
+// Sum of partials[0..n) in a fixed order: lane t < kFoldWidth folds
+// t, t+kFoldWidth, ...; the remaining lanes add exact zeros, so any
+// workgroup size >= kFoldWidth gives the same bits.
+constexpr int kFoldWidth = 256;
+__device__ inline double fold_partials(const double* __restrict__ part, int n, double* s_w) {
   double v = 0;
-  for (int i = threadIdx.x; i < (int)gridDim.x; i += BS)
-    v += __hip_atomic_load(&rs->partial[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __syncthreads();
-  const double tot = block_sum(v, s_w);
-  if (threadIdx.x == 0) __hip_atomic_store(&rs->ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return tot;
+  if (threadIdx.x < kFoldWidth)
+    for (int i = threadIdx.x; i < n; i += kFoldWidth) v += part[i];
+  return block_allsum(v, s_w);
 }
 
-template <typename T>
-__global__ __launch_bounds__(BS) void k_set(dlong N, T alpha, T* __restrict__ x) {
-  for (dlong n = blockIdx.x * BS + threadIdx.x; n < N; n += gridDim.x * BS) x[n] = alpha;
-}
-
-template <typename T>
-__global__ __launch_bounds__(BS) void k_axpy(dlong N, T alpha, const T* __restrict__ x, T beta,
-                                             T* __restrict__ y) {
-  for (dlong n = blockIdx.x * BS + threadIdx.x; n < N; n += gridDim.x * BS) y[n] = alpha * x[n] + beta * y[n];
-}
-
-__global__ __launch_bounds__(BS) void k_axpy_f32_f64(dlong N, double alpha, const float* __restrict__ x,
-                                                     double beta, double* __restrict__ y) {
-  for (dlong n = blockIdx.x * BS + threadIdx.x; n < N; n += gridDim.x * BS)
-    y[n] = alpha * (double)x[n] + beta * y[n];
-}
-
-__global__ __launch_bounds__(BS) void k_copy_f64_f32(dlong N, double scale, const double* __restrict__ x,
-                                                     float* __restrict__ y) {
-  for (dlong n = blockIdx.x * BS + threadIdx.x; n < N; n += gridDim.x * BS) y[n] = (float)(scale * x[n]);
-}
-
-// four independent strided accumulators per lane
+// Grid-strided sum with four independent accumulators per lane.
 template <typename F>
 __device__ inline double strided_sum(dlong N, F term) {
-  const long long stride = (long long)gridDim.x * BS;
-  long long n = (long long)blockIdx.x * BS + threadIdx.x;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
   for (; n + 3 * stride < N; n += 4 * stride) {
     a0 += term(n);
@@ -124,92 +82,116 @@ __device__ inline double strided_sum(dlong N, F term) {
 }
 
 template <typename T>
-__global__ __launch_bounds__(BS) void k_dot(dlong N, const T* __restrict__ x, const T* __restrict__ y,
-                                            ReduceScratch* rs, double* out) {
-  __shared__ double s_w[kWaves];
-  __shared__ bool s_last;
-  const double v = strided_sum(N, [&](long long n) { return (double)x[n] * (double)y[n]; });
-  const double bsum = block_sum(v, s_w);
-  if (!finish_partials(rs, bsum, &s_last)) return;
-  const double tot = total_of_partials(rs, s_w);
-  if (threadIdx.x == 0) *out = tot;
+__global__ __launch_bounds__(ST) void k_set(dlong N, T alpha, T* __restrict__ x) {
+  for (dlong n = blockIdx.x * ST + threadIdx.x; n < N; n += gridDim.x * ST) x[n] = alpha;
 }
 
-// r <- r - Ap (x0 residual) and r.r; sets up the stopping threshold
-// (linearSolverCG.cpp:79-87: rdotr = norm2(r)^2, TOL = max(tol^2 rdotr, tol^2)).
 template <typename T>
-__global__ __launch_bounds__(BS) void k_cg_init(dlong N, const T* __restrict__ Ap, T* __restrict__ r,
-                                                ReduceScratch* rs, CgState* st, double tol, int maxit,
-                                                double* hist_rr) {
-  __shared__ double s_w[kWaves];
-  __shared__ bool s_last;
+__global__ __launch_bounds__(ST) void k_axpy(dlong N, T alpha, const T* __restrict__ x, T beta,
+                                             T* __restrict__ y) {
+  for (dlong n = blockIdx.x * ST + threadIdx.x; n < N; n += gridDim.x * ST) y[n] = alpha * x[n] + beta * y[n];
+}
+
+__global__ __launch_bounds__(ST) void k_axpy_f32_f64(dlong N, double alpha, const float* __restrict__ x,
+                                                     double beta, double* __restrict__ y) {
+  for (dlong n = blockIdx.x * ST + threadIdx.x; n < N; n += gridDim.x * ST)
+    y[n] = alpha * (double)x[n] + beta * y[n];
+}
+
+__global__ __launch_bounds__(ST) void k_copy_f64_f32(dlong N, double scale, const double* __restrict__ x,
+                                                     float* __restrict__ y) {
+  for (dlong n = blockIdx.x * ST + threadIdx.x; n < N; n += gridDim.x * ST) y[n] = (float)(scale * x[n]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(RT) void k_dot_partials(dlong N, const T* __restrict__ x, const T* __restrict__ y,
+                                                     double* __restrict__ part) {
+  __shared__ double s_w[33];
+  const double v = strided_sum(N, [&](long long n) { return (double)x[n] * (double)y[n]; });
+  const double t = block_allsum(v, s_w);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(RT) void k_fold(const double* __restrict__ part, int n, double* __restrict__ out) {
+  __shared__ double s_w[33];
+  const double t = fold_partials(part, n, s_w);
+  if (threadIdx.x == 0) *out = t;
+}
+
+// r <- r - Ap (residual of the initial guess) with r.r partials
+template <typename T>
+__global__ __launch_bounds__(RT) void k_cg_init(dlong N, const T* __restrict__ Ap, T* __restrict__ r,
+                                                double* __restrict__ part_rr) {
+  __shared__ double s_w[33];
   const double v = strided_sum(N, [&](long long n) {
     const T rn = T(-1) * Ap[n] + T(1) * r[n];
     r[n] = rn;
     return (double)rn * (double)rn;
   });
-  const double bsum = block_sum(v, s_w);
-  if (!finish_partials(rs, bsum, &s_last)) return;
-  const double tot = total_of_partials(rs, s_w);
-  if (threadIdx.x == 0) {
-    const double nrm = sqrt(tot);
-    const double rdotr = nrm * nrm;
-    st->rdotr = rdotr;
-    st->rdotr_prev = 0.0;
-    st->alpha = 0.0;
-    st->pAp = 0.0;
-    st->tol2 = fmax(tol * tol * rdotr, tol * tol);
-    st->iter = 0;
-    st->maxit = maxit;
-    st->breakdown = isfinite(rdotr) ? 0 : 1;
-    st->done = (rdotr <= st->tol2 || maxit <= 0 || st->breakdown) ? 1 : 0;
-    if (hist_rr) hist_rr[0] = rdotr;
-  }
+  const double t = block_allsum(v, s_w);
+  if (threadIdx.x == 0) part_rr[blockIdx.x] = t;
 }
 
-// p <- r + beta p, beta = rdotr/rdotr_prev (0 on the first iteration)
+// Start of iteration k: rr_k = r.r from the partials, stopping test,
+// beta = rr_k / rr_{k-1}, p <- r + beta p (p <- r at k = 0).
 template <typename T>
-__global__ __launch_bounds__(BS) void k_cg_update_p(dlong N, const T* __restrict__ r, T* __restrict__ p,
-                                                    const CgState* st) {
-  if (st->done) return;
-  // first iteration: beta = 0, p = r (also when p holds stale values)
-  const bool first = (st->iter == 0);
-  const T b = first ? T(0) : (T)(st->rdotr / st->rdotr_prev);
-  for (dlong n = blockIdx.x * BS + threadIdx.x; n < N; n += gridDim.x * BS)
+__global__ __launch_bounds__(ST) void k_cg_update_p(dlong N, int k, const T* __restrict__ r, T* __restrict__ p,
+                                                    const double* __restrict__ part_rr, int nrr, CgState* st,
+                                                    double* __restrict__ hist_rr, double tol) {
+  __shared__ double s_w[33];
+  if (k > 0 && st->done) return;
+  double rr = fold_partials(part_rr, nrr, s_w);
+  double tol2;
+  if (k == 0) {
+    const double nrm = sqrt(rr);  // rdotr = norm2(r)^2 (linearSolverCG.cpp:84-85)
+    rr = nrm * nrm;
+    tol2 = fmax(tol * tol * rr, tol * tol);
+  } else {
+    tol2 = st->tol2;
+  }
+  const bool stop = !(rr > tol2) || !isfinite(rr);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    hist_rr[k] = rr;
+    if (k == 0) {
+      st->tol2 = tol2;
+      st->breakdown = 0;
+      st->done = 0;
+    }
+    if (stop) {
+      st->done = 1;
+      st->iter = k;
+      if (!isfinite(rr)) st->breakdown = 1;
+    }
+  }
+  if (stop) return;
+  const bool first = (k == 0);
+  const T b = first ? T(0) : (T)(rr / hist_rr[k - 1]);
+  for (dlong n = blockIdx.x * ST + threadIdx.x; n < N; n += gridDim.x * ST)
     p[n] = first ? r[n] : T(1) * r[n] + b * p[n];
 }
 
-// p.Ap, then alpha = rdotr / pAp on the device
+// End of iteration k: alpha = rr_k / p.Ap, r <- r - alpha Ap, x <- x + alpha p,
+// r.r partials for iteration k+1.
 template <typename T>
-__global__ __launch_bounds__(BS) void k_cg_pAp(dlong N, const T* __restrict__ p, const T* __restrict__ Ap,
-                                               ReduceScratch* rs, CgState* st) {
-  __shared__ double s_w[kWaves];
-  __shared__ bool s_last;
+__global__ __launch_bounds__(RT) void k_cg_update_xr(dlong N, int k, const T* __restrict__ p,
+                                                     const T* __restrict__ Ap, T* __restrict__ x,
+                                                     T* __restrict__ r, const double* __restrict__ part_pAp,
+                                                     int npAp, double* __restrict__ part_rr, CgState* st,
+                                                     const double* __restrict__ hist_rr,
+                                                     double* __restrict__ hist_alpha) {
+  __shared__ double s_w[33];
   if (st->done) return;
-  const double v = strided_sum(N, [&](long long n) { return (double)p[n] * (double)Ap[n]; });
-  const double bsum = block_sum(v, s_w);
-  if (!finish_partials(rs, bsum, &s_last)) return;
-  const double tot = total_of_partials(rs, s_w);
-  if (threadIdx.x == 0) {
-    st->pAp = tot;
-    const double alpha = st->rdotr / tot;
-    st->alpha = alpha;
-    if (!isfinite(alpha) || !(tot > 0.0)) {
+  const double pAp = fold_partials(part_pAp, npAp, s_w);
+  const double alpha = hist_rr[k] / pAp;
+  if (!isfinite(alpha) || !(pAp > 0.0)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
       st->breakdown = 1;
       st->done = 1;
+      st->iter = k;
     }
+    return;
   }
-}
-
-// r <- r - alpha Ap, x <- x + alpha p, r.r; advances the iteration
-template <typename T>
-__global__ __launch_bounds__(BS) void k_cg_update_xr(dlong N, const T* __restrict__ p, const T* __restrict__ Ap,
-                                                     T* __restrict__ x, T* __restrict__ r, ReduceScratch* rs,
-                                                     CgState* st, double* hist_rr, double* hist_alpha) {
-  __shared__ double s_w[kWaves];
-  __shared__ bool s_last;
-  if (st->done) return;
-  const double alpha = st->alpha;
+  if (blockIdx.x == 0 && threadIdx.x == 0) hist_alpha[k] = alpha;
   const T a = (T)alpha;
   const double v = strided_sum(N, [&](long long n) {
     const T rn = r[n] - a * Ap[n];
@@ -217,30 +199,81 @@ __global__ __launch_bounds__(BS) void k_cg_update_xr(dlong N, const T* __restric
     x[n] = a * p[n] + T(1) * x[n];
     return (double)rn * (double)rn;
   });
-  const double bsum = block_sum(v, s_w);
-  if (!finish_partials(rs, bsum, &s_last)) return;
-  const double tot = total_of_partials(rs, s_w);
+  const double t = block_allsum(v, s_w);
+  if (threadIdx.x == 0) part_rr[blockIdx.x] = t;
+}
+
+// After the last queued iteration kend: record rr_kend and the state.
+__global__ __launch_bounds__(RT) void k_cg_finish(int kend, int maxit, const double* __restrict__ part_rr,
+                                                  int nrr, CgState* st, double* __restrict__ hist_rr) {
+  __shared__ double s_w[33];
+  if (st->done) return;
+  const double rr = fold_partials(part_rr, nrr, s_w);
   if (threadIdx.x == 0) {
-    st->rdotr_prev = st->rdotr;
-    st->rdotr = tot;
-    const int it = st->iter + 1;
-    st->iter = it;
-    if (hist_rr) hist_rr[it] = tot;
-    if (hist_alpha) hist_alpha[it - 1] = alpha;
-    if (!isfinite(tot)) st->breakdown = 1;
-    st->done = (tot <= st->tol2 || it >= st->maxit || st->breakdown) ? 1 : 0;
+    hist_rr[kend] = rr;
+    st->iter = kend;
+    st->maxit = maxit;
+    if (!isfinite(rr)) st->breakdown = 1;
+    if (!(rr > st->tol2) || !isfinite(rr) || kend >= maxit) st->done = 1;
   }
 }
 
-}  // namespace
+// Reliable update, part 1: x64 += x32 (FP32 correction), x32 = 0.
+__global__ __launch_bounds__(ST) void k_accumulate_f32(dlong N, float* c32, double* __restrict__ x64) {
+  for (dlong n = blockIdx.x * ST + threadIdx.x; n < N; n += gridDim.x * ST) {
+    x64[n] += (double)c32[n];
+    c32[n] = 0.0f;
+  }
+}
 
-size_t reduce_scratch_bytes() { return sizeof(ReduceScratch); }
+// Reliable update, part 2: true residual r = b - Ax in FP64, handed to the
+// FP32 recurrence as r32 = float(r), with partials of r.r (FP64, for the
+// stopping test) and of r32.r32 (the recurrence's own r.r).
+__global__ __launch_bounds__(RT) void k_true_residual(dlong N, const double* __restrict__ b,
+                                                      const double* __restrict__ Ax, float* __restrict__ r32,
+                                                      double* __restrict__ part_true,
+                                                      double* __restrict__ part_rr32) {
+  __shared__ double s_w[33];
+  double t0 = 0, t1 = 0;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x; n < N; n += stride) {
+    const double rn = b[n] - Ax[n];
+    const float rf = (float)rn;
+    r32[n] = rf;
+    t0 += rn * rn;
+    t1 += (double)rf * (double)rf;
+  }
+  const double s0 = block_allsum(t0, s_w);
+  const double s1 = block_allsum(t1, s_w);
+  if (threadIdx.x == 0) {
+    part_true[blockIdx.x] = s0;
+    part_rr32[blockIdx.x] = s1;
+  }
+}
+
+// cg::UpdateCG as one pass: r <- r - alpha Ap, x <- x + alpha p, r.r partials
+template <typename T>
+__global__ __launch_bounds__(RT) void k_update_cg(dlong N, T alpha, const T* __restrict__ p,
+                                                  const T* __restrict__ Ap, T* __restrict__ x, T* __restrict__ r,
+                                                  double* __restrict__ part) {
+  __shared__ double s_w[33];
+  const double v = strided_sum(N, [&](long long n) {
+    const T rn = r[n] - alpha * Ap[n];
+    r[n] = rn;
+    x[n] = alpha * p[n] + T(1) * x[n];
+    return (double)rn * (double)rn;
+  });
+  const double t = block_allsum(v, s_w);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+}  // namespace
 
 template <typename T>
 int launch_set(hipStream_t s, dlong N, T alpha, T* x) {
   if (N < 0 || (N > 0 && !x)) return set_error(BPX_ERR_INVALID_ARGUMENT, "set: bad arguments");
   if (N == 0) return BPX_OK;
-  hipLaunchKernelGGL(k_set<T>, dim3(stream_blocks(N)), dim3(BS), 0, s, N, alpha, x);
+  hipLaunchKernelGGL(k_set<T>, dim3(stream_blocks(N)), dim3(ST), 0, s, N, alpha, x);
   BPX_HIP(hipGetLastError());
   return BPX_OK;
 }
@@ -249,78 +282,111 @@ template <typename T>
 int launch_axpy(hipStream_t s, dlong N, T alpha, const T* x, T beta, T* y) {
   if (N < 0 || (N > 0 && (!x || !y))) return set_error(BPX_ERR_INVALID_ARGUMENT, "axpy: bad arguments");
   if (N == 0) return BPX_OK;
-  hipLaunchKernelGGL(k_axpy<T>, dim3(stream_blocks(N)), dim3(BS), 0, s, N, alpha, x, beta, y);
+  hipLaunchKernelGGL(k_axpy<T>, dim3(stream_blocks(N)), dim3(ST), 0, s, N, alpha, x, beta, y);
   BPX_HIP(hipGetLastError());
   return BPX_OK;
 }
 
 int launch_axpy_f32_to_f64(hipStream_t s, dlong N, double alpha, const float* x, double beta, double* y) {
   if (N <= 0) return BPX_OK;
-  hipLaunchKernelGGL(k_axpy_f32_f64, dim3(stream_blocks(N)), dim3(BS), 0, s, N, alpha, x, beta, y);
+  hipLaunchKernelGGL(k_axpy_f32_f64, dim3(stream_blocks(N)), dim3(ST), 0, s, N, alpha, x, beta, y);
   BPX_HIP(hipGetLastError());
   return BPX_OK;
 }
 
 int launch_copy_f64_to_f32(hipStream_t s, dlong N, double scale, const double* x, float* y) {
   if (N <= 0) return BPX_OK;
-  hipLaunchKernelGGL(k_copy_f64_f32, dim3(stream_blocks(N)), dim3(BS), 0, s, N, scale, x, y);
+  hipLaunchKernelGGL(k_copy_f64_f32, dim3(stream_blocks(N)), dim3(ST), 0, s, N, scale, x, y);
   BPX_HIP(hipGetLastError());
   return BPX_OK;
 }
 
 template <typename T>
-int launch_dot(hipStream_t s, dlong N, const T* x, const T* y, void* scratch, double* out) {
-  if (N < 0 || !out || !scratch || (N > 0 && (!x || !y)))
-    return set_error(BPX_ERR_INVALID_ARGUMENT, "dot: bad arguments");
-  hipLaunchKernelGGL(k_dot<T>, dim3(reduce_blocks(N)), dim3(BS), 0, s, N, x, y,
-                     static_cast<ReduceScratch*>(scratch), out);
+int launch_dot_partials(hipStream_t s, dlong N, const T* x, const T* y, double* part) {
+  if (N < 0 || !part || (N > 0 && (!x || !y))) return set_error(BPX_ERR_INVALID_ARGUMENT, "dot: bad arguments");
+  hipLaunchKernelGGL(k_dot_partials<T>, dim3(RB), dim3(RT), 0, s, N, x, y, part);
+  BPX_HIP(hipGetLastError());
+  return BPX_OK;
+}
+
+int launch_fold(hipStream_t s, const double* part, int n, double* out) {
+  hipLaunchKernelGGL(k_fold, dim3(1), dim3(RT), 0, s, part, n, out);
   BPX_HIP(hipGetLastError());
   return BPX_OK;
 }
 
 template <typename T>
-int launch_cg_init(hipStream_t s, dlong N, const T* Ap, T* r, void* scratch, CgState* st, double tol,
-                   int maxit, double* hist_rr) {
-  hipLaunchKernelGGL(k_cg_init<T>, dim3(reduce_blocks(N)), dim3(BS), 0, s, N, Ap, r,
-                     static_cast<ReduceScratch*>(scratch), st, tol, maxit, hist_rr);
+int launch_dot(hipStream_t s, dlong N, const T* x, const T* y, double* part, double* out) {
+  if (int st = launch_dot_partials<T>(s, N, x, y, part)) return st;
+  return launch_fold(s, part, RB, out);
+}
+
+template <typename T>
+int launch_cg_init(hipStream_t s, dlong N, const T* Ap, T* r, double* part_rr) {
+  hipLaunchKernelGGL(k_cg_init<T>, dim3(RB), dim3(RT), 0, s, N, Ap, r, part_rr);
   BPX_HIP(hipGetLastError());
   return BPX_OK;
 }
 
 template <typename T>
-int launch_cg_update_p(hipStream_t s, dlong N, const T* r, T* p, const CgState* st) {
-  hipLaunchKernelGGL(k_cg_update_p<T>, dim3(stream_blocks(N)), dim3(BS), 0, s, N, r, p, st);
+int launch_cg_update_p(hipStream_t s, dlong N, int k, const T* r, T* p, const double* part_rr, int nrr,
+                       CgState* st, double* hist_rr, double tol) {
+  hipLaunchKernelGGL(k_cg_update_p<T>, dim3(stream_blocks(N)), dim3(ST), 0, s, N, k, r, p, part_rr, nrr, st,
+                     hist_rr, tol);
   BPX_HIP(hipGetLastError());
   return BPX_OK;
 }
 
 template <typename T>
-int launch_cg_pAp(hipStream_t s, dlong N, const T* p, const T* Ap, void* scratch, CgState* st) {
-  hipLaunchKernelGGL(k_cg_pAp<T>, dim3(reduce_blocks(N)), dim3(BS), 0, s, N, p, Ap,
-                     static_cast<ReduceScratch*>(scratch), st);
+int launch_cg_update_xr(hipStream_t s, dlong N, int k, const T* p, const T* Ap, T* x, T* r,
+                        const double* part_pAp, int npAp, double* part_rr, CgState* st, const double* hist_rr,
+                        double* hist_alpha) {
+  hipLaunchKernelGGL(k_cg_update_xr<T>, dim3(RB), dim3(RT), 0, s, N, k, p, Ap, x, r, part_pAp, npAp, part_rr,
+                     st, hist_rr, hist_alpha);
   BPX_HIP(hipGetLastError());
   return BPX_OK;
 }
 
-template <typename T>
-int launch_cg_update_xr(hipStream_t s, dlong N, const T* p, const T* Ap, T* x, T* r, void* scratch,
-                        CgState* st, double* hist_rr, double* hist_alpha) {
-  hipLaunchKernelGGL(k_cg_update_xr<T>, dim3(reduce_blocks(N)), dim3(BS), 0, s, N, p, Ap, x, r,
-                     static_cast<ReduceScratch*>(scratch), st, hist_rr, hist_alpha);
+int launch_cg_finish(hipStream_t s, int kend, int maxit, const double* part_rr, int nrr, CgState* st,
+                     double* hist_rr) {
+  hipLaunchKernelGGL(k_cg_finish, dim3(1), dim3(RT), 0, s, kend, maxit, part_rr, nrr, st, hist_rr);
   BPX_HIP(hipGetLastError());
   return BPX_OK;
 }
 
-#define BPX_LINALG_INST(T)                                                                         \
-  template int launch_set<T>(hipStream_t, dlong, T, T*);                                            \
-  template int launch_axpy<T>(hipStream_t, dlong, T, const T*, T, T*);                              \
-  template int launch_dot<T>(hipStream_t, dlong, const T*, const T*, void*, double*);               \
-  template int launch_cg_init<T>(hipStream_t, dlong, const T*, T*, void*, CgState*, double, int,    \
-                                 double*);                                                          \
-  template int launch_cg_update_p<T>(hipStream_t, dlong, const T*, T*, const CgState*);             \
-  template int launch_cg_pAp<T>(hipStream_t, dlong, const T*, const T*, void*, CgState*);           \
-  template int launch_cg_update_xr<T>(hipStream_t, dlong, const T*, const T*, T*, T*, void*,        \
-                                      CgState*, double*, double*);
+int launch_accumulate_f32(hipStream_t s, dlong N, float* c32, double* x64) {
+  if (N <= 0) return BPX_OK;
+  hipLaunchKernelGGL(k_accumulate_f32, dim3(stream_blocks(N)), dim3(ST), 0, s, N, c32, x64);
+  BPX_HIP(hipGetLastError());
+  return BPX_OK;
+}
+
+int launch_true_residual(hipStream_t s, dlong N, const double* b, const double* Ax, float* r32, double* part_true,
+                         double* part_rr32) {
+  hipLaunchKernelGGL(k_true_residual, dim3(RB), dim3(RT), 0, s, N, b, Ax, r32, part_true, part_rr32);
+  BPX_HIP(hipGetLastError());
+  return BPX_OK;
+}
+
+int launch_update_cg(hipStream_t s, dlong N, double alpha, const double* p, const double* Ap, double* x,
+                     double* r, double* part, double* out) {
+  if (N < 0 || !part || !out || (N > 0 && (!p || !Ap || !x || !r)))
+    return set_error(BPX_ERR_INVALID_ARGUMENT, "update_cg: bad arguments");
+  hipLaunchKernelGGL(k_update_cg<double>, dim3(RB), dim3(RT), 0, s, N, alpha, p, Ap, x, r, part);
+  BPX_HIP(hipGetLastError());
+  return launch_fold(s, part, RB, out);
+}
+
+#define BPX_LINALG_INST(T)                                                                                   \
+  template int launch_set<T>(hipStream_t, dlong, T, T*);                                                      \
+  template int launch_axpy<T>(hipStream_t, dlong, T, const T*, T, T*);                                        \
+  template int launch_dot_partials<T>(hipStream_t, dlong, const T*, const T*, double*);                       \
+  template int launch_dot<T>(hipStream_t, dlong, const T*, const T*, double*, double*);                       \
+  template int launch_cg_init<T>(hipStream_t, dlong, const T*, T*, double*);                                  \
+  template int launch_cg_update_p<T>(hipStream_t, dlong, int, const T*, T*, const double*, int, CgState*,     \
+                                     double*, double);                                                        \
+  template int launch_cg_update_xr<T>(hipStream_t, dlong, int, const T*, const T*, T*, T*, const double*, int, \
+                                      double*, CgState*, const double*, double*);
 BPX_LINALG_INST(double)
 BPX_LINALG_INST(float)
 

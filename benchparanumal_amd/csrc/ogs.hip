@@ -5,14 +5,15 @@
 // then scatter back). K vectors are interleaved: entry k of node n at
 // k + n*K.
 //
-// Design (gfx950): atomic-free segmented sums over the precomputed sorted
-// segment lists (rowStarts/colIds). A workgroup takes one of the <=512-nnz
-// row blocks built at setup (blockStarts); each lane owns whole rows and
-// sums its segment in column order, so every row is summed in exactly the
-// reference order and the result is bit-reproducible. Consecutive rows
-// have adjacent segments, so a wave's index loads cover a few cache lines
-// and the value loads follow the first-appearance numbering (mostly
-// ascending local ids).
+// Design (gfx950): atomic-free segmented sums over the sorted segment lists
+// (rowStarts/colIds). A workgroup owns kSuper consecutive row blocks of at
+// most 512 entries (the setup's blockStarts, so no row straddles two
+// workgroups): its lanes load the column ids with unit stride and the
+// gathered values in column order into LDS (all loads independent, so they
+// are in flight together), then each lane sums whole rows out of LDS in
+// column order, which reproduces the reference's summation order bit for
+// bit. The gather can also emit one FP64 partial of p.Ap per workgroup
+// (fused CG dot product), saving a pass over Ap.
 #include <hip/hip_runtime.h>
 
 #include "bpx_internal.hpp"
@@ -20,52 +21,102 @@
 namespace bpx {
 namespace {
 
-constexpr int kOgsThreads = 256;
+constexpr int kThreads = 512;
+constexpr int kSuper = 8;            // row blocks per workgroup
+constexpr int kMaxNnz = 512 * kSuper;
 
-template <typename T>
-__global__ __launch_bounds__(kOgsThreads) void k_gather(int Nblocks, int K, const int* __restrict__ blockStarts,
-                                                        const int* __restrict__ rowStarts,
-                                                        const int* __restrict__ colIds,
-                                                        const T* __restrict__ q, T* __restrict__ gq) {
-  const int blk = blockIdx.x;
+template <typename T, bool DOT>
+__global__ __launch_bounds__(kThreads) void k_gather(int Nblocks, int K, const int* __restrict__ blockStarts,
+                                                     const int* __restrict__ rowStarts,
+                                                     const int* __restrict__ colIds, const T* __restrict__ q,
+                                                     T* __restrict__ gq, const T* __restrict__ p,
+                                                     double* __restrict__ part) {
+  __shared__ T s_v[kMaxNnz];
+  __shared__ double s_w[kThreads / 64];
   const int k = blockIdx.y;
-  const int r0 = blockStarts[blk], r1 = blockStarts[blk + 1];
-  for (int row = r0 + threadIdx.x; row < r1; row += kOgsThreads) {
-    const int g0 = rowStarts[row], g1 = rowStarts[row + 1];
+  const int b0 = blockIdx.x * kSuper;
+  const int b1 = min(b0 + kSuper, Nblocks);
+  const int r0 = blockStarts[b0], r1 = blockStarts[b1];
+  const int g0 = rowStarts[r0], g1 = rowStarts[r1];
+  // caller-built blocks may exceed 512 entries: then sum straight from memory
+  const bool staged = (g1 - g0) <= kMaxNnz;
+  if (staged)
+    for (int g = g0 + threadIdx.x; g < g1; g += kThreads) s_v[g - g0] = q[k + (size_t)colIds[g] * K];
+  __syncthreads();
+  double dot = 0;
+  for (int row = r0 + threadIdx.x; row < r1; row += kThreads) {
+    const int a = rowStarts[row] - g0, e = rowStarts[row + 1] - g0;
     T acc = T(0);
-    for (int g = g0; g < g1; ++g) acc += q[k + (size_t)colIds[g] * K];
+    if (staged)
+      for (int i = a; i < e; ++i) acc += s_v[i];
+    else
+      for (int i = a; i < e; ++i) acc += q[k + (size_t)colIds[g0 + i] * K];
     gq[k + (size_t)row * K] = acc;
+    if (DOT) dot += (double)p[row] * (double)acc;
+  }
+  if (DOT) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) dot += __shfl_xor(dot, off, 64);
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = dot;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0;
+      for (int i = 0; i < kThreads / 64; ++i) t += s_w[i];
+      part[blockIdx.x] = t;
+    }
   }
 }
 
 template <typename T>
-__global__ __launch_bounds__(kOgsThreads) void k_scatter(int Nblocks, int K, const int* __restrict__ blockStarts,
-                                                         const int* __restrict__ rowStarts,
-                                                         const int* __restrict__ colIds,
-                                                         const T* __restrict__ gq, T* __restrict__ q) {
-  const int blk = blockIdx.x;
+__global__ __launch_bounds__(kThreads) void k_scatter(int Nblocks, int K, const int* __restrict__ blockStarts,
+                                                      const int* __restrict__ rowStarts,
+                                                      const int* __restrict__ colIds, const T* __restrict__ gq,
+                                                      T* __restrict__ q) {
   const int k = blockIdx.y;
-  const int r0 = blockStarts[blk], r1 = blockStarts[blk + 1];
-  for (int row = r0 + threadIdx.x; row < r1; row += kOgsThreads) {
+  const int b0 = blockIdx.x * kSuper;
+  const int b1 = min(b0 + kSuper, Nblocks);
+  const int r0 = blockStarts[b0], r1 = blockStarts[b1];
+  for (int row = r0 + threadIdx.x; row < r1; row += kThreads) {
     const T v = gq[k + (size_t)row * K];
-    const int g0 = rowStarts[row], g1 = rowStarts[row + 1];
-    for (int g = g0; g < g1; ++g) q[k + (size_t)colIds[g] * K] = v;
+    const int a = rowStarts[row], e = rowStarts[row + 1];
+    for (int g = a; g < e; ++g) q[k + (size_t)colIds[g] * K] = v;
   }
 }
 
 template <typename T>
-__global__ __launch_bounds__(kOgsThreads) void k_gather_scatter(int Nblocks, int K,
-                                                                const int* __restrict__ blockStarts,
-                                                                const int* __restrict__ rowStarts,
-                                                                const int* __restrict__ colIds, T* q) {
-  const int blk = blockIdx.x;
+__global__ __launch_bounds__(kThreads) void k_gather_scatter(int Nblocks, int K, const int* __restrict__ blockStarts,
+                                                             const int* __restrict__ rowStarts,
+                                                             const int* __restrict__ colIds, T* q) {
+  __shared__ T s_v[kMaxNnz];
   const int k = blockIdx.y;
-  const int r0 = blockStarts[blk], r1 = blockStarts[blk + 1];
-  for (int row = r0 + threadIdx.x; row < r1; row += kOgsThreads) {
-    const int g0 = rowStarts[row], g1 = rowStarts[row + 1];
+  const int b0 = blockIdx.x * kSuper;
+  const int b1 = min(b0 + kSuper, Nblocks);
+  const int r0 = blockStarts[b0], r1 = blockStarts[b1];
+  const int g0 = rowStarts[r0], g1 = rowStarts[r1];
+  if ((g1 - g0) > kMaxNnz) {  // caller-built oversized blocks: rows from memory
+    for (int row = r0 + threadIdx.x; row < r1; row += kThreads) {
+      const int a = rowStarts[row], e = rowStarts[row + 1];
+      T acc = T(0);
+      for (int g = a; g < e; ++g) acc += q[k + (size_t)colIds[g] * K];
+      for (int g = a; g < e; ++g) q[k + (size_t)colIds[g] * K] = acc;
+    }
+    return;
+  }
+  for (int g = g0 + threadIdx.x; g < g1; g += kThreads) s_v[g - g0] = q[k + (size_t)colIds[g] * K];
+  __syncthreads();
+  for (int row = r0 + threadIdx.x; row < r1; row += kThreads) {
+    const int a = rowStarts[row] - g0, e = rowStarts[row + 1] - g0;
+    if (e <= a) continue;
     T acc = T(0);
-    for (int g = g0; g < g1; ++g) acc += q[k + (size_t)colIds[g] * K];
-    for (int g = g0; g < g1; ++g) q[k + (size_t)colIds[g] * K] = acc;
+    for (int i = a; i < e; ++i) acc += s_v[i];
+    s_v[a] = acc;  // park the sum; entries of one row belong to one lane
+  }
+  __syncthreads();
+  for (int row = r0 + threadIdx.x; row < r1; row += kThreads) {
+    const int a = rowStarts[row] - g0, e = rowStarts[row + 1] - g0;
+    if (e <= a) continue;
+    const T v = s_v[a];
+    for (int i = a; i < e; ++i) q[k + (size_t)colIds[g0 + i] * K] = v;
   }
 }
 
@@ -77,15 +128,30 @@ int check(int Nblocks, int K, const void* bs, const void* rs, const void* ci, co
   return BPX_OK;
 }
 
+inline int super_blocks(int Nblocks) { return (Nblocks + kSuper - 1) / kSuper; }
+
 }  // namespace
+
+int ogs_gather_partials(int Nblocks) { return super_blocks(Nblocks); }
 
 template <typename T>
 int launch_ogs_gather(hipStream_t s, int Nblocks, int K, const int* blockStarts, const int* rowStarts,
                       const int* colIds, const T* q, T* gq) {
   if (int st = check(Nblocks, K, blockStarts, rowStarts, colIds, q, gq)) return st;
   if (Nblocks == 0) return BPX_OK;
-  hipLaunchKernelGGL(k_gather<T>, dim3(Nblocks, K), dim3(kOgsThreads), 0, s, Nblocks, K, blockStarts,
-                     rowStarts, colIds, q, gq);
+  hipLaunchKernelGGL((k_gather<T, false>), dim3(super_blocks(Nblocks), K), dim3(kThreads), 0, s, Nblocks, K,
+                     blockStarts, rowStarts, colIds, q, gq, (const T*)nullptr, (double*)nullptr);
+  BPX_HIP(hipGetLastError());
+  return BPX_OK;
+}
+
+template <typename T>
+int launch_ogs_gather_dot(hipStream_t s, int Nblocks, const int* blockStarts, const int* rowStarts,
+                          const int* colIds, const T* q, T* gq, const T* p, double* part) {
+  if (int st = check(Nblocks, 1, blockStarts, rowStarts, colIds, q, gq)) return st;
+  if (Nblocks == 0) return BPX_OK;
+  hipLaunchKernelGGL((k_gather<T, true>), dim3(super_blocks(Nblocks), 1), dim3(kThreads), 0, s, Nblocks, 1,
+                     blockStarts, rowStarts, colIds, q, gq, p, part);
   BPX_HIP(hipGetLastError());
   return BPX_OK;
 }
@@ -95,18 +161,18 @@ int launch_ogs_scatter(hipStream_t s, int Nblocks, int K, const int* blockStarts
                        const int* colIds, const T* gq, T* q) {
   if (int st = check(Nblocks, K, blockStarts, rowStarts, colIds, gq, q)) return st;
   if (Nblocks == 0) return BPX_OK;
-  hipLaunchKernelGGL(k_scatter<T>, dim3(Nblocks, K), dim3(kOgsThreads), 0, s, Nblocks, K, blockStarts,
-                     rowStarts, colIds, gq, q);
+  hipLaunchKernelGGL(k_scatter<T>, dim3(super_blocks(Nblocks), K), dim3(kThreads), 0, s, Nblocks, K,
+                     blockStarts, rowStarts, colIds, gq, q);
   BPX_HIP(hipGetLastError());
   return BPX_OK;
 }
 
 template <typename T>
-int launch_ogs_gather_scatter(hipStream_t s, int Nblocks, int K, const int* blockStarts,
-                              const int* rowStarts, const int* colIds, T* q) {
+int launch_ogs_gather_scatter(hipStream_t s, int Nblocks, int K, const int* blockStarts, const int* rowStarts,
+                              const int* colIds, T* q) {
   if (int st = check(Nblocks, K, blockStarts, rowStarts, colIds, q, q)) return st;
   if (Nblocks == 0) return BPX_OK;
-  hipLaunchKernelGGL(k_gather_scatter<T>, dim3(Nblocks, K), dim3(kOgsThreads), 0, s, Nblocks, K,
+  hipLaunchKernelGGL(k_gather_scatter<T>, dim3(super_blocks(Nblocks), K), dim3(kThreads), 0, s, Nblocks, K,
                      blockStarts, rowStarts, colIds, q);
   BPX_HIP(hipGetLastError());
   return BPX_OK;
@@ -116,11 +182,15 @@ template int launch_ogs_gather<double>(hipStream_t, int, int, const int*, const 
                                        const double*, double*);
 template int launch_ogs_gather<float>(hipStream_t, int, int, const int*, const int*, const int*,
                                       const float*, float*);
+template int launch_ogs_gather_dot<double>(hipStream_t, int, const int*, const int*, const int*, const double*,
+                                           double*, const double*, double*);
+template int launch_ogs_gather_dot<float>(hipStream_t, int, const int*, const int*, const int*, const float*,
+                                          float*, const float*, double*);
 template int launch_ogs_scatter<double>(hipStream_t, int, int, const int*, const int*, const int*,
                                         const double*, double*);
 template int launch_ogs_scatter<float>(hipStream_t, int, int, const int*, const int*, const int*,
                                        const float*, float*);
-template int launch_ogs_gather_scatter<double>(hipStream_t, int, int, const int*, const int*,
-                                               const int*, double*);
+template int launch_ogs_gather_scatter<double>(hipStream_t, int, int, const int*, const int*, const int*,
+                                               double*);
 
 }  // namespace bpx

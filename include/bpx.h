@@ -252,12 +252,14 @@ int bpx_bp_rhs_forcing(bpx_bp b, double* o_b);
  * BPX_OK and *iterations. */
 int bpx_bp_solve(bpx_bp b, double* o_x, double* o_r, double tol, int maxit, int verbose,
                  int* iterations, double* rnorm_hist);
-/* FP32 inner CG inside an FP64 defect-correction loop until
- * ||b - A x|| / ||b|| < tol (FP64 residual, norm and update on the GPU).
- * inner_tol: relative reduction per inner solve; stats (host, may be NULL):
- * [outer iterations, total inner iterations, final relative residual]. */
-int bpx_bp_solve_mixed(bpx_bp b, double* o_x, const double* o_b, double tol, double inner_tol,
-                       int max_outer, int max_inner, double* stats);
+/* No reference counterpart: CG in FP32 (FP32 operator and vectors, FP64
+ * dot accumulation) with an FP64 defect correction every `interval`
+ * iterations (x += c; r = b - A x in FP64; the FP32 recurrence continues
+ * from float(r)) until ||b - A x|| / ||b|| < tol. o_x: initial guess and
+ * solution (device, FP64). stats (host, may be NULL): [defect corrections,
+ * FP32 iterations, final FP64 relative residual]. */
+int bpx_bp_solve_mixed(bpx_bp b, double* o_x, const double* o_b, double tol, int interval,
+                       int maxit, double* stats);
 
 #ifdef __cplusplus
 }

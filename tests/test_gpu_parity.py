@@ -156,12 +156,14 @@ def test_cg_reference_forcing_semantics(plat):
     x = torch.zeros_like(rhs)
     its, hist = b.solve(x, rhs, tol=0.0, maxit=100)
     assert its == it_ref
-    # this smooth problem converges to ~1e-18 within 100 iterations; below
-    # ~1e-7 of the initial residual the iterates are rounding noise (the
-    # oracle built with and without FMA already differs there), so compare
-    # the history down to 1e-6
-    live = hist_ref > 1e-6 * hist_ref[0]
-    assert live.sum() >= 10
+    # this smooth problem converges to ~1e-18 within 100 iterations, so late
+    # iterates carry rounding noise of the order eps*||r0|| that another
+    # (equally valid) summation order changes in relative terms: the whole
+    # history is compared to 1e-9*||r0||, and per iteration to 1e-9 while the
+    # residual is above 1e-3*||r0||
+    np.testing.assert_allclose(hist, hist_ref, rtol=0, atol=CG_RTOL * hist_ref[0])
+    live = hist_ref > 1e-3 * hist_ref[0]
+    assert live.sum() >= 5
     np.testing.assert_allclose(hist[live], hist_ref[live], rtol=CG_RTOL)
     assert np.abs(rhs_np).max() > 0
 
@@ -215,8 +217,11 @@ def test_mixed_precision_reaches_1e10(plat, bp):
     rhs = capi.random_normal(9, b.Ngather)
     bd = dev(rhs)
     x = torch.zeros(b.Ngather, dtype=torch.float64, device="cuda")
-    st = b.solve_mixed(x, bd, tol=1e-10, inner_tol=1e-4)
-    assert st["rel_res"] < 1e-10
+    st = b.solve_mixed(x, bd, tol=1e-10, interval=32)
+    assert st["rel_res"] < 1e-10 and st["updates"] >= 1
+    # the FP32 recurrence keeps its Krylov space: close to the FP64 count
+    _, it64, _, _ = P.cg(rhs, 20000, tol=1e-10)
+    assert st["iterations"] <= 1.3 * it64 + 32
     Ax = torch.zeros_like(x)
     b.operator(x, Ax)
     assert np.linalg.norm(rhs - host(Ax)) / np.linalg.norm(rhs) < 1e-10

@@ -1,6 +1,9 @@
 """Pins the oracle's Ax loop nests (okl/bp{1..6}Ax{,Affine}Hex3D.okl), the
 gather, the reductions and CG (libs/core/linearSolverCG.cpp) against dense
 brute-force assembly, invariants and the README's sample run."""
+import json
+import os
+
 import numpy as np
 import numpy.polynomial.legendre as L
 import pytest
@@ -8,6 +11,7 @@ import pytest
 import brute
 
 LAM = 1.3
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 def dense_reference(P, N, bp):
@@ -135,13 +139,13 @@ def test_cg_readme_sample(oracle):
     sample counted 68,921 = 41^3 DOFs, i.e. it ran without the Dirichlet mask
     (README.md:115). On that problem rounding alone (FMA on/off) moves the
     iteration-100 norm by ~2e-3, so the match is soft."""
-    P = oracle.Problem(bp=5, N=4, nx=10, lam=1.0)
-    P.set_mask(False)
-    assert P.Ngather == 68921
-    x, it, hist, alpha = P.cg(P.rhs_forcing(), 100)
-    ref_r = np.array([1.405229334496e-04, 1.375460859099e-04, 1.198097786957e-04,
-                      1.108821042895e-04, 9.086922290200e-05])
-    ref_a = np.array([2.686587, 2.540830, 2.780510, 2.907639, 2.946219])
-    assert it == 100
-    np.testing.assert_allclose(hist[96:101], ref_r, rtol=5e-3)
-    np.testing.assert_allclose(alpha[95:100], ref_a, rtol=3e-3)
+    g = json.load(open(os.path.join(GOLDEN, "bp5_p4_nx10_unmasked_cg100.json")))
+    P = oracle.Problem(bp=g["bp"], N=g["N"], nx=g["nx"], lam=g["lambda"])
+    P.set_mask(g["mask"])
+    assert P.Ngather == g["Ndofs"]
+    x, it, hist, alpha = P.cg(P.rhs_forcing(), g["iterations"])
+    its = np.array(g["it"])
+    assert it == g["iterations"]
+    # hist[k] is the norm after iteration k; alpha[k-1] is iteration k's step
+    np.testing.assert_allclose(hist[its], g["r_norm"], rtol=5e-3)
+    np.testing.assert_allclose(alpha[its - 1], g["alpha"], rtol=3e-3)
