@@ -48,6 +48,21 @@ def physical_cores():
         return os.cpu_count() or 1
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_bp5_ax_fp64_p7_24cube_hbm_traffic.txt")
+
+
+def measured_traffic():
+    """HBM bytes per Ax launch from the committed counter summary (FETCH_SIZE and
+    WRITE_SIZE scaled by a plain copy of known size), or None."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def bk5_bytes(Ndofs, Nel, Np):
     # BK/BK5/src/bk5Run.cpp:116-122 (non-affine hex, FP64, 4-byte dlong)
     return Ndofs * 8 + Nel * (Np * 7 * 8 + 4 + Np * 4 + Np * 8)
@@ -95,10 +110,14 @@ def main():
         dst.copy_(src)
         q = torch.from_numpy(capi.random_normal(2, Ndofs)).cuda()
         AqL = torch.empty(bp.NLocal, dtype=torch.float64, device="cuda")
+        Aq = torch.empty_like(q)
         for _ in range(args.ax_launches):
             bp.ax_local(q, AqL)
+        for _ in range(args.ax_launches):
+            bp.operator(q, Aq)
         torch.cuda.synchronize()
-        log(f"ax-only: {args.ax_launches} BK5 Ax launches and one 512 MiB copy done")
+        log(f"ax-only: {args.ax_launches} BK5 Ax launches, {args.ax_launches} operator (Ax + gather) "
+            f"launches and one 512 MiB copy done")
         return
 
     b_host = capi.random_normal(7, Ndofs)
@@ -159,6 +178,19 @@ def main():
     achieved = nbytes / t_ax / 1e9
     log(f"BK5 Ax: {t_ax * 1e6:.1f} us, {achieved:.0f} GB/s, {Ndofs / t_ax / 1e9:.1f} GDOF/s")
 
+    # plain device copy of 512 MiB on the same stream: the attainable HBM rate
+    src = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    ev0.record()
+    for _ in range(10):
+        dst.copy_(src)
+    ev1.record()
+    torch.cuda.synchronize()
+    copy_gbs = 2 * src.numel() * 8 * 10 / (ev0.elapsed_time(ev1) / 1e3) / 1e9
+    del src, dst
+    log(f"512 MiB device copy: {copy_gbs:.0f} GB/s (read + write)")
+
     # ---- CPU baseline: the oracle's FP64 CG on the host cores, bounded sample
     cpu = None
     if not args.no_cpu_baseline:
@@ -205,9 +237,11 @@ def main():
                   "own_rate": Ndofs * st["iterations"] / tmix / 1e9,
                   "speedup_vs_fp64": t64 / tmix},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "kernel": "k_bp5_ax (FP64, non-affine, Nq=8)", "us_per_launch": t_ax * 1e6,
-                     "algorithmic_bytes": nbytes, "gdof_per_s": Ndofs / t_ax / 1e9},
+                     "algorithmic_bytes": nbytes, "gdof_per_s": Ndofs / t_ax / 1e9,
+                     "device_copy_gbs": copy_gbs},
+        "tuning": bp.tuning(),
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

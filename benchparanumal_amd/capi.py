@@ -70,6 +70,8 @@ _SIGS = {
     "bpx_bp_operator": (_I, [_P, _P, _P]),
     "bpx_bp_ax_local": (_I, [_P, _P, _P]),
     "bpx_bp_operator_f32": (_I, [_P, _P, _P]),
+    "bpx_bp_operator_krhs": (_I, [_P, _I, _I, _P, _P]),
+    "bpx_bp_tuning": (_I, [_P, _P, _I]),
     "bpx_bp_rhs_forcing": (_I, [_P, _P]),
     "bpx_bp_solve": (_I, [_P, _P, _P, _D, _I, _I, C.POINTER(_I), _P]),
     "bpx_bp_solve_mixed": (_I, [_P, _P, _P, _D, _I, _I, _P]),
@@ -81,6 +83,8 @@ for _name in ("bpx_bp1_ax_hex", "bpx_bp1_ax_affine_hex", "bpx_bp2_ax_hex", "bpx_
     _SIGS[_name] = (_I, [_P, _I, _I, _P, _P] + [_P] * 3 + [_P, _P])
 for _name in ("bpx_bp3_ax_hex", "bpx_bp3_ax_affine_hex", "bpx_bp4_ax_hex", "bpx_bp4_ax_affine_hex"):
     _SIGS[_name] = (_I, [_P, _I, _I, _P, _P] + [_P] * 7 + [_D, _P, _P])
+for _name in ("bpx_bp5_ax_hex_krhs", "bpx_bp5_ax_affine_hex_krhs"):
+    _SIGS[_name] = (_I, [_P, _I, _I, _I, _I, _P, _P] + [_P] * 7 + [_D, _P, _P])
 _SIGS["bpx_bp5_ax_hex_f32"] = (_I, [_P, _I, _I, _P, _P] + [_P] * 7 + [_F, _P, _P])
 _SIGS["bpx_bp3_ax_hex_f32"] = (_I, [_P, _I, _I, _P, _P] + [_P] * 7 + [_F, _P, _P])
 
@@ -255,6 +259,16 @@ class BP:
 
     def operator_f32(self, q, Aq):
         check(lib().bpx_bp_operator_f32(self.h, ptr(q), ptr(Aq)))
+
+    def tuning(self):
+        """Launch choices timed at setup: gather order and its times (us)."""
+        out = np.zeros(3)
+        n = lib().bpx_bp_tuning(self.h, ptr(out), 3)
+        return {"gather_mode": int(out[0]), "gather_us": [float(out[1]), float(out[2])]}
+
+    def operator_krhs(self, q, Aq, K, chunk=0):
+        """Operator on K interleaved right-hand sides ([Ngather][K]), BP5."""
+        check(lib().bpx_bp_operator_krhs(self.h, K, chunk, ptr(q), ptr(Aq)))
 
     def rhs_forcing(self, b):
         check(lib().bpx_bp_rhs_forcing(self.h, ptr(b)))

@@ -16,6 +16,12 @@
 // slab k computes, keeping the HBM stream of ggeo/wJ (56 B per node in
 // FP64) in flight.
 //
+// k right-hand sides (an extension; the reference applies one vector per
+// launch): vectors are interleaved as in the ogs K form, and each element is
+// swept K/chunk times with chunk right-hand sides in registers; the sweeps
+// of one element run back to back in one workgroup, so its geometry comes
+// from HBM once and from L2 after that.
+//
 // This file is compiled once per Nq (-DBPX_NQ=n) so the instantiations
 // build in parallel; ax_dispatch.cpp picks the instantiation at run time.
 #include <hip/hip_runtime.h>
@@ -29,6 +35,13 @@
 namespace bpx {
 namespace {
 
+// All arithmetic is written as explicit fused multiply-adds and the file is
+// compiled with -ffp-contract=off, so every instantiation (one or several
+// right-hand sides, any workgroup shape) rounds identically and gives the
+// same bits for the same input.
+__device__ inline double fmad(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ inline float fmad(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
 // elements per workgroup: ~256 lanes for small Nq, one element from Nq=8
 template <int NQ>
 constexpr int bp5_epb() {
@@ -36,20 +49,28 @@ constexpr int bp5_epb() {
 }
 
 // occupancy target handed to the register allocator
-template <int NQ, int NF>
+template <int NQ, int NF, bool KRHS>
 constexpr int bp5_waves() {
+  if (KRHS) return NF <= 2 && NQ <= 8 ? 2 : 1;
   return NF == 1 ? (NQ <= 8 ? 3 : 2) : (NQ <= 8 ? 2 : 1);
 }
 
-template <typename T, int NQ, int NF, int EPB, bool AFFINE, int WAVES>
+// D rows/columns in registers (saves 4*Nq LDS reads per node) for one
+// field per sweep; with several the registers hold the extra pencils
+template <int NQ, int NF>
+constexpr bool bp5_dreg() {
+  return NQ <= 10 && NF == 1;
+}
+
+template <typename T, int NQ, int NF, int EPB, bool AFFINE, int WAVES, bool KRHS>
 __global__ __launch_bounds__(NQ* NQ* EPB) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_bp5_ax(
-    int Nelements, const int* __restrict__ elementList, const int* __restrict__ G2L,
+    int Nelements, int K, const int* __restrict__ elementList, const int* __restrict__ G2L,
     const T* __restrict__ wJ, const T* __restrict__ ggeo, const T* __restrict__ gllw,
     const T* __restrict__ D, T lambda, const T* __restrict__ q, T* __restrict__ Aq) {
   constexpr int NQ2 = NQ * NQ;
   constexpr int NP = NQ2 * NQ;
   constexpr int LD = NQ + 1;  // padded LDS row
-  constexpr bool DREG = NQ <= 10;
+  constexpr bool DREG = bp5_dreg<NQ, NF>();
 
   __shared__ T s_D[NQ][LD];
   __shared__ T s_q[NF][EPB][NQ][LD];
@@ -67,17 +88,38 @@ __global__ __launch_bounds__(NQ* NQ* EPB) __attribute__((amdgpu_waves_per_eu(WAV
 
   for (int n = t; n < NQ2; n += NQ2 * EPB) s_D[n / NQ][n % NQ] = D[n];
 
+  // KRHS: the pencil's gather rows, read once for all sweeps
+  int gid[KRHS ? NQ : 1];
+  if (KRHS) {
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) gid[k] = active ? G2L[(size_t)e * NP + k * NQ2 + ij] : -1;
+  }
+
+  const int nsweep = KRHS ? K / NF : 1;
+  for (int c = 0; c < nsweep; ++c) {
   T rq[NF][NQ];
   T rA[NF][NQ];
-  const size_t lbase = (size_t)e * NP * NF + ij;
-#pragma unroll
-  for (int f = 0; f < NF; ++f)
+  if (KRHS) {
 #pragma unroll
     for (int k = 0; k < NQ; ++k) {
-      const int id = active ? G2L[lbase + f * NP + k * NQ2] : -1;
-      rq[f][k] = (id >= 0) ? q[id] : T(0);
-      rA[f][k] = T(0);
+      const T* src = q + (size_t)max(gid[k], 0) * K + c * NF;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        rq[f][k] = (gid[k] >= 0) ? src[f] : T(0);
+        rA[f][k] = T(0);
+      }
     }
+  } else {
+    const size_t lbase = (size_t)e * NP * NF + ij;
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int k = 0; k < NQ; ++k) {
+        const int id = active ? G2L[lbase + f * NP + k * NQ2] : -1;
+        rq[f][k] = (id >= 0) ? q[id] : T(0);
+        rA[f][k] = T(0);
+      }
+  }
 
   // geometry of the current slab (non-affine: prefetched one slab ahead)
   T g0 = 0, g1 = 0, g2 = 0, g3 = 0, g4 = 0, g5 = 0, gw = 0;
@@ -138,18 +180,18 @@ __global__ __launch_bounds__(NQ* NQ* EPB) __attribute__((amdgpu_waves_per_eu(WAV
       T qr = 0, qs = 0, qt = 0;
 #pragma unroll
       for (int m = 0; m < NQ; ++m) {
-        qr += (DREG ? Di[m] : s_D[i][m]) * s_q[f][es][j][m];
-        qs += (DREG ? Dj[m] : s_D[j][m]) * s_q[f][es][m][i];
-        qt += s_D[k][m] * rq[f][m];
+        qr = fmad((DREG ? Di[m] : s_D[i][m]), s_q[f][es][j][m], qr);
+        qs = fmad((DREG ? Dj[m] : s_D[j][m]), s_q[f][es][m][i], qs);
+        qt = fmad(s_D[k][m], rq[f][m], qt);
       }
-      const T Gr = G00 * qr + G01 * qs + G02 * qt;
-      const T Gs = G01 * qr + G11 * qs + G12 * qt;
-      const T Gt = G02 * qr + G12 * qs + G22 * qt;
+      const T Gr = fmad(G02, qt, fmad(G01, qs, G00 * qr));
+      const T Gs = fmad(G12, qt, fmad(G11, qs, G01 * qr));
+      const T Gt = fmad(G22, qt, fmad(G12, qs, G02 * qr));
       s_r[f][es][j][i] = Gr;
       s_s[f][es][j][i] = Gs;
-      rA[f][k] += lambda * W * rq[f][k];
+      rA[f][k] = fmad(lambda * W, rq[f][k], rA[f][k]);
 #pragma unroll
-      for (int m = 0; m < NQ; ++m) rA[f][m] += s_D[k][m] * Gt;
+      for (int m = 0; m < NQ; ++m) rA[f][m] = fmad(s_D[k][m], Gt, rA[f][m]);
     }
     __syncthreads();
     asm volatile("" ::: "memory");
@@ -159,44 +201,70 @@ __global__ __launch_bounds__(NQ* NQ* EPB) __attribute__((amdgpu_waves_per_eu(WAV
       T acc = 0;
 #pragma unroll
       for (int m = 0; m < NQ; ++m) {
-        acc += (DREG ? DTi[m] : s_D[m][i]) * s_r[f][es][j][m];
-        acc += (DREG ? DTj[m] : s_D[m][j]) * s_s[f][es][m][i];
+        acc = fmad((DREG ? DTi[m] : s_D[m][i]), s_r[f][es][j][m], acc);
+        acc = fmad((DREG ? DTj[m] : s_D[m][j]), s_s[f][es][m][i], acc);
       }
       rA[f][k] += acc;
     }
   }
 
   if (active) {
-    T* out = Aq + (size_t)e * NP * NF + ij;
+    if (KRHS) {
 #pragma unroll
-    for (int f = 0; f < NF; ++f)
+      for (int k = 0; k < NQ; ++k) {
+        T* out = Aq + ((size_t)e * NP + k * NQ2 + ij) * K + c * NF;
 #pragma unroll
-      for (int k = 0; k < NQ; ++k) out[f * NP + k * NQ2] = rA[f][k];
+        for (int f = 0; f < NF; ++f) out[f] = rA[f][k];
+      }
+    } else {
+      T* out = Aq + (size_t)e * NP * NF + ij;
+#pragma unroll
+      for (int f = 0; f < NF; ++f)
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) out[f * NP + k * NQ2] = rA[f][k];
+    }
   }
+  }  // sweeps
+}
+
+template <typename T, int NQ, int NF, bool KRHS>
+int launch_bp5(hipStream_t s, const AxArgs<T>& a) {
+  constexpr int EPB = bp5_epb<NQ>();
+  constexpr int WV = bp5_waves<NQ, NF, KRHS>();
+  const int blocks = (a.Nelements + EPB - 1) / EPB;
+  if (blocks == 0) return BPX_OK;
+  if (a.affine)
+    hipLaunchKernelGGL((k_bp5_ax<T, NQ, NF, EPB, true, WV, KRHS>), dim3(blocks), dim3(NQ * NQ * EPB), 0,
+                       s, a.Nelements, a.K, a.elementList, a.GlobalToLocal, a.wJ, a.ggeo, a.gllw, a.D,
+                       a.lambda, a.q, a.Aq);
+  else
+    hipLaunchKernelGGL((k_bp5_ax<T, NQ, NF, EPB, false, WV, KRHS>), dim3(blocks), dim3(NQ * NQ * EPB), 0,
+                       s, a.Nelements, a.K, a.elementList, a.GlobalToLocal, a.wJ, a.ggeo, a.gllw, a.D,
+                       a.lambda, a.q, a.Aq);
+  BPX_HIP(hipGetLastError());
+  return BPX_OK;
 }
 
 }  // namespace
 
 template <typename T, int NQ, int NF>
 int launch_bp5_nq(hipStream_t s, const AxArgs<T>& a) {
-  constexpr int EPB = bp5_epb<NQ>();
-  constexpr int WV = bp5_waves<NQ, NF>();
-  const int blocks = (a.Nelements + EPB - 1) / EPB;
-  if (blocks == 0) return BPX_OK;
-  if (a.affine)
-    hipLaunchKernelGGL((k_bp5_ax<T, NQ, NF, EPB, true, WV>), dim3(blocks), dim3(NQ * NQ * EPB), 0,
-                       s, a.Nelements, a.elementList, a.GlobalToLocal, a.wJ, a.ggeo, a.gllw, a.D,
-                       a.lambda, a.q, a.Aq);
-  else
-    hipLaunchKernelGGL((k_bp5_ax<T, NQ, NF, EPB, false, WV>), dim3(blocks), dim3(NQ * NQ * EPB), 0,
-                       s, a.Nelements, a.elementList, a.GlobalToLocal, a.wJ, a.ggeo, a.gllw, a.D,
-                       a.lambda, a.q, a.Aq);
-  BPX_HIP(hipGetLastError());
-  return BPX_OK;
+  return launch_bp5<T, NQ, NF, false>(s, a);
+}
+
+template <typename T, int NQ>
+int launch_bp5_krhs_nq(hipStream_t s, const AxArgs<T>& a, int chunk) {
+  switch (chunk) {
+    case 1: return launch_bp5<T, NQ, 1, true>(s, a);
+    case 2: return launch_bp5<T, NQ, 2, true>(s, a);
+    case 4: return launch_bp5<T, NQ, 4, true>(s, a);
+    default: return set_error(BPX_ERR_INVALID_ARGUMENT, "bp5 k-RHS Ax: chunk must be 1, 2 or 4");
+  }
 }
 
 template int launch_bp5_nq<double, BPX_NQ, 1>(hipStream_t, const AxArgs<double>&);
 template int launch_bp5_nq<double, BPX_NQ, 3>(hipStream_t, const AxArgs<double>&);
 template int launch_bp5_nq<float, BPX_NQ, 1>(hipStream_t, const AxArgs<float>&);
+template int launch_bp5_krhs_nq<double, BPX_NQ>(hipStream_t, const AxArgs<double>&, int);
 
 }  // namespace bpx

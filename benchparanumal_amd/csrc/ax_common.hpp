@@ -7,6 +7,7 @@
 namespace bpx {
 
 template <typename T, int NQ, int NF> int launch_bp5_nq(hipStream_t s, const AxArgs<T>& a);
+template <typename T, int NQ> int launch_bp5_krhs_nq(hipStream_t s, const AxArgs<T>& a, int chunk);
 template <typename T, int NQ, int NF> int launch_bp1_nq(hipStream_t s, const AxArgs<T>& a);
 template <typename T, int NQ, int NF> int launch_bp3_nq(hipStream_t s, const AxArgs<T>& a);
 

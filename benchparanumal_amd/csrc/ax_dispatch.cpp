@@ -65,6 +65,31 @@ int launch_bp3_ax(hipStream_t s, const AxArgs<T>& a) {
   BPX_AX_SWITCH(launch_bp3_nq)
 }
 
+#define BPX_KRHS_CASE(NQ) \
+  case NQ:                 \
+    return launch_bp5_krhs_nq<T, NQ>(s, a, chunk);
+
+template <typename T>
+int launch_bp5_ax_krhs(hipStream_t s, const AxArgs<T>& a, int chunk) {
+  if (int st = check("bp5 k-RHS Ax", a.Nelements, a.q, a.Aq, a.GlobalToLocal, a.elementList)) return st;
+  if (a.K < 1) return set_error(BPX_ERR_INVALID_ARGUMENT, "bp5 k-RHS Ax: K < 1");
+  if (a.Nfields != 1) return set_error(BPX_ERR_UNSUPPORTED, "bp5 k-RHS Ax: one field per right-hand side");
+  if (chunk == 0) chunk = a.K % 2 == 0 ? 2 : 1;
+  if ((chunk != 1 && chunk != 2 && chunk != 4) || a.K % chunk != 0)
+    return set_error(BPX_ERR_INVALID_ARGUMENT, "bp5 k-RHS Ax: chunk must be 1, 2 or 4 and divide K");
+  if (a.Nelements == 0) return BPX_OK;
+  if (!a.wJ || !a.ggeo || !a.D || (a.affine && !a.gllw))
+    return set_error(BPX_ERR_INVALID_ARGUMENT, "bp5 k-RHS Ax: null geometry/operator pointer");
+  switch (a.N + 1) {
+    BPX_KRHS_CASE(2) BPX_KRHS_CASE(3) BPX_KRHS_CASE(4) BPX_KRHS_CASE(5) BPX_KRHS_CASE(6)
+    BPX_KRHS_CASE(7) BPX_KRHS_CASE(8) BPX_KRHS_CASE(9) BPX_KRHS_CASE(10) BPX_KRHS_CASE(11)
+    BPX_KRHS_CASE(12) BPX_KRHS_CASE(13) BPX_KRHS_CASE(14) BPX_KRHS_CASE(15) BPX_KRHS_CASE(16)
+    default:
+      return set_error(BPX_ERR_UNSUPPORTED, "bp5 k-RHS Ax: degree N must be in 1..15");
+  }
+}
+
+template int launch_bp5_ax_krhs<double>(hipStream_t, const AxArgs<double>&, int);
 template int launch_bp5_ax<double>(hipStream_t, const AxArgs<double>&);
 template int launch_bp5_ax<float>(hipStream_t, const AxArgs<float>&);
 template int launch_bp1_ax<double>(hipStream_t, const AxArgs<double>&);

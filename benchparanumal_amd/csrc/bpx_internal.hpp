@@ -63,10 +63,12 @@ int launch_rhs(hipStream_t s, int N, int Nfields, int affine, int Nelements, con
                const double* gllw, const double* x, const double* y, const double* z,
                double lambda, double* rhs);
 
-// ogs (libs/ogs/okl/ogsKernels.okl semantics, K interleaved vectors)
+// ogs (libs/ogs/okl/ogsKernels.okl semantics, K interleaved vectors).
+// mode: order in which the gather's workgroups take super-blocks (0 strided,
+// 1 XCD-blocked); it changes speed only, never a result bit.
 template <typename T>
 int launch_ogs_gather(hipStream_t s, int Nblocks, int K, const int* blockStarts,
-                      const int* rowStarts, const int* colIds, const T* q, T* gq);
+                      const int* rowStarts, const int* colIds, const T* q, T* gq, int mode = 0);
 template <typename T>
 int launch_ogs_scatter(hipStream_t s, int Nblocks, int K, const int* blockStarts,
                        const int* rowStarts, const int* colIds, const T* gq, T* q);
@@ -74,12 +76,12 @@ template <typename T>
 int launch_ogs_gather_scatter(hipStream_t s, int Nblocks, int K, const int* blockStarts,
                               const int* rowStarts, const int* colIds, T* q);
 
-// gather with a fused p.Ap: one FP64 partial per workgroup, in part[0..n)
-// with n = ogs_gather_partials(Nblocks)
+// gather with a fused p.Ap: one FP64 partial per super-block of row
+// blocks, in part[0..n) with n = ogs_gather_partials(Nblocks)
 int ogs_gather_partials(int Nblocks);
 template <typename T>
 int launch_ogs_gather_dot(hipStream_t s, int Nblocks, const int* blockStarts, const int* rowStarts,
-                          const int* colIds, const T* q, T* gq, const T* p, double* part);
+                          const int* colIds, const T* q, T* gq, const T* p, double* part, int mode = 0);
 
 // linAlg. Reductions run on a fixed grid of kRedBlocks workgroups and write
 // kRedBlocks FP64 partials; launch_fold sums partials into a device scalar.

@@ -146,6 +146,37 @@ int bpx_bp5_ax_affine_hex(bpx_platform p, int N, int Nelements, const int* eleme
   (void)vgeo; (void)S_; (void)MM;
   return bp5_entry(p, N, Nelements, elementList, GlobalToLocal, wJ, ggeo, gllw, D, lambda, q, Aq, 1, 1);
 }
+static int bp5_krhs_entry(bpx_platform p, int N, int K, int chunk, int Nelements, const int* elementList,
+                          const int* GlobalToLocal, const double* wJ, const double* ggeo, const double* gllw,
+                          const double* D, double lambda, const double* q, double* Aq, int affine) {
+  if (int st = need(p, "platform")) return st;
+  AX_COMMON(double)
+  a.affine = affine;
+  a.wJ = wJ;
+  a.ggeo = ggeo;
+  a.gllw = gllw;
+  a.D = D;
+  a.lambda = lambda;
+  a.K = K;
+  return launch_bp5_ax_krhs<double>(S(p), a, chunk);
+}
+
+int bpx_bp5_ax_hex_krhs(bpx_platform p, int N, int K, int chunk, int Nelements, const int* elementList,
+                        const int* GlobalToLocal, const double* wJ, const double* vgeo, const double* ggeo,
+                        const double* gllw, const double* D, const double* S_, const double* MM, double lambda,
+                        const double* q, double* Aq) {
+  (void)vgeo; (void)S_; (void)MM;
+  return bp5_krhs_entry(p, N, K, chunk, Nelements, elementList, GlobalToLocal, wJ, ggeo, gllw, D, lambda, q, Aq,
+                        0);
+}
+int bpx_bp5_ax_affine_hex_krhs(bpx_platform p, int N, int K, int chunk, int Nelements, const int* elementList,
+                               const int* GlobalToLocal, const double* wJ, const double* vgeo, const double* ggeo,
+                               const double* gllw, const double* D, const double* S_, const double* MM,
+                               double lambda, const double* q, double* Aq) {
+  (void)vgeo; (void)S_; (void)MM;
+  return bp5_krhs_entry(p, N, K, chunk, Nelements, elementList, GlobalToLocal, wJ, ggeo, gllw, D, lambda, q, Aq,
+                        1);
+}
 int bpx_bp6_ax_hex(bpx_platform p, int N, int Nelements, const int* elementList, const int* GlobalToLocal,
                    const double* wJ, const double* vgeo, const double* ggeo, const double* gllw, const double* D,
                    const double* S_, const double* MM, double lambda, const double* q, double* Aq) {
@@ -499,6 +530,15 @@ int bpx_bp_info(bpx_bp b, int64_t* out, int capacity) {
   return n;
 }
 
+int bpx_bp_tuning(bpx_bp b, double* out, int capacity) {
+  if (int st = need(b, "bp")) return st;
+  const auto& t = b->bp.tuning;
+  const double v[] = {(double)t.gatherMode, t.gather_us[0], t.gather_us[1]};
+  const int n = 3;
+  for (int i = 0; i < n && i < capacity; ++i) out[i] = v[i];
+  return n;
+}
+
 int bpx_bp_operator(bpx_bp b, const double* o_q, double* o_Aq) {
   if (int st = need(b, "bp")) return st;
   return b->bp.Operator(o_q, o_Aq);
@@ -512,6 +552,13 @@ int bpx_bp_ax_local(bpx_bp b, const double* o_q, double* o_AqL) {
 int bpx_bp_operator_f32(bpx_bp b, const float* o_q, float* o_Aq) {
   if (int st = need(b, "bp")) return st;
   return b->bp.Operator32(o_q, o_Aq);
+}
+
+int bpx_bp_operator_krhs(bpx_bp b, int K, int chunk, const double* o_q, double* o_Aq) {
+  BPX_GUARD_BEGIN
+  if (int st = need(b, "bp")) return st;
+  return b->bp.OperatorKrhs(K, o_q, o_Aq, chunk);
+  BPX_GUARD_END
 }
 
 int bpx_bp_rhs_forcing(bpx_bp b, double* o_b) {

@@ -132,6 +132,7 @@ class ogs_t {
   int GatherScatter(platform_t& p, dfloat* o_v, int K) const;
 
   dlong N = 0, Ngather = 0, nnzT = 0, NrowBlocksT = 0;
+  int gatherMode = 0;  // workgroup order of the gather (set by the tuner; same results either way)
   std::vector<dlong> rowStartsT, colIdsT, blockRowStartsT;
   deviceBuffer<dlong> o_rowStartsT, o_colIdsT, o_blockRowStartsT;
 };
@@ -209,9 +210,20 @@ class bp_t : public solver_t {
   int OperatorDot(const dfloat* o_q, dfloat* o_Aq, double* part, int* npart) override;
   int OperatorDot32(const float* o_q, float* o_Aq, double* part, int* npart);
   int GatherPartials() const { return ogs_gather_partials(ogs.NrowBlocksT); }
+  // times the launch variants that give identical results (gather
+  // workgroup order) on this problem and keeps the fastest
+  int Tune();
+  struct Tuning {
+    int gatherMode = 0;
+    double gather_us[2] = {0, 0};
+  } tuning;
   int AxLocal(const dfloat* o_q, dfloat* o_AqL);
   int Operator32(const float* o_q, float* o_Aq);
   int Forcing(dfloat* o_b);
+  // K right-hand sides at once (BP5): o_q is [Ngather][K], o_Aq the same,
+  // o_AqL [NLocal][K]; chunk as for launch_bp5_ax_krhs (0 = default)
+  int AxLocalKrhs(int K, const dfloat* o_q, dfloat* o_AqL, int chunk);
+  int OperatorKrhs(int K, const dfloat* o_q, dfloat* o_Aq, int chunk);
   // FP32 CG with FP64 reliable updates every `interval` iterations until
   // ||b - A x|| / ||b|| < tol; stats = {updates, FP32 iterations, rel. residual}
   int SolveMixed(dfloat* o_x, const dfloat* o_b, double tol, int interval, int maxit, double* stats);
@@ -235,6 +247,7 @@ class bp_t : public solver_t {
   deviceBuffer<dlong> o_GlobalToLocal, o_elementList;
   deviceBuffer<double> o_wJ, o_ggeo, o_gllw, o_D, o_I, o_MM, o_invV;
   deviceBuffer<double> o_AqL;
+  deviceBuffer<double> o_AqLk;  // [NLocal][K] for OperatorKrhs, grown on demand
   // FP32 copies (created on first use)
   int Prepare32();
   bool have32 = false;

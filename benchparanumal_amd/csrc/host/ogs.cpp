@@ -134,7 +134,7 @@ int ogs_t::Upload(platform_t& p) {
 template <typename T>
 int ogs_t::Gather(platform_t& p, T* o_gv, const T* o_v, int K) const {
   return launch_ogs_gather<T>(p.stream, NrowBlocksT, K, o_blockRowStartsT.ptr(), o_rowStartsT.ptr(),
-                              o_colIdsT.ptr(), o_v, o_gv);
+                              o_colIdsT.ptr(), o_v, o_gv, gatherMode);
 }
 
 template <typename T>

@@ -257,6 +257,7 @@ int bp_t::Setup(platform_t& p, mesh_t& m, int bp_, dfloat lambda_) {
     if (affine)
       if (int st = o_invV.upload(m.invV, s)) return st;
   }
+  if (int st = Tune()) return st;
   return p.finish();
 }
 
@@ -301,7 +302,38 @@ int bp_t::OperatorDot(const dfloat* o_q, dfloat* o_Aq, double* part, int* npart)
   if (int st = AxLocal(o_q, o_AqL.ptr())) return st;
   *npart = GatherPartials();
   return launch_ogs_gather_dot<double>(platform->stream, ogs.NrowBlocksT, ogs.o_blockRowStartsT.ptr(),
-                                       ogs.o_rowStartsT.ptr(), ogs.o_colIdsT.ptr(), o_AqL.ptr(), o_Aq, o_q, part);
+                                       ogs.o_rowStartsT.ptr(), ogs.o_colIdsT.ptr(), o_AqL.ptr(), o_Aq, o_q, part,
+                                       ogs.gatherMode);
+}
+
+int bp_t::AxLocalKrhs(int K, const dfloat* o_q, dfloat* o_AqL, int chunk) {
+  if (bp != 5) return set_error(BPX_ERR_UNSUPPORTED, "bp: k right-hand sides are implemented for BP5");
+  AxArgs<double> a;
+  a.N = mesh->N;
+  a.Nelements = mesh->Nelements;
+  a.affine = mesh->opts.affine;
+  a.elementList = o_elementList.ptr();
+  a.GlobalToLocal = o_GlobalToLocal.ptr();
+  a.wJ = o_wJ.ptr();
+  a.ggeo = o_ggeo.ptr();
+  a.gllw = o_gllw.ptr();
+  a.D = o_D.ptr();
+  a.lambda = lambda;
+  a.q = o_q;
+  a.Aq = o_AqL;
+  a.K = K;
+  return launch_bp5_ax_krhs<double>(platform->stream, a, chunk);
+}
+
+int bp_t::OperatorKrhs(int K, const dfloat* o_q, dfloat* o_Aq, int chunk) {
+  if (K < 1) return set_error(BPX_ERR_INVALID_ARGUMENT, "bp: K < 1");
+  const size_t need = (size_t)NLocal * K;
+  if (o_AqLk.length() < need) {
+    if (int st = platform->finish()) return st;  // the old buffer may still be in use
+    if (int st = o_AqLk.alloc(need)) return st;
+  }
+  if (int st = AxLocalKrhs(K, o_q, o_AqLk.ptr(), chunk)) return st;
+  return ogs.Gather<double>(*platform, o_Aq, o_AqLk.ptr(), K);
 }
 
 int bp_t::OperatorDot32(const float* o_q, float* o_Aq, double* part, int* npart) {
@@ -312,7 +344,48 @@ int bp_t::OperatorDot32(const float* o_q, float* o_Aq, double* part, int* npart)
     return st;
   *npart = GatherPartials();
   return launch_ogs_gather_dot<float>(platform->stream, ogs.NrowBlocksT, ogs.o_blockRowStartsT.ptr(),
-                                      ogs.o_rowStartsT.ptr(), ogs.o_colIdsT.ptr(), o_AqL32.ptr(), o_Aq, o_q, part);
+                                      ogs.o_rowStartsT.ptr(), ogs.o_colIdsT.ptr(), o_AqL32.ptr(), o_Aq, o_q, part,
+                                      ogs.gatherMode);
+}
+
+int bp_t::Tune() {
+  hipStream_t s = platform->stream;
+  const dlong Ng = ogs.Ngather;
+  if (Ng == 0 || ogs.NrowBlocksT == 0) return BPX_OK;
+  deviceBuffer<double> v, part;
+  if (int st = v.alloc(2 * (size_t)Ng)) return st;
+  if (int st = part.alloc(GatherPartials())) return st;
+  BPX_HIP(hipMemsetAsync(o_AqL.ptr(), 0, (size_t)NLocal * sizeof(double), s));
+  BPX_HIP(hipMemsetAsync(v.ptr(), 0, 2 * (size_t)Ng * sizeof(double), s));
+  hipEvent_t e0, e1;
+  BPX_HIP(hipEventCreate(&e0));
+  BPX_HIP(hipEventCreate(&e1));
+  int st = BPX_OK;
+  for (int mode = 0; mode < 2 && st == BPX_OK; ++mode) {
+    auto run = [&] {
+      return launch_ogs_gather_dot<double>(s, ogs.NrowBlocksT, ogs.o_blockRowStartsT.ptr(), ogs.o_rowStartsT.ptr(),
+                                           ogs.o_colIdsT.ptr(), o_AqL.ptr(), v.ptr(), v.ptr() + Ng, part.ptr(),
+                                           mode);
+    };
+    if ((st = run())) break;
+    float best = 1e30f;
+    for (int rep = 0; rep < 3 && st == BPX_OK; ++rep) {
+      float ms = 0;
+      if (hipEventRecord(e0, s) != hipSuccess || (st = run()) || hipEventRecord(e1, s) != hipSuccess ||
+          hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+        if (!st) st = set_error(BPX_ERR_HIP, "tune: event timing failed");
+        break;
+      }
+      best = std::min(best, ms);
+    }
+    tuning.gather_us[mode] = best * 1e3;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (st) return st;
+  tuning.gatherMode = tuning.gather_us[1] < tuning.gather_us[0] ? 1 : 0;
+  ogs.gatherMode = tuning.gatherMode;
+  return BPX_OK;
 }
 
 int bp_t::Prepare32() {

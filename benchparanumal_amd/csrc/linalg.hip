@@ -19,6 +19,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdint>
+#include <type_traits>
 
 #include "bpx_internal.hpp"
 
@@ -54,15 +56,37 @@ __device__ inline double block_allsum(double v, double* s_w) {
   return s_w[32];
 }
 
-// Sum of partials[0..n) in a fixed order: lane t < kFoldWidth folds
-// t, t+kFoldWidth, ...; the remaining lanes add exact zeros, so any
-// workgroup size >= kFoldWidth gives the same bits.
-constexpr int kFoldWidth = 256;
+// Sum of partials[0..n) in a fixed order, by one wavefront: lane l folds
+// l, l+64, ... into four accumulators (so its loads are in flight
+// together), then an xor butterfly (each step adds the same two operands on
+// both lanes of a pair, so every lane ends with the same bits). Every
+// wavefront that folds the same partials gets the same value, whatever
+// kernel or workgroup size it runs in.
+__device__ inline double wave_fold(const double* __restrict__ part, int n) {
+  const int lane = threadIdx.x & 63;
+  double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+  int i = lane;
+  for (; i + 192 < n; i += 256) {
+    a0 += part[i];
+    a1 += part[i + 64];
+    a2 += part[i + 128];
+    a3 += part[i + 192];
+  }
+  for (; i < n; i += 64) a0 += part[i];
+  double v = (a0 + a1) + (a2 + a3);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// wave_fold by wavefront 0, broadcast to the workgroup through LDS
 __device__ inline double fold_partials(const double* __restrict__ part, int n, double* s_w) {
-  double v = 0;
-  if (threadIdx.x < kFoldWidth)
-    for (int i = threadIdx.x; i < n; i += kFoldWidth) v += part[i];
-  return block_allsum(v, s_w);
+  if (threadIdx.x < 64) {
+    const double v = wave_fold(part, n);
+    if (threadIdx.x == 0) s_w[32] = v;
+  }
+  __syncthreads();
+  return s_w[32];
 }
 
 // Grid-strided sum with four independent accumulators per lane.
@@ -80,6 +104,33 @@ __device__ inline double strided_sum(dlong N, F term) {
   for (; n < N; n += stride) a0 += term(n);
   return (a0 + a1) + (a2 + a3);
 }
+
+// Grid-strided loop, four independent elements per lane and step.
+template <typename F>
+__device__ inline void strided_for(dlong N, F body) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; n + 3 * stride < N; n += 4 * stride) {
+    body(n);
+    body(n + stride);
+    body(n + 2 * stride);
+    body(n + 3 * stride);
+  }
+  for (; n < N; n += stride) body(n);
+}
+
+// 16-byte vector view of a float stream: V4 floats per lane and access,
+// used when every pointer of the stream is 16-byte aligned
+template <typename T>
+struct vec4;
+template <>
+struct vec4<float> {
+  typedef float4 type;
+};
+template <>
+struct vec4<double> {
+  typedef double4 type;
+};
 
 template <typename T>
 __global__ __launch_bounds__(ST) void k_set(dlong N, T alpha, T* __restrict__ x) {
@@ -133,9 +184,10 @@ __global__ __launch_bounds__(RT) void k_cg_init(dlong N, const T* __restrict__ A
 }
 
 // Start of iteration k: rr_k = r.r from the partials, stopping test,
-// beta = rr_k / rr_{k-1}, p <- r + beta p (p <- r at k = 0).
-template <typename T>
-__global__ __launch_bounds__(ST) void k_cg_update_p(dlong N, int k, const T* __restrict__ r, T* __restrict__ p,
+// beta = rr_k / rr_{k-1}, p <- r + beta p (p <- r at k = 0). VEC: float4
+// accesses (all pointers 16-byte aligned, float only).
+template <typename T, bool VEC>
+__global__ __launch_bounds__(RT) void k_cg_update_p(dlong N, int k, const T* __restrict__ r, T* __restrict__ p,
                                                     const double* __restrict__ part_rr, int nrr, CgState* st,
                                                     double* __restrict__ hist_rr, double tol) {
   __shared__ double s_w[33];
@@ -166,13 +218,32 @@ __global__ __launch_bounds__(ST) void k_cg_update_p(dlong N, int k, const T* __r
   if (stop) return;
   const bool first = (k == 0);
   const T b = first ? T(0) : (T)(rr / hist_rr[k - 1]);
-  for (dlong n = blockIdx.x * ST + threadIdx.x; n < N; n += gridDim.x * ST)
-    p[n] = first ? r[n] : T(1) * r[n] + b * p[n];
+  if (VEC) {
+    typedef typename vec4<T>::type V;
+    const V* r4 = reinterpret_cast<const V*>(r);
+    V* p4 = reinterpret_cast<V*>(p);
+    strided_for(N / 4, [&](long long n) {
+      const V rv = r4[n];
+      V pv = first ? rv : p4[n];
+      if (!first) {
+        pv.x = T(1) * rv.x + b * pv.x;
+        pv.y = T(1) * rv.y + b * pv.y;
+        pv.z = T(1) * rv.z + b * pv.z;
+        pv.w = T(1) * rv.w + b * pv.w;
+      }
+      p4[n] = pv;
+    });
+    const long long tail = (long long)(N / 4) * 4;
+    const long long n = tail + (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n < N) p[n] = first ? r[n] : T(1) * r[n] + b * p[n];
+  } else {
+    strided_for(N, [&](long long n) { p[n] = first ? r[n] : T(1) * r[n] + b * p[n]; });
+  }
 }
 
 // End of iteration k: alpha = rr_k / p.Ap, r <- r - alpha Ap, x <- x + alpha p,
-// r.r partials for iteration k+1.
-template <typename T>
+// r.r partials for iteration k+1. VEC: float4 accesses (float only).
+template <typename T, bool VEC>
 __global__ __launch_bounds__(RT) void k_cg_update_xr(dlong N, int k, const T* __restrict__ p,
                                                      const T* __restrict__ Ap, T* __restrict__ x,
                                                      T* __restrict__ r, const double* __restrict__ part_pAp,
@@ -193,12 +264,41 @@ __global__ __launch_bounds__(RT) void k_cg_update_xr(dlong N, int k, const T* __
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) hist_alpha[k] = alpha;
   const T a = (T)alpha;
-  const double v = strided_sum(N, [&](long long n) {
-    const T rn = r[n] - a * Ap[n];
-    r[n] = rn;
-    x[n] = a * p[n] + T(1) * x[n];
-    return (double)rn * (double)rn;
-  });
+  double v;
+  if (VEC) {
+    typedef typename vec4<T>::type V;
+    const V* p4 = reinterpret_cast<const V*>(p);
+    const V* A4 = reinterpret_cast<const V*>(Ap);
+    V* x4 = reinterpret_cast<V*>(x);
+    V* r4 = reinterpret_cast<V*>(r);
+    v = strided_sum(N / 4, [&](long long n) {
+      V rv = r4[n];
+      const V Av = A4[n], pv = p4[n];
+      V xv = x4[n];
+      rv.x = rv.x - a * Av.x; rv.y = rv.y - a * Av.y; rv.z = rv.z - a * Av.z; rv.w = rv.w - a * Av.w;
+      xv.x = a * pv.x + T(1) * xv.x; xv.y = a * pv.y + T(1) * xv.y;
+      xv.z = a * pv.z + T(1) * xv.z; xv.w = a * pv.w + T(1) * xv.w;
+      r4[n] = rv;
+      x4[n] = xv;
+      return ((double)rv.x * (double)rv.x + (double)rv.y * (double)rv.y) +
+             ((double)rv.z * (double)rv.z + (double)rv.w * (double)rv.w);
+    });
+    const long long tail = (long long)(N / 4) * 4;
+    const long long n = tail + (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n < N) {
+      const T rn = r[n] - a * Ap[n];
+      r[n] = rn;
+      x[n] = a * p[n] + T(1) * x[n];
+      v += (double)rn * (double)rn;
+    }
+  } else {
+    v = strided_sum(N, [&](long long n) {
+      const T rn = r[n] - a * Ap[n];
+      r[n] = rn;
+      x[n] = a * p[n] + T(1) * x[n];
+      return (double)rn * (double)rn;
+    });
+  }
   const double t = block_allsum(v, s_w);
   if (threadIdx.x == 0) part_rr[blockIdx.x] = t;
 }
@@ -328,11 +428,19 @@ int launch_cg_init(hipStream_t s, dlong N, const T* Ap, T* r, double* part_rr) {
   return BPX_OK;
 }
 
+inline bool aligned16(const void* a) { return ((uintptr_t)a & 15) == 0; }
+
+// float streams use 16-byte accesses when every pointer allows it; FP64
+// keeps the scalar form, whose partial sums the parity tests pin
 template <typename T>
 int launch_cg_update_p(hipStream_t s, dlong N, int k, const T* r, T* p, const double* part_rr, int nrr,
                        CgState* st, double* hist_rr, double tol) {
-  hipLaunchKernelGGL(k_cg_update_p<T>, dim3(stream_blocks(N)), dim3(ST), 0, s, N, k, r, p, part_rr, nrr, st,
-                     hist_rr, tol);
+  if (std::is_same<T, float>::value && aligned16(r) && aligned16(p))
+    hipLaunchKernelGGL((k_cg_update_p<T, true>), dim3(RB), dim3(RT), 0, s, N, k, r, p, part_rr, nrr, st, hist_rr,
+                       tol);
+  else
+    hipLaunchKernelGGL((k_cg_update_p<T, false>), dim3(RB), dim3(RT), 0, s, N, k, r, p, part_rr, nrr, st,
+                       hist_rr, tol);
   BPX_HIP(hipGetLastError());
   return BPX_OK;
 }
@@ -341,8 +449,12 @@ template <typename T>
 int launch_cg_update_xr(hipStream_t s, dlong N, int k, const T* p, const T* Ap, T* x, T* r,
                         const double* part_pAp, int npAp, double* part_rr, CgState* st, const double* hist_rr,
                         double* hist_alpha) {
-  hipLaunchKernelGGL(k_cg_update_xr<T>, dim3(RB), dim3(RT), 0, s, N, k, p, Ap, x, r, part_pAp, npAp, part_rr,
-                     st, hist_rr, hist_alpha);
+  if (std::is_same<T, float>::value && aligned16(p) && aligned16(Ap) && aligned16(x) && aligned16(r))
+    hipLaunchKernelGGL((k_cg_update_xr<T, true>), dim3(RB), dim3(RT), 0, s, N, k, p, Ap, x, r, part_pAp, npAp,
+                       part_rr, st, hist_rr, hist_alpha);
+  else
+    hipLaunchKernelGGL((k_cg_update_xr<T, false>), dim3(RB), dim3(RT), 0, s, N, k, p, Ap, x, r, part_pAp, npAp,
+                       part_rr, st, hist_rr, hist_alpha);
   BPX_HIP(hipGetLastError());
   return BPX_OK;
 }

@@ -86,6 +86,23 @@ int bpx_bp5_ax_affine_hex(bpx_platform p, int N, int Nelements, const int* eleme
                           const double* ggeo, const double* gllw, const double* D,
                           const double* S, const double* MM, double lambda, const double* q,
                           double* Aq);
+/* No reference counterpart (the reference applies Ax to one vector per
+ * launch; okl/bp5AxHex3D.okl:415-427 is the single-vector form): BP5 Ax on K
+ * right-hand sides at once, geometry streamed once. Vectors are interleaved
+ * as in the ogs K form (libs/ogs/okl/ogsKernels.okl:86-122): q[r + K*id],
+ * Aq[r + K*(e*Np + n)]. chunk = right-hand sides per register sweep (1, 2
+ * or 4, dividing K; 0 = default). Each right-hand side equals the
+ * single-vector bpx_bp5_ax_hex bit for bit. */
+int bpx_bp5_ax_hex_krhs(bpx_platform p, int N, int K, int chunk, int Nelements,
+                        const int* elementList, const int* GlobalToLocal, const double* wJ,
+                        const double* vgeo, const double* ggeo, const double* gllw,
+                        const double* D, const double* S, const double* MM, double lambda,
+                        const double* q, double* Aq);
+int bpx_bp5_ax_affine_hex_krhs(bpx_platform p, int N, int K, int chunk, int Nelements,
+                               const int* elementList, const int* GlobalToLocal,
+                               const double* wJ, const double* vgeo, const double* ggeo,
+                               const double* gllw, const double* D, const double* S,
+                               const double* MM, double lambda, const double* q, double* Aq);
 /* okl/bp6AxHex3D.okl:555-567 (3 fields, local layout [e][f][n]) */
 int bpx_bp6_ax_hex(bpx_platform p, int N, int Nelements, const int* elementList,
                    const int* GlobalToLocal, const double* wJ, const double* vgeo,
@@ -236,6 +253,11 @@ int bpx_bp_create(bpx_platform p, bpx_mesh m, int bp, double lambda, bpx_bp* out
 int bpx_bp_destroy(bpx_bp b);
 /* counts: bp Nfields Ngather NLocal Nelements N (int64 each) */
 int bpx_bp_info(bpx_bp b, int64_t* out, int capacity);
+/* No reference counterpart: launch choices the setup timed on this
+ * problem, all giving identical results. out: [gather workgroup order
+ * (0 strided, 1 XCD-blocked), its time in us for order 0, for order 1].
+ * Returns the number of values. */
+int bpx_bp_tuning(bpx_bp b, double* out, int capacity);
 /* bpN_t::Operator(o_q, o_Aq) (BP/BP5/src/bp5Operator.cpp:29-85): Ax over the
  * local elements, then the T gather. Device vectors of length Ngather. */
 int bpx_bp_operator(bpx_bp b, const double* o_q, double* o_Aq);
@@ -243,6 +265,10 @@ int bpx_bp_operator(bpx_bp b, const double* o_q, double* o_Aq);
 int bpx_bp_ax_local(bpx_bp b, const double* o_q, double* o_AqL);
 /* FP32 operator on the same mesh (factors rounded to float). */
 int bpx_bp_operator_f32(bpx_bp b, const float* o_q, float* o_Aq);
+/* No reference counterpart: bpN_t::Operator (BP/BP5/src/bp5Operator.cpp:29-85)
+ * on K interleaved right-hand sides (BP5 only): o_q and o_Aq are
+ * [Ngather][K]; k-RHS Ax, then the ogs gather with K. */
+int bpx_bp_operator_krhs(bpx_bp b, int K, int chunk, const double* o_q, double* o_Aq);
 /* b = gather(forcing) as bpN_t::Run does (BP/BP5/src/bp5Run.cpp:51-64). */
 int bpx_bp_rhs_forcing(bpx_bp b, double* o_b);
 /* cg::Solve (libs/core/linearSolverCG.cpp:62-129) with device-resident

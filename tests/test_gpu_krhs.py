@@ -1,0 +1,121 @@
+"""k right-hand sides at once (BP5): every right-hand side of the batched
+operator equals the single-vector operator to 1e-15 relative (same
+arithmetic), and the single-vector operator is itself pinned against the
+oracle; one case is also checked against the oracle directly."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle_lib
+from benchparanumal_amd import capi
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+KRHS_RTOL = 1e-15
+
+
+@pytest.fixture(scope="module")
+def plat():
+    assert torch.cuda.is_available(), "GPU tests need a visible MI355X"
+    return capi.Platform(0)
+
+
+def rel_err(got, ref):
+    return float(np.abs(got - ref).max() / max(np.abs(ref).max(), 1e-300))
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def single_columns(b, qk, K, local):
+    """Apply the single-vector path to each column of qk ([n][K])."""
+    n_out = b.NLocal if local else b.Ngather
+    out = np.zeros((n_out, K))
+    for r in range(K):
+        q = torch.from_numpy(np.ascontiguousarray(qk[:, r])).cuda()
+        A = torch.zeros(n_out, dtype=torch.float64, device="cuda")
+        (b.ax_local if local else b.operator)(q, A)
+        out[:, r] = host(A)
+    return out
+
+
+def oracle_arrays(P):
+    return {k: torch.from_numpy(np.ascontiguousarray(P.array(k))).cuda()
+            for k in ("elementList", "GlobalToLocal", "wJ", "ggeo", "gllw", "D")}
+
+
+def krhs_kernel(plat, P, arr, qk, K, chunk, affine, lam):
+    """bpx_bp5_ax[_affine]_hex_krhs on the oracle problem's arrays."""
+    out = torch.zeros(P.NLocal * K, dtype=torch.float64, device="cuda")
+    qd = torch.from_numpy(np.ascontiguousarray(qk.ravel())).cuda()
+    fn = capi.lib().bpx_bp5_ax_affine_hex_krhs if affine else capi.lib().bpx_bp5_ax_hex_krhs
+    capi.check(fn(plat.h, P.N, K, chunk, P.Nelements, capi.ptr(arr["elementList"]), capi.ptr(arr["GlobalToLocal"]),
+                  capi.ptr(arr["wJ"]), None, capi.ptr(arr["ggeo"]), capi.ptr(arr["gllw"]), capi.ptr(arr["D"]),
+                  None, None, lam, capi.ptr(qd), capi.ptr(out)))
+    return host(out).reshape(P.NLocal, K)
+
+
+def single_kernel(plat, P, arr, qk, K, affine, lam):
+    """bpx_bp5_ax[_affine]_hex once per right-hand side, same arrays."""
+    out = np.zeros((P.NLocal, K))
+    fn = capi.lib().bpx_bp5_ax_affine_hex if affine else capi.lib().bpx_bp5_ax_hex
+    for r in range(K):
+        qd = torch.from_numpy(np.ascontiguousarray(qk[:, r])).cuda()
+        o = torch.zeros(P.NLocal, dtype=torch.float64, device="cuda")
+        capi.check(fn(plat.h, P.N, P.Nelements, capi.ptr(arr["elementList"]), capi.ptr(arr["GlobalToLocal"]),
+                      capi.ptr(arr["wJ"]), None, capi.ptr(arr["ggeo"]), capi.ptr(arr["gllw"]), capi.ptr(arr["D"]),
+                      None, None, lam, capi.ptr(qd), capi.ptr(o)))
+        out[:, r] = host(o)
+    return out
+
+
+@pytest.mark.parametrize("affine", [False, True])
+@pytest.mark.parametrize("N,K,chunk", [(7, 1, 1), (7, 2, 2), (7, 4, 4), (7, 8, 2), (7, 8, 4), (7, 3, 1),
+                                       (3, 4, 2), (4, 6, 2), (10, 4, 4), (15, 2, 2)])
+def test_krhs_ax_local_equals_single(plat, N, K, chunk, affine):
+    """Kernel entry: every right-hand side equals the single-vector kernel on
+    the same arrays (both run the same fused multiply-adds in the same order)."""
+    nx = 2 if N >= 10 else 3
+    lam = 1.3
+    P = oracle_lib.Problem(bp=5, N=N, nx=nx, affine=affine, perturb=0.0 if affine else 0.1, seed=3, lam=lam)
+    arr = oracle_arrays(P)
+    qk = np.random.default_rng(N * 10 + K).standard_normal((P.Ngather, K))
+    got = krhs_kernel(plat, P, arr, qk, K, chunk, affine, lam)
+    ref = single_kernel(plat, P, arr, qk, K, affine, lam)
+    assert rel_err(got, ref) <= KRHS_RTOL
+    # and against the oracle for the first and last right-hand side
+    for r in (0, K - 1):
+        assert rel_err(got[:, r], P.ax_local(np.ascontiguousarray(qk[:, r]))) <= 1e-12
+
+
+@pytest.mark.parametrize("K", [2, 4, 8, 16, 32])
+def test_krhs_operator_equals_single_20cube(plat, K):
+    """BP5 operator (Ax + gather) on K right-hand sides at 20^3 p=7 (K <= 8)
+    or 8^3 (K > 8, to bound the single-vector reference loop)."""
+    nx = 20 if K <= 8 else 8
+    m = capi.Mesh(N=7, nx=nx, perturb=0.1, seed=1)
+    b = capi.BP(plat, m, 5, lam=1.0)
+    qk = np.random.default_rng(K).standard_normal((b.Ngather, K))
+    Aq = torch.zeros(b.Ngather * K, dtype=torch.float64, device="cuda")
+    b.operator_krhs(torch.from_numpy(qk.ravel()).cuda(), Aq, K)
+    got = host(Aq).reshape(b.Ngather, K)
+    ref = single_columns(b, qk, K, local=False)
+    assert rel_err(got, ref) <= KRHS_RTOL
+
+
+def test_krhs_argument_errors(plat):
+    m = capi.Mesh(N=3, nx=2)
+    b = capi.BP(plat, m, 5, lam=1.0)
+    q = torch.zeros(b.Ngather * 3, dtype=torch.float64, device="cuda")
+    Aq = torch.zeros_like(q)
+    with pytest.raises(capi.BpxError):
+        b.operator_krhs(q, Aq, 3, chunk=2)  # chunk must divide K
+    with pytest.raises(capi.BpxError):
+        b.operator_krhs(q, Aq, 0)
+    b3 = capi.BP(plat, capi.Mesh(N=3, nx=2, cubature=True), 3, lam=1.0)
+    with pytest.raises(capi.BpxError):
+        b3.operator_krhs(q, Aq, 1)

@@ -156,15 +156,17 @@ def test_cg_reference_forcing_semantics(plat):
     x = torch.zeros_like(rhs)
     its, hist = b.solve(x, rhs, tol=0.0, maxit=100)
     assert its == it_ref
-    # this smooth problem converges to ~1e-18 within 100 iterations, so late
-    # iterates carry rounding noise of the order eps*||r0|| that another
-    # (equally valid) summation order changes in relative terms: the whole
-    # history is compared to 1e-9*||r0||, and per iteration to 1e-9 while the
-    # residual is above 1e-3*||r0||
-    np.testing.assert_allclose(hist, hist_ref, rtol=0, atol=CG_RTOL * hist_ref[0])
+    # the forcing is close to an eigenvector, so the residual collapses from
+    # 3e-5 to 1e-7 of ||r0|| at iteration 20 and then sits at rounding level;
+    # there the history depends on the summation order: the oracle compiled
+    # with and without FMA contraction differs by 1.7e-8*||r0|| (15%) at
+    # iteration 20, but by 5e-14 relative while the residual is above
+    # 1e-3*||r0||. Compare per iteration to 1e-9 above that level, and the
+    # whole history to 1e-6*||r0||.
     live = hist_ref > 1e-3 * hist_ref[0]
-    assert live.sum() >= 5
+    assert live.sum() >= 15
     np.testing.assert_allclose(hist[live], hist_ref[live], rtol=CG_RTOL)
+    np.testing.assert_allclose(hist, hist_ref, rtol=0, atol=1e-6 * hist_ref[0])
     assert np.abs(rhs_np).max() > 0
 
 
