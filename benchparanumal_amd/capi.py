@@ -72,6 +72,14 @@ _SIGS = {
     "bpx_bp_operator_f32": (_I, [_P, _P, _P]),
     "bpx_bp_operator_krhs": (_I, [_P, _I, _I, _P, _P]),
     "bpx_bp_tuning": (_I, [_P, _P, _I]),
+    "bpx_bp_set_kappa": (_I, [_P, _P]),
+    "bpx_batch_create": (_I, [_P, _I, _P, _D, C.POINTER(_P)]),
+    "bpx_batch_destroy": (_I, [_P]),
+    "bpx_batch_info": (_I, [_P, _I64P, _I]),
+    "bpx_batch_offsets": (_I, [_P, _I64P]),
+    "bpx_batch_tuning": (_I, [_P, _P, _I]),
+    "bpx_batch_solve": (_I, [_P, _P, _P, _D, _I, _P, _P]),
+    "bpx_bp_kappa_gradient": (_I, [_P, _P, _P, _P]),
     "bpx_bp_rhs_forcing": (_I, [_P, _P]),
     "bpx_bp_solve": (_I, [_P, _P, _P, _D, _I, _I, C.POINTER(_I), _P]),
     "bpx_bp_solve_mixed": (_I, [_P, _P, _P, _D, _I, _I, _P]),
@@ -198,6 +206,7 @@ class Mesh:
         keys = ["N", "Nq", "Np", "cubNq", "cubNp", "Nelements"]
         self.info = {k: int(buf[i]) for i, k in enumerate(keys[:n])}
         self.__dict__.update(self.info)
+        self.nx, self.ny, self.nz = o.nx, o.ny, o.nz
 
     def array(self, name):
         return _read_array(lib().bpx_mesh_array, self.h, name)
@@ -247,9 +256,10 @@ class BP:
         self.h = h
         self.platform = platform
         self.mesh = mesh
-        buf = (C.c_int64 * 6)()
-        lib().bpx_bp_info(self.h, buf, 6)
-        self.bp, self.Nfields, self.Ngather, self.NLocal, self.Nelements, self.N = (int(v) for v in buf)
+        buf = (C.c_int64 * 7)()
+        lib().bpx_bp_info(self.h, buf, 7)
+        self.bp, self.Nfields, self.Ngather, self.NLocal, self.Nelements, self.N, self.Nnodes = (
+            int(v) for v in buf)
 
     def operator(self, q, Aq):
         check(lib().bpx_bp_operator(self.h, ptr(q), ptr(Aq)))
@@ -259,6 +269,14 @@ class BP:
 
     def operator_f32(self, q, Aq):
         check(lib().bpx_bp_operator_f32(self.h, ptr(q), ptr(Aq)))
+
+    def set_kappa(self, kappa):
+        """Variable coefficient on the node lattice (device tensor of Nnodes), or None."""
+        check(lib().bpx_bp_set_kappa(self.h, None if kappa is None else ptr(kappa)))
+
+    def kappa_gradient(self, u, lam, grad):
+        """grad (Nnodes) = dL/dkappa from the solution u and the adjoint lam."""
+        check(lib().bpx_bp_kappa_gradient(self.h, ptr(u), ptr(lam), ptr(grad)))
 
     def tuning(self):
         """Launch choices timed at setup: gather order and its times (us)."""
@@ -289,6 +307,60 @@ class BP:
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
             _lib.bpx_bp_destroy(self.h)
+            self.h = None
+
+
+class Batch:
+    """Ragged batch of BP5 problems, one CG each, replayed from a hipGraph.
+
+    problems: list of dicts with Mesh arguments (N, nx, ny, nz, perturb, seed, ...)."""
+
+    def __init__(self, platform, problems, lam=1.0):
+        n = len(problems)
+        arr = (_MeshOpts * n)()
+        for i, pr in enumerate(problems):
+            o = arr[i]
+            o.N, o.nx = pr["N"], pr["nx"]
+            o.ny = pr.get("ny", pr["nx"])
+            o.nz = pr.get("nz", pr["nx"])
+            o.affine, o.cubature = 0, 0
+            lo, dim = pr.get("lo", (-1.0, -1.0, -1.0)), pr.get("dim", (2.0, 2.0, 2.0))
+            for c in range(3):
+                o.lo[c], o.dim[c] = lo[c], dim[c]
+            o.perturb, o.seed = pr.get("perturb", 0.0), pr.get("seed", 0)
+        h = _P()
+        check(lib().bpx_batch_create(platform.h, n, C.cast(arr, _P), lam, C.byref(h)))
+        self.h = h
+        self.platform = platform
+        self.nprob = n
+        off = np.zeros(n + 1, dtype=np.int64)
+        check(lib().bpx_batch_offsets(self.h, off.ctypes.data_as(_I64P)))
+        self.offsets = off
+        self.Ntot = int(off[-1])
+
+    def info(self):
+        buf = (C.c_int64 * 7)()
+        lib().bpx_batch_info(self.h, buf, 7)
+        keys = ["nprob", "Ntot", "ngroups", "nchunks", "nsupers", "NLocal", "replays"]
+        return {k: int(buf[i]) for i, k in enumerate(keys)}
+
+    def tuning(self):
+        n = lib().bpx_batch_tuning(self.h, None, 0)
+        out = np.zeros(max(n, 1))
+        lib().bpx_batch_tuning(self.h, ptr(out), n)
+        return [{"N": int(out[6 * g]), "shape": int(out[6 * g + 1]), "elements_per_workgroup": int(out[6 * g + 2]),
+                 "us": [float(v) for v in out[6 * g + 3:6 * g + 6]]} for g in range(n // 6)]
+
+    def solve(self, x, b, tol=1e-8, maxit=10000):
+        """Solve every problem; returns (iterations, final residual norms) per problem."""
+        its = np.zeros(self.nprob, dtype=np.int32)
+        rn = np.zeros(self.nprob)
+        check(lib().bpx_batch_solve(self.h, ptr(x), ptr(b), tol, maxit, its.ctypes.data_as(_P), ptr(rn)))
+        return its, rn
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.bpx_batch_destroy(self.h)
             self.h = None
 
 

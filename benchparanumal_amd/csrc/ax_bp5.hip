@@ -62,11 +62,16 @@ constexpr bool bp5_dreg() {
   return NQ <= 10 && NF == 1;
 }
 
-template <typename T, int NQ, int NF, int EPB, bool AFFINE, int WAVES, bool KRHS>
+// KRHS: q is [Ndofs][K], Aq [NLocal][K]; the element is swept K/NF times,
+// NF right-hand sides per sweep, re-reading its geometry from L2.
+// BATCH: elements of many problems; an element whose problem
+// (elemPid[e]) is done is skipped.
+template <typename T, int NQ, int NF, int EPB, bool AFFINE, int WAVES, bool KRHS, bool BATCH>
 __global__ __launch_bounds__(NQ* NQ* EPB) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_bp5_ax(
     int Nelements, int K, const int* __restrict__ elementList, const int* __restrict__ G2L,
     const T* __restrict__ wJ, const T* __restrict__ ggeo, const T* __restrict__ gllw,
-    const T* __restrict__ D, T lambda, const T* __restrict__ q, T* __restrict__ Aq) {
+    const T* __restrict__ D, T lambda, const T* __restrict__ q, T* __restrict__ Aq,
+    const int* __restrict__ elemPid, const BatchState* __restrict__ bst) {
   constexpr int NQ2 = NQ * NQ;
   constexpr int NP = NQ2 * NQ;
   constexpr int LD = NQ + 1;  // padded LDS row
@@ -83,8 +88,12 @@ __global__ __launch_bounds__(NQ* NQ* EPB) __attribute__((amdgpu_waves_per_eu(WAV
   const int i = ij % NQ;
   const int j = ij / NQ;
   const int eo = blockIdx.x * EPB + es;
-  const bool active = eo < Nelements;
+  bool active = eo < Nelements;
   const int e = active ? elementList[eo] : 0;
+  if (BATCH) {
+    if (active && bst[elemPid[e]].done) active = false;
+    if (EPB == 1 && !active) return;  // one element per workgroup: uniform exit
+  }
 
   for (int n = t; n < NQ2; n += NQ2 * EPB) s_D[n / NQ][n % NQ] = D[n];
 
@@ -227,6 +236,76 @@ __global__ __launch_bounds__(NQ* NQ* EPB) __attribute__((amdgpu_waves_per_eu(WAV
   }  // sweeps
 }
 
+// Adjoint gradient of the variable-coefficient BP5 (an extension):
+// gL[e*Np + n] = -(D lam)_n . G_n (D u)_n with the unscaled factors G_n,
+// u and lam global vectors read through GlobalToLocal (-1 -> 0). Same lane
+// sheet and slab sweep as the Ax kernel, without the weak divergence.
+template <int NQ, int EPB, bool AFFINE>
+__global__ __launch_bounds__(NQ* NQ* EPB) void k_bp5_kappa_grad(int Nelements, const int* __restrict__ G2L,
+                                                               const double* __restrict__ ggeo,
+                                                               const double* __restrict__ gllw,
+                                                               const double* __restrict__ D,
+                                                               const double* __restrict__ u,
+                                                               const double* __restrict__ lam,
+                                                               double* __restrict__ gL) {
+  constexpr int NQ2 = NQ * NQ;
+  constexpr int NP = NQ2 * NQ;
+  constexpr int LD = NQ + 1;
+  __shared__ double s_D[NQ][LD];
+  __shared__ double s_u[EPB][NQ][LD];
+  __shared__ double s_l[EPB][NQ][LD];
+
+  const int t = threadIdx.x;
+  const int es = t / NQ2;
+  const int ij = t - es * NQ2;
+  const int i = ij % NQ;
+  const int j = ij / NQ;
+  const int e = blockIdx.x * EPB + es;
+  const bool active = e < Nelements;
+  for (int n = t; n < NQ2; n += NQ2 * EPB) s_D[n / NQ][n % NQ] = D[n];
+
+  double ru[NQ], rl[NQ];
+#pragma unroll
+  for (int k = 0; k < NQ; ++k) {
+    const int id = active ? G2L[(size_t)e * NP + k * NQ2 + ij] : -1;
+    ru[k] = id >= 0 ? u[id] : 0.0;
+    rl[k] = id >= 0 ? lam[id] : 0.0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NQ; ++k) {
+    s_u[es][j][i] = ru[k];
+    s_l[es][j][i] = rl[k];
+    __syncthreads();
+    double ur = 0, us = 0, ut = 0, lr = 0, ls = 0, lt = 0;
+#pragma unroll
+    for (int m = 0; m < NQ; ++m) {
+      ur = fmad(s_D[i][m], s_u[es][j][m], ur);
+      us = fmad(s_D[j][m], s_u[es][m][i], us);
+      ut = fmad(s_D[k][m], ru[m], ut);
+      lr = fmad(s_D[i][m], s_l[es][j][m], lr);
+      ls = fmad(s_D[j][m], s_l[es][m][i], ls);
+      lt = fmad(s_D[k][m], rl[m], lt);
+    }
+    if (active) {
+      double G00, G01, G02, G11, G12, G22;
+      if (AFFINE) {
+        const double* g = ggeo + (size_t)6 * e;
+        const double w = gllw[i] * gllw[j] * gllw[k];
+        G00 = g[0] * w; G01 = g[1] * w; G02 = g[2] * w; G11 = g[3] * w; G12 = g[4] * w; G22 = g[5] * w;
+      } else {
+        const double* g = ggeo + 6 * ((size_t)e * NP + k * NQ2 + ij);
+        G00 = g[0]; G01 = g[1]; G02 = g[2]; G11 = g[3]; G12 = g[4]; G22 = g[5];
+      }
+      const double gr = fmad(G02, ut, fmad(G01, us, G00 * ur));
+      const double gs = fmad(G12, ut, fmad(G11, us, G01 * ur));
+      const double gt = fmad(G22, ut, fmad(G12, us, G02 * ur));
+      gL[(size_t)e * NP + k * NQ2 + ij] = -fmad(lt, gt, fmad(ls, gs, lr * gr));
+    }
+    __syncthreads();
+  }
+}
+
 template <typename T, int NQ, int NF, bool KRHS>
 int launch_bp5(hipStream_t s, const AxArgs<T>& a) {
   constexpr int EPB = bp5_epb<NQ>();
@@ -234,13 +313,13 @@ int launch_bp5(hipStream_t s, const AxArgs<T>& a) {
   const int blocks = (a.Nelements + EPB - 1) / EPB;
   if (blocks == 0) return BPX_OK;
   if (a.affine)
-    hipLaunchKernelGGL((k_bp5_ax<T, NQ, NF, EPB, true, WV, KRHS>), dim3(blocks), dim3(NQ * NQ * EPB), 0,
+    hipLaunchKernelGGL((k_bp5_ax<T, NQ, NF, EPB, true, WV, KRHS, false>), dim3(blocks), dim3(NQ * NQ * EPB), 0,
                        s, a.Nelements, a.K, a.elementList, a.GlobalToLocal, a.wJ, a.ggeo, a.gllw, a.D,
-                       a.lambda, a.q, a.Aq);
+                       a.lambda, a.q, a.Aq, nullptr, nullptr);
   else
-    hipLaunchKernelGGL((k_bp5_ax<T, NQ, NF, EPB, false, WV, KRHS>), dim3(blocks), dim3(NQ * NQ * EPB), 0,
+    hipLaunchKernelGGL((k_bp5_ax<T, NQ, NF, EPB, false, WV, KRHS, false>), dim3(blocks), dim3(NQ * NQ * EPB), 0,
                        s, a.Nelements, a.K, a.elementList, a.GlobalToLocal, a.wJ, a.ggeo, a.gllw, a.D,
-                       a.lambda, a.q, a.Aq);
+                       a.lambda, a.q, a.Aq, nullptr, nullptr);
   BPX_HIP(hipGetLastError());
   return BPX_OK;
 }
@@ -262,6 +341,69 @@ int launch_bp5_krhs_nq(hipStream_t s, const AxArgs<T>& a, int chunk) {
   }
 }
 
+template <int NQ>
+int launch_bp5_kappa_grad_nq(hipStream_t s, int Nelements, int affine, const int* G2L, const double* ggeo,
+                             const double* gllw, const double* D, const double* u, const double* lam,
+                             double* gL) {
+  constexpr int EPB = bp5_epb<NQ>();
+  const int blocks = (Nelements + EPB - 1) / EPB;
+  if (blocks == 0) return BPX_OK;
+  if (affine)
+    hipLaunchKernelGGL((k_bp5_kappa_grad<NQ, EPB, true>), dim3(blocks), dim3(NQ * NQ * EPB), 0, s, Nelements,
+                       G2L, ggeo, gllw, D, u, lam, gL);
+  else
+    hipLaunchKernelGGL((k_bp5_kappa_grad<NQ, EPB, false>), dim3(blocks), dim3(NQ * NQ * EPB), 0, s, Nelements,
+                       G2L, ggeo, gllw, D, u, lam, gL);
+  BPX_HIP(hipGetLastError());
+  return BPX_OK;
+}
+
+// Batch variants for the setup-time tuner: workgroup shapes that round
+// identically (the default elements per workgroup, twice that, half of
+// it), so the tuner's choice never changes a result bit.
+template <int NQ, int V>
+constexpr int batch_epb() {
+  return V == 0 ? bp5_epb<NQ>() : V == 1 ? 2 * bp5_epb<NQ>() : bp5_epb<NQ>() / 2;
+}
+
+template <int NQ, int V>
+int launch_batch_v(hipStream_t s, int Nelements, const int* elementList, const int* G2L, const double* wJ,
+                   const double* ggeo, const double* D, double lambda, const double* q, double* Aq,
+                   const int* elemPid, const BatchState* done) {
+  constexpr int EPB = batch_epb<NQ, V>();
+  static_assert(EPB >= 1 && NQ * NQ * EPB <= 1024, "workgroup shape");
+  constexpr int WV = bp5_waves<NQ, 1, false>();
+  const int blocks = (Nelements + EPB - 1) / EPB;
+  if (blocks == 0) return BPX_OK;
+  hipLaunchKernelGGL((k_bp5_ax<double, NQ, 1, EPB, false, WV, false, true>), dim3(blocks), dim3(NQ * NQ * EPB), 0,
+                     s, Nelements, 1, elementList, G2L, wJ, ggeo, (const double*)nullptr, D, lambda, q, Aq,
+                     elemPid, done);
+  BPX_HIP(hipGetLastError());
+  return BPX_OK;
+}
+
+template <int NQ>
+int launch_bp5_batch_nq(hipStream_t s, int variant, int Nelements, const int* elementList, const int* G2L,
+                        const double* wJ, const double* ggeo, const double* D, double lambda, const double* q,
+                        double* Aq, const int* elemPid, const BatchState* done) {
+  if (variant == 0)
+    return launch_batch_v<NQ, 0>(s, Nelements, elementList, G2L, wJ, ggeo, D, lambda, q, Aq, elemPid, done);
+  if constexpr (NQ * NQ * 2 * bp5_epb<NQ>() <= 1024) {
+    if (variant == 1)
+      return launch_batch_v<NQ, 1>(s, Nelements, elementList, G2L, wJ, ggeo, D, lambda, q, Aq, elemPid, done);
+  }
+  if constexpr (bp5_epb<NQ>() >= 2) {
+    if (variant == 2)
+      return launch_batch_v<NQ, 2>(s, Nelements, elementList, G2L, wJ, ggeo, D, lambda, q, Aq, elemPid, done);
+  }
+  return set_error(BPX_ERR_INVALID_ARGUMENT, "batch Ax: variant not available for this degree");
+}
+
+template int launch_bp5_batch_nq<BPX_NQ>(hipStream_t, int, int, const int*, const int*, const double*,
+                                         const double*, const double*, double, const double*, double*, const int*,
+                                         const BatchState*);
+template int launch_bp5_kappa_grad_nq<BPX_NQ>(hipStream_t, int, int, const int*, const double*, const double*,
+                                              const double*, const double*, const double*, double*);
 template int launch_bp5_nq<double, BPX_NQ, 1>(hipStream_t, const AxArgs<double>&);
 template int launch_bp5_nq<double, BPX_NQ, 3>(hipStream_t, const AxArgs<double>&);
 template int launch_bp5_nq<float, BPX_NQ, 1>(hipStream_t, const AxArgs<float>&);

@@ -90,6 +90,50 @@ int launch_bp5_ax_krhs(hipStream_t s, const AxArgs<T>& a, int chunk) {
 }
 
 template int launch_bp5_ax_krhs<double>(hipStream_t, const AxArgs<double>&, int);
+
+#define BPX_BATCH_CASE(NQ) \
+  case NQ:                  \
+    return launch_bp5_batch_nq<NQ>(s, variant, Nelements, elementList, G2L, wJ, ggeo, D, lambda, q, Aq, elemPid, done);
+
+int launch_bp5_batch(hipStream_t s, int N, int variant, int Nelements, const int* elementList, const int* G2L,
+                     const double* wJ, const double* ggeo, const double* D, double lambda, const double* q,
+                     double* Aq, const int* elemPid, const BatchState* done) {
+  if (Nelements <= 0) return BPX_OK;
+  switch (N + 1) {
+    BPX_BATCH_CASE(2) BPX_BATCH_CASE(3) BPX_BATCH_CASE(4) BPX_BATCH_CASE(5) BPX_BATCH_CASE(6)
+    BPX_BATCH_CASE(7) BPX_BATCH_CASE(8) BPX_BATCH_CASE(9) BPX_BATCH_CASE(10) BPX_BATCH_CASE(11)
+    BPX_BATCH_CASE(12) BPX_BATCH_CASE(13) BPX_BATCH_CASE(14) BPX_BATCH_CASE(15) BPX_BATCH_CASE(16)
+    default:
+      return set_error(BPX_ERR_UNSUPPORTED, "batch Ax: degree N must be in 1..15");
+  }
+}
+
+int bp5_batch_epb(int Nq, int variant) {
+  const int epb = Nq >= 8 ? 1 : 256 / (Nq * Nq);
+  if (variant == 0) return epb;
+  if (variant == 1) return Nq * Nq * 2 * epb <= 1024 ? 2 * epb : 0;
+  if (variant == 2) return epb >= 2 ? epb / 2 : 0;
+  return 0;
+}
+
+#define BPX_KGRAD_CASE(NQ) \
+  case NQ:                  \
+    return launch_bp5_kappa_grad_nq<NQ>(s, Nelements, affine, G2L, ggeo, gllw, D, u, lam, gL);
+
+int launch_bp5_kappa_grad(hipStream_t s, int N, int Nelements, int affine, const int* G2L, const double* ggeo,
+                          const double* gllw, const double* D, const double* u, const double* lam, double* gL) {
+  if (Nelements < 0) return set_error(BPX_ERR_INVALID_ARGUMENT, "kappa gradient: Nelements < 0");
+  if (Nelements == 0) return BPX_OK;
+  if (!G2L || !ggeo || !D || !u || !lam || !gL || (affine && !gllw))
+    return set_error(BPX_ERR_INVALID_ARGUMENT, "kappa gradient: null device pointer");
+  switch (N + 1) {
+    BPX_KGRAD_CASE(2) BPX_KGRAD_CASE(3) BPX_KGRAD_CASE(4) BPX_KGRAD_CASE(5) BPX_KGRAD_CASE(6)
+    BPX_KGRAD_CASE(7) BPX_KGRAD_CASE(8) BPX_KGRAD_CASE(9) BPX_KGRAD_CASE(10) BPX_KGRAD_CASE(11)
+    BPX_KGRAD_CASE(12) BPX_KGRAD_CASE(13) BPX_KGRAD_CASE(14) BPX_KGRAD_CASE(15) BPX_KGRAD_CASE(16)
+    default:
+      return set_error(BPX_ERR_UNSUPPORTED, "kappa gradient: degree N must be in 1..15");
+  }
+}
 template int launch_bp5_ax<double>(hipStream_t, const AxArgs<double>&);
 template int launch_bp5_ax<float>(hipStream_t, const AxArgs<float>&);
 template int launch_bp1_ax<double>(hipStream_t, const AxArgs<double>&);

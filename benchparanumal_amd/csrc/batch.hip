@@ -1,0 +1,266 @@
+// Ragged batch CG: many independent BP5 problems (any sizes, degrees)
+// solved together, one CG per problem. No reference counterpart; each
+// problem runs cg::Solve's recurrence (libs/core/linearSolverCG.cpp:62-129).
+//
+// Layout: the problems' global vectors are concatenated (problem i owns
+// [off_i, off_i + Ngather_i)). The streaming work is cut into chunks that
+// never straddle two problems, and the gather's super-blocks likewise, so
+// every reduction partial belongs to one problem: a problem's scalars are
+// folds over its own contiguous range of partials, done by every workgroup
+// that needs them (no atomics, no inter-workgroup waits, bit-reproducible).
+//
+// Per-problem state lives on the device, so iterations never wait on the
+// host and a whole block of iterations can be captured in one hipGraph and
+// replayed. Writers and readers of each state field are split across
+// kernels so no kernel reads a field it also writes:
+//   update_p  (iteration k): reads rr_prev, iter; first chunk writes rr_k,
+//             tol2 (k = 0), and done/rnorm when the problem stops;
+//   update_xr (iteration k): reads rr_k; first chunk writes rr_prev = rr_k,
+//             iter = k + 1, and done/breakdown on a non-positive p.Ap;
+// every chunk recomputes the scalars it acts on, so all chunks of a
+// problem take the same branch. A converged problem's elements, rows and
+// chunks return at once in every kernel.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "bpx_internal.hpp"
+
+namespace bpx {
+namespace {
+
+constexpr int BT = 256;  // threads of the streaming kernels
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// fixed-order fold of part[a..b) by wavefront 0, broadcast through LDS
+__device__ inline double fold_range(const double* __restrict__ part, int a, int b, double* s_w) {
+  if (threadIdx.x < 64) {
+    double a0 = 0, a1 = 0;
+    int i = a + (int)threadIdx.x;
+    for (; i + 64 < b; i += 128) {
+      a0 += part[i];
+      a1 += part[i + 64];
+    }
+    if (i < b) a0 += part[i];
+    const double v = wave_sum(a0 + a1);
+    if (threadIdx.x == 0) s_w[8] = v;
+  }
+  __syncthreads();
+  return s_w[8];
+}
+
+__device__ inline double block_sum(double v, double* s_w) {
+  v = wave_sum(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < BT / 64; ++w) t += s_w[w];
+  return t;
+}
+
+// r = b, x = 0, partial r.r per chunk; state reset
+__global__ __launch_bounds__(BT) void k_batch_init(const BatchChunk* __restrict__ chunks, const double* __restrict__ b,
+                                                   double* __restrict__ x, double* __restrict__ r,
+                                                   double* __restrict__ part_rr, BatchState* st, int maxit) {
+  __shared__ double s_w[16];
+  const BatchChunk c = chunks[blockIdx.x];
+  double v = 0;
+  for (int n = c.n0 + threadIdx.x; n < c.n1; n += BT) {
+    const double rn = b[n];
+    r[n] = rn;
+    x[n] = 0.0;
+    v += rn * rn;
+  }
+  const double t = block_sum(v, s_w);
+  if (threadIdx.x == 0) {
+    part_rr[blockIdx.x] = t;
+    if (c.first) {
+      BatchState& s = st[c.pid];
+      s.rr_k = 0;
+      s.rr_prev = 0;
+      s.tol2 = 0;
+      s.rnorm = 0;
+      s.iter = 0;
+      s.done = 0;
+      s.breakdown = 0;
+      s.maxit = maxit;
+    }
+  }
+}
+
+__global__ __launch_bounds__(BT) void k_batch_update_p(const BatchChunk* __restrict__ chunks,
+                                                       const int2* __restrict__ rrRange, const double* __restrict__ r,
+                                                       double* __restrict__ p, const double* __restrict__ part_rr,
+                                                       BatchState* st, double tol) {
+  __shared__ double s_w[16];
+  const BatchChunk c = chunks[blockIdx.x];
+  BatchState* s = st + c.pid;
+  if (s->done) return;
+  const int2 range = rrRange[c.pid];
+  double rr = fold_range(part_rr, range.x, range.y, s_w);
+  const int k = s->iter;
+  double tol2;
+  if (k == 0) {
+    const double nrm = sqrt(rr);  // rdotr = norm2(r)^2 (linearSolverCG.cpp:84-85)
+    rr = nrm * nrm;
+    tol2 = fmax(tol * tol * rr, tol * tol);
+  } else {
+    tol2 = s->tol2;
+  }
+  const bool stop = !(rr > tol2) || !isfinite(rr) || k >= s->maxit;
+  if (c.first && threadIdx.x == 0) {
+    s->rr_k = rr;
+    if (k == 0) s->tol2 = tol2;
+    if (stop) {
+      s->rnorm = sqrt(rr);
+      s->done = 1;
+      if (!isfinite(rr)) s->breakdown = 1;
+    }
+  }
+  if (stop) return;
+  const double beta = k == 0 ? 0.0 : rr / s->rr_prev;
+  for (int n = c.n0 + threadIdx.x; n < c.n1; n += BT) p[n] = k == 0 ? r[n] : 1.0 * r[n] + beta * p[n];
+}
+
+__global__ __launch_bounds__(BT) void k_batch_update_xr(const BatchChunk* __restrict__ chunks,
+                                                        const int2* __restrict__ pApRange,
+                                                        const double* __restrict__ p, const double* __restrict__ Ap,
+                                                        double* __restrict__ x, double* __restrict__ r,
+                                                        const double* __restrict__ part_pAp,
+                                                        double* __restrict__ part_rr, BatchState* st) {
+  __shared__ double s_w[16];
+  const BatchChunk c = chunks[blockIdx.x];
+  BatchState* s = st + c.pid;
+  if (s->done) return;
+  const int2 range = pApRange[c.pid];
+  const double pAp = fold_range(part_pAp, range.x, range.y, s_w);
+  const double rr = s->rr_k;
+  const double alpha = rr / pAp;
+  if (!isfinite(alpha) || !(pAp > 0.0)) {
+    if (c.first && threadIdx.x == 0) {
+      s->breakdown = 1;
+      s->done = 1;
+      s->rnorm = sqrt(rr);
+    }
+    return;
+  }
+  double v = 0;
+  for (int n = c.n0 + threadIdx.x; n < c.n1; n += BT) {
+    const double rn = r[n] - alpha * Ap[n];
+    r[n] = rn;
+    x[n] = alpha * p[n] + 1.0 * x[n];
+    v += rn * rn;
+  }
+  const double t = block_sum(v, s_w);
+  if (threadIdx.x == 0) {
+    part_rr[blockIdx.x] = t;
+    if (c.first) {
+      s->rr_prev = rr;
+      s->iter = s->iter + 1;
+    }
+  }
+}
+
+// Gather with one p.Ap partial per super-block; super-blocks never straddle
+// problems and a converged problem's super-blocks return at once.
+constexpr int GT = 512;
+constexpr int GMAX = 4096;  // entries staged per super-block (8 row blocks of <= 512)
+
+__global__ __launch_bounds__(GT) void k_batch_gather(const BatchSuper* __restrict__ supers,
+                                                     const int* __restrict__ rowStarts,
+                                                     const int* __restrict__ colIds, const double* __restrict__ q,
+                                                     double* __restrict__ gq, const double* __restrict__ p,
+                                                     double* __restrict__ part, const BatchState* __restrict__ st) {
+  __shared__ double s_v[GMAX];
+  __shared__ double s_w[GT / 64];
+  const BatchSuper sb = supers[blockIdx.x];
+  if (st[sb.pid].done) return;
+  const int r0 = sb.r0, r1 = sb.r1;
+  const int g0 = rowStarts[r0], g1 = rowStarts[r1];
+  const bool staged = (g1 - g0) <= GMAX;
+  if (staged)
+    for (int g = g0 + threadIdx.x; g < g1; g += GT) s_v[g - g0] = q[colIds[g]];
+  __syncthreads();
+  double dot = 0;
+  for (int row = r0 + threadIdx.x; row < r1; row += GT) {
+    const int a = rowStarts[row], e = rowStarts[row + 1];
+    double acc = 0;
+    if (staged)
+      for (int g = a; g < e; ++g) acc += s_v[g - g0];
+    else
+      for (int g = a; g < e; ++g) acc += q[colIds[g]];
+    gq[row] = acc;
+    dot += p[row] * acc;
+  }
+  dot = wave_sum(dot);
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = dot;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0;
+    for (int w = 0; w < GT / 64; ++w) t += s_w[w];
+    part[blockIdx.x] = t;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_batch_count_done(int nprob, const BatchState* __restrict__ st,
+                                                          int* __restrict__ count) {
+  __shared__ int s_c[4];
+  int c = 0;
+  for (int i = threadIdx.x; i < nprob; i += 256) c += st[i].done ? 1 : 0;
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+  if ((threadIdx.x & 63) == 0) s_c[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) *count = s_c[0] + s_c[1] + s_c[2] + s_c[3];
+}
+
+}  // namespace
+
+int launch_batch_init(hipStream_t s, int nchunks, const BatchChunk* chunks, const double* b, double* x, double* r,
+                      double* part_rr, BatchState* st, int maxit) {
+  if (nchunks <= 0) return BPX_OK;
+  hipLaunchKernelGGL(k_batch_init, dim3(nchunks), dim3(BT), 0, s, chunks, b, x, r, part_rr, st, maxit);
+  BPX_HIP(hipGetLastError());
+  return BPX_OK;
+}
+
+int launch_batch_update_p(hipStream_t s, int nchunks, const BatchChunk* chunks, const int2* rrRange, const double* r,
+                          double* p, const double* part_rr, BatchState* st, double tol) {
+  if (nchunks <= 0) return BPX_OK;
+  hipLaunchKernelGGL(k_batch_update_p, dim3(nchunks), dim3(BT), 0, s, chunks, rrRange, r, p, part_rr, st, tol);
+  BPX_HIP(hipGetLastError());
+  return BPX_OK;
+}
+
+int launch_batch_update_xr(hipStream_t s, int nchunks, const BatchChunk* chunks, const int2* pApRange,
+                           const double* p, const double* Ap, double* x, double* r, const double* part_pAp,
+                           double* part_rr, BatchState* st) {
+  if (nchunks <= 0) return BPX_OK;
+  hipLaunchKernelGGL(k_batch_update_xr, dim3(nchunks), dim3(BT), 0, s, chunks, pApRange, p, Ap, x, r, part_pAp,
+                     part_rr, st);
+  BPX_HIP(hipGetLastError());
+  return BPX_OK;
+}
+
+int launch_batch_gather(hipStream_t s, int nsupers, const BatchSuper* supers, const int* rowStarts,
+                        const int* colIds, const double* q, double* gq, const double* p, double* part,
+                        const BatchState* st) {
+  if (nsupers <= 0) return BPX_OK;
+  hipLaunchKernelGGL(k_batch_gather, dim3(nsupers), dim3(GT), 0, s, supers, rowStarts, colIds, q, gq, p, part, st);
+  BPX_HIP(hipGetLastError());
+  return BPX_OK;
+}
+
+int launch_batch_count_done(hipStream_t s, int nprob, const BatchState* st, int* count) {
+  hipLaunchKernelGGL(k_batch_count_done, dim3(1), dim3(256), 0, s, nprob, st, count);
+  BPX_HIP(hipGetLastError());
+  return BPX_OK;
+}
+
+}  // namespace bpx

@@ -59,6 +59,50 @@ template <typename T> int launch_bp5_ax_krhs(hipStream_t s, const AxArgs<T>& a, 
 template <typename T> int launch_bp1_ax(hipStream_t s, const AxArgs<T>& a);
 template <typename T> int launch_bp3_ax(hipStream_t s, const AxArgs<T>& a);
 
+// variable-coefficient BP5: kappa-scaled geometry (affine meshes expanded
+// to per-node factors) and the adjoint gradient per local node
+int launch_kappa_geometry(hipStream_t s, long long NLocal, int Nq, int affine, const int* lattice,
+                          const double* kappa, const double* ggeo, const double* wJ, const double* gllw,
+                          double* ggeoK, double* wJK);
+int launch_bp5_kappa_grad(hipStream_t s, int N, int Nelements, int affine, const int* G2L, const double* ggeo,
+                          const double* gllw, const double* D, const double* u, const double* lam, double* gL);
+
+// ------------------------------------------------ ragged batch CG ------
+// A chunk of one problem's slice of the concatenated vectors; first marks
+// the chunk that writes the problem's state.
+struct BatchChunk {
+  int pid, n0, n1, first;
+};
+// Rows [r0, r1) of the concatenated gather maps, all of problem pid.
+struct BatchSuper {
+  int pid, r0, r1, pad;
+};
+// Device-resident CG state of one problem.
+struct BatchState {
+  double rr_k;     // r.r of the current iteration
+  double rr_prev;  // r.r of the previous iteration
+  double tol2;
+  double rnorm;    // ||r|| when the problem stopped
+  int iter, done, breakdown, maxit;
+};
+// BP5 Ax over the elements of one degree group; variant = workgroup shape
+// (bp5_batch_epb: elements per workgroup, 0 if the variant does not exist)
+int launch_bp5_batch(hipStream_t s, int N, int variant, int Nelements, const int* elementList, const int* G2L,
+                     const double* wJ, const double* ggeo, const double* D, double lambda, const double* q,
+                     double* Aq, const int* elemPid, const BatchState* done);
+int bp5_batch_epb(int Nq, int variant);
+int launch_batch_init(hipStream_t s, int nchunks, const BatchChunk* chunks, const double* b, double* x, double* r,
+                      double* part_rr, BatchState* st, int maxit);
+int launch_batch_update_p(hipStream_t s, int nchunks, const BatchChunk* chunks, const int2* rrRange, const double* r,
+                          double* p, const double* part_rr, BatchState* st, double tol);
+int launch_batch_update_xr(hipStream_t s, int nchunks, const BatchChunk* chunks, const int2* pApRange,
+                           const double* p, const double* Ap, double* x, double* r, const double* part_pAp,
+                           double* part_rr, BatchState* st);
+int launch_batch_gather(hipStream_t s, int nsupers, const BatchSuper* supers, const int* rowStarts,
+                        const int* colIds, const double* q, double* gq, const double* p, double* part,
+                        const BatchState* st);
+int launch_batch_count_done(hipStream_t s, int nprob, const BatchState* st, int* count);
+
 int launch_rhs(hipStream_t s, int N, int Nfields, int affine, int Nelements, const double* wJ,
                const double* gllw, const double* x, const double* y, const double* z,
                double lambda, double* rhs);

@@ -28,6 +28,26 @@ struct bpx_bp_s {
   cg* solver = nullptr;
   ~bpx_bp_s() { delete solver; }
 };
+struct bpx_batch_s {
+  bpx_platform_s* plat = nullptr;
+  bp_batch_t b;
+};
+
+static int to_mesh_opts(const bpx_mesh_opts& o, MeshOpts& mo) {
+  mo.N = o.N;
+  mo.nx = o.nx;
+  mo.ny = o.ny;
+  mo.nz = o.nz;
+  mo.affine = o.affine;
+  mo.cubature = o.cubature;
+  for (int c = 0; c < 3; ++c) {
+    mo.lo[c] = o.lo[c];
+    mo.dim[c] = o.dim[c];
+  }
+  mo.perturb = o.perturb;
+  mo.seed = o.seed;
+  return BPX_OK;
+}
 
 #define BPX_GUARD_BEGIN try {
 #define BPX_GUARD_END                                                        \
@@ -363,18 +383,7 @@ int bpx_mesh_create(const bpx_mesh_opts* o, bpx_mesh* out) {
   if (int st = need(o, "opts")) return st;
   if (int st = need(out, "out")) return st;
   MeshOpts mo;
-  mo.N = o->N;
-  mo.nx = o->nx;
-  mo.ny = o->ny;
-  mo.nz = o->nz;
-  mo.affine = o->affine;
-  mo.cubature = o->cubature;
-  for (int c = 0; c < 3; ++c) {
-    mo.lo[c] = o->lo[c];
-    mo.dim[c] = o->dim[c];
-  }
-  mo.perturb = o->perturb;
-  mo.seed = o->seed;
+  to_mesh_opts(*o, mo);
   bpx_mesh_s* m = new bpx_mesh_s;
   if (int st = m->m.Setup(mo)) {
     delete m;
@@ -524,8 +533,8 @@ int bpx_bp_destroy(bpx_bp b) {
 int bpx_bp_info(bpx_bp b, int64_t* out, int capacity) {
   if (int st = need(b, "bp")) return st;
   const int64_t v[] = {b->bp.bp, b->bp.Nfields, b->bp.ogs.Ngather, b->bp.NLocal, b->bp.mesh->Nelements,
-                       b->bp.mesh->N};
-  const int n = 6;
+                       b->bp.mesh->N, b->bp.LatticeNodes()};
+  const int n = 7;
   for (int i = 0; i < n && i < capacity; ++i) out[i] = v[i];
   return n;
 }
@@ -561,6 +570,21 @@ int bpx_bp_operator_krhs(bpx_bp b, int K, int chunk, const double* o_q, double* 
   BPX_GUARD_END
 }
 
+int bpx_bp_set_kappa(bpx_bp b, const double* o_kappa) {
+  BPX_GUARD_BEGIN
+  if (int st = need(b, "bp")) return st;
+  return b->bp.SetKappa(o_kappa);
+  BPX_GUARD_END
+}
+
+int bpx_bp_kappa_gradient(bpx_bp b, const double* o_u, const double* o_lam, double* o_grad) {
+  BPX_GUARD_BEGIN
+  if (int st = need(b, "bp")) return st;
+  if (!o_u || !o_lam || !o_grad) return set_error(BPX_ERR_INVALID_ARGUMENT, "kappa gradient: null pointer");
+  return b->bp.KappaGradient(o_u, o_lam, o_grad);
+  BPX_GUARD_END
+}
+
 int bpx_bp_rhs_forcing(bpx_bp b, double* o_b) {
   if (int st = need(b, "bp")) return st;
   return b->bp.Forcing(o_b);
@@ -587,6 +611,69 @@ int bpx_bp_solve_mixed(bpx_bp b, double* o_x, const double* o_b, double tol, int
   BPX_GUARD_BEGIN
   if (int st = need(b, "bp")) return st;
   return b->bp.SolveMixed(o_x, o_b, tol, interval, maxit, stats);
+  BPX_GUARD_END
+}
+
+// ----------------------------------------------------------- batch ----
+int bpx_batch_create(bpx_platform p, int nprob, const bpx_mesh_opts* opts, double lambda, bpx_batch* out) {
+  BPX_GUARD_BEGIN
+  if (int st = need(p, "platform")) return st;
+  if (int st = need(opts, "opts")) return st;
+  if (int st = need(out, "out")) return st;
+  if (nprob < 1) return set_error(BPX_ERR_INVALID_ARGUMENT, "batch: nprob < 1");
+  std::vector<MeshOpts> v(nprob);
+  for (int i = 0; i < nprob; ++i)
+    if (int st = to_mesh_opts(opts[i], v[i])) return st;
+  bpx_batch_s* b = new bpx_batch_s;
+  b->plat = p;
+  if (int st = b->b.Setup(*p->p, v, lambda)) {
+    delete b;
+    return st;
+  }
+  *out = b;
+  return BPX_OK;
+  BPX_GUARD_END
+}
+
+int bpx_batch_destroy(bpx_batch b) {
+  delete b;
+  return BPX_OK;
+}
+
+int bpx_batch_info(bpx_batch b, int64_t* out, int capacity) {
+  if (int st = need(b, "batch")) return st;
+  const int64_t v[] = {b->b.nprob, b->b.Ntot, (int64_t)b->b.groups.size(), b->b.nchunks, b->b.nsupers,
+                       b->b.NLocalTot, b->b.replays};
+  const int n = 7;
+  for (int i = 0; i < n && i < capacity; ++i) out[i] = v[i];
+  return n;
+}
+
+int bpx_batch_offsets(bpx_batch b, int64_t* off) {
+  if (int st = need(b, "batch")) return st;
+  if (int st = need(off, "off")) return st;
+  for (int i = 0; i <= b->b.nprob; ++i) off[i] = b->b.off[i];
+  return BPX_OK;
+}
+
+int bpx_batch_tuning(bpx_batch b, double* out, int capacity) {
+  if (int st = need(b, "batch")) return st;
+  const int n = 6 * (int)b->b.groups.size();
+  for (size_t g = 0; g < b->b.groups.size(); ++g) {
+    const auto& G = b->b.groups[g];
+    const double v[6] = {(double)G.N, (double)G.variant, (double)bp5_batch_epb(G.Nq, G.variant), G.us[0], G.us[1],
+                         G.us[2]};
+    for (int k = 0; k < 6; ++k)
+      if ((int)(6 * g + k) < capacity) out[6 * g + k] = v[k];
+  }
+  return n;
+}
+
+int bpx_batch_solve(bpx_batch b, double* o_x, const double* o_b, double tol, int maxit, int* iterations,
+                    double* rnorm) {
+  BPX_GUARD_BEGIN
+  if (int st = need(b, "batch")) return st;
+  return b->b.Solve(o_x, o_b, tol, maxit, iterations, rnorm, nullptr);
   BPX_GUARD_END
 }
 

@@ -10,6 +10,7 @@
 #include <functional>
 #include <memory>
 #include <string>
+#include <deque>
 #include <vector>
 
 #include "../bpx_internal.hpp"
@@ -123,6 +124,7 @@ namespace ogs {
 class ogs_t {
  public:
   int Setup(dlong N, const hlong* ids);
+  int SetupRows(dlong N, dlong Nrows, const dlong* rowOf);
   void SetupGlobalToLocalMapping(dlong* GlobalToLocal) const;
   int Upload(platform_t& p);  // device copies of the T maps
   template <typename T>
@@ -224,6 +226,13 @@ class bp_t : public solver_t {
   // o_AqL [NLocal][K]; chunk as for launch_bp5_ax_krhs (0 = default)
   int AxLocalKrhs(int K, const dfloat* o_q, dfloat* o_AqL, int chunk);
   int OperatorKrhs(int K, const dfloat* o_q, dfloat* o_Aq, int chunk);
+  // Variable coefficient (BP5): kappa on the node lattice (device, Nnodes
+  // values); nullptr restores the constant coefficient. The operator then
+  // scales G by kappa at each node.
+  int SetKappa(const dfloat* o_kappa);
+  // o_grad (Nnodes) = -sum over local copies of (D lam).G(D u)
+  int KappaGradient(const dfloat* o_u, const dfloat* o_lam, dfloat* o_grad);
+  hlong LatticeNodes() const;
   // FP32 CG with FP64 reliable updates every `interval` iterations until
   // ||b - A x|| / ||b|| < tol; stats = {updates, FP32 iterations, rel. residual}
   int SolveMixed(dfloat* o_x, const dfloat* o_b, double tol, int interval, int maxit, double* stats);
@@ -248,6 +257,12 @@ class bp_t : public solver_t {
   deviceBuffer<double> o_wJ, o_ggeo, o_gllw, o_D, o_I, o_MM, o_invV;
   deviceBuffer<double> o_AqL;
   deviceBuffer<double> o_AqLk;  // [NLocal][K] for OperatorKrhs, grown on demand
+  // variable coefficient: lattice point of each local node, the lattice
+  // sum maps, kappa-scaled factors (per node) and, for affine meshes, wJ per node
+  bool haveKappa = false;
+  deviceBuffer<dlong> o_lattice;
+  ogs::ogs_t latticeOgs;
+  deviceBuffer<double> o_ggeoK, o_wJK, o_gradL;
   // FP32 copies (created on first use)
   int Prepare32();
   bool have32 = false;
@@ -259,6 +274,54 @@ class bp_t : public solver_t {
   template <typename T>
   int ax(const T* q, T* AqL, const T* wJ, const T* ggeo, const T* gllw, const T* D, const T* I,
          const T* MM, const T* invV);
+};
+
+// Ragged batch of BP5 problems (box meshes of any size and degree), one CG
+// per problem, all problems advanced together from a replayed hipGraph.
+class bp_batch_t {
+ public:
+  ~bp_batch_t();
+  int Setup(platform_t& p, const std::vector<MeshOpts>& opts, dfloat lambda);
+  // x = A_i^{-1} b per problem (x starts from 0); per problem: iterations,
+  // final ||r||, breakdown flag (host arrays of nprob, may be null)
+  int Solve(double* o_x, const double* o_b, double tol, int maxit, int* iterations, double* rnorm,
+            int* breakdown);
+
+  struct Group {  // problems of one degree: one Ax launch over their elements
+    int N = 0, Nq = 0;
+    dlong Nel = 0;
+    long long loff = 0;  // start in the concatenated local vector
+    deviceBuffer<dlong> elementList, G2L, elemPid;
+    deviceBuffer<double> wJ, ggeo, D;
+    int variant = 0;             // workgroup shape chosen by the tuner
+    double us[3] = {-1, -1, -1};  // its timings (us; -1 = shape not available)
+  };
+  platform_t* platform = nullptr;
+  dfloat lambda = 1.0;
+  int nprob = 0, nchunks = 0, nsupers = 0;
+  long long replays = 0;
+  std::vector<dlong> off;  // global offsets, nprob+1
+  dlong Ntot = 0, NLocalTot = 0;
+  std::deque<Group> groups;  // deque: Group holds device buffers, which do not move
+
+ private:
+  int Tune();
+  int AxGroup(const Group& G, int variant, const double* q, double* AqL) const;
+  int Iteration(double* o_x, double tol);
+  int Capture(double* o_x, double tol);
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int* h_count = nullptr;
+  hipGraphExec_t graphExec = nullptr;
+  double* graphX = nullptr;
+  double graphTol = -1;
+  deviceBuffer<dlong> o_rowStarts, o_colIds;
+  deviceBuffer<BatchSuper> o_supers;
+  deviceBuffer<BatchChunk> o_chunks;
+  deviceBuffer<int2> o_pApRange, o_rrRange;
+  deviceBuffer<BatchState> o_state;
+  deviceBuffer<double> o_r, o_p, o_Ap, o_AqL, o_part_rr, o_part_pAp;
+  deviceBuffer<int> o_count;
 };
 
 }  // namespace bpx

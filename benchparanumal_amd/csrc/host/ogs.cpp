@@ -117,6 +117,30 @@ int ogs_t::Setup(dlong N_, const hlong* ids) {
   return BPX_OK;
 }
 
+// Maps with caller-numbered rows: local node n joins row rowOf[n] (or none
+// when rowOf[n] < 0); each row lists its nodes in ascending order. Used for
+// sums over the node lattice, whose rows are the lattice points in order.
+int ogs_t::SetupRows(dlong N_, dlong Nrows, const dlong* rowOf) {
+  if (N_ < 0 || Nrows < 0) return set_error(BPX_ERR_INVALID_ARGUMENT, "ogs: N or Nrows < 0");
+  N = N_;
+  Ngather = Nrows;
+  rowStartsT.assign((size_t)Nrows + 1, 0);
+  for (dlong n = 0; n < N; ++n) {
+    if (rowOf[n] >= Nrows) return set_error(BPX_ERR_INVALID_ARGUMENT, "ogs: row index out of range");
+    if (rowOf[n] >= 0) ++rowStartsT[rowOf[n] + 1];
+  }
+  for (dlong r = 0; r < Nrows; ++r) rowStartsT[r + 1] += rowStartsT[r];
+  nnzT = rowStartsT[Nrows];
+  colIdsT.assign(nnzT, 0);
+  std::vector<dlong> fill(rowStartsT.begin(), rowStartsT.end() - 1);
+  for (dlong n = 0; n < N; ++n)
+    if (rowOf[n] >= 0) colIdsT[fill[rowOf[n]]++] = n;
+  int st = block_rows(Nrows, rowStartsT, blockRowStartsT);
+  if (st) return st;
+  NrowBlocksT = blockRowStartsT.empty() ? 0 : (dlong)blockRowStartsT.size() - 1;
+  return BPX_OK;
+}
+
 void ogs_t::SetupGlobalToLocalMapping(dlong* G2L) const {
   std::fill(G2L, G2L + N, -1);
 #pragma omp parallel for

@@ -268,7 +268,7 @@ int bp_t::ax(const T* q, T* AqL, const T* wJ, const T* ggeo, const T* gllw, cons
   a.N = mesh->N;
   a.Nelements = mesh->Nelements;
   a.Nfields = Nfields;
-  a.affine = mesh->opts.affine;
+  a.affine = haveKappa ? 0 : mesh->opts.affine;  // kappa-scaled factors are per node
   a.elementList = o_elementList.ptr();
   a.GlobalToLocal = o_GlobalToLocal.ptr();
   a.wJ = wJ;
@@ -287,8 +287,55 @@ int bp_t::ax(const T* q, T* AqL, const T* wJ, const T* ggeo, const T* gllw, cons
 }
 
 int bp_t::AxLocal(const dfloat* o_q, dfloat* o_AqL) {
+  if (haveKappa)
+    return ax<double>(o_q, o_AqL, mesh->opts.affine ? o_wJK.ptr() : o_wJ.ptr(), o_ggeoK.ptr(), o_gllw.ptr(),
+                      o_D.ptr(), nullptr, nullptr, nullptr);
   return ax<double>(o_q, o_AqL, o_wJ.ptr(), o_ggeo.ptr(), o_gllw.ptr(), o_D.ptr(), o_I.ptr(), o_MM.ptr(),
                     o_invV.ptr());
+}
+
+hlong bp_t::LatticeNodes() const {
+  const MeshOpts& o = mesh->opts;
+  return ((hlong)o.nx * mesh->N + 1) * ((hlong)o.ny * mesh->N + 1) * ((hlong)o.nz * mesh->N + 1);
+}
+
+int bp_t::SetKappa(const dfloat* o_kappa) {
+  if (!o_kappa) {
+    haveKappa = false;
+    return BPX_OK;
+  }
+  if (bp != 5) return set_error(BPX_ERR_UNSUPPORTED, "bp: a variable coefficient is implemented for BP5");
+  hipStream_t s = platform->stream;
+  const mesh_t& m = *mesh;
+  const hlong Nnodes = LatticeNodes();
+  if (Nnodes >= ((hlong)1 << 31)) return set_error(BPX_ERR_UNSUPPORTED, "bp: lattice above 2^31 nodes");
+  if (!o_lattice.ptr()) {
+    // lattice point of each local node: the mesh's global ids are 1 + lattice index
+    std::vector<dlong> lat(NLocal);
+    for (dlong l = 0; l < NLocal; ++l) lat[l] = (dlong)(m.globalIds[l] - 1);
+    if (int st = o_lattice.upload(lat, s)) return st;
+    if (int st = latticeOgs.SetupRows(NLocal, (dlong)Nnodes, lat.data())) return st;
+    if (int st = latticeOgs.Upload(*platform)) return st;
+    if (int st = o_ggeoK.alloc((size_t)NLocal * 6)) return st;
+    if (m.opts.affine)
+      if (int st = o_wJK.alloc(NLocal)) return st;
+    if (int st = o_gradL.alloc(NLocal)) return st;
+  }
+  if (int st = launch_kappa_geometry(s, NLocal, m.Nq, m.opts.affine, o_lattice.ptr(), o_kappa, o_ggeo.ptr(),
+                                     o_wJ.ptr(), o_gllw.ptr(), o_ggeoK.ptr(), o_wJK.ptr()))
+    return st;
+  haveKappa = true;
+  return BPX_OK;
+}
+
+int bp_t::KappaGradient(const dfloat* o_u, const dfloat* o_lam, dfloat* o_grad) {
+  if (bp != 5) return set_error(BPX_ERR_UNSUPPORTED, "bp: the kappa gradient is implemented for BP5");
+  if (!o_lattice.ptr()) return set_error(BPX_ERR_INVALID_ARGUMENT, "bp: set a coefficient (kappa) first");
+  hipStream_t s = platform->stream;
+  if (int st = launch_bp5_kappa_grad(s, mesh->N, mesh->Nelements, mesh->opts.affine, o_GlobalToLocal.ptr(),
+                                     o_ggeo.ptr(), o_gllw.ptr(), o_D.ptr(), o_u, o_lam, o_gradL.ptr()))
+    return st;
+  return latticeOgs.Gather<double>(*platform, o_grad, o_gradL.ptr(), 1);
 }
 
 // bpN_t::Operator (BP/BP5/src/bp5Operator.cpp:29-85). On one GPU the halo
@@ -308,6 +355,7 @@ int bp_t::OperatorDot(const dfloat* o_q, dfloat* o_Aq, double* part, int* npart)
 
 int bp_t::AxLocalKrhs(int K, const dfloat* o_q, dfloat* o_AqL, int chunk) {
   if (bp != 5) return set_error(BPX_ERR_UNSUPPORTED, "bp: k right-hand sides are implemented for BP5");
+  if (haveKappa) return set_error(BPX_ERR_UNSUPPORTED, "bp: k right-hand sides use the constant coefficient");
   AxArgs<double> a;
   a.N = mesh->N;
   a.Nelements = mesh->Nelements;
@@ -338,6 +386,7 @@ int bp_t::OperatorKrhs(int K, const dfloat* o_q, dfloat* o_Aq, int chunk) {
 
 int bp_t::OperatorDot32(const float* o_q, float* o_Aq, double* part, int* npart) {
   if (Nfields != 1) return set_error(BPX_ERR_UNSUPPORTED, "bp: the FP32 operator covers one field (BP1/3/5)");
+  if (haveKappa) return set_error(BPX_ERR_UNSUPPORTED, "bp: the FP32 operator has no variable coefficient");
   if (int st = Prepare32()) return st;
   if (int st = ax<float>(o_q, o_AqL32.ptr(), o_wJ32.ptr(), o_ggeo32.ptr(), o_gllw32.ptr(), o_D32.ptr(),
                          o_I32.ptr(), o_MM32.ptr(), o_invV32.ptr()))
@@ -419,6 +468,7 @@ int bp_t::Prepare32() {
 
 int bp_t::Operator32(const float* o_q, float* o_Aq) {
   if (Nfields != 1) return set_error(BPX_ERR_UNSUPPORTED, "bp: the FP32 operator covers one field (BP1/3/5)");
+  if (haveKappa) return set_error(BPX_ERR_UNSUPPORTED, "bp: the FP32 operator has no variable coefficient");
   if (int st = Prepare32()) return st;
   if (int st = ax<float>(o_q, o_AqL32.ptr(), o_wJ32.ptr(), o_ggeo32.ptr(), o_gllw32.ptr(), o_D32.ptr(),
                          o_I32.ptr(), o_MM32.ptr(), o_invV32.ptr()))

@@ -49,6 +49,7 @@ typedef struct bpx_platform_s* bpx_platform; /* replaces platform_t (include/pla
 typedef struct bpx_mesh_s* bpx_mesh;         /* replaces mesh_t (include/mesh.hpp) */
 typedef struct bpx_ogs_s* bpx_ogs;           /* replaces ogs::ogs_t (include/ogs.hpp:216-341) */
 typedef struct bpx_bp_s* bpx_bp;             /* replaces bpN_t (BP/BP5/bp5.hpp:43-74) */
+typedef struct bpx_batch_s* bpx_batch;       /* no reference counterpart: many BP5 problems */
 
 const char* bpx_last_error(void);
 int bpx_version(void);
@@ -251,7 +252,8 @@ int bpx_ogs_scatter_dev(bpx_platform p, bpx_ogs o, int K, const double* gv, doub
  * for bp in 1..6: masked ogs, GlobalToLocal, operator data on the device. */
 int bpx_bp_create(bpx_platform p, bpx_mesh m, int bp, double lambda, bpx_bp* out);
 int bpx_bp_destroy(bpx_bp b);
-/* counts: bp Nfields Ngather NLocal Nelements N (int64 each) */
+/* counts: bp Nfields Ngather NLocal Nelements N Nnodes (int64 each; Nnodes =
+ * points of the node lattice, (nx*N+1)(ny*N+1)(nz*N+1)) */
 int bpx_bp_info(bpx_bp b, int64_t* out, int capacity);
 /* No reference counterpart: launch choices the setup timed on this
  * problem, all giving identical results. out: [gather workgroup order
@@ -269,6 +271,19 @@ int bpx_bp_operator_f32(bpx_bp b, const float* o_q, float* o_Aq);
  * on K interleaved right-hand sides (BP5 only): o_q and o_Aq are
  * [Ngather][K]; k-RHS Ax, then the ogs gather with K. */
 int bpx_bp_operator_krhs(bpx_bp b, int K, int chunk, const double* o_q, double* o_Aq);
+/* No reference counterpart (the reference's BP5 coefficient is constant,
+ * okl/bp5AxHex3D.okl:415-569): variable coefficient -div(kappa grad u) +
+ * lambda u for BP5. o_kappa (device, Nnodes values, lattice order
+ * X + (nx*N+1)*(Y + (ny*N+1)*Z)) scales the geometric factors G at every
+ * node; bpx_bp_operator and bpx_bp_solve then use it. NULL restores the
+ * constant coefficient. Call again after changing kappa. */
+int bpx_bp_set_kappa(bpx_bp b, const double* o_kappa);
+/* No reference counterpart: adjoint gradient for the variable coefficient.
+ * With A(kappa) u = b and A lam = dL/du (A is symmetric), dL/dkappa at
+ * lattice point p is -sum over the local copies of p of (D lam).G(D u);
+ * o_u, o_lam: device, Ngather; o_grad: device, Nnodes. Requires
+ * bpx_bp_set_kappa first. */
+int bpx_bp_kappa_gradient(bpx_bp b, const double* o_u, const double* o_lam, double* o_grad);
 /* b = gather(forcing) as bpN_t::Run does (BP/BP5/src/bp5Run.cpp:51-64). */
 int bpx_bp_rhs_forcing(bpx_bp b, double* o_b);
 /* cg::Solve (libs/core/linearSolverCG.cpp:62-129) with device-resident
@@ -286,6 +301,32 @@ int bpx_bp_solve(bpx_bp b, double* o_x, double* o_r, double tol, int maxit, int 
  * FP32 iterations, final FP64 relative residual]. */
 int bpx_bp_solve_mixed(bpx_bp b, double* o_x, const double* o_b, double tol, int interval,
                        int maxit, double* stats);
+
+/* ---------------------------------------------------------------- batch --
+ * No reference counterpart. A ragged batch of BP5 problems (box meshes of
+ * any size and degree, each set up as bp5_t does, BP/BP5/src/bp5Setup.cpp:
+ * 29-95), solved together by one CG per problem (the recurrence of
+ * cg::Solve, libs/core/linearSolverCG.cpp:62-129) whose scalars stay on the
+ * device; blocks of iterations are captured once in a hipGraph and
+ * replayed until every problem has stopped. Problems of one degree share one
+ * Ax launch; the setup times the workgroup shapes of each degree's Ax (all
+ * give identical results) and keeps the fastest. Vectors are the problems'
+ * global vectors concatenated in order (bpx_batch_offsets). */
+int bpx_batch_create(bpx_platform p, int nprob, const bpx_mesh_opts* opts, double lambda,
+                     bpx_batch* out);
+int bpx_batch_destroy(bpx_batch b);
+/* counts: nprob Ntot ngroups nchunks nsupers NLocal replays-of-last-solve */
+int bpx_batch_info(bpx_batch b, int64_t* out, int capacity);
+/* off[0..nprob]: problem i owns [off[i], off[i+1]) of the vectors */
+int bpx_batch_offsets(bpx_batch b, int64_t* off);
+/* per degree group, 6 values: N, chosen shape, elements per workgroup,
+ * time (us) of shapes 0, 1, 2 (-1 = not available). Returns the count. */
+int bpx_batch_tuning(bpx_batch b, double* out, int capacity);
+/* x_i = A_i^{-1} b_i to ||r_i|| <= max(tol ||b_i||, tol) or maxit
+ * iterations, x from 0. iterations, rnorm: host arrays of nprob (may be
+ * NULL). BPX_ERR_BREAKDOWN if any problem broke down. */
+int bpx_batch_solve(bpx_batch b, double* o_x, const double* o_b, double tol, int maxit,
+                    int* iterations, double* rnorm);
 
 #ifdef __cplusplus
 }

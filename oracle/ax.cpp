@@ -42,6 +42,10 @@ void bp5_element(const Problem& P, dlong e, int f, const double* q, double* Aq) 
           G00 = g[0]; G01 = g[1]; G02 = g[2]; G11 = g[3]; G12 = g[4]; G22 = g[5];
           wj = P.wJ[gb];
         }
+        if (!P.kappaL.empty()) {  // variable coefficient: kappa scales G
+          const double kap = P.kappaL[(size_t)e * Np + n];
+          G00 *= kap; G01 *= kap; G02 *= kap; G11 *= kap; G12 *= kap; G22 *= kap;
+        }
         double tmp = 0.0;
         for (int m = 0; m < Nq; ++m) tmp += D[k * Nq + m] * sq[i + j * Nq + m * Nq * Nq];
         double gr = G02 * tmp, gs = G12 * tmp, gt = G22 * tmp;

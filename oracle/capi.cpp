@@ -137,6 +137,27 @@ void orc_operator(void* h, const double* q, double* Aq) {
 
 void orc_set_lambda(void* h, double lambda) { static_cast<Problem*>(h)->lambda = lambda; }
 
+// variable coefficient: kappa on the node lattice (nullptr = constant 1)
+int orc_set_kappa(void* h, const double* kappa) {
+  Problem& P = *static_cast<Problem*>(h);
+  const int st = set_kappa(P, kappa);
+  if (st) g_err = P.error;
+  return st;
+}
+long long orc_lattice_nodes(void* h) { return lattice_nodes(*static_cast<Problem*>(h)); }
+void orc_lattice_index(void* h, long long* out) {
+  const Problem& P = *static_cast<Problem*>(h);
+  for (dlong e = 0; e < P.Nelements; ++e)
+    for (int n = 0; n < P.Np; ++n) out[(size_t)e * P.Np + n] = lattice_index(P, e, n);
+}
+// dL/dkappa on the lattice from the global solution u and adjoint lam
+void orc_kappa_grad(void* h, const double* u, const double* lam, double* grad) {
+  const Problem& P = *static_cast<Problem*>(h);
+  std::vector<double> gL(P.NLocal);
+  kappa_grad_local(P, u, lam, gL.data());
+  lattice_sum(P, gL.data(), grad);
+}
+
 // Rebuild the ogs maps without (mask=0) or with (mask=1) the Dirichlet mask.
 int orc_problem_set_mask(void* h, int mask) {
   Problem& P = *static_cast<Problem*>(h);

@@ -90,6 +90,10 @@ struct Problem {
   dlong NLocal = 0, Ngather = 0, nnzT = 0, NrowBlocksT = 0;
   std::vector<dlong> rowStartsT, colIdsT, blockRowStartsT, GlobalToLocal;
 
+  // variable coefficient (extension, BP5 only): kappa per local node,
+  // scaling the stiffness part; empty = 1
+  std::vector<double> kappaL;
+
   std::string error;
 };
 
@@ -102,6 +106,13 @@ void ax_local(const Problem& P, const double* q, double* AqL);
 void ogs_gather(const Problem& P, const double* vL, double* v);
 void ogs_scatter(const Problem& P, const double* v, double* vL);
 void rhs_forcing_local(const Problem& P, double* rhsL);
+
+// ------------------------------------- variable coefficient and adjoint --
+hlong lattice_index(const Problem& P, dlong e, int n);
+int set_kappa(Problem& P, const double* kappa);  // lattice-ordered; nullptr clears
+hlong lattice_nodes(const Problem& P);
+void kappa_grad_local(const Problem& P, const double* u, const double* lam, double* gradL);
+void lattice_sum(const Problem& P, const double* vL, double* v);
 
 // --------------------------------------------------------------- linAlg --
 void la_set(dlong N, double alpha, double* x);

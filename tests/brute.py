@@ -55,10 +55,12 @@ def trilinear_point(X, Y, Z, r, s, t):
     return np.array([c @ X, c @ Y, c @ Z])
 
 
-def element_matrix(N, Xv, Yv, Zv, qpts, qw, lam, stiff=True):
+def element_matrix(N, Xv, Yv, Zv, qpts, qw, lam, stiff=True, kappa=None):
     """Dense element matrix (Np x Np) of lam*M + K for the GLL Lagrange basis
     of degree N, integrated with the tensor rule (qpts, qw), on the trilinear
-    element with vertices Xv, Yv, Zv. Node index n = i + j*Nq + k*Nq^2."""
+    element with vertices Xv, Yv, Zv. Node index n = i + j*Nq + k*Nq^2.
+    kappa (optional): coefficient of the stiffness term at each quadrature
+    point, indexed a + b*nq + c*nq^2."""
     x, _ = gll_nodes(N)
     V, Vd = lagrange(x, qpts)
     nq = len(qpts)
@@ -80,19 +82,23 @@ def element_matrix(N, Xv, Yv, Zv, qpts, qw, lam, stiff=True):
                     Gref = np.stack([gr, gs, gt])  # 3 x Np
                     Jinv = np.linalg.inv(Jm)       # rows = grad r, grad s, grad t
                     G = w * J * (Jinv @ Jinv.T)
+                    if kappa is not None:
+                        G = G * kappa[a + b * nq + c * nq * nq]
                     A += Gref.T @ G @ Gref
     return A
 
 
-def assembled_operator(N, EX, EY, EZ, G2L, Ngather, qpts, qw, lam, stiff=True, nfields=1):
+def assembled_operator(N, EX, EY, EZ, G2L, Ngather, qpts, qw, lam, stiff=True, nfields=1, kappaL=None):
     """Assembled dense operator Z^T blockdiag(A_e) Z using the local->global
-    map G2L (-1 = masked)."""
+    map G2L (-1 = masked). kappaL (optional): stiffness coefficient per local
+    node (quadrature at the GLL nodes)."""
     Np = (N + 1) ** 3
     Nel = len(EX) // 8
     A = np.zeros((Ngather, Ngather))
     for e in range(Nel):
         Ae = element_matrix(N, EX[8 * e:8 * e + 8], EY[8 * e:8 * e + 8], EZ[8 * e:8 * e + 8],
-                            qpts, qw, lam, stiff)
+                            qpts, qw, lam, stiff,
+                            kappa=None if kappaL is None else kappaL[e * Np:(e + 1) * Np])
         for f in range(nfields):
             ids = G2L[e * Np * nfields + f * Np:e * Np * nfields + (f + 1) * Np]
             keep = ids >= 0

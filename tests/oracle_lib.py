@@ -52,6 +52,11 @@ def lib():
     L.orc_set_lambda.argtypes = [C.c_void_p, C.c_double]
     L.orc_cg_solve.argtypes = [C.c_void_p, _D, _D, C.c_double, C.c_int, _D, _D]
     L.orc_problem_set_mask.argtypes = [C.c_void_p, C.c_int]
+    L.orc_set_kappa.argtypes = [C.c_void_p, C.c_void_p]
+    L.orc_lattice_nodes.argtypes = [C.c_void_p]
+    L.orc_lattice_nodes.restype = C.c_longlong
+    L.orc_lattice_index.argtypes = [C.c_void_p, C.c_void_p]
+    L.orc_kappa_grad.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     L.orc_block_rows.argtypes = [C.c_int, _I, _I]
     L.orc_la_axpy.argtypes = [C.c_int, C.c_double, _D, C.c_double, _D]
     L.orc_la_inner_prod.argtypes = [C.c_int, _D, _D]
@@ -189,6 +194,34 @@ class Problem:
         rL = np.zeros(self.NLocal)
         lib().orc_rhs_forcing_local(self.h, dptr(rL))
         return self.gather(rL)
+
+    # ---- variable coefficient (kappa on the node lattice) and adjoint gradient
+    def lattice_nodes(self):
+        return int(lib().orc_lattice_nodes(self.h))
+
+    def lattice_index(self):
+        out = np.zeros(self.NLocal, dtype=np.int64)
+        lib().orc_lattice_index(self.h, out.ctypes.data_as(C.c_void_p))
+        return out
+
+    def set_kappa(self, kappa):
+        """kappa: lattice-ordered array, or None for the constant coefficient."""
+        if kappa is None:
+            lib().orc_set_kappa(self.h, None)
+            self._kappa = None
+            return
+        self._kappa = np.ascontiguousarray(kappa, dtype=np.float64)
+        assert self._kappa.size == self.lattice_nodes()
+        if lib().orc_set_kappa(self.h, dptr(self._kappa)):
+            raise RuntimeError(lib().orc_last_error().decode())
+
+    def kappa_grad(self, u, lam):
+        """dL/dkappa on the lattice, -sum (D lam).G(D u) over node copies."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        lam = np.ascontiguousarray(lam, dtype=np.float64)
+        out = np.zeros(self.lattice_nodes())
+        lib().orc_kappa_grad(self.h, dptr(u), dptr(lam), dptr(out))
+        return out
 
     def cg(self, b, maxit, tol=0.0, x0=None):
         x = np.zeros(self.Ngather) if x0 is None else np.array(x0, dtype=np.float64)

@@ -1,0 +1,110 @@
+"""Ragged batch CG (one CG per problem, hipGraph-replayed): 64 seeded BP5
+problems with nx, ny, nz in [2, 8] and p in {3..8}, solved together to
+1e-8. Each problem must match its standalone solve (iterations within 1,
+same solution to rounding) and the oracle's CG solution."""
+import numpy as np
+import pytest
+
+import oracle_lib
+from benchparanumal_amd import capi
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def plat():
+    assert torch.cuda.is_available(), "GPU tests need a visible MI355X"
+    return capi.Platform(0)
+
+
+def ragged_problems(n=64, seed=2024):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        nx, ny, nz = (int(v) for v in rng.integers(2, 9, size=3))
+        out.append({"N": int(rng.integers(3, 9)), "nx": nx, "ny": ny, "nz": nz, "perturb": 0.1, "seed": 100 + i})
+    return out
+
+
+@pytest.fixture(scope="module")
+def solved(plat):
+    probs = ragged_problems()
+    B = capi.Batch(plat, probs, lam=1.0)
+    rhs = torch.from_numpy(capi.random_normal(77, B.Ntot)).cuda()
+    x = torch.zeros(B.Ntot, dtype=torch.float64, device="cuda")
+    its, rn = B.solve(x, rhs, tol=1e-8, maxit=5000)
+    torch.cuda.synchronize()
+    return probs, B, rhs, x.cpu().numpy(), its, rn
+
+
+def test_batch_matches_standalone_solves(plat, solved):
+    probs, B, rhs, x, its, rn = solved
+    rhs_h = rhs.cpu().numpy()
+    assert len(set(its.tolist())) > 10  # ragged: the problems stop at different iterations
+    for i, pr in enumerate(probs):
+        a, b = int(B.offsets[i]), int(B.offsets[i + 1])
+        m = capi.Mesh(N=pr["N"], nx=pr["nx"], ny=pr["ny"], nz=pr["nz"], perturb=0.1, seed=pr["seed"])
+        bp = capi.BP(plat, m, 5, lam=1.0)
+        assert bp.Ngather == b - a
+        xs = torch.zeros(b - a, dtype=torch.float64, device="cuda")
+        r = torch.from_numpy(rhs_h[a:b].copy()).cuda()
+        it1, hist = bp.solve(xs, r, tol=1e-8, maxit=5000)
+        assert abs(int(its[i]) - it1) <= 1, (i, pr, int(its[i]), it1)
+        xs_h = xs.cpu().numpy()
+        nb = np.linalg.norm(rhs_h[a:b])
+        if int(its[i]) == it1:  # same iterates up to the rounding of the reductions
+            assert np.linalg.norm(x[a:b] - xs_h) <= 1e-8 * np.linalg.norm(xs_h), i
+        # the batch solution's true residual meets the tolerance
+        Ax = torch.zeros_like(xs)
+        bp.operator(torch.from_numpy(x[a:b].copy()).cuda(), Ax)
+        assert np.linalg.norm(rhs_h[a:b] - Ax.cpu().numpy()) <= 1.01e-8 * nb, i
+        assert rn[i] <= 1e-8 * nb * (1 + 1e-12)
+
+
+def test_batch_matches_oracle(plat, solved):
+    probs, B, rhs, x, its, rn = solved
+    rhs_h = rhs.cpu().numpy()
+    oracle_lib.set_threads(16)
+    for i in range(0, len(probs), 4):
+        pr = probs[i]
+        a, b = int(B.offsets[i]), int(B.offsets[i + 1])
+        P = oracle_lib.Problem(bp=5, N=pr["N"], nx=pr["nx"], ny=pr["ny"], nz=pr["nz"], lam=1.0, perturb=0.1,
+                               seed=pr["seed"])
+        xr, it_ref, _, _ = P.cg(rhs_h[a:b].copy(), 5000, tol=1e-8)
+        assert abs(int(its[i]) - it_ref) <= 1, (i, int(its[i]), it_ref)
+        if int(its[i]) == it_ref:
+            assert np.linalg.norm(x[a:b] - xr) <= 1e-8 * np.linalg.norm(xr), i
+        # against the oracle's converged solution: both within the CG error at 1e-8
+        xt, _, _, _ = P.cg(rhs_h[a:b].copy(), 20000, tol=1e-13)
+        assert np.linalg.norm(x[a:b] - xt) <= 1e-5 * np.linalg.norm(xt), i
+
+
+def test_batch_deterministic_and_tuned(plat, solved):
+    probs, B, rhs, x, its, rn = solved
+    x2 = torch.zeros(B.Ntot, dtype=torch.float64, device="cuda")
+    its2, _ = B.solve(x2, rhs, tol=1e-8, maxit=5000)
+    np.testing.assert_array_equal(its2, its)
+    np.testing.assert_array_equal(x2.cpu().numpy(), x)
+    tun = B.tuning()
+    assert sorted(t["N"] for t in tun) == sorted({p["N"] for p in probs})
+    for t in tun:
+        times = [u for u in t["us"] if u >= 0]
+        assert t["us"][t["shape"]] == min(times)
+    info = B.info()
+    assert info["replays"] <= 5000 // 8 + 2
+
+
+def test_batch_single_problem_and_maxit(plat):
+    B = capi.Batch(plat, [{"N": 5, "nx": 3, "ny": 4, "nz": 2, "perturb": 0.1, "seed": 9}], lam=1.0)
+    rhs = torch.from_numpy(capi.random_normal(5, B.Ntot)).cuda()
+    x = torch.zeros(B.Ntot, dtype=torch.float64, device="cuda")
+    its, rn = B.solve(x, rhs, tol=0.0, maxit=37)
+    assert int(its[0]) == 37
+    m = capi.Mesh(N=5, nx=3, ny=4, nz=2, perturb=0.1, seed=9)
+    bp = capi.BP(plat, m, 5, lam=1.0)
+    xs = torch.zeros_like(x)
+    r = rhs.clone()
+    it1, hist = bp.solve(xs, r, tol=0.0, maxit=37)
+    assert it1 == 37
+    assert abs(rn[0] - hist[-1]) <= 1e-9 * hist[0]
