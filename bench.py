@@ -68,6 +68,212 @@ def bk5_bytes(Ndofs, Nel, Np):
     return Ndofs * 8 + Nel * (Np * 7 * 8 + 4 + Np * 4 + Np * 8)
 
 
+def krhs_bytes(K, Ndofs, Nel, Np):
+    # k right-hand sides: vectors K times, geometry and GlobalToLocal once
+    return K * (Ndofs * 8 + Nel * Np * 8) + Nel * (Np * 7 * 8 + 4 + Np * 4)
+
+
+def event_time(fn, reps, warm=2):
+    """Average seconds per call of fn, HIP events on torch's current stream
+    (the stream libbpx launches on)."""
+    import torch
+
+    for _ in range(warm):
+        fn()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / 1e3 / reps
+
+
+def bench_krhs(capi, plat):
+    """k-RHS BP5 Ax at 20^3, p=7 (kernel entry on the mesh's own arrays)."""
+    import numpy as np
+    import torch
+
+    m = capi.Mesh(N=7, nx=20, perturb=0.1, seed=1)
+    o = capi.Ogs(mesh=m)
+    bp = capi.BP(plat, m, 5, lam=1.0)
+    Ndofs, Nel, Np, NL = bp.Ngather, m.Nelements, m.Np, bp.NLocal
+    arr = {k: torch.from_numpy(np.ascontiguousarray(m.array(k))).cuda() for k in ("wJ", "ggeo", "gllw", "D")}
+    G2L = torch.from_numpy(np.ascontiguousarray(o.array("GlobalToLocal"))).cuda()
+    elist = torch.arange(Nel, dtype=torch.int32, device="cuda")
+    L = capi.lib()
+    out = {"workload": "BP5 Ax, 20^3 hex p=7 perturbed, K right-hand sides stored one after another",
+           "Ndofs": Ndofs}
+    q1 = torch.from_numpy(capi.random_normal(3, Ndofs)).cuda()
+    A1 = torch.empty(NL, dtype=torch.float64, device="cuda")
+    t1 = event_time(lambda: bp.ax_local(q1, A1), 20)
+    out["single_us"] = t1 * 1e6
+    rows = []
+    for K in (4, 8, 16, 32):
+        q = torch.from_numpy(capi.random_normal(4, Ndofs * K)).cuda()
+        A = torch.empty(NL * K, dtype=torch.float64, device="cuda")
+        best = None
+        for chunk in (1, 2, 4):
+            def run():
+                capi.check(L.bpx_bp5_ax_hex_krhs(plat.h, 7, K, chunk, Nel, capi.ptr(elist), capi.ptr(G2L),
+                                                 capi.ptr(arr["wJ"]), None, capi.ptr(arr["ggeo"]),
+                                                 capi.ptr(arr["gllw"]), capi.ptr(arr["D"]), None, None, 1.0,
+                                                 capi.ptr(q), Ndofs, capi.ptr(A), NL))
+            t = event_time(run, 10)
+            if best is None or t < best[1]:
+                best = (chunk, t)
+        chunk, t = best
+        nb = krhs_bytes(K, Ndofs, Nel, Np)
+        rows.append({"K": K, "chunk": chunk, "us": t * 1e6, "gdof_per_s": K * Ndofs / t / 1e9,
+                     "gbs": nb / t / 1e9, "algorithmic_bytes": nb, "speedup_vs_K_single": K * t1 / t})
+        del q, A
+    out["rows"] = rows
+    return out
+
+
+def bench_batch(capi, plat):
+    """64 ragged BP5 problems (nx, ny, nz in [2, 8], p in 3..8) to 1e-8: one
+    batched solve against the same problems solved one after another."""
+    import numpy as np
+    import time
+    import torch
+
+    rng = np.random.default_rng(2024)
+    probs = []
+    for i in range(64):
+        nx, ny, nz = (int(v) for v in rng.integers(2, 9, size=3))
+        probs.append({"N": int(rng.integers(3, 9)), "nx": nx, "ny": ny, "nz": nz, "perturb": 0.1, "seed": 100 + i})
+    B = capi.Batch(plat, probs, lam=1.0)
+    rhs = torch.from_numpy(capi.random_normal(77, B.Ntot)).cuda()
+    x = torch.zeros(B.Ntot, dtype=torch.float64, device="cuda")
+    B.solve(x, rhs, tol=1e-8, maxit=5000)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    its, _ = B.solve(x, rhs, tol=1e-8, maxit=5000)
+    torch.cuda.synchronize()
+    tb = time.perf_counter() - t0
+    sizes = np.diff(B.offsets)
+    work = float((sizes * its).sum())
+    # the same problems one by one with the single-problem CG
+    bps = []
+    for pr in probs:
+        m = capi.Mesh(N=pr["N"], nx=pr["nx"], ny=pr["ny"], nz=pr["nz"], perturb=0.1, seed=pr["seed"])
+        bps.append((m, capi.BP(plat, m, 5, lam=1.0)))
+    def seq():
+        for i, (m, bp) in enumerate(bps):
+            a, b = int(B.offsets[i]), int(B.offsets[i + 1])
+            xs = torch.zeros(b - a, dtype=torch.float64, device="cuda")
+            r = rhs[a:b].clone()
+            bp.solve(xs, r, tol=1e-8, maxit=5000, history=False)
+    seq()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    seq()
+    torch.cuda.synchronize()
+    ts = time.perf_counter() - t0
+    return {"workload": "64 BP5 problems, nx/ny/nz in [2,8], p in 3..8, N(0,1) rhs, CG to 1e-8",
+            "Ntot": B.Ntot, "iterations_min": int(its.min()), "iterations_max": int(its.max()),
+            "ms_per_solve": tb * 1e3, "gdof_it_per_s": work / tb / 1e9,
+            "sequential_ms": ts * 1e3, "speedup_vs_sequential": ts / tb, "tuning": B.tuning(),
+            "graph_replays": B.info()["replays"]}
+
+
+def bench_p15(capi, plat):
+    """BK5 Ax at 24^3, p = 15 (Np 4096, ggeo 2.7 GB)."""
+    import torch
+
+    m = capi.Mesh(N=15, nx=24, perturb=0.1, seed=1)
+    bp = capi.BP(plat, m, 5, lam=1.0)
+    q = torch.from_numpy(capi.random_normal(2, bp.Ngather)).cuda()
+    AqL = torch.empty(bp.NLocal, dtype=torch.float64, device="cuda")
+    t = event_time(lambda: bp.ax_local(q, AqL), 20)
+    nb = bk5_bytes(bp.Ngather, m.Nelements, m.Np)
+    return {"workload": "BK5 Ax, 24^3 hex p=15 perturbed", "Ndofs": bp.Ngather, "us": t * 1e6,
+            "gdof_per_s": bp.Ngather / t / 1e9, "algorithmic_bytes": nb, "gbs": nb / t / 1e9}
+
+
+def bench_bp3(capi, plat, args):
+    """BP3 (over-integrated stiffness, cubNq = p+2) at 24^3, p = 7: Ax, and
+    the FP64 CG against the FP32 CG with FP64 defect correction, both to tol."""
+    import time
+    import torch
+
+    m = capi.Mesh(N=7, nx=24, cubature=True, perturb=0.1, seed=1)
+    bp = capi.BP(plat, m, 3, lam=1.0)
+    Nd, Nel, Np, cubNp = bp.Ngather, m.Nelements, m.Np, m.cubNp
+    q = torch.from_numpy(capi.random_normal(2, Nd)).cuda()
+    AqL = torch.empty(bp.NLocal, dtype=torch.float64, device="cuda")
+    t_ax = event_time(lambda: bp.ax_local(q, AqL), 20)
+    nb = Nd * 8 + Nel * (cubNp * 7 * 8 + 4 + Np * 12)  # BP/BP3/src/bp3Run.cpp:114-118
+    b = torch.from_numpy(capi.random_normal(7, Nd)).cuda()
+    x = torch.zeros_like(b)
+    r = torch.empty_like(b)
+
+    def fp64():
+        x.zero_()
+        r.copy_(b)
+        return bp.solve(x, r, tol=args.tol, maxit=100000, history=False)[0]
+
+    def mixed():
+        x.zero_()
+        return bp.solve_mixed(x, b, tol=args.tol, interval=args.interval, maxit=100000)
+
+    fp64()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    its = fp64()
+    torch.cuda.synchronize()
+    t64 = time.perf_counter() - t0
+    mixed()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = mixed()
+    torch.cuda.synchronize()
+    tm = time.perf_counter() - t0
+    return {"workload": f"BP3 hex p=7 24^3 perturbed, lambda=1, N(0,1) rhs, tol {args.tol:g}", "Ndofs": Nd,
+            "ax_us": t_ax * 1e6, "ax_gbs": nb / t_ax / 1e9, "ax_algorithmic_bytes": nb,
+            "fp64_iterations": its, "fp64_ms": t64 * 1e3, "mixed_fp32_iterations": st["iterations"],
+            "mixed_corrections": st["updates"], "mixed_ms": tm * 1e3, "speedup_vs_fp64": t64 / tm,
+            "fp64_equivalent_gdof_it_per_s": Nd * its / tm / 1e9}
+
+
+def bench_adjoint(capi, plat):
+    """dL/dkappa at 16^3, p=7: forward solve, adjoint solve and gradient to 1e-10."""
+    import time
+    import numpy as np
+    import torch
+    from benchparanumal_amd import adjoint
+
+    m = capi.Mesh(N=7, nx=16, perturb=0.1, seed=1)
+    bp = capi.BP(plat, m, 5, lam=0.0)
+    o = capi.Ogs(mesh=m)
+    W = torch.zeros(bp.Ngather, dtype=torch.float64, device="cuda")
+    o.gather(plat, torch.from_numpy(np.ascontiguousarray(m.array("wJ"))).cuda(), W)
+    kap = torch.from_numpy(np.exp(adjoint.smooth_lattice_field(adjoint.lattice_shape(m), 31).ravel())).cuda()
+    u_obs = torch.from_numpy(capi.random_normal(33, bp.Ngather)).cuda()
+
+    def step():
+        kt = kap.clone().requires_grad_(True)
+        u = adjoint.kappa_solve(bp, kt, W, tol=1e-10)
+        L = 0.5 * ((u - u_obs) ** 2 * W).sum()
+        L.backward()
+        return kt.grad
+
+    step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    step()
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    u = torch.randn(bp.Ngather, dtype=torch.float64, device="cuda")
+    g = torch.empty(bp.Nnodes, dtype=torch.float64, device="cuda")
+    bp.set_kappa(kap)
+    tg = event_time(lambda: bp.kappa_gradient(u, u, g), 10)
+    return {"workload": "16^3 hex p=7, kappa = exp(smoothed N(0,1), std 0.5), L = 1/2 |u - u_obs|_M^2, tol 1e-10",
+            "Ndofs": bp.Ngather, "ms_forward_adjoint_gradient": t * 1e3, "gradient_kernel_us": tg * 1e6}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,6 +287,8 @@ def main():
     ap.add_argument("--ax-launches", type=int, default=50)
     ap.add_argument("--cpu-its", type=int, default=1000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extensions", action="store_true",
+                    help="skip the k-RHS, batch and adjoint measurements")
     ap.add_argument("--ax-only", action="store_true",
                     help="only the Ax launches and a 512 MiB device copy (for counter runs)")
     args = ap.parse_args()
@@ -210,6 +418,17 @@ def main():
                          f"{args.cpu_its} iterations in {tcpu:.1f}s"}
         log(f"cpu baseline: {cpu['value']:.4f} GDOF*it/s on {threads} threads")
 
+    ext = None
+    if not args.no_extensions:
+        ext = {"krhs_ax": bench_krhs(capi, plat), "ragged_batch_cg": bench_batch(capi, plat),
+               "adjoint": bench_adjoint(capi, plat), "bp3": bench_bp3(capi, plat, args),
+               "bk5_p15": bench_p15(capi, plat)}
+        log("extensions: " + json.dumps({k: {kk: vv for kk, vv in v.items() if kk in
+                                                 ("ms_per_solve", "speedup_vs_sequential",
+                                                  "ms_forward_adjoint_gradient", "single_us",
+                                                  "speedup_vs_fp64", "gbs", "ax_gbs")}
+                                             for k, v in ext.items()}))
+
     value = Ndofs * its64 / tmix / 1e9
     line = {
         "metric": METRIC,
@@ -243,6 +462,7 @@ def main():
                      "device_copy_gbs": copy_gbs},
         "tuning": bp.tuning(),
         "cpu_baseline": cpu,
+        "extensions": ext,
     }
     print(json.dumps(line), flush=True)
 

@@ -92,7 +92,7 @@ for _name in ("bpx_bp1_ax_hex", "bpx_bp1_ax_affine_hex", "bpx_bp2_ax_hex", "bpx_
 for _name in ("bpx_bp3_ax_hex", "bpx_bp3_ax_affine_hex", "bpx_bp4_ax_hex", "bpx_bp4_ax_affine_hex"):
     _SIGS[_name] = (_I, [_P, _I, _I, _P, _P] + [_P] * 7 + [_D, _P, _P])
 for _name in ("bpx_bp5_ax_hex_krhs", "bpx_bp5_ax_affine_hex_krhs"):
-    _SIGS[_name] = (_I, [_P, _I, _I, _I, _I, _P, _P] + [_P] * 7 + [_D, _P, _P])
+    _SIGS[_name] = (_I, [_P, _I, _I, _I, _I, _P, _P] + [_P] * 7 + [_D, _P, C.c_int64, _P, C.c_int64])
 _SIGS["bpx_bp5_ax_hex_f32"] = (_I, [_P, _I, _I, _P, _P] + [_P] * 7 + [_F, _P, _P])
 _SIGS["bpx_bp3_ax_hex_f32"] = (_I, [_P, _I, _I, _P, _P] + [_P] * 7 + [_F, _P, _P])
 
@@ -285,7 +285,7 @@ class BP:
         return {"gather_mode": int(out[0]), "gather_us": [float(out[1]), float(out[2])]}
 
     def operator_krhs(self, q, Aq, K, chunk=0):
-        """Operator on K interleaved right-hand sides ([Ngather][K]), BP5."""
+        """Operator on K right-hand sides stored one after another (K x Ngather), BP5."""
         check(lib().bpx_bp_operator_krhs(self.h, K, chunk, ptr(q), ptr(Aq)))
 
     def rhs_forcing(self, b):

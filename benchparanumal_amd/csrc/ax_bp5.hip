@@ -17,10 +17,11 @@
 // FP64) in flight.
 //
 // k right-hand sides (an extension; the reference applies one vector per
-// launch): vectors are interleaved as in the ogs K form, and each element is
-// swept K/chunk times with chunk right-hand sides in registers; the sweeps
-// of one element run back to back in one workgroup, so its geometry comes
-// from HBM once and from L2 after that.
+// launch): the right-hand sides are stored one after another (leading
+// dimensions ldq, ldA), and each element is swept K/chunk times with chunk
+// right-hand sides in registers; the sweeps of one element run back to back
+// in one workgroup, so its geometry comes from HBM once and from cache
+// after that, while every sweep reads and writes whole cache lines.
 //
 // This file is compiled once per Nq (-DBPX_NQ=n) so the instantiations
 // build in parallel; ax_dispatch.cpp picks the instantiation at run time.
@@ -62,16 +63,18 @@ constexpr bool bp5_dreg() {
   return NQ <= 10 && NF == 1;
 }
 
-// KRHS: q is [Ndofs][K], Aq [NLocal][K]; the element is swept K/NF times,
-// NF right-hand sides per sweep, re-reading its geometry from L2.
+// KRHS: K right-hand sides stored one after another, r-th at q + r*ldq and
+// Aq + r*ldA; the element is swept K/NF times, NF right-hand sides per
+// sweep, re-reading its geometry (from cache: the sweeps of one element run
+// back to back in one workgroup).
 // BATCH: elements of many problems; an element whose problem
 // (elemPid[e]) is done is skipped.
 template <typename T, int NQ, int NF, int EPB, bool AFFINE, int WAVES, bool KRHS, bool BATCH>
 __global__ __launch_bounds__(NQ* NQ* EPB) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_bp5_ax(
     int Nelements, int K, const int* __restrict__ elementList, const int* __restrict__ G2L,
     const T* __restrict__ wJ, const T* __restrict__ ggeo, const T* __restrict__ gllw,
-    const T* __restrict__ D, T lambda, const T* __restrict__ q, T* __restrict__ Aq,
-    const int* __restrict__ elemPid, const BatchState* __restrict__ bst) {
+    const T* __restrict__ D, T lambda, const T* __restrict__ q, T* __restrict__ Aq, long long ldq,
+    long long ldA, const int* __restrict__ elemPid, const BatchState* __restrict__ bst) {
   constexpr int NQ2 = NQ * NQ;
   constexpr int NP = NQ2 * NQ;
   constexpr int LD = NQ + 1;  // padded LDS row
@@ -111,10 +114,10 @@ __global__ __launch_bounds__(NQ* NQ* EPB) __attribute__((amdgpu_waves_per_eu(WAV
   if (KRHS) {
 #pragma unroll
     for (int k = 0; k < NQ; ++k) {
-      const T* src = q + (size_t)max(gid[k], 0) * K + c * NF;
+      const T* src = q + (size_t)(c * NF) * ldq + max(gid[k], 0);
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
-        rq[f][k] = (gid[k] >= 0) ? src[f] : T(0);
+        rq[f][k] = (gid[k] >= 0) ? src[(size_t)f * ldq] : T(0);
         rA[f][k] = T(0);
       }
     }
@@ -221,9 +224,9 @@ __global__ __launch_bounds__(NQ* NQ* EPB) __attribute__((amdgpu_waves_per_eu(WAV
     if (KRHS) {
 #pragma unroll
       for (int k = 0; k < NQ; ++k) {
-        T* out = Aq + ((size_t)e * NP + k * NQ2 + ij) * K + c * NF;
+        T* out = Aq + (size_t)(c * NF) * ldA + (size_t)e * NP + k * NQ2 + ij;
 #pragma unroll
-        for (int f = 0; f < NF; ++f) out[f] = rA[f][k];
+        for (int f = 0; f < NF; ++f) out[(size_t)f * ldA] = rA[f][k];
       }
     } else {
       T* out = Aq + (size_t)e * NP * NF + ij;
@@ -315,11 +318,11 @@ int launch_bp5(hipStream_t s, const AxArgs<T>& a) {
   if (a.affine)
     hipLaunchKernelGGL((k_bp5_ax<T, NQ, NF, EPB, true, WV, KRHS, false>), dim3(blocks), dim3(NQ * NQ * EPB), 0,
                        s, a.Nelements, a.K, a.elementList, a.GlobalToLocal, a.wJ, a.ggeo, a.gllw, a.D,
-                       a.lambda, a.q, a.Aq, nullptr, nullptr);
+                       a.lambda, a.q, a.Aq, a.ldq, a.ldAq, nullptr, nullptr);
   else
     hipLaunchKernelGGL((k_bp5_ax<T, NQ, NF, EPB, false, WV, KRHS, false>), dim3(blocks), dim3(NQ * NQ * EPB), 0,
                        s, a.Nelements, a.K, a.elementList, a.GlobalToLocal, a.wJ, a.ggeo, a.gllw, a.D,
-                       a.lambda, a.q, a.Aq, nullptr, nullptr);
+                       a.lambda, a.q, a.Aq, a.ldq, a.ldAq, nullptr, nullptr);
   BPX_HIP(hipGetLastError());
   return BPX_OK;
 }
@@ -376,8 +379,8 @@ int launch_batch_v(hipStream_t s, int Nelements, const int* elementList, const i
   const int blocks = (Nelements + EPB - 1) / EPB;
   if (blocks == 0) return BPX_OK;
   hipLaunchKernelGGL((k_bp5_ax<double, NQ, 1, EPB, false, WV, false, true>), dim3(blocks), dim3(NQ * NQ * EPB), 0,
-                     s, Nelements, 1, elementList, G2L, wJ, ggeo, (const double*)nullptr, D, lambda, q, Aq,
-                     elemPid, done);
+                     s, Nelements, 1, elementList, G2L, wJ, ggeo, (const double*)nullptr, D, lambda, q, Aq, 0LL,
+                     0LL, elemPid, done);
   BPX_HIP(hipGetLastError());
   return BPX_OK;
 }

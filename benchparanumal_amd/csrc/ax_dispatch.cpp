@@ -73,6 +73,8 @@ template <typename T>
 int launch_bp5_ax_krhs(hipStream_t s, const AxArgs<T>& a, int chunk) {
   if (int st = check("bp5 k-RHS Ax", a.Nelements, a.q, a.Aq, a.GlobalToLocal, a.elementList)) return st;
   if (a.K < 1) return set_error(BPX_ERR_INVALID_ARGUMENT, "bp5 k-RHS Ax: K < 1");
+  if (a.K > 1 && (a.ldq < 1 || a.ldAq < 1))
+    return set_error(BPX_ERR_INVALID_ARGUMENT, "bp5 k-RHS Ax: leading dimensions must be positive");
   if (a.Nfields != 1) return set_error(BPX_ERR_UNSUPPORTED, "bp5 k-RHS Ax: one field per right-hand side");
   if (chunk == 0) chunk = a.K % 2 == 0 ? 2 : 1;
   if ((chunk != 1 && chunk != 2 && chunk != 4) || a.K % chunk != 0)

@@ -46,9 +46,10 @@ struct AxArgs {
   T lambda = 0;
   const T* q = nullptr;
   T* Aq = nullptr;
-  // k right-hand sides (BP5 only): q is [Ndofs][K] and Aq [NLocal][K], so
-  // entry r of node n sits at r + n*K, the ogs interleaving
+  // k right-hand sides (BP5 only): right-hand side r of q at q + r*ldq,
+  // of Aq at Aq + r*ldAq (one vector after another)
   int K = 1;
+  long long ldq = 0, ldAq = 0;
 };
 
 // Launchers (defined in the .hip files). Return BPX_OK or an error code.

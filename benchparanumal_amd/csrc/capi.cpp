@@ -168,7 +168,8 @@ int bpx_bp5_ax_affine_hex(bpx_platform p, int N, int Nelements, const int* eleme
 }
 static int bp5_krhs_entry(bpx_platform p, int N, int K, int chunk, int Nelements, const int* elementList,
                           const int* GlobalToLocal, const double* wJ, const double* ggeo, const double* gllw,
-                          const double* D, double lambda, const double* q, double* Aq, int affine) {
+                          const double* D, double lambda, const double* q, int64_t ldq, double* Aq, int64_t ldAq,
+                          int affine) {
   if (int st = need(p, "platform")) return st;
   AX_COMMON(double)
   a.affine = affine;
@@ -178,24 +179,26 @@ static int bp5_krhs_entry(bpx_platform p, int N, int K, int chunk, int Nelements
   a.D = D;
   a.lambda = lambda;
   a.K = K;
+  a.ldq = ldq;
+  a.ldAq = ldAq;
   return launch_bp5_ax_krhs<double>(S(p), a, chunk);
 }
 
 int bpx_bp5_ax_hex_krhs(bpx_platform p, int N, int K, int chunk, int Nelements, const int* elementList,
                         const int* GlobalToLocal, const double* wJ, const double* vgeo, const double* ggeo,
                         const double* gllw, const double* D, const double* S_, const double* MM, double lambda,
-                        const double* q, double* Aq) {
+                        const double* q, int64_t ldq, double* Aq, int64_t ldAq) {
   (void)vgeo; (void)S_; (void)MM;
-  return bp5_krhs_entry(p, N, K, chunk, Nelements, elementList, GlobalToLocal, wJ, ggeo, gllw, D, lambda, q, Aq,
-                        0);
+  return bp5_krhs_entry(p, N, K, chunk, Nelements, elementList, GlobalToLocal, wJ, ggeo, gllw, D, lambda, q, ldq,
+                        Aq, ldAq, 0);
 }
 int bpx_bp5_ax_affine_hex_krhs(bpx_platform p, int N, int K, int chunk, int Nelements, const int* elementList,
                                const int* GlobalToLocal, const double* wJ, const double* vgeo, const double* ggeo,
                                const double* gllw, const double* D, const double* S_, const double* MM,
-                               double lambda, const double* q, double* Aq) {
+                               double lambda, const double* q, int64_t ldq, double* Aq, int64_t ldAq) {
   (void)vgeo; (void)S_; (void)MM;
-  return bp5_krhs_entry(p, N, K, chunk, Nelements, elementList, GlobalToLocal, wJ, ggeo, gllw, D, lambda, q, Aq,
-                        1);
+  return bp5_krhs_entry(p, N, K, chunk, Nelements, elementList, GlobalToLocal, wJ, ggeo, gllw, D, lambda, q, ldq,
+                        Aq, ldAq, 1);
 }
 int bpx_bp6_ax_hex(bpx_platform p, int N, int Nelements, const int* elementList, const int* GlobalToLocal,
                    const double* wJ, const double* vgeo, const double* ggeo, const double* gllw, const double* D,

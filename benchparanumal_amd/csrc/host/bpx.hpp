@@ -222,8 +222,9 @@ class bp_t : public solver_t {
   int AxLocal(const dfloat* o_q, dfloat* o_AqL);
   int Operator32(const float* o_q, float* o_Aq);
   int Forcing(dfloat* o_b);
-  // K right-hand sides at once (BP5): o_q is [Ngather][K], o_Aq the same,
-  // o_AqL [NLocal][K]; chunk as for launch_bp5_ax_krhs (0 = default)
+  // K right-hand sides at once (BP5), one vector after another: o_q and
+  // o_Aq hold K vectors of Ngather, o_AqL K of NLocal; chunk as for
+  // launch_bp5_ax_krhs (0 = default)
   int AxLocalKrhs(int K, const dfloat* o_q, dfloat* o_AqL, int chunk);
   int OperatorKrhs(int K, const dfloat* o_q, dfloat* o_Aq, int chunk);
   // Variable coefficient (BP5): kappa on the node lattice (device, Nnodes
@@ -256,7 +257,7 @@ class bp_t : public solver_t {
   deviceBuffer<dlong> o_GlobalToLocal, o_elementList;
   deviceBuffer<double> o_wJ, o_ggeo, o_gllw, o_D, o_I, o_MM, o_invV;
   deviceBuffer<double> o_AqL;
-  deviceBuffer<double> o_AqLk;  // [NLocal][K] for OperatorKrhs, grown on demand
+  deviceBuffer<double> o_AqLk;  // K local vectors for OperatorKrhs, grown on demand
   // variable coefficient: lattice point of each local node, the lattice
   // sum maps, kappa-scaled factors (per node) and, for affine meshes, wJ per node
   bool haveKappa = false;

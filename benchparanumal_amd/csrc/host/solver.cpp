@@ -370,6 +370,8 @@ int bp_t::AxLocalKrhs(int K, const dfloat* o_q, dfloat* o_AqL, int chunk) {
   a.q = o_q;
   a.Aq = o_AqL;
   a.K = K;
+  a.ldq = ogs.Ngather;
+  a.ldAq = NLocal;
   return launch_bp5_ax_krhs<double>(platform->stream, a, chunk);
 }
 
@@ -381,7 +383,10 @@ int bp_t::OperatorKrhs(int K, const dfloat* o_q, dfloat* o_Aq, int chunk) {
     if (int st = o_AqLk.alloc(need)) return st;
   }
   if (int st = AxLocalKrhs(K, o_q, o_AqLk.ptr(), chunk)) return st;
-  return ogs.Gather<double>(*platform, o_Aq, o_AqLk.ptr(), K);
+  for (int r = 0; r < K; ++r)  // one gather per right-hand side, same maps
+    if (int st = ogs.Gather<double>(*platform, o_Aq + (size_t)r * ogs.Ngather, o_AqLk.ptr() + (size_t)r * NLocal, 1))
+      return st;
+  return BPX_OK;
 }
 
 int bp_t::OperatorDot32(const float* o_q, float* o_Aq, double* part, int* npart) {

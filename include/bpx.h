@@ -89,21 +89,23 @@ int bpx_bp5_ax_affine_hex(bpx_platform p, int N, int Nelements, const int* eleme
                           double* Aq);
 /* No reference counterpart (the reference applies Ax to one vector per
  * launch; okl/bp5AxHex3D.okl:415-427 is the single-vector form): BP5 Ax on K
- * right-hand sides at once, geometry streamed once. Vectors are interleaved
- * as in the ogs K form (libs/ogs/okl/ogsKernels.okl:86-122): q[r + K*id],
- * Aq[r + K*(e*Np + n)]. chunk = right-hand sides per register sweep (1, 2
- * or 4, dividing K; 0 = default). Each right-hand side equals the
- * single-vector bpx_bp5_ax_hex bit for bit. */
+ * right-hand sides at once, geometry streamed once. The right-hand sides are
+ * stored one after another: q + r*ldq (global, indexed through
+ * GlobalToLocal) and Aq + r*ldAq (local, e*Np + n), r = 0..K-1. chunk =
+ * right-hand sides per register sweep (1, 2 or 4, dividing K; 0 =
+ * default). Each right-hand side equals the single-vector bpx_bp5_ax_hex
+ * bit for bit. */
 int bpx_bp5_ax_hex_krhs(bpx_platform p, int N, int K, int chunk, int Nelements,
                         const int* elementList, const int* GlobalToLocal, const double* wJ,
                         const double* vgeo, const double* ggeo, const double* gllw,
                         const double* D, const double* S, const double* MM, double lambda,
-                        const double* q, double* Aq);
+                        const double* q, int64_t ldq, double* Aq, int64_t ldAq);
 int bpx_bp5_ax_affine_hex_krhs(bpx_platform p, int N, int K, int chunk, int Nelements,
                                const int* elementList, const int* GlobalToLocal,
                                const double* wJ, const double* vgeo, const double* ggeo,
                                const double* gllw, const double* D, const double* S,
-                               const double* MM, double lambda, const double* q, double* Aq);
+                               const double* MM, double lambda, const double* q, int64_t ldq,
+                               double* Aq, int64_t ldAq);
 /* okl/bp6AxHex3D.okl:555-567 (3 fields, local layout [e][f][n]) */
 int bpx_bp6_ax_hex(bpx_platform p, int N, int Nelements, const int* elementList,
                    const int* GlobalToLocal, const double* wJ, const double* vgeo,
@@ -268,8 +270,8 @@ int bpx_bp_ax_local(bpx_bp b, const double* o_q, double* o_AqL);
 /* FP32 operator on the same mesh (factors rounded to float). */
 int bpx_bp_operator_f32(bpx_bp b, const float* o_q, float* o_Aq);
 /* No reference counterpart: bpN_t::Operator (BP/BP5/src/bp5Operator.cpp:29-85)
- * on K interleaved right-hand sides (BP5 only): o_q and o_Aq are
- * [Ngather][K]; k-RHS Ax, then the ogs gather with K. */
+ * on K right-hand sides (BP5 only), stored one after another: o_q and o_Aq
+ * hold K vectors of Ngather. k-RHS Ax, then one gather per right-hand side. */
 int bpx_bp_operator_krhs(bpx_bp b, int K, int chunk, const double* o_q, double* o_Aq);
 /* No reference counterpart (the reference's BP5 coefficient is constant,
  * okl/bp5AxHex3D.okl:415-569): variable coefficient -div(kappa grad u) +

@@ -50,13 +50,13 @@ def oracle_arrays(P):
 
 def krhs_kernel(plat, P, arr, qk, K, chunk, affine, lam):
     """bpx_bp5_ax[_affine]_hex_krhs on the oracle problem's arrays."""
-    out = torch.zeros(P.NLocal * K, dtype=torch.float64, device="cuda")
-    qd = torch.from_numpy(np.ascontiguousarray(qk.ravel())).cuda()
+    out = torch.zeros(K * P.NLocal, dtype=torch.float64, device="cuda")
+    qd = torch.from_numpy(np.ascontiguousarray(qk.T.ravel())).cuda()  # right-hand side after right-hand side
     fn = capi.lib().bpx_bp5_ax_affine_hex_krhs if affine else capi.lib().bpx_bp5_ax_hex_krhs
     capi.check(fn(plat.h, P.N, K, chunk, P.Nelements, capi.ptr(arr["elementList"]), capi.ptr(arr["GlobalToLocal"]),
                   capi.ptr(arr["wJ"]), None, capi.ptr(arr["ggeo"]), capi.ptr(arr["gllw"]), capi.ptr(arr["D"]),
-                  None, None, lam, capi.ptr(qd), capi.ptr(out)))
-    return host(out).reshape(P.NLocal, K)
+                  None, None, lam, capi.ptr(qd), P.Ngather, capi.ptr(out), P.NLocal))
+    return host(out).reshape(K, P.NLocal).T
 
 
 def single_kernel(plat, P, arr, qk, K, affine, lam):
@@ -100,9 +100,9 @@ def test_krhs_operator_equals_single_20cube(plat, K):
     m = capi.Mesh(N=7, nx=nx, perturb=0.1, seed=1)
     b = capi.BP(plat, m, 5, lam=1.0)
     qk = np.random.default_rng(K).standard_normal((b.Ngather, K))
-    Aq = torch.zeros(b.Ngather * K, dtype=torch.float64, device="cuda")
-    b.operator_krhs(torch.from_numpy(qk.ravel()).cuda(), Aq, K)
-    got = host(Aq).reshape(b.Ngather, K)
+    Aq = torch.zeros(K * b.Ngather, dtype=torch.float64, device="cuda")
+    b.operator_krhs(torch.from_numpy(np.ascontiguousarray(qk.T.ravel())).cuda(), Aq, K)
+    got = host(Aq).reshape(K, b.Ngather).T
     ref = single_columns(b, qk, K, local=False)
     assert rel_err(got, ref) <= KRHS_RTOL
 
