@@ -72,6 +72,8 @@ _SIGS = {
     "bpx_bp_operator_f32": (_I, [_P, _P, _P]),
     "bpx_bp_operator_krhs": (_I, [_P, _I, _I, _P, _P]),
     "bpx_bp_tuning": (_I, [_P, _P, _I]),
+    "bpx_bp_operator_dot": (_I, [_P, _P, _P, C.POINTER(_D)]),
+    "bpx_bp_set_tuning": (_I, [_P, _I, _I, _I, _I]),
     "bpx_bp_set_kappa": (_I, [_P, _P]),
     "bpx_batch_create": (_I, [_P, _I, _P, _D, C.POINTER(_P)]),
     "bpx_batch_destroy": (_I, [_P]),
@@ -279,10 +281,23 @@ class BP:
         check(lib().bpx_bp_kappa_gradient(self.h, ptr(u), ptr(lam), ptr(grad)))
 
     def tuning(self):
-        """Launch choices timed at setup: gather order and its times (us)."""
-        out = np.zeros(3)
-        n = lib().bpx_bp_tuning(self.h, ptr(out), 3)
-        return {"gather_mode": int(out[0]), "gather_us": [float(out[1]), float(out[2])]}
+        """Launch choices timed at setup (identical results whichever is chosen)."""
+        out = np.zeros(16)
+        lib().bpx_bp_tuning(self.h, ptr(out), 16)
+        f = [float(v) for v in out]
+        return {"gather_mode": int(f[0]), "gather_us": f[1:3], "ax_shape": int(f[3]), "ax_us": f[4:7],
+                "fused_build": int(f[7]), "fused_us": f[8:11], "unfused_us": f[11],
+                "fused32_shape": int(f[12]), "fused32_us": f[13:16]}
+
+    def set_tuning(self, gather_mode=-1, ax_shape=-1, fused_shape=-1, fused32_shape=-1):
+        """Pin launch choices (-1 keeps the timed one)."""
+        check(lib().bpx_bp_set_tuning(self.h, gather_mode, ax_shape, fused_shape, fused32_shape))
+
+    def operator_dot(self, q, Aq):
+        """Aq = A q as CG applies it (fused for BP5); returns q.Aq."""
+        r = _D()
+        check(lib().bpx_bp_operator_dot(self.h, ptr(q), ptr(Aq), C.byref(r)))
+        return r.value
 
     def operator_krhs(self, q, Aq, K, chunk=0):
         """Operator on K right-hand sides stored one after another (K x Ngather), BP5."""

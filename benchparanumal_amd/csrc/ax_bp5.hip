@@ -27,6 +27,8 @@
 // build in parallel; ax_dispatch.cpp picks the instantiation at run time.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "ax_common.hpp"
 
 #ifndef BPX_NQ
@@ -69,12 +71,27 @@ constexpr bool bp5_dreg() {
 // back to back in one workgroup).
 // BATCH: elements of many problems; an element whose problem
 // (elemPid[e]) is done is skipped.
-template <typename T, int NQ, int NF, int EPB, bool AFFINE, int WAVES, bool KRHS, bool BATCH>
+// FUSED (one field): G2L entries carry a flag (bit 30) on nodes whose global
+// row has this node as its only copy; those results go straight to Ap[row]
+// (a one-entry gather row sums to the value itself) with their p.Ap terms
+// summed into part[workgroup]; the other nodes go to Aq (the local vector)
+// for the gather of the shared rows.
+enum { VAR_PLAIN = 0, VAR_KRHS = 1, VAR_BATCH = 2, VAR_FUSED = 3 };
+constexpr int kSingleFlag = 1 << 30;
+
+// PF: non-affine geometry slabs in flight (slab k+PF loads while slab k
+// computes); more depth trades registers (occupancy) for bytes in flight.
+template <typename T, int NQ, int NF, int EPB, bool AFFINE, int WAVES, int VAR, int PF = 1>
 __global__ __launch_bounds__(NQ* NQ* EPB) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_bp5_ax(
     int Nelements, int K, const int* __restrict__ elementList, const int* __restrict__ G2L,
     const T* __restrict__ wJ, const T* __restrict__ ggeo, const T* __restrict__ gllw,
     const T* __restrict__ D, T lambda, const T* __restrict__ q, T* __restrict__ Aq, long long ldq,
-    long long ldA, const int* __restrict__ elemPid, const BatchState* __restrict__ bst) {
+    long long ldA, const int* __restrict__ elemPid, const BatchState* __restrict__ bst, T* __restrict__ Ap,
+    double* __restrict__ part) {
+  constexpr bool KRHS = VAR == VAR_KRHS;
+  constexpr bool BATCH = VAR == VAR_BATCH;
+  constexpr bool FUSED = VAR == VAR_FUSED;
+  static_assert(!FUSED || NF == 1, "the fused operator covers one field");
   constexpr int NQ2 = NQ * NQ;
   constexpr int NP = NQ2 * NQ;
   constexpr int LD = NQ + 1;  // padded LDS row
@@ -101,7 +118,7 @@ __global__ __launch_bounds__(NQ* NQ* EPB) __attribute__((amdgpu_waves_per_eu(WAV
   for (int n = t; n < NQ2; n += NQ2 * EPB) s_D[n / NQ][n % NQ] = D[n];
 
   // KRHS: the pencil's gather rows, read once for all sweeps
-  int gid[KRHS ? NQ : 1];
+  int gid[(KRHS || FUSED) ? NQ : 1];
   if (KRHS) {
 #pragma unroll
     for (int k = 0; k < NQ; ++k) gid[k] = active ? G2L[(size_t)e * NP + k * NQ2 + ij] : -1;
@@ -121,6 +138,14 @@ __global__ __launch_bounds__(NQ* NQ* EPB) __attribute__((amdgpu_waves_per_eu(WAV
         rA[f][k] = T(0);
       }
     }
+  } else if (FUSED) {
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) {
+      const int v = active ? G2L[(size_t)e * NP + k * NQ2 + ij] : -1;
+      gid[k] = v;
+      rq[0][k] = (v >= 0) ? q[v & (kSingleFlag - 1)] : T(0);
+      rA[0][k] = T(0);
+    }
   } else {
     const size_t lbase = (size_t)e * NP * NF + ij;
 #pragma unroll
@@ -133,18 +158,25 @@ __global__ __launch_bounds__(NQ* NQ* EPB) __attribute__((amdgpu_waves_per_eu(WAV
       }
   }
 
-  // geometry of the current slab (non-affine: prefetched one slab ahead)
+  // geometry: affine, one set per element; non-affine, a ring of PF slabs
+  // (slab k in ring[k % PF], slab k+PF loads while slab k computes)
   T g0 = 0, g1 = 0, g2 = 0, g3 = 0, g4 = 0, g5 = 0, gw = 0;
   T wij = 0;
+  T ring[AFFINE ? 1 : PF][7];
   if (AFFINE) {
     const T* ge = ggeo + (size_t)6 * e;
     g0 = ge[0]; g1 = ge[1]; g2 = ge[2]; g3 = ge[3]; g4 = ge[4]; g5 = ge[5];
     gw = wJ[e];
     wij = gllw[i] * gllw[j];
   } else {
-    const T* gp = ggeo + 6 * ((size_t)e * NP + ij);
-    g0 = gp[0]; g1 = gp[1]; g2 = gp[2]; g3 = gp[3]; g4 = gp[4]; g5 = gp[5];
-    gw = wJ[(size_t)e * NP + ij];
+#pragma unroll
+    for (int d = 0; d < PF && d < NQ; ++d) {
+      const size_t gb = (size_t)e * NP + d * NQ2 + ij;
+      const T* gp = ggeo + 6 * gb;
+#pragma unroll
+      for (int c = 0; c < 6; ++c) ring[d][c] = gp[c];
+      ring[d][6] = wJ[gb];
+    }
   }
   __syncthreads();
 
@@ -174,12 +206,16 @@ __global__ __launch_bounds__(NQ* NQ* EPB) __attribute__((amdgpu_waves_per_eu(WAV
       G00 = g0 * w; G01 = g1 * w; G02 = g2 * w; G11 = g3 * w; G12 = g4 * w; G22 = g5 * w;
       W = gw * w;
     } else {
-      G00 = g0; G01 = g1; G02 = g2; G11 = g3; G12 = g4; G22 = g5; W = gw;
-      if (k + 1 < NQ) {
-        const size_t gb = (size_t)e * NP + (k + 1) * NQ2 + ij;
+      constexpr int RS = AFFINE ? 1 : PF;
+      const int slot = k % RS;
+      G00 = ring[slot][0]; G01 = ring[slot][1]; G02 = ring[slot][2];
+      G11 = ring[slot][3]; G12 = ring[slot][4]; G22 = ring[slot][5]; W = ring[slot][6];
+      if (k + PF < NQ) {
+        const size_t gb = (size_t)e * NP + (k + PF) * NQ2 + ij;
         const T* gp = ggeo + 6 * gb;
-        g0 = gp[0]; g1 = gp[1]; g2 = gp[2]; g3 = gp[3]; g4 = gp[4]; g5 = gp[5];
-        gw = wJ[gb];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) ring[slot][c] = gp[c];
+        ring[slot][6] = wJ[gb];
       }
     }
 #pragma unroll
@@ -220,7 +256,38 @@ __global__ __launch_bounds__(NQ* NQ* EPB) __attribute__((amdgpu_waves_per_eu(WAV
     }
   }
 
-  if (active) {
+  if (FUSED) {
+    double pd = 0;
+    if (active) {
+#pragma unroll
+      for (int k = 0; k < NQ; ++k) {
+        const int v = gid[k];
+        if (v >= 0) {
+          if (v & kSingleFlag) {
+            Ap[v & (kSingleFlag - 1)] = rA[0][k];
+            pd += (double)rq[0][k] * (double)rA[0][k];
+          } else {
+            Aq[(size_t)e * NP + k * NQ2 + ij] = rA[0][k];
+          }
+        }
+      }
+    }
+    // workgroup sum of the p.Ap terms (fixed order)
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) pd += __shfl_xor(pd, off, 64);
+    if (EPB * NQ2 <= 64) {
+      if (t == 0) part[blockIdx.x] = pd;
+    } else {
+      __shared__ double s_pd[(NQ2 * EPB + 63) / 64];
+      if ((t & 63) == 0) s_pd[t >> 6] = pd;
+      __syncthreads();
+      if (t == 0) {
+        double acc = 0;
+        for (int w = 0; w < (NQ2 * EPB + 63) / 64; ++w) acc += s_pd[w];
+        part[blockIdx.x] = acc;
+      }
+    }
+  } else if (active) {
     if (KRHS) {
 #pragma unroll
       for (int k = 0; k < NQ; ++k) {
@@ -309,37 +376,58 @@ __global__ __launch_bounds__(NQ* NQ* EPB) void k_bp5_kappa_grad(int Nelements, c
   }
 }
 
-template <typename T, int NQ, int NF, bool KRHS>
-int launch_bp5(hipStream_t s, const AxArgs<T>& a) {
+template <typename T, int NQ, int NF, int VAR, int WVSET = 0, int PF = 1>
+int launch_bp5(hipStream_t s, const AxArgs<T>& a, T* Ap = nullptr, double* part = nullptr) {
   constexpr int EPB = bp5_epb<NQ>();
-  constexpr int WV = bp5_waves<NQ, NF, KRHS>();
+  constexpr int WV = WVSET > 0 ? WVSET : bp5_waves<NQ, NF, VAR == VAR_KRHS>();
   const int blocks = (a.Nelements + EPB - 1) / EPB;
   if (blocks == 0) return BPX_OK;
   if (a.affine)
-    hipLaunchKernelGGL((k_bp5_ax<T, NQ, NF, EPB, true, WV, KRHS, false>), dim3(blocks), dim3(NQ * NQ * EPB), 0,
-                       s, a.Nelements, a.K, a.elementList, a.GlobalToLocal, a.wJ, a.ggeo, a.gllw, a.D,
-                       a.lambda, a.q, a.Aq, a.ldq, a.ldAq, nullptr, nullptr);
+    hipLaunchKernelGGL((k_bp5_ax<T, NQ, NF, EPB, true, WV, VAR>), dim3(blocks), dim3(NQ * NQ * EPB), 0, s,
+                       a.Nelements, a.K, a.elementList, a.GlobalToLocal, a.wJ, a.ggeo, a.gllw, a.D, a.lambda, a.q,
+                       a.Aq, a.ldq, a.ldAq, nullptr, nullptr, Ap, part);
   else
-    hipLaunchKernelGGL((k_bp5_ax<T, NQ, NF, EPB, false, WV, KRHS, false>), dim3(blocks), dim3(NQ * NQ * EPB), 0,
-                       s, a.Nelements, a.K, a.elementList, a.GlobalToLocal, a.wJ, a.ggeo, a.gllw, a.D,
-                       a.lambda, a.q, a.Aq, a.ldq, a.ldAq, nullptr, nullptr);
+    hipLaunchKernelGGL((k_bp5_ax<T, NQ, NF, EPB, false, WV, VAR, PF>), dim3(blocks), dim3(NQ * NQ * EPB), 0, s,
+                       a.Nelements, a.K, a.elementList, a.GlobalToLocal, a.wJ, a.ggeo, a.gllw, a.D, a.lambda, a.q,
+                       a.Aq, a.ldq, a.ldAq, nullptr, nullptr, Ap, part);
   BPX_HIP(hipGetLastError());
   return BPX_OK;
 }
 
 }  // namespace
 
+// Launch shapes of the one-field kernels, chosen by the setup's timing (all
+// give the same bits): 0 = the default occupancy with one geometry slab in
+// flight; 1 and 2 = 2 and 3 slabs in flight, FP64 with one wave per SIMD
+// fewer (registers), FP32 with one more (it has registers to spare).
+template <typename T, int NQ>
+constexpr int shape_waves(int sh) {
+  constexpr int W0 = bp5_waves<NQ, 1, false>();
+  if (sh == 0) return W0;
+  if (std::is_same<T, float>::value) return W0 + 1;
+  return W0 > 1 ? W0 - 1 : 1;
+}
+
+template <typename T, int NQ, int VAR>
+int launch_shaped(hipStream_t s, const AxArgs<T>& a, T* Ap, double* part) {
+  if (a.affine || a.shape == 0) return launch_bp5<T, NQ, 1, VAR>(s, a, Ap, part);
+  if (a.shape == 1) return launch_bp5<T, NQ, 1, VAR, shape_waves<T, NQ>(1), 2>(s, a, Ap, part);
+  if (a.shape == 2) return launch_bp5<T, NQ, 1, VAR, shape_waves<T, NQ>(2), 3>(s, a, Ap, part);
+  return set_error(BPX_ERR_INVALID_ARGUMENT, "bp5 Ax: launch shape must be 0, 1 or 2");
+}
+
 template <typename T, int NQ, int NF>
 int launch_bp5_nq(hipStream_t s, const AxArgs<T>& a) {
-  return launch_bp5<T, NQ, NF, false>(s, a);
+  if constexpr (NF == 1) return launch_shaped<T, NQ, VAR_PLAIN>(s, a, nullptr, nullptr);
+  return launch_bp5<T, NQ, NF, VAR_PLAIN>(s, a);
 }
 
 template <typename T, int NQ>
 int launch_bp5_krhs_nq(hipStream_t s, const AxArgs<T>& a, int chunk) {
   switch (chunk) {
-    case 1: return launch_bp5<T, NQ, 1, true>(s, a);
-    case 2: return launch_bp5<T, NQ, 2, true>(s, a);
-    case 4: return launch_bp5<T, NQ, 4, true>(s, a);
+    case 1: return launch_bp5<T, NQ, 1, VAR_KRHS>(s, a);
+    case 2: return launch_bp5<T, NQ, 2, VAR_KRHS>(s, a);
+    case 4: return launch_bp5<T, NQ, 4, VAR_KRHS>(s, a);
     default: return set_error(BPX_ERR_INVALID_ARGUMENT, "bp5 k-RHS Ax: chunk must be 1, 2 or 4");
   }
 }
@@ -378,9 +466,9 @@ int launch_batch_v(hipStream_t s, int Nelements, const int* elementList, const i
   constexpr int WV = bp5_waves<NQ, 1, false>();
   const int blocks = (Nelements + EPB - 1) / EPB;
   if (blocks == 0) return BPX_OK;
-  hipLaunchKernelGGL((k_bp5_ax<double, NQ, 1, EPB, false, WV, false, true>), dim3(blocks), dim3(NQ * NQ * EPB), 0,
+  hipLaunchKernelGGL((k_bp5_ax<double, NQ, 1, EPB, false, WV, VAR_BATCH>), dim3(blocks), dim3(NQ * NQ * EPB), 0,
                      s, Nelements, 1, elementList, G2L, wJ, ggeo, (const double*)nullptr, D, lambda, q, Aq, 0LL,
-                     0LL, elemPid, done);
+                     0LL, elemPid, done, (double*)nullptr, (double*)nullptr);
   BPX_HIP(hipGetLastError());
   return BPX_OK;
 }
@@ -407,6 +495,16 @@ template int launch_bp5_batch_nq<BPX_NQ>(hipStream_t, int, int, const int*, cons
                                          const BatchState*);
 template int launch_bp5_kappa_grad_nq<BPX_NQ>(hipStream_t, int, int, const int*, const double*, const double*,
                                               const double*, const double*, const double*, double*);
+// variant = launch shape (see launch_shaped). Same arithmetic, same bits.
+template <typename T, int NQ>
+int launch_bp5_fused_nq(hipStream_t s, const AxArgs<T>& a, T* Ap, double* part, int variant) {
+  AxArgs<T> b = a;
+  b.shape = variant;
+  return launch_shaped<T, NQ, VAR_FUSED>(s, b, Ap, part);
+}
+
+template int launch_bp5_fused_nq<double, BPX_NQ>(hipStream_t, const AxArgs<double>&, double*, double*, int);
+template int launch_bp5_fused_nq<float, BPX_NQ>(hipStream_t, const AxArgs<float>&, float*, double*, int);
 template int launch_bp5_nq<double, BPX_NQ, 1>(hipStream_t, const AxArgs<double>&);
 template int launch_bp5_nq<double, BPX_NQ, 3>(hipStream_t, const AxArgs<double>&);
 template int launch_bp5_nq<float, BPX_NQ, 1>(hipStream_t, const AxArgs<float>&);

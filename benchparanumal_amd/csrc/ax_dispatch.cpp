@@ -93,6 +93,33 @@ int launch_bp5_ax_krhs(hipStream_t s, const AxArgs<T>& a, int chunk) {
 
 template int launch_bp5_ax_krhs<double>(hipStream_t, const AxArgs<double>&, int);
 
+#define BPX_FUSED_CASE(NQ) \
+  case NQ:                  \
+    return launch_bp5_fused_nq<T, NQ>(s, a, Ap, part, variant);
+
+template <typename T>
+int launch_bp5_fused(hipStream_t s, const AxArgs<T>& a, T* Ap, double* part, int variant) {
+  if (int st = check("bp5 fused Ax", a.Nelements, a.q, a.Aq, a.GlobalToLocal, a.elementList)) return st;
+  if (a.Nelements == 0) return BPX_OK;
+  if (a.Nfields != 1 || !Ap || !part) return set_error(BPX_ERR_INVALID_ARGUMENT, "bp5 fused Ax: bad arguments");
+  switch (a.N + 1) {
+    BPX_FUSED_CASE(2) BPX_FUSED_CASE(3) BPX_FUSED_CASE(4) BPX_FUSED_CASE(5) BPX_FUSED_CASE(6)
+    BPX_FUSED_CASE(7) BPX_FUSED_CASE(8) BPX_FUSED_CASE(9) BPX_FUSED_CASE(10) BPX_FUSED_CASE(11)
+    BPX_FUSED_CASE(12) BPX_FUSED_CASE(13) BPX_FUSED_CASE(14) BPX_FUSED_CASE(15) BPX_FUSED_CASE(16)
+    default:
+      return set_error(BPX_ERR_UNSUPPORTED, "bp5 fused Ax: degree N must be in 1..15");
+  }
+}
+
+template int launch_bp5_fused<double>(hipStream_t, const AxArgs<double>&, double*, double*, int);
+template int launch_bp5_fused<float>(hipStream_t, const AxArgs<float>&, float*, double*, int);
+
+int bp5_workgroups(int N, int Nelements) {
+  const int Nq = N + 1;
+  const int epb = Nq >= 8 ? 1 : 256 / (Nq * Nq);
+  return (Nelements + epb - 1) / epb;
+}
+
 #define BPX_BATCH_CASE(NQ) \
   case NQ:                  \
     return launch_bp5_batch_nq<NQ>(s, variant, Nelements, elementList, G2L, wJ, ggeo, D, lambda, q, Aq, elemPid, done);

@@ -50,6 +50,9 @@ struct AxArgs {
   // of Aq at Aq + r*ldAq (one vector after another)
   int K = 1;
   long long ldq = 0, ldAq = 0;
+  // BP5 one-field launch shape (0, 1, 2): occupancy against geometry
+  // slabs in flight; every shape gives the same bits
+  int shape = 0;
 };
 
 // Launchers (defined in the .hip files). Return BPX_OK or an error code.
@@ -57,6 +60,14 @@ template <typename T> int launch_bp5_ax(hipStream_t s, const AxArgs<T>& a);
 // a.K right-hand sides in one launch; chunk = right-hand sides held in
 // registers per sweep (1, 2 or 4, dividing K; 0 picks the default)
 template <typename T> int launch_bp5_ax_krhs(hipStream_t s, const AxArgs<T>& a, int chunk);
+// Fused operator path (one field): a.GlobalToLocal carries kFusedSingle on
+// nodes that are the only copy of their row; their results go to Ap[row]
+// with p.Ap partials in part[bp5_workgroups(N, Nelements)], the others to
+// a.Aq (the local vector) for launch_ogs_gather_dot_rows. variant: 0 or 1,
+// two builds of the same arithmetic.
+constexpr int kFusedSingle = 1 << 30;
+template <typename T> int launch_bp5_fused(hipStream_t s, const AxArgs<T>& a, T* Ap, double* part, int variant);
+int bp5_workgroups(int N, int Nelements);
 template <typename T> int launch_bp1_ax(hipStream_t s, const AxArgs<T>& a);
 template <typename T> int launch_bp3_ax(hipStream_t s, const AxArgs<T>& a);
 
@@ -124,6 +135,13 @@ int launch_ogs_gather_scatter(hipStream_t s, int Nblocks, int K, const int* bloc
 // gather with a fused p.Ap: one FP64 partial per super-block of row
 // blocks, in part[0..n) with n = ogs_gather_partials(Nblocks)
 int ogs_gather_partials(int Nblocks);
+// the same over a subset of rows: map row i of the maps to rows[i] of gq
+// and p; super-block sb also adds axpart[sb*nax/nsb .. (sb+1)*nax/nsb) to
+// its partial (nsb = ogs_gather_partials(Nblocks))
+template <typename T>
+int launch_ogs_gather_dot_rows(hipStream_t s, int Nblocks, const int* blockStarts, const int* rowStarts,
+                               const int* colIds, const int* rows, const T* q, T* gq, const T* p, double* part,
+                               const double* axpart, int nax, int mode = 0);
 template <typename T>
 int launch_ogs_gather_dot(hipStream_t s, int Nblocks, const int* blockStarts, const int* rowStarts,
                           const int* colIds, const T* q, T* gq, const T* p, double* part, int mode = 0);

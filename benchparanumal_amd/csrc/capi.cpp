@@ -26,6 +26,7 @@ struct bpx_bp_s {
   bpx_mesh_s* mesh = nullptr;
   bp_t bp;
   cg* solver = nullptr;
+  deviceBuffer<double> dotPart;  // partials of bpx_bp_operator_dot
   ~bpx_bp_s() { delete solver; }
 };
 struct bpx_batch_s {
@@ -545,8 +546,12 @@ int bpx_bp_info(bpx_bp b, int64_t* out, int capacity) {
 int bpx_bp_tuning(bpx_bp b, double* out, int capacity) {
   if (int st = need(b, "bp")) return st;
   const auto& t = b->bp.tuning;
-  const double v[] = {(double)t.gatherMode, t.gather_us[0], t.gather_us[1]};
-  const int n = 3;
+  const double v[] = {(double)t.gatherMode, t.gather_us[0], t.gather_us[1],
+                      (double)t.axShape, t.ax_us[0], t.ax_us[1], t.ax_us[2],
+                      b->bp.fused ? (double)t.fusedVariant : -1.0, t.fused_us[0], t.fused_us[1], t.fused_us[2],
+                      t.unfused_us,
+                      b->bp.fused ? (double)t.fused32Shape : -1.0, t.fused32_us[0], t.fused32_us[1], t.fused32_us[2]};
+  const int n = 16;
   for (int i = 0; i < n && i < capacity; ++i) out[i] = v[i];
   return n;
 }
@@ -554,6 +559,42 @@ int bpx_bp_tuning(bpx_bp b, double* out, int capacity) {
 int bpx_bp_operator(bpx_bp b, const double* o_q, double* o_Aq) {
   if (int st = need(b, "bp")) return st;
   return b->bp.Operator(o_q, o_Aq);
+}
+
+int bpx_bp_set_tuning(bpx_bp b, int gatherMode, int axShape, int fusedShape, int fused32Shape) {
+  BPX_GUARD_BEGIN
+  if (int st = need(b, "bp")) return st;
+  auto bad = [](int v, int hi) { return v < -1 || v > hi; };
+  if (bad(gatherMode, 1) || bad(axShape, 2) || bad(fusedShape, 2) || bad(fused32Shape, 2))
+    return set_error(BPX_ERR_INVALID_ARGUMENT, "bp: tuning choice out of range");
+  auto& t = b->bp.tuning;
+  if (gatherMode >= 0) t.gatherMode = b->bp.ogs.gatherMode = gatherMode;
+  if (axShape >= 0) t.axShape = axShape;
+  if (fusedShape >= 0) t.fusedVariant = fusedShape;
+  if (fused32Shape >= 0) {
+    if (int st = b->bp.Prepare32()) return st;  // so a later first use does not re-time
+    t.fused32Shape = fused32Shape;
+  }
+  return BPX_OK;
+  BPX_GUARD_END
+}
+
+int bpx_bp_operator_dot(bpx_bp b, const double* o_q, double* o_Aq, double* qAq) {
+  BPX_GUARD_BEGIN
+  if (int st = need(b, "bp")) return st;
+  if (int st = need(qAq, "qAq")) return st;
+  platform_t& P = *b->plat->p;
+  const int n = b->bp.GatherPartials();
+  if (b->dotPart.length() < (size_t)n)
+    if (int st = b->dotPart.alloc(n)) return st;
+  int np = 0;
+  if (int st = b->bp.OperatorDot(o_q, o_Aq, b->dotPart.ptr(), &np)) return st;
+  if (int st = launch_fold(P.stream, b->dotPart.ptr(), np, P.d_scalar)) return st;
+  BPX_HIP(hipMemcpyAsync(P.h_scalar, P.d_scalar, sizeof(double), hipMemcpyDeviceToHost, P.stream));
+  BPX_HIP(hipStreamSynchronize(P.stream));
+  *qAq = *P.h_scalar;
+  return BPX_OK;
+  BPX_GUARD_END
 }
 
 int bpx_bp_ax_local(bpx_bp b, const double* o_q, double* o_AqL) {

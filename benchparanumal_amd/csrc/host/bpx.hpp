@@ -10,6 +10,7 @@
 #include <functional>
 #include <memory>
 #include <string>
+#include <algorithm>
 #include <deque>
 #include <vector>
 
@@ -211,14 +212,30 @@ class bp_t : public solver_t {
   bool HasOperatorDot() const override { return true; }
   int OperatorDot(const dfloat* o_q, dfloat* o_Aq, double* part, int* npart) override;
   int OperatorDot32(const float* o_q, float* o_Aq, double* part, int* npart);
-  int GatherPartials() const { return ogs_gather_partials(ogs.NrowBlocksT); }
+  int GatherPartials() const {
+    return std::max({ogs_gather_partials(ogs.NrowBlocksT), ogs_gather_partials(sharedOgs.NrowBlocksT), 1});
+  }
   // times the launch variants that give identical results (gather
   // workgroup order) on this problem and keeps the fastest
   int Tune();
   struct Tuning {
     int gatherMode = 0;
     double gather_us[2] = {0, 0};
+    int axShape = 0;                        // FP64 Ax launch shape (AxLocal, Operator)
+    double ax_us[3] = {-1, -1, -1};
+    int fusedVariant = 0;                   // FP64 fused operator launch shape (CG)
+    double fused_us[3] = {-1, -1, -1};      // fused operator (Ax + shared-row gather)
+    double unfused_us = -1;                 // Ax + full gather, for comparison
+    int fused32Shape = 0;                   // FP32 fused operator (mixed precision), timed on first use
+    double fused32_us[3] = {-1, -1, -1};
   } tuning;
+  int Tune32();
+  // Fused operator for CG (BP5, one field): single-copy nodes written by
+  // the Ax straight into Aq, shared nodes gathered (see launch_bp5_fused).
+  bool fused = false;
+  int PrepareFused();
+  template <typename T>
+  int OperatorDotFused(const T* o_q, T* o_Aq, double* part, int* npart, int variant);
   int AxLocal(const dfloat* o_q, dfloat* o_AqL);
   int Operator32(const float* o_q, float* o_Aq);
   int Forcing(dfloat* o_b);
@@ -264,6 +281,12 @@ class bp_t : public solver_t {
   deviceBuffer<dlong> o_lattice;
   ogs::ogs_t latticeOgs;
   deviceBuffer<double> o_ggeoK, o_wJK, o_gradL;
+  // fused operator: flagged GlobalToLocal, the shared rows' maps and their
+  // row numbers, the Ax kernel's p.Ap partials
+  deviceBuffer<dlong> o_G2Lf, o_sharedRows;
+  ogs::ogs_t sharedOgs;
+  deviceBuffer<double> o_partAx;
+  int naxParts = 0;
   // FP32 copies (created on first use)
   int Prepare32();
   bool have32 = false;

@@ -5,6 +5,8 @@
 
 #include <algorithm>
 
+#include <type_traits>
+
 #include "bpx.hpp"
 
 namespace bpx {
@@ -257,6 +259,7 @@ int bp_t::Setup(platform_t& p, mesh_t& m, int bp_, dfloat lambda_) {
     if (affine)
       if (int st = o_invV.upload(m.invV, s)) return st;
   }
+  if (int st = PrepareFused()) return st;
   if (int st = Tune()) return st;
   return p.finish();
 }
@@ -281,6 +284,7 @@ int bp_t::ax(const T* q, T* AqL, const T* wJ, const T* ggeo, const T* gllw, cons
   a.lambda = (T)lambda;
   a.q = q;
   a.Aq = AqL;
+  if (std::is_same<T, double>::value && Nfields == 1) a.shape = tuning.axShape;
   if (bp == 5 || bp == 6) return launch_bp5_ax<T>(platform->stream, a);
   if (bp == 1 || bp == 2) return launch_bp1_ax<T>(platform->stream, a);
   return launch_bp3_ax<T>(platform->stream, a);
@@ -346,12 +350,97 @@ int bp_t::Operator(const dfloat* o_q, dfloat* o_Aq) {
 }
 
 int bp_t::OperatorDot(const dfloat* o_q, dfloat* o_Aq, double* part, int* npart) {
+  if (fused) return OperatorDotFused<double>(o_q, o_Aq, part, npart, tuning.fusedVariant);
   if (int st = AxLocal(o_q, o_AqL.ptr())) return st;
-  *npart = GatherPartials();
+  *npart = ogs_gather_partials(ogs.NrowBlocksT);
   return launch_ogs_gather_dot<double>(platform->stream, ogs.NrowBlocksT, ogs.o_blockRowStartsT.ptr(),
                                        ogs.o_rowStartsT.ptr(), ogs.o_colIdsT.ptr(), o_AqL.ptr(), o_Aq, o_q, part,
                                        ogs.gatherMode);
 }
+
+// Flagged GlobalToLocal (single-copy nodes marked) and the maps of the rows
+// with two or more copies, in ascending row order with ascending columns:
+// the shared rows are summed exactly as the full gather sums them, and a
+// single-copy row's sum is its one value, so the fused operator's output
+// equals the unfused one bit for bit.
+int bp_t::PrepareFused() {
+  fused = false;
+  if (bp != 5 || Nfields != 1) return BPX_OK;
+  const dlong Ng = ogs.Ngather;
+  if (Ng >= kFusedSingle) return BPX_OK;  // row numbers must leave the flag bit free
+  std::vector<dlong> sidx(Ng, -1), sharedRows;
+  for (dlong r = 0; r < Ng; ++r)
+    if (ogs.rowStartsT[r + 1] - ogs.rowStartsT[r] >= 2) {
+      sidx[r] = (dlong)sharedRows.size();
+      sharedRows.push_back(r);
+    }
+  std::vector<dlong> G2Lf(NLocal), rowOf(NLocal, -1);
+  for (dlong l = 0; l < NLocal; ++l) {
+    const dlong row = GlobalToLocal[l];
+    if (row < 0) {
+      G2Lf[l] = -1;
+    } else if (sidx[row] < 0) {
+      G2Lf[l] = row | kFusedSingle;
+    } else {
+      G2Lf[l] = row;
+      rowOf[l] = sidx[row];
+    }
+  }
+  if (int st = sharedOgs.SetupRows(NLocal, (dlong)sharedRows.size(), rowOf.data())) return st;
+  if (int st = sharedOgs.Upload(*platform)) return st;
+  hipStream_t s = platform->stream;
+  if (int st = o_G2Lf.upload(G2Lf, s)) return st;
+  if (int st = o_sharedRows.upload(sharedRows, s)) return st;
+  naxParts = bp5_workgroups(mesh->N, mesh->Nelements);
+  if (int st = o_partAx.alloc(std::max(naxParts, 1))) return st;
+  fused = true;
+  return BPX_OK;
+}
+
+template <typename T>
+int bp_t::OperatorDotFused(const T* o_q, T* o_Aq, double* part, int* npart, int variant) {
+  hipStream_t s = platform->stream;
+  AxArgs<T> a;
+  a.N = mesh->N;
+  a.Nelements = mesh->Nelements;
+  a.Nfields = 1;
+  a.elementList = o_elementList.ptr();
+  a.GlobalToLocal = o_G2Lf.ptr();
+  a.lambda = (T)lambda;
+  a.q = o_q;
+  T* AqL;
+  if constexpr (std::is_same<T, double>::value) {
+    a.affine = haveKappa ? 0 : mesh->opts.affine;
+    a.wJ = (haveKappa && mesh->opts.affine) ? o_wJK.ptr() : o_wJ.ptr();
+    a.ggeo = haveKappa ? o_ggeoK.ptr() : o_ggeo.ptr();
+    a.gllw = o_gllw.ptr();
+    a.D = o_D.ptr();
+    AqL = o_AqL.ptr();
+  } else {
+    if (haveKappa) return set_error(BPX_ERR_UNSUPPORTED, "bp: the FP32 operator has no variable coefficient");
+    if (int st = Prepare32()) return st;
+    a.affine = mesh->opts.affine;
+    a.wJ = o_wJ32.ptr();
+    a.ggeo = o_ggeo32.ptr();
+    a.gllw = o_gllw32.ptr();
+    a.D = o_D32.ptr();
+    AqL = o_AqL32.ptr();
+  }
+  a.Aq = AqL;
+  if (int st = launch_bp5_fused<T>(s, a, o_Aq, o_partAx.ptr(), variant)) return st;
+  if (sharedOgs.NrowBlocksT > 0) {
+    *npart = ogs_gather_partials(sharedOgs.NrowBlocksT);
+    return launch_ogs_gather_dot_rows<T>(s, sharedOgs.NrowBlocksT, sharedOgs.o_blockRowStartsT.ptr(),
+                                         sharedOgs.o_rowStartsT.ptr(), sharedOgs.o_colIdsT.ptr(),
+                                         o_sharedRows.ptr(), AqL, o_Aq, o_q, part, o_partAx.ptr(), naxParts,
+                                         ogs.gatherMode);
+  }
+  *npart = 1;  // no shared rows (a single element): the Ax partials are the whole dot
+  return launch_fold(s, o_partAx.ptr(), naxParts, part);
+}
+
+template int bp_t::OperatorDotFused<double>(const double*, double*, double*, int*, int);
+template int bp_t::OperatorDotFused<float>(const float*, float*, double*, int*, int);
 
 int bp_t::AxLocalKrhs(int K, const dfloat* o_q, dfloat* o_AqL, int chunk) {
   if (bp != 5) return set_error(BPX_ERR_UNSUPPORTED, "bp: k right-hand sides are implemented for BP5");
@@ -392,11 +481,15 @@ int bp_t::OperatorKrhs(int K, const dfloat* o_q, dfloat* o_Aq, int chunk) {
 int bp_t::OperatorDot32(const float* o_q, float* o_Aq, double* part, int* npart) {
   if (Nfields != 1) return set_error(BPX_ERR_UNSUPPORTED, "bp: the FP32 operator covers one field (BP1/3/5)");
   if (haveKappa) return set_error(BPX_ERR_UNSUPPORTED, "bp: the FP32 operator has no variable coefficient");
+  if (fused) {
+    if (int st = Prepare32()) return st;
+    return OperatorDotFused<float>(o_q, o_Aq, part, npart, tuning.fused32Shape);
+  }
   if (int st = Prepare32()) return st;
   if (int st = ax<float>(o_q, o_AqL32.ptr(), o_wJ32.ptr(), o_ggeo32.ptr(), o_gllw32.ptr(), o_D32.ptr(),
                          o_I32.ptr(), o_MM32.ptr(), o_invV32.ptr()))
     return st;
-  *npart = GatherPartials();
+  *npart = ogs_gather_partials(ogs.NrowBlocksT);
   return launch_ogs_gather_dot<float>(platform->stream, ogs.NrowBlocksT, ogs.o_blockRowStartsT.ptr(),
                                       ogs.o_rowStartsT.ptr(), ogs.o_colIdsT.ptr(), o_AqL32.ptr(), o_Aq, o_q, part,
                                       ogs.gatherMode);
@@ -439,6 +532,107 @@ int bp_t::Tune() {
   if (st) return st;
   tuning.gatherMode = tuning.gather_us[1] < tuning.gather_us[0] ? 1 : 0;
   ogs.gatherMode = tuning.gatherMode;
+
+  // FP64 Ax launch shapes (BP5/BP1/BP3 one field; only BP5 has shapes)
+  if (bp == 5 && Nfields == 1 && !mesh->opts.affine) {
+    hipEvent_t t0, t1;
+    BPX_HIP(hipEventCreate(&t0));
+    BPX_HIP(hipEventCreate(&t1));
+    for (int sh = 0; sh < 3 && st == BPX_OK; ++sh) {
+      tuning.axShape = sh;
+      if ((st = AxLocal(v.ptr(), o_AqL.ptr()))) break;
+      float best = 1e30f;
+      for (int rep = 0; rep < 3 && st == BPX_OK; ++rep) {
+        float ms = 0;
+        if (hipEventRecord(t0, s) != hipSuccess || (st = AxLocal(v.ptr(), o_AqL.ptr())) ||
+            hipEventRecord(t1, s) != hipSuccess || hipEventSynchronize(t1) != hipSuccess ||
+            hipEventElapsedTime(&ms, t0, t1) != hipSuccess) {
+          if (!st) st = set_error(BPX_ERR_HIP, "tune: event timing failed");
+          break;
+        }
+        best = std::min(best, ms);
+      }
+      tuning.ax_us[sh] = best * 1e3;
+    }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    if (st) return st;
+    tuning.axShape = (int)(std::min_element(tuning.ax_us, tuning.ax_us + 3) - tuning.ax_us);
+  }
+  if (!fused) return BPX_OK;
+
+  // the fused operator's two builds (identical results); the unfused
+  // operator is timed beside them for the record
+  deviceBuffer<double> part2;
+  if (int st2 = part2.alloc(GatherPartials())) return st2;
+  BPX_HIP(hipEventCreate(&e0));
+  BPX_HIP(hipEventCreate(&e1));
+  auto timeit = [&](auto&& fn, double& us) -> int {
+    if (int st2 = fn()) return st2;
+    float best = 1e30f;
+    for (int rep = 0; rep < 3; ++rep) {
+      float ms = 0;
+      if (hipEventRecord(e0, s) != hipSuccess) return set_error(BPX_ERR_HIP, "tune: event");
+      if (int st2 = fn()) return st2;
+      if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+        return set_error(BPX_ERR_HIP, "tune: event timing failed");
+      best = std::min(best, ms);
+    }
+    us = best * 1e3;
+    return BPX_OK;
+  };
+  int np = 0;
+  for (int var = 0; var < 3 && st == BPX_OK; ++var)
+    st = timeit([&] { return OperatorDotFused<double>(v.ptr(), v.ptr() + Ng, part2.ptr(), &np, var); },
+                tuning.fused_us[var]);
+  if (st == BPX_OK) {
+    fused = false;
+    st = timeit([&] { return OperatorDot(v.ptr(), v.ptr() + Ng, part2.ptr(), &np); }, tuning.unfused_us);
+    fused = true;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (st) return st;
+  tuning.fusedVariant = (int)(std::min_element(tuning.fused_us, tuning.fused_us + 3) - tuning.fused_us);
+  return BPX_OK;
+}
+
+// FP32 fused operator shapes, timed when the FP32 operator is first built
+int bp_t::Tune32() {
+  if (!fused) return BPX_OK;
+  hipStream_t s = platform->stream;
+  const dlong Ng = ogs.Ngather;
+  if (Ng == 0) return BPX_OK;
+  deviceBuffer<float> v;
+  deviceBuffer<double> part;
+  if (int st = v.alloc(2 * (size_t)Ng)) return st;
+  if (int st = part.alloc(GatherPartials())) return st;
+  BPX_HIP(hipMemsetAsync(v.ptr(), 0, 2 * (size_t)Ng * sizeof(float), s));
+  hipEvent_t e0, e1;
+  BPX_HIP(hipEventCreate(&e0));
+  BPX_HIP(hipEventCreate(&e1));
+  int st = BPX_OK, np = 0;
+  for (int sh = 0; sh < 3 && st == BPX_OK; ++sh) {
+    if ((st = OperatorDotFused<float>(v.ptr(), v.ptr() + Ng, part.ptr(), &np, sh))) break;
+    float best = 1e30f;
+    for (int rep = 0; rep < 3 && st == BPX_OK; ++rep) {
+      float ms = 0;
+      if (hipEventRecord(e0, s) != hipSuccess ||
+          (st = OperatorDotFused<float>(v.ptr(), v.ptr() + Ng, part.ptr(), &np, sh)) ||
+          hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+        if (!st) st = set_error(BPX_ERR_HIP, "tune: event timing failed");
+        break;
+      }
+      best = std::min(best, ms);
+    }
+    tuning.fused32_us[sh] = best * 1e3;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (st) return st;
+  tuning.fused32Shape = (int)(std::min_element(tuning.fused32_us, tuning.fused32_us + 3) - tuning.fused32_us);
   return BPX_OK;
 }
 
@@ -468,7 +662,8 @@ int bp_t::Prepare32() {
   if ((st = o_AqL32.alloc(NLocal))) return st;
   if ((st = platform->finish())) return st;
   have32 = true;
-  return BPX_OK;
+  if ((st = Tune32())) return st;
+  return platform->finish();
 }
 
 int bp_t::Operator32(const float* o_q, float* o_Aq) {

@@ -49,12 +49,16 @@ __device__ inline int gather_sb(int mode, int w, int it, int G, int nsb) {
 // issues every global load at once (column ids, the gathered values into
 // LDS, and for the lane's own rows their rowStarts and p), phase 2 sums
 // rows out of LDS in column order.
-template <typename T, bool DOT>
+// ROWS: the maps cover a subset of the rows; map row i is row rows[i] of
+// gq and p, and super-block sb also folds its share of the fused Ax
+// kernel's partials (axpart) into its own.
+template <typename T, bool DOT, bool ROWS>
 __global__ __launch_bounds__(kThreads) void k_gather(int Nblocks, int mode, const int* __restrict__ blockStarts,
                                                      const int* __restrict__ rowStarts,
                                                      const int* __restrict__ colIds, const T* __restrict__ q,
                                                      T* __restrict__ gq, const T* __restrict__ p,
-                                                     double* __restrict__ part) {
+                                                     double* __restrict__ part, const int* __restrict__ rows,
+                                                     const double* __restrict__ axpart, int nax) {
   constexpr int RPL = kGatherNnz / kThreads;  // rows per lane when the rows fit the staging
   __shared__ T s_v[kGatherNnz];
   __shared__ double s_w[kThreads / 64];
@@ -79,7 +83,7 @@ __global__ __launch_bounds__(kThreads) void k_gather(int Nblocks, int mode, cons
         if (row < r1) {
           ra[m] = rowStarts[row];
           re[m] = rowStarts[row + 1];
-          if (DOT) pr[m] = p[row];
+          if (DOT) pr[m] = p[ROWS ? rows[row] : row];
         }
       }
     }
@@ -92,7 +96,7 @@ __global__ __launch_bounds__(kThreads) void k_gather(int Nblocks, int mode, cons
         if (row < r1) {
           T acc = T(0);
           for (int i = ra[m] - g0; i < re[m] - g0; ++i) acc += s_v[i];
-          gq[row] = acc;
+          gq[ROWS ? rows[row] : row] = acc;
           if (DOT) dot += (double)pr[m] * (double)acc;
         }
       }
@@ -100,8 +104,9 @@ __global__ __launch_bounds__(kThreads) void k_gather(int Nblocks, int mode, cons
       for (int row = r0 + threadIdx.x; row < r1; row += kThreads) {
         T acc = T(0);
         for (int g = rowStarts[row]; g < rowStarts[row + 1]; ++g) acc += q[colIds[g]];
-        gq[row] = acc;
-        if (DOT) dot += (double)p[row] * (double)acc;
+        const int grow = ROWS ? rows[row] : row;
+        gq[grow] = acc;
+        if (DOT) dot += (double)p[grow] * (double)acc;
       }
     }
     if (DOT) {
@@ -113,6 +118,10 @@ __global__ __launch_bounds__(kThreads) void k_gather(int Nblocks, int mode, cons
     if (DOT && threadIdx.x == 0) {
       double t = 0;
       for (int i = 0; i < kThreads / 64; ++i) t += s_w[i];
+      if (ROWS) {
+        const int a0 = (int)((long long)sb * nax / nsb), a1 = (int)((long long)(sb + 1) * nax / nsb);
+        for (int i = a0; i < a1; ++i) t += axpart[i];
+      }
       part[sb] = t;
     }
   }
@@ -235,8 +244,9 @@ int launch_ogs_gather(hipStream_t s, int Nblocks, int K, const int* blockStarts,
     hipLaunchKernelGGL(k_gather_k<T>, dim3(super_blocks(Nblocks)), dim3(kThreads), 0, s, Nblocks, K,
                        blockStarts, rowStarts, colIds, q, gq);
   else
-    hipLaunchKernelGGL((k_gather<T, false>), dim3(gather_grid(Nblocks)), dim3(kThreads), 0, s, Nblocks, mode,
-                       blockStarts, rowStarts, colIds, q, gq, (const T*)nullptr, (double*)nullptr);
+    hipLaunchKernelGGL((k_gather<T, false, false>), dim3(gather_grid(Nblocks)), dim3(kThreads), 0, s, Nblocks,
+                       mode, blockStarts, rowStarts, colIds, q, gq, (const T*)nullptr, (double*)nullptr,
+                       (const int*)nullptr, (const double*)nullptr, 0);
   BPX_HIP(hipGetLastError());
   return BPX_OK;
 }
@@ -246,8 +256,21 @@ int launch_ogs_gather_dot(hipStream_t s, int Nblocks, const int* blockStarts, co
                           const int* colIds, const T* q, T* gq, const T* p, double* part, int mode) {
   if (int st = check(Nblocks, 1, blockStarts, rowStarts, colIds, q, gq)) return st;
   if (Nblocks == 0) return BPX_OK;
-  hipLaunchKernelGGL((k_gather<T, true>), dim3(gather_grid(Nblocks)), dim3(kThreads), 0, s, Nblocks, mode,
-                     blockStarts, rowStarts, colIds, q, gq, p, part);
+  hipLaunchKernelGGL((k_gather<T, true, false>), dim3(gather_grid(Nblocks)), dim3(kThreads), 0, s, Nblocks, mode,
+                     blockStarts, rowStarts, colIds, q, gq, p, part, (const int*)nullptr, (const double*)nullptr, 0);
+  BPX_HIP(hipGetLastError());
+  return BPX_OK;
+}
+
+template <typename T>
+int launch_ogs_gather_dot_rows(hipStream_t s, int Nblocks, const int* blockStarts, const int* rowStarts,
+                               const int* colIds, const int* rows, const T* q, T* gq, const T* p, double* part,
+                               const double* axpart, int nax, int mode) {
+  if (int st = check(Nblocks, 1, blockStarts, rowStarts, colIds, q, gq)) return st;
+  if (Nblocks == 0) return BPX_OK;
+  if (!rows || !part || (nax > 0 && !axpart)) return set_error(BPX_ERR_INVALID_ARGUMENT, "ogs rows: null pointer");
+  hipLaunchKernelGGL((k_gather<T, true, true>), dim3(gather_grid(Nblocks)), dim3(kThreads), 0, s, Nblocks, mode,
+                     blockStarts, rowStarts, colIds, q, gq, p, part, rows, axpart, nax);
   BPX_HIP(hipGetLastError());
   return BPX_OK;
 }
@@ -286,6 +309,12 @@ template int launch_ogs_gather_dot<double>(hipStream_t, int, const int*, const i
                                            double*, const double*, double*, int);
 template int launch_ogs_gather_dot<float>(hipStream_t, int, const int*, const int*, const int*, const float*,
                                           float*, const float*, double*, int);
+template int launch_ogs_gather_dot_rows<double>(hipStream_t, int, const int*, const int*, const int*, const int*,
+                                                const double*, double*, const double*, double*, const double*,
+                                                int, int);
+template int launch_ogs_gather_dot_rows<float>(hipStream_t, int, const int*, const int*, const int*, const int*,
+                                               const float*, float*, const float*, double*, const double*, int,
+                                               int);
 template int launch_ogs_scatter<double>(hipStream_t, int, int, const int*, const int*, const int*,
                                         const double*, double*);
 template int launch_ogs_scatter<float>(hipStream_t, int, int, const int*, const int*, const int*,

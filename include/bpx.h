@@ -258,13 +258,28 @@ int bpx_bp_destroy(bpx_bp b);
  * points of the node lattice, (nx*N+1)(ny*N+1)(nz*N+1)) */
 int bpx_bp_info(bpx_bp b, int64_t* out, int capacity);
 /* No reference counterpart: launch choices the setup timed on this
- * problem, all giving identical results. out: [gather workgroup order
- * (0 strided, 1 XCD-blocked), its time in us for order 0, for order 1].
- * Returns the number of values. */
+ * problem, all giving identical results (times in us, -1 = not timed).
+ * out[0..2]: gather workgroup order (0 strided, 1 XCD-blocked), time of
+ * order 0, of order 1; out[3..6]: FP64 Ax launch shape, time of shapes
+ * 0, 1, 2; out[7..11]: shape of the fused CG operator (-1 = not fused),
+ * time of shapes 0, 1, 2, time of the unfused operator; out[12..15]: shape
+ * of the FP32 fused operator (timed when the FP32 operator is first used),
+ * time of shapes 0, 1, 2. Returns the number of values. */
 int bpx_bp_tuning(bpx_bp b, double* out, int capacity);
+/* No reference counterpart: pin launch choices instead of the timed ones
+ * (for reproducible timings; results do not depend on them): gather order
+ * 0/1, FP64 Ax shape 0-2, fused FP64 operator shape 0-2, fused FP32
+ * operator shape 0-2; -1 keeps a choice. */
+int bpx_bp_set_tuning(bpx_bp b, int gatherMode, int axShape, int fusedShape, int fused32Shape);
 /* bpN_t::Operator(o_q, o_Aq) (BP/BP5/src/bp5Operator.cpp:29-85): Ax over the
  * local elements, then the T gather. Device vectors of length Ngather. */
 int bpx_bp_operator(bpx_bp b, const double* o_q, double* o_Aq);
+/* The operator as CG applies it: cg::Solve's Operator followed by
+ * innerProd(p, Ap) (libs/core/linearSolverCG.cpp:99-106) in one pass. For
+ * BP5 the Ax kernel writes nodes that are the only copy of their global row
+ * straight into o_Aq and the gather sums only the shared rows; o_Aq equals
+ * bpx_bp_operator's bit for bit. *qAq (host) = q.Aq. Synchronises. */
+int bpx_bp_operator_dot(bpx_bp b, const double* o_q, double* o_Aq, double* qAq);
 /* Ax alone into the local vector (BK/BK5/src/bk5Run.cpp:59-71). */
 int bpx_bp_ax_local(bpx_bp b, const double* o_q, double* o_AqL);
 /* FP32 operator on the same mesh (factors rounded to float). */
