@@ -73,7 +73,7 @@ _SIGS = {
     "bpx_bp_operator_krhs": (_I, [_P, _I, _I, _P, _P]),
     "bpx_bp_tuning": (_I, [_P, _P, _I]),
     "bpx_bp_operator_dot": (_I, [_P, _P, _P, C.POINTER(_D)]),
-    "bpx_bp_set_tuning": (_I, [_P, _I, _I, _I, _I]),
+    "bpx_bp_set_tuning": (_I, [_P, _I, _I, _I]),
     "bpx_bp_set_kappa": (_I, [_P, _P]),
     "bpx_batch_create": (_I, [_P, _I, _P, _D, C.POINTER(_P)]),
     "bpx_batch_destroy": (_I, [_P]),
@@ -282,16 +282,15 @@ class BP:
 
     def tuning(self):
         """Launch choices timed at setup (identical results whichever is chosen)."""
-        out = np.zeros(16)
-        lib().bpx_bp_tuning(self.h, ptr(out), 16)
+        out = np.zeros(9)
+        lib().bpx_bp_tuning(self.h, ptr(out), 9)
         f = [float(v) for v in out]
-        return {"gather_mode": int(f[0]), "gather_us": f[1:3], "ax_shape": int(f[3]), "ax_us": f[4:7],
-                "fused_build": int(f[7]), "fused_us": f[8:11], "unfused_us": f[11],
-                "fused32_shape": int(f[12]), "fused32_us": f[13:16]}
+        return {"gather_mode": int(f[0]), "gather_us": f[1:3], "fused": int(f[3]), "fused_us": f[4],
+                "unfused_us": f[5], "fused32": int(f[6]), "fused32_us": f[7], "unfused32_us": f[8]}
 
-    def set_tuning(self, gather_mode=-1, ax_shape=-1, fused_shape=-1, fused32_shape=-1):
+    def set_tuning(self, gather_mode=-1, fused=-1, fused32=-1):
         """Pin launch choices (-1 keeps the timed one)."""
-        check(lib().bpx_bp_set_tuning(self.h, gather_mode, ax_shape, fused_shape, fused32_shape))
+        check(lib().bpx_bp_set_tuning(self.h, gather_mode, fused, fused32))
 
     def operator_dot(self, q, Aq):
         """Aq = A q as CG applies it (fused for BP5); returns q.Aq."""

@@ -79,9 +79,7 @@ constexpr bool bp5_dreg() {
 enum { VAR_PLAIN = 0, VAR_KRHS = 1, VAR_BATCH = 2, VAR_FUSED = 3 };
 constexpr int kSingleFlag = 1 << 30;
 
-// PF: non-affine geometry slabs in flight (slab k+PF loads while slab k
-// computes); more depth trades registers (occupancy) for bytes in flight.
-template <typename T, int NQ, int NF, int EPB, bool AFFINE, int WAVES, int VAR, int PF = 1>
+template <typename T, int NQ, int NF, int EPB, bool AFFINE, int WAVES, int VAR>
 __global__ __launch_bounds__(NQ* NQ* EPB) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_bp5_ax(
     int Nelements, int K, const int* __restrict__ elementList, const int* __restrict__ G2L,
     const T* __restrict__ wJ, const T* __restrict__ ggeo, const T* __restrict__ gllw,
@@ -158,25 +156,18 @@ __global__ __launch_bounds__(NQ* NQ* EPB) __attribute__((amdgpu_waves_per_eu(WAV
       }
   }
 
-  // geometry: affine, one set per element; non-affine, a ring of PF slabs
-  // (slab k in ring[k % PF], slab k+PF loads while slab k computes)
+  // geometry of the current slab (non-affine: prefetched one slab ahead)
   T g0 = 0, g1 = 0, g2 = 0, g3 = 0, g4 = 0, g5 = 0, gw = 0;
   T wij = 0;
-  T ring[AFFINE ? 1 : PF][7];
   if (AFFINE) {
     const T* ge = ggeo + (size_t)6 * e;
     g0 = ge[0]; g1 = ge[1]; g2 = ge[2]; g3 = ge[3]; g4 = ge[4]; g5 = ge[5];
     gw = wJ[e];
     wij = gllw[i] * gllw[j];
   } else {
-#pragma unroll
-    for (int d = 0; d < PF && d < NQ; ++d) {
-      const size_t gb = (size_t)e * NP + d * NQ2 + ij;
-      const T* gp = ggeo + 6 * gb;
-#pragma unroll
-      for (int c = 0; c < 6; ++c) ring[d][c] = gp[c];
-      ring[d][6] = wJ[gb];
-    }
+    const T* gp = ggeo + 6 * ((size_t)e * NP + ij);
+    g0 = gp[0]; g1 = gp[1]; g2 = gp[2]; g3 = gp[3]; g4 = gp[4]; g5 = gp[5];
+    gw = wJ[(size_t)e * NP + ij];
   }
   __syncthreads();
 
@@ -206,16 +197,12 @@ __global__ __launch_bounds__(NQ* NQ* EPB) __attribute__((amdgpu_waves_per_eu(WAV
       G00 = g0 * w; G01 = g1 * w; G02 = g2 * w; G11 = g3 * w; G12 = g4 * w; G22 = g5 * w;
       W = gw * w;
     } else {
-      constexpr int RS = AFFINE ? 1 : PF;
-      const int slot = k % RS;
-      G00 = ring[slot][0]; G01 = ring[slot][1]; G02 = ring[slot][2];
-      G11 = ring[slot][3]; G12 = ring[slot][4]; G22 = ring[slot][5]; W = ring[slot][6];
-      if (k + PF < NQ) {
-        const size_t gb = (size_t)e * NP + (k + PF) * NQ2 + ij;
+      G00 = g0; G01 = g1; G02 = g2; G11 = g3; G12 = g4; G22 = g5; W = gw;
+      if (k + 1 < NQ) {
+        const size_t gb = (size_t)e * NP + (k + 1) * NQ2 + ij;
         const T* gp = ggeo + 6 * gb;
-#pragma unroll
-        for (int c = 0; c < 6; ++c) ring[slot][c] = gp[c];
-        ring[slot][6] = wJ[gb];
+        g0 = gp[0]; g1 = gp[1]; g2 = gp[2]; g3 = gp[3]; g4 = gp[4]; g5 = gp[5];
+        gw = wJ[gb];
       }
     }
 #pragma unroll
@@ -376,10 +363,10 @@ __global__ __launch_bounds__(NQ* NQ* EPB) void k_bp5_kappa_grad(int Nelements, c
   }
 }
 
-template <typename T, int NQ, int NF, int VAR, int WVSET = 0, int PF = 1>
+template <typename T, int NQ, int NF, int VAR>
 int launch_bp5(hipStream_t s, const AxArgs<T>& a, T* Ap = nullptr, double* part = nullptr) {
   constexpr int EPB = bp5_epb<NQ>();
-  constexpr int WV = WVSET > 0 ? WVSET : bp5_waves<NQ, NF, VAR == VAR_KRHS>();
+  constexpr int WV = bp5_waves<NQ, NF, VAR == VAR_KRHS>();
   const int blocks = (a.Nelements + EPB - 1) / EPB;
   if (blocks == 0) return BPX_OK;
   if (a.affine)
@@ -387,7 +374,7 @@ int launch_bp5(hipStream_t s, const AxArgs<T>& a, T* Ap = nullptr, double* part 
                        a.Nelements, a.K, a.elementList, a.GlobalToLocal, a.wJ, a.ggeo, a.gllw, a.D, a.lambda, a.q,
                        a.Aq, a.ldq, a.ldAq, nullptr, nullptr, Ap, part);
   else
-    hipLaunchKernelGGL((k_bp5_ax<T, NQ, NF, EPB, false, WV, VAR, PF>), dim3(blocks), dim3(NQ * NQ * EPB), 0, s,
+    hipLaunchKernelGGL((k_bp5_ax<T, NQ, NF, EPB, false, WV, VAR>), dim3(blocks), dim3(NQ * NQ * EPB), 0, s,
                        a.Nelements, a.K, a.elementList, a.GlobalToLocal, a.wJ, a.ggeo, a.gllw, a.D, a.lambda, a.q,
                        a.Aq, a.ldq, a.ldAq, nullptr, nullptr, Ap, part);
   BPX_HIP(hipGetLastError());
@@ -396,29 +383,8 @@ int launch_bp5(hipStream_t s, const AxArgs<T>& a, T* Ap = nullptr, double* part 
 
 }  // namespace
 
-// Launch shapes of the one-field kernels, chosen by the setup's timing (all
-// give the same bits): 0 = the default occupancy with one geometry slab in
-// flight; 1 and 2 = 2 and 3 slabs in flight, FP64 with one wave per SIMD
-// fewer (registers), FP32 with one more (it has registers to spare).
-template <typename T, int NQ>
-constexpr int shape_waves(int sh) {
-  constexpr int W0 = bp5_waves<NQ, 1, false>();
-  if (sh == 0) return W0;
-  if (std::is_same<T, float>::value) return W0 + 1;
-  return W0 > 1 ? W0 - 1 : 1;
-}
-
-template <typename T, int NQ, int VAR>
-int launch_shaped(hipStream_t s, const AxArgs<T>& a, T* Ap, double* part) {
-  if (a.affine || a.shape == 0) return launch_bp5<T, NQ, 1, VAR>(s, a, Ap, part);
-  if (a.shape == 1) return launch_bp5<T, NQ, 1, VAR, shape_waves<T, NQ>(1), 2>(s, a, Ap, part);
-  if (a.shape == 2) return launch_bp5<T, NQ, 1, VAR, shape_waves<T, NQ>(2), 3>(s, a, Ap, part);
-  return set_error(BPX_ERR_INVALID_ARGUMENT, "bp5 Ax: launch shape must be 0, 1 or 2");
-}
-
 template <typename T, int NQ, int NF>
 int launch_bp5_nq(hipStream_t s, const AxArgs<T>& a) {
-  if constexpr (NF == 1) return launch_shaped<T, NQ, VAR_PLAIN>(s, a, nullptr, nullptr);
   return launch_bp5<T, NQ, NF, VAR_PLAIN>(s, a);
 }
 
@@ -495,16 +461,13 @@ template int launch_bp5_batch_nq<BPX_NQ>(hipStream_t, int, int, const int*, cons
                                          const BatchState*);
 template int launch_bp5_kappa_grad_nq<BPX_NQ>(hipStream_t, int, int, const int*, const double*, const double*,
                                               const double*, const double*, const double*, double*);
-// variant = launch shape (see launch_shaped). Same arithmetic, same bits.
 template <typename T, int NQ>
-int launch_bp5_fused_nq(hipStream_t s, const AxArgs<T>& a, T* Ap, double* part, int variant) {
-  AxArgs<T> b = a;
-  b.shape = variant;
-  return launch_shaped<T, NQ, VAR_FUSED>(s, b, Ap, part);
+int launch_bp5_fused_nq(hipStream_t s, const AxArgs<T>& a, T* Ap, double* part) {
+  return launch_bp5<T, NQ, 1, VAR_FUSED>(s, a, Ap, part);
 }
 
-template int launch_bp5_fused_nq<double, BPX_NQ>(hipStream_t, const AxArgs<double>&, double*, double*, int);
-template int launch_bp5_fused_nq<float, BPX_NQ>(hipStream_t, const AxArgs<float>&, float*, double*, int);
+template int launch_bp5_fused_nq<double, BPX_NQ>(hipStream_t, const AxArgs<double>&, double*, double*);
+template int launch_bp5_fused_nq<float, BPX_NQ>(hipStream_t, const AxArgs<float>&, float*, double*);
 template int launch_bp5_nq<double, BPX_NQ, 1>(hipStream_t, const AxArgs<double>&);
 template int launch_bp5_nq<double, BPX_NQ, 3>(hipStream_t, const AxArgs<double>&);
 template int launch_bp5_nq<float, BPX_NQ, 1>(hipStream_t, const AxArgs<float>&);

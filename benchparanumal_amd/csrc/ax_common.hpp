@@ -8,7 +8,7 @@ namespace bpx {
 
 template <typename T, int NQ, int NF> int launch_bp5_nq(hipStream_t s, const AxArgs<T>& a);
 template <typename T, int NQ>
-int launch_bp5_fused_nq(hipStream_t s, const AxArgs<T>& a, T* Ap, double* part, int variant);
+int launch_bp5_fused_nq(hipStream_t s, const AxArgs<T>& a, T* Ap, double* part);
 template <typename T, int NQ> int launch_bp5_krhs_nq(hipStream_t s, const AxArgs<T>& a, int chunk);
 template <int NQ>
 int launch_bp5_batch_nq(hipStream_t s, int variant, int Nelements, const int* elementList, const int* G2L,

@@ -95,10 +95,10 @@ template int launch_bp5_ax_krhs<double>(hipStream_t, const AxArgs<double>&, int)
 
 #define BPX_FUSED_CASE(NQ) \
   case NQ:                  \
-    return launch_bp5_fused_nq<T, NQ>(s, a, Ap, part, variant);
+    return launch_bp5_fused_nq<T, NQ>(s, a, Ap, part);
 
 template <typename T>
-int launch_bp5_fused(hipStream_t s, const AxArgs<T>& a, T* Ap, double* part, int variant) {
+int launch_bp5_fused(hipStream_t s, const AxArgs<T>& a, T* Ap, double* part) {
   if (int st = check("bp5 fused Ax", a.Nelements, a.q, a.Aq, a.GlobalToLocal, a.elementList)) return st;
   if (a.Nelements == 0) return BPX_OK;
   if (a.Nfields != 1 || !Ap || !part) return set_error(BPX_ERR_INVALID_ARGUMENT, "bp5 fused Ax: bad arguments");
@@ -111,8 +111,8 @@ int launch_bp5_fused(hipStream_t s, const AxArgs<T>& a, T* Ap, double* part, int
   }
 }
 
-template int launch_bp5_fused<double>(hipStream_t, const AxArgs<double>&, double*, double*, int);
-template int launch_bp5_fused<float>(hipStream_t, const AxArgs<float>&, float*, double*, int);
+template int launch_bp5_fused<double>(hipStream_t, const AxArgs<double>&, double*, double*);
+template int launch_bp5_fused<float>(hipStream_t, const AxArgs<float>&, float*, double*);
 
 int bp5_workgroups(int N, int Nelements) {
   const int Nq = N + 1;

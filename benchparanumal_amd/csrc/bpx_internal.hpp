@@ -50,9 +50,6 @@ struct AxArgs {
   // of Aq at Aq + r*ldAq (one vector after another)
   int K = 1;
   long long ldq = 0, ldAq = 0;
-  // BP5 one-field launch shape (0, 1, 2): occupancy against geometry
-  // slabs in flight; every shape gives the same bits
-  int shape = 0;
 };
 
 // Launchers (defined in the .hip files). Return BPX_OK or an error code.
@@ -63,10 +60,9 @@ template <typename T> int launch_bp5_ax_krhs(hipStream_t s, const AxArgs<T>& a, 
 // Fused operator path (one field): a.GlobalToLocal carries kFusedSingle on
 // nodes that are the only copy of their row; their results go to Ap[row]
 // with p.Ap partials in part[bp5_workgroups(N, Nelements)], the others to
-// a.Aq (the local vector) for launch_ogs_gather_dot_rows. variant: 0 or 1,
-// two builds of the same arithmetic.
+// a.Aq (the local vector) for launch_ogs_gather_dot_rows.
 constexpr int kFusedSingle = 1 << 30;
-template <typename T> int launch_bp5_fused(hipStream_t s, const AxArgs<T>& a, T* Ap, double* part, int variant);
+template <typename T> int launch_bp5_fused(hipStream_t s, const AxArgs<T>& a, T* Ap, double* part);
 int bp5_workgroups(int N, int Nelements);
 template <typename T> int launch_bp1_ax(hipStream_t s, const AxArgs<T>& a);
 template <typename T> int launch_bp3_ax(hipStream_t s, const AxArgs<T>& a);

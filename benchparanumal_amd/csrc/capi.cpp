@@ -547,11 +547,9 @@ int bpx_bp_tuning(bpx_bp b, double* out, int capacity) {
   if (int st = need(b, "bp")) return st;
   const auto& t = b->bp.tuning;
   const double v[] = {(double)t.gatherMode, t.gather_us[0], t.gather_us[1],
-                      (double)t.axShape, t.ax_us[0], t.ax_us[1], t.ax_us[2],
-                      b->bp.fused ? (double)t.fusedVariant : -1.0, t.fused_us[0], t.fused_us[1], t.fused_us[2],
-                      t.unfused_us,
-                      b->bp.fused ? (double)t.fused32Shape : -1.0, t.fused32_us[0], t.fused32_us[1], t.fused32_us[2]};
-  const int n = 16;
+                      b->bp.fused ? (double)t.fused64 : -1.0, t.fused_us, t.unfused_us,
+                      b->bp.fused ? (double)t.fused32 : -1.0, t.fused32_us, t.unfused32_us};
+  const int n = 9;
   for (int i = 0; i < n && i < capacity; ++i) out[i] = v[i];
   return n;
 }
@@ -561,19 +559,20 @@ int bpx_bp_operator(bpx_bp b, const double* o_q, double* o_Aq) {
   return b->bp.Operator(o_q, o_Aq);
 }
 
-int bpx_bp_set_tuning(bpx_bp b, int gatherMode, int axShape, int fusedShape, int fused32Shape) {
+int bpx_bp_set_tuning(bpx_bp b, int gatherMode, int fused, int fused32) {
   BPX_GUARD_BEGIN
   if (int st = need(b, "bp")) return st;
-  auto bad = [](int v, int hi) { return v < -1 || v > hi; };
-  if (bad(gatherMode, 1) || bad(axShape, 2) || bad(fusedShape, 2) || bad(fused32Shape, 2))
+  auto bad = [](int v) { return v < -1 || v > 1; };
+  if (bad(gatherMode) || bad(fused) || bad(fused32))
     return set_error(BPX_ERR_INVALID_ARGUMENT, "bp: tuning choice out of range");
+  if ((fused == 1 || fused32 == 1) && !b->bp.fused)
+    return set_error(BPX_ERR_UNSUPPORTED, "bp: no fused operator for this problem (BP5, one field)");
   auto& t = b->bp.tuning;
   if (gatherMode >= 0) t.gatherMode = b->bp.ogs.gatherMode = gatherMode;
-  if (axShape >= 0) t.axShape = axShape;
-  if (fusedShape >= 0) t.fusedVariant = fusedShape;
-  if (fused32Shape >= 0) {
+  if (fused >= 0) t.fused64 = fused;
+  if (fused32 >= 0) {
     if (int st = b->bp.Prepare32()) return st;  // so a later first use does not re-time
-    t.fused32Shape = fused32Shape;
+    t.fused32 = fused32;
   }
   return BPX_OK;
   BPX_GUARD_END

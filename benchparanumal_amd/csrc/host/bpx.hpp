@@ -215,27 +215,26 @@ class bp_t : public solver_t {
   int GatherPartials() const {
     return std::max({ogs_gather_partials(ogs.NrowBlocksT), ogs_gather_partials(sharedOgs.NrowBlocksT), 1});
   }
-  // times the launch variants that give identical results (gather
-  // workgroup order) on this problem and keeps the fastest
+  // times the launch choices that give identical results (gather
+  // workgroup order; fused or unfused CG operator) and keeps the fastest
   int Tune();
   struct Tuning {
     int gatherMode = 0;
     double gather_us[2] = {0, 0};
-    int axShape = 0;                        // FP64 Ax launch shape (AxLocal, Operator)
-    double ax_us[3] = {-1, -1, -1};
-    int fusedVariant = 0;                   // FP64 fused operator launch shape (CG)
-    double fused_us[3] = {-1, -1, -1};      // fused operator (Ax + shared-row gather)
-    double unfused_us = -1;                 // Ax + full gather, for comparison
-    int fused32Shape = 0;                   // FP32 fused operator (mixed precision), timed on first use
-    double fused32_us[3] = {-1, -1, -1};
+    int fused64 = 0;             // FP64 CG operator: 1 = fused Ax + shared-row gather, 0 = Ax + full gather
+    double fused_us = -1, unfused_us = -1;
+    int fused32 = 0;             // the same choice for the FP32 operator, timed on its first use
+    double fused32_us = -1, unfused32_us = -1;
   } tuning;
   int Tune32();
   // Fused operator for CG (BP5, one field): single-copy nodes written by
   // the Ax straight into Aq, shared nodes gathered (see launch_bp5_fused).
+  // fused = the maps are built; tuning.fused64/fused32 = it is used.
   bool fused = false;
   int PrepareFused();
   template <typename T>
-  int OperatorDotFused(const T* o_q, T* o_Aq, double* part, int* npart, int variant);
+  int OperatorDotFused(const T* o_q, T* o_Aq, double* part, int* npart);
+  int OperatorDotUnfused32(const float* o_q, float* o_Aq, double* part, int* npart);
   int AxLocal(const dfloat* o_q, dfloat* o_AqL);
   int Operator32(const float* o_q, float* o_Aq);
   int Forcing(dfloat* o_b);

@@ -284,7 +284,6 @@ int bp_t::ax(const T* q, T* AqL, const T* wJ, const T* ggeo, const T* gllw, cons
   a.lambda = (T)lambda;
   a.q = q;
   a.Aq = AqL;
-  if (std::is_same<T, double>::value && Nfields == 1) a.shape = tuning.axShape;
   if (bp == 5 || bp == 6) return launch_bp5_ax<T>(platform->stream, a);
   if (bp == 1 || bp == 2) return launch_bp1_ax<T>(platform->stream, a);
   return launch_bp3_ax<T>(platform->stream, a);
@@ -350,7 +349,7 @@ int bp_t::Operator(const dfloat* o_q, dfloat* o_Aq) {
 }
 
 int bp_t::OperatorDot(const dfloat* o_q, dfloat* o_Aq, double* part, int* npart) {
-  if (fused) return OperatorDotFused<double>(o_q, o_Aq, part, npart, tuning.fusedVariant);
+  if (fused && tuning.fused64) return OperatorDotFused<double>(o_q, o_Aq, part, npart);
   if (int st = AxLocal(o_q, o_AqL.ptr())) return st;
   *npart = ogs_gather_partials(ogs.NrowBlocksT);
   return launch_ogs_gather_dot<double>(platform->stream, ogs.NrowBlocksT, ogs.o_blockRowStartsT.ptr(),
@@ -398,7 +397,7 @@ int bp_t::PrepareFused() {
 }
 
 template <typename T>
-int bp_t::OperatorDotFused(const T* o_q, T* o_Aq, double* part, int* npart, int variant) {
+int bp_t::OperatorDotFused(const T* o_q, T* o_Aq, double* part, int* npart) {
   hipStream_t s = platform->stream;
   AxArgs<T> a;
   a.N = mesh->N;
@@ -427,7 +426,7 @@ int bp_t::OperatorDotFused(const T* o_q, T* o_Aq, double* part, int* npart, int 
     AqL = o_AqL32.ptr();
   }
   a.Aq = AqL;
-  if (int st = launch_bp5_fused<T>(s, a, o_Aq, o_partAx.ptr(), variant)) return st;
+  if (int st = launch_bp5_fused<T>(s, a, o_Aq, o_partAx.ptr())) return st;
   if (sharedOgs.NrowBlocksT > 0) {
     *npart = ogs_gather_partials(sharedOgs.NrowBlocksT);
     return launch_ogs_gather_dot_rows<T>(s, sharedOgs.NrowBlocksT, sharedOgs.o_blockRowStartsT.ptr(),
@@ -439,8 +438,8 @@ int bp_t::OperatorDotFused(const T* o_q, T* o_Aq, double* part, int* npart, int 
   return launch_fold(s, o_partAx.ptr(), naxParts, part);
 }
 
-template int bp_t::OperatorDotFused<double>(const double*, double*, double*, int*, int);
-template int bp_t::OperatorDotFused<float>(const float*, float*, double*, int*, int);
+template int bp_t::OperatorDotFused<double>(const double*, double*, double*, int*);
+template int bp_t::OperatorDotFused<float>(const float*, float*, double*, int*);
 
 int bp_t::AxLocalKrhs(int K, const dfloat* o_q, dfloat* o_AqL, int chunk) {
   if (bp != 5) return set_error(BPX_ERR_UNSUPPORTED, "bp: k right-hand sides are implemented for BP5");
@@ -481,11 +480,12 @@ int bp_t::OperatorKrhs(int K, const dfloat* o_q, dfloat* o_Aq, int chunk) {
 int bp_t::OperatorDot32(const float* o_q, float* o_Aq, double* part, int* npart) {
   if (Nfields != 1) return set_error(BPX_ERR_UNSUPPORTED, "bp: the FP32 operator covers one field (BP1/3/5)");
   if (haveKappa) return set_error(BPX_ERR_UNSUPPORTED, "bp: the FP32 operator has no variable coefficient");
-  if (fused) {
-    if (int st = Prepare32()) return st;
-    return OperatorDotFused<float>(o_q, o_Aq, part, npart, tuning.fused32Shape);
-  }
   if (int st = Prepare32()) return st;
+  if (fused && tuning.fused32) return OperatorDotFused<float>(o_q, o_Aq, part, npart);
+  return OperatorDotUnfused32(o_q, o_Aq, part, npart);
+}
+
+int bp_t::OperatorDotUnfused32(const float* o_q, float* o_Aq, double* part, int* npart) {
   if (int st = ax<float>(o_q, o_AqL32.ptr(), o_wJ32.ptr(), o_ggeo32.ptr(), o_gllw32.ptr(), o_D32.ptr(),
                          o_I32.ptr(), o_MM32.ptr(), o_invV32.ptr()))
     return st;
@@ -494,6 +494,39 @@ int bp_t::OperatorDot32(const float* o_q, float* o_Aq, double* part, int* npart)
                                       ogs.o_rowStartsT.ptr(), ogs.o_colIdsT.ptr(), o_AqL32.ptr(), o_Aq, o_q, part,
                                       ogs.gatherMode);
 }
+
+namespace {
+// best of three event-timed runs after one untimed run, in microseconds
+template <typename F>
+int time_best(hipStream_t s, F&& fn, double& us) {
+  hipEvent_t e0, e1;
+  BPX_HIP(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return set_error(BPX_ERR_HIP, "tune: event create failed");
+  }
+  int st = fn();
+  float best = 1e30f;
+  for (int rep = 0; rep < 3 && st == BPX_OK; ++rep) {
+    float ms = 0;
+    if (hipEventRecord(e0, s) != hipSuccess) {
+      st = set_error(BPX_ERR_HIP, "tune: event record failed");
+      break;
+    }
+    if ((st = fn())) break;
+    if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+      st = set_error(BPX_ERR_HIP, "tune: event timing failed");
+      break;
+    }
+    best = std::min(best, ms);
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (st == BPX_OK) us = best * 1e3;
+  return st;
+}
+}  // namespace
 
 int bp_t::Tune() {
   hipStream_t s = platform->stream;
@@ -504,101 +537,33 @@ int bp_t::Tune() {
   if (int st = part.alloc(GatherPartials())) return st;
   BPX_HIP(hipMemsetAsync(o_AqL.ptr(), 0, (size_t)NLocal * sizeof(double), s));
   BPX_HIP(hipMemsetAsync(v.ptr(), 0, 2 * (size_t)Ng * sizeof(double), s));
-  hipEvent_t e0, e1;
-  BPX_HIP(hipEventCreate(&e0));
-  BPX_HIP(hipEventCreate(&e1));
-  int st = BPX_OK;
-  for (int mode = 0; mode < 2 && st == BPX_OK; ++mode) {
+  for (int mode = 0; mode < 2; ++mode) {
     auto run = [&] {
       return launch_ogs_gather_dot<double>(s, ogs.NrowBlocksT, ogs.o_blockRowStartsT.ptr(), ogs.o_rowStartsT.ptr(),
                                            ogs.o_colIdsT.ptr(), o_AqL.ptr(), v.ptr(), v.ptr() + Ng, part.ptr(),
                                            mode);
     };
-    if ((st = run())) break;
-    float best = 1e30f;
-    for (int rep = 0; rep < 3 && st == BPX_OK; ++rep) {
-      float ms = 0;
-      if (hipEventRecord(e0, s) != hipSuccess || (st = run()) || hipEventRecord(e1, s) != hipSuccess ||
-          hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
-        if (!st) st = set_error(BPX_ERR_HIP, "tune: event timing failed");
-        break;
-      }
-      best = std::min(best, ms);
-    }
-    tuning.gather_us[mode] = best * 1e3;
+    if (int st = time_best(s, run, tuning.gather_us[mode])) return st;
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  if (st) return st;
   tuning.gatherMode = tuning.gather_us[1] < tuning.gather_us[0] ? 1 : 0;
   ogs.gatherMode = tuning.gatherMode;
-
-  // FP64 Ax launch shapes (BP5/BP1/BP3 one field; only BP5 has shapes)
-  if (bp == 5 && Nfields == 1 && !mesh->opts.affine) {
-    hipEvent_t t0, t1;
-    BPX_HIP(hipEventCreate(&t0));
-    BPX_HIP(hipEventCreate(&t1));
-    for (int sh = 0; sh < 3 && st == BPX_OK; ++sh) {
-      tuning.axShape = sh;
-      if ((st = AxLocal(v.ptr(), o_AqL.ptr()))) break;
-      float best = 1e30f;
-      for (int rep = 0; rep < 3 && st == BPX_OK; ++rep) {
-        float ms = 0;
-        if (hipEventRecord(t0, s) != hipSuccess || (st = AxLocal(v.ptr(), o_AqL.ptr())) ||
-            hipEventRecord(t1, s) != hipSuccess || hipEventSynchronize(t1) != hipSuccess ||
-            hipEventElapsedTime(&ms, t0, t1) != hipSuccess) {
-          if (!st) st = set_error(BPX_ERR_HIP, "tune: event timing failed");
-          break;
-        }
-        best = std::min(best, ms);
-      }
-      tuning.ax_us[sh] = best * 1e3;
-    }
-    (void)hipEventDestroy(t0);
-    (void)hipEventDestroy(t1);
-    if (st) return st;
-    tuning.axShape = (int)(std::min_element(tuning.ax_us, tuning.ax_us + 3) - tuning.ax_us);
-  }
   if (!fused) return BPX_OK;
 
-  // the fused operator's two builds (identical results); the unfused
-  // operator is timed beside them for the record
-  deviceBuffer<double> part2;
-  if (int st2 = part2.alloc(GatherPartials())) return st2;
-  BPX_HIP(hipEventCreate(&e0));
-  BPX_HIP(hipEventCreate(&e1));
-  auto timeit = [&](auto&& fn, double& us) -> int {
-    if (int st2 = fn()) return st2;
-    float best = 1e30f;
-    for (int rep = 0; rep < 3; ++rep) {
-      float ms = 0;
-      if (hipEventRecord(e0, s) != hipSuccess) return set_error(BPX_ERR_HIP, "tune: event");
-      if (int st2 = fn()) return st2;
-      if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-          hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
-        return set_error(BPX_ERR_HIP, "tune: event timing failed");
-      best = std::min(best, ms);
-    }
-    us = best * 1e3;
-    return BPX_OK;
-  };
+  // the CG operator: fused (Ax writes single-copy rows, shared rows
+  // gathered) against unfused (Ax, then the full gather); same bits
   int np = 0;
-  for (int var = 0; var < 3 && st == BPX_OK; ++var)
-    st = timeit([&] { return OperatorDotFused<double>(v.ptr(), v.ptr() + Ng, part2.ptr(), &np, var); },
-                tuning.fused_us[var]);
-  if (st == BPX_OK) {
-    fused = false;
-    st = timeit([&] { return OperatorDot(v.ptr(), v.ptr() + Ng, part2.ptr(), &np); }, tuning.unfused_us);
-    fused = true;
-  }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  if (st) return st;
-  tuning.fusedVariant = (int)(std::min_element(tuning.fused_us, tuning.fused_us + 3) - tuning.fused_us);
+  if (int st = time_best(s, [&] { return OperatorDotFused<double>(v.ptr(), v.ptr() + Ng, part.ptr(), &np); },
+                         tuning.fused_us))
+    return st;
+  tuning.fused64 = 0;
+  if (int st = time_best(s, [&] { return OperatorDot(v.ptr(), v.ptr() + Ng, part.ptr(), &np); },
+                         tuning.unfused_us))
+    return st;
+  tuning.fused64 = tuning.fused_us < tuning.unfused_us ? 1 : 0;
   return BPX_OK;
 }
 
-// FP32 fused operator shapes, timed when the FP32 operator is first built
+// The same choice for the FP32 operator, timed when it is first built
 int bp_t::Tune32() {
   if (!fused) return BPX_OK;
   hipStream_t s = platform->stream;
@@ -609,30 +574,14 @@ int bp_t::Tune32() {
   if (int st = v.alloc(2 * (size_t)Ng)) return st;
   if (int st = part.alloc(GatherPartials())) return st;
   BPX_HIP(hipMemsetAsync(v.ptr(), 0, 2 * (size_t)Ng * sizeof(float), s));
-  hipEvent_t e0, e1;
-  BPX_HIP(hipEventCreate(&e0));
-  BPX_HIP(hipEventCreate(&e1));
-  int st = BPX_OK, np = 0;
-  for (int sh = 0; sh < 3 && st == BPX_OK; ++sh) {
-    if ((st = OperatorDotFused<float>(v.ptr(), v.ptr() + Ng, part.ptr(), &np, sh))) break;
-    float best = 1e30f;
-    for (int rep = 0; rep < 3 && st == BPX_OK; ++rep) {
-      float ms = 0;
-      if (hipEventRecord(e0, s) != hipSuccess ||
-          (st = OperatorDotFused<float>(v.ptr(), v.ptr() + Ng, part.ptr(), &np, sh)) ||
-          hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-          hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
-        if (!st) st = set_error(BPX_ERR_HIP, "tune: event timing failed");
-        break;
-      }
-      best = std::min(best, ms);
-    }
-    tuning.fused32_us[sh] = best * 1e3;
-  }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  if (st) return st;
-  tuning.fused32Shape = (int)(std::min_element(tuning.fused32_us, tuning.fused32_us + 3) - tuning.fused32_us);
+  int np = 0;
+  if (int st = time_best(s, [&] { return OperatorDotFused<float>(v.ptr(), v.ptr() + Ng, part.ptr(), &np); },
+                         tuning.fused32_us))
+    return st;
+  if (int st = time_best(s, [&] { return OperatorDotUnfused32(v.ptr(), v.ptr() + Ng, part.ptr(), &np); },
+                         tuning.unfused32_us))
+    return st;
+  tuning.fused32 = tuning.fused32_us < tuning.unfused32_us ? 1 : 0;
   return BPX_OK;
 }
 

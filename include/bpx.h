@@ -260,17 +260,17 @@ int bpx_bp_info(bpx_bp b, int64_t* out, int capacity);
 /* No reference counterpart: launch choices the setup timed on this
  * problem, all giving identical results (times in us, -1 = not timed).
  * out[0..2]: gather workgroup order (0 strided, 1 XCD-blocked), time of
- * order 0, of order 1; out[3..6]: FP64 Ax launch shape, time of shapes
- * 0, 1, 2; out[7..11]: shape of the fused CG operator (-1 = not fused),
- * time of shapes 0, 1, 2, time of the unfused operator; out[12..15]: shape
- * of the FP32 fused operator (timed when the FP32 operator is first used),
- * time of shapes 0, 1, 2. Returns the number of values. */
+ * order 0, of order 1; out[3..5]: FP64 CG operator fused (1: the Ax writes
+ * single-copy rows, the gather sums the shared rows) or unfused (0: Ax,
+ * then the full gather; -1 = no fused form for this problem), time fused,
+ * time unfused; out[6..8]: the same for the FP32 operator (timed when it is
+ * first used). Returns the number of values. */
 int bpx_bp_tuning(bpx_bp b, double* out, int capacity);
 /* No reference counterpart: pin launch choices instead of the timed ones
  * (for reproducible timings; results do not depend on them): gather order
- * 0/1, FP64 Ax shape 0-2, fused FP64 operator shape 0-2, fused FP32
- * operator shape 0-2; -1 keeps a choice. */
-int bpx_bp_set_tuning(bpx_bp b, int gatherMode, int axShape, int fusedShape, int fused32Shape);
+ * 0/1, FP64 operator fused 0/1, FP32 operator fused 0/1; -1 keeps a
+ * choice. */
+int bpx_bp_set_tuning(bpx_bp b, int gatherMode, int fused, int fused32);
 /* bpN_t::Operator(o_q, o_Aq) (BP/BP5/src/bp5Operator.cpp:29-85): Ax over the
  * local elements, then the T gather. Device vectors of length Ngather. */
 int bpx_bp_operator(bpx_bp b, const double* o_q, double* o_Aq);

@@ -44,7 +44,7 @@ def make_pair(bp, N, nx, ny=None, nz=None, affine=False, perturb=0.1, seed=4, la
 
 
 @pytest.mark.parametrize("affine", [False, True])
-@pytest.mark.parametrize("bp,N", [(1, 3), (1, 7), (2, 3), (3, 3), (3, 7), (4, 2), (5, 1), (5, 3), (5, 7),
+@pytest.mark.parametrize("bp,N", [(1, 3), (1, 7), (2, 3), (3, 3), (3, 7), (4, 2), (5, 1), (5, 3), (5, 4), (5, 5), (5, 7),
                                   (5, 8), (5, 11), (5, 15), (6, 3), (6, 7)])
 def test_bp_operator_matches_oracle(plat, bp, N, affine):
     nx = 2 if N >= 11 else 3
@@ -232,39 +232,43 @@ def test_mixed_precision_reaches_1e10(plat, bp):
     assert np.linalg.norm(host(x) - x_ref) / np.linalg.norm(x_ref) < 1e-7
 
 
-@pytest.mark.parametrize("N,nx,affine", [(1, 3, False), (3, 3, False), (5, 2, True), (7, 4, False), (7, 1, False),
-                                         (8, 3, False), (11, 2, False)])
+@pytest.mark.parametrize("N,nx,affine", [(1, 3, False), (3, 3, False), (4, 3, False), (5, 2, True), (5, 3, False),
+                                         (7, 4, False), (7, 1, False), (8, 3, False), (11, 2, False)])
 def test_fused_operator_bit_exact(plat, N, nx, affine):
-    """The operator as CG applies it (BP5: single-copy nodes written by the Ax
-    straight into Aq, shared rows gathered) equals bpx_bp_operator bit for
-    bit; q.Aq matches the dot product of the result. nx = 1 has no shared
-    rows at all."""
+    """The operator as CG applies it, fused (BP5: single-copy nodes written
+    by the Ax straight into Aq, shared rows gathered) and unfused, equals
+    bpx_bp_operator bit for bit; q.Aq matches the dot product of the result.
+    nx = 1 has no shared rows at all."""
     m, P = make_pair(5, N, nx, affine=affine, lam=1.1)
     b = capi.BP(plat, m, 5, lam=1.1)
+    t = b.tuning()
+    assert t["fused"] in (0, 1)
+    assert t["fused"] == (1 if t["fused_us"] < t["unfused_us"] else 0)
     q = np.random.default_rng(N * 7 + nx).standard_normal(b.Ngather)
     A1 = torch.zeros(b.Ngather, dtype=torch.float64, device="cuda")
-    A2 = torch.full_like(A1, float("nan"))
     b.operator(dev(q), A1)
-    qAq = b.operator_dot(dev(q), A2)
-    np.testing.assert_array_equal(host(A2), host(A1))
     ref = float(np.dot(q, host(A1)))
-    assert abs(qAq - ref) <= 1e-13 * np.abs(q * host(A1)).sum()
-    t = b.tuning()
-    assert t["fused_build"] in (0, 1)
+    for fz in (1, 0):
+        b.set_tuning(fused=fz)
+        A2 = torch.full_like(A1, float("nan"))
+        qAq = b.operator_dot(dev(q), A2)
+        np.testing.assert_array_equal(host(A2), host(A1))
+        assert abs(qAq - ref) <= 1e-13 * np.abs(q * host(A1)).sum()
+    assert rel_err(host(A1), P.operator(q)) <= AX_RTOL
 
 
-@pytest.mark.parametrize("N,affine", [(7, False), (4, False), (15, False), (7, True)])
-def test_launch_shapes_bit_identical(plat, N, affine):
-    """Every launch choice the tuner may pick (gather order, Ax shape, fused
-    operator shape, FP32 fused shape) gives the same bits."""
+@pytest.mark.parametrize("N,affine", [(7, False), (4, False), (5, False), (8, False), (15, False), (7, True)])
+def test_launch_choices_bit_identical(plat, N, affine):
+    """Every launch choice the tuner may pick (gather order, fused or
+    unfused FP64 and FP32 operator) gives the same bits."""
     nx = 2 if N >= 11 else 3
     m, P = make_pair(5, N, nx, affine=affine, lam=1.0)
     b = capi.BP(plat, m, 5, lam=1.0)
     q = dev(np.random.default_rng(N).standard_normal(b.Ngather))
     ref = None
     for g in (0, 1):
-        for sh in (0, 1, 2):
-            b.set_tuning(gather_mode=g, ax_shape=sh, fused_shape=sh)
+        for fz in (0, 1):
+            b.set_tuning(gather_mode=g, fused=fz)
             A1 = torch.zeros(b.Ngather, dtype=torch.float64, device="cuda")
             A2 = torch.zeros_like(A1)
             b.operator(q, A1)
@@ -274,15 +278,27 @@ def test_launch_shapes_bit_identical(plat, N, affine):
                 ref = host(A1)
             np.testing.assert_array_equal(host(A1), ref)
     assert rel_err(ref, P.operator(host(q))) <= AX_RTOL
-    # FP32 fused shapes, through the mixed-precision solve's operator: same
-    # iterations and bits whatever the shape
+    # FP32 fused and unfused operator, through the mixed-precision solve:
+    # same iterations and bits either way
     rhs = dev(capi.random_normal(3, b.Ngather))
     res = []
-    for sh in (0, 1, 2):
-        b.set_tuning(fused32_shape=sh)
+    for fz in (0, 1):
+        b.set_tuning(fused32=fz)
         x = torch.zeros_like(rhs)
         st = b.solve_mixed(x, rhs, tol=1e-9, interval=16)
         res.append((st["iterations"], host(x)))
-    for its, x in res[1:]:
-        assert its == res[0][0]
-        np.testing.assert_array_equal(x, res[0][1])
+    assert res[1][0] == res[0][0]
+    np.testing.assert_array_equal(res[1][1], res[0][1])
+    t = b.tuning()
+    assert t["fused32"] in (0, 1)
+
+
+def test_set_tuning_errors(plat):
+    m = capi.Mesh(N=3, nx=2, cubature=True)
+    b3 = capi.BP(plat, m, 3, lam=1.0)
+    assert b3.tuning()["fused"] == -1
+    with pytest.raises(capi.BpxError):
+        b3.set_tuning(fused=1)  # BP3 has no fused operator
+    b5 = capi.BP(plat, capi.Mesh(N=3, nx=2), 5, lam=1.0)
+    with pytest.raises(capi.BpxError):
+        b5.set_tuning(gather_mode=2)
