@@ -71,13 +71,15 @@ constexpr bool bp5_dreg() {
 // back to back in one workgroup).
 // BATCH: elements of many problems; an element whose problem
 // (elemPid[e]) is done is skipped.
-// FUSED (one field): G2L entries carry a flag (bit 30) on nodes whose global
-// row has this node as its only copy; those results go straight to Ap[row]
-// (a one-entry gather row sums to the value itself) with their p.Ap terms
-// summed into part[workgroup]; the other nodes go to Aq (the local vector)
-// for the gather of the shared rows.
+// FUSED (one field), rows numbered so that element e's interior nodes are
+// rows S0 + e*fused_interior_pad(NQ) + (interior index in local order), the
+// only copies of their rows: their results go straight to Ap (a one-entry
+// gather row sums to the value itself) with the p.Ap terms summed into
+// part[workgroup]. The element's surface nodes go, in local order, to
+// Aq + e*fused_surface_pad(NQ) for the gather of the shared rows. S0 comes
+// in ldq. Padding entries of both runs are written with zeros, so every
+// cache line of Ap and Aq is written whole by one workgroup.
 enum { VAR_PLAIN = 0, VAR_KRHS = 1, VAR_BATCH = 2, VAR_FUSED = 3 };
-constexpr int kSingleFlag = 1 << 30;
 
 template <typename T, int NQ, int NF, int EPB, bool AFFINE, int WAVES, int VAR>
 __global__ __launch_bounds__(NQ* NQ* EPB) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_bp5_ax(
@@ -116,7 +118,7 @@ __global__ __launch_bounds__(NQ* NQ* EPB) __attribute__((amdgpu_waves_per_eu(WAV
   for (int n = t; n < NQ2; n += NQ2 * EPB) s_D[n / NQ][n % NQ] = D[n];
 
   // KRHS: the pencil's gather rows, read once for all sweeps
-  int gid[(KRHS || FUSED) ? NQ : 1];
+  int gid[KRHS ? NQ : 1];
   if (KRHS) {
 #pragma unroll
     for (int k = 0; k < NQ; ++k) gid[k] = active ? G2L[(size_t)e * NP + k * NQ2 + ij] : -1;
@@ -135,14 +137,6 @@ __global__ __launch_bounds__(NQ* NQ* EPB) __attribute__((amdgpu_waves_per_eu(WAV
         rq[f][k] = (gid[k] >= 0) ? src[(size_t)f * ldq] : T(0);
         rA[f][k] = T(0);
       }
-    }
-  } else if (FUSED) {
-#pragma unroll
-    for (int k = 0; k < NQ; ++k) {
-      const int v = active ? G2L[(size_t)e * NP + k * NQ2 + ij] : -1;
-      gid[k] = v;
-      rq[0][k] = (v >= 0) ? q[v & (kSingleFlag - 1)] : T(0);
-      rA[0][k] = T(0);
     }
   } else {
     const size_t lbase = (size_t)e * NP * NF + ij;
@@ -244,20 +238,36 @@ __global__ __launch_bounds__(NQ* NQ* EPB) __attribute__((amdgpu_waves_per_eu(WAV
   }
 
   if (FUSED) {
+    // the element's results go through LDS (every lane writes all of its
+    // nodes, as the plain kernel does to the local vector); then the
+    // element's lanes copy its interior run to Ap and its surface run to
+    // Aq, each as one contiguous, coalesced sweep with zero padding
+    constexpr int NI = fused_interior(NQ);
+    constexpr int NIP = fused_interior_pad(NQ);
+    constexpr int NS = NP - NI;
+    constexpr int NSP = fused_surface_pad(NQ);
+    __shared__ T s_out[FUSED ? EPB : 1][FUSED ? NP : 1];
+    __shared__ T s_pin[FUSED ? EPB : 1][FUSED ? NP : 1];
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) {
+      s_out[es][k * NQ2 + ij] = rA[0][k];
+      s_pin[es][k * NQ2 + ij] = rq[0][k];
+    }
+    __syncthreads();
     double pd = 0;
     if (active) {
-#pragma unroll
-      for (int k = 0; k < NQ; ++k) {
-        const int v = gid[k];
-        if (v >= 0) {
-          if (v & kSingleFlag) {
-            Ap[v & (kSingleFlag - 1)] = rA[0][k];
-            pd += (double)rq[0][k] * (double)rA[0][k];
-          } else {
-            Aq[(size_t)e * NP + k * NQ2 + ij] = rA[0][k];
-          }
+      T* inner = Ap + ldq + (size_t)e * NIP;
+      for (int n = ij; n < NIP; n += NQ2) {
+        T v = T(0);
+        if (n < NI) {
+          const int node = fused_interior_node(NQ, n);
+          v = s_out[es][node];
+          pd = fma((double)s_pin[es][node], (double)v, pd);
         }
+        inner[n] = v;
       }
+      T* surf = Aq + (size_t)e * NSP;
+      for (int n = ij; n < NSP; n += NQ2) surf[n] = (n < NS) ? s_out[es][fused_surface_node(NQ, n)] : T(0);
     }
     // workgroup sum of the p.Ap terms (fixed order)
 #pragma unroll

@@ -57,11 +57,54 @@ template <typename T> int launch_bp5_ax(hipStream_t s, const AxArgs<T>& a);
 // a.K right-hand sides in one launch; chunk = right-hand sides held in
 // registers per sweep (1, 2 or 4, dividing K; 0 picks the default)
 template <typename T> int launch_bp5_ax_krhs(hipStream_t s, const AxArgs<T>& a, int chunk);
-// Fused operator path (one field): a.GlobalToLocal carries kFusedSingle on
-// nodes that are the only copy of their row; their results go to Ap[row]
-// with p.Ap partials in part[bp5_workgroups(N, Nelements)], the others to
-// a.Aq (the local vector) for launch_ogs_gather_dot_rows.
-constexpr int kFusedSingle = 1 << 30;
+// Output layout of the fused operator (BP5, one field, Nq nodes per
+// direction). Element e's interior nodes are Ap rows a.ldq + e *
+// fused_interior_pad(Nq) + fused_interior_index(...); its surface nodes go
+// to a.Aq + e * fused_surface_pad(Nq) + fused_surface_index(...), both in
+// local node order (k, j, i); runs are padded to 32 entries (whole cache
+// lines) and the padding is written with zeros. p.Ap partials of the
+// interior rows go to part[bp5_workgroups(N, Nelements)].
+constexpr int fused_pad(int n) { return (n + 31) / 32 * 32; }
+constexpr int fused_interior(int Nq) { return Nq > 2 ? (Nq - 2) * (Nq - 2) * (Nq - 2) : 0; }
+constexpr int fused_interior_pad(int Nq) { return fused_pad(fused_interior(Nq)); }
+constexpr int fused_surface_pad(int Nq) { return fused_pad(Nq * Nq * Nq - fused_interior(Nq)); }
+constexpr int fused_interior_index(int Nq, int i, int j, int k) {
+  return ((k - 1) * (Nq - 2) + (j - 1)) * (Nq - 2) + (i - 1);
+}
+// -1 for an interior node; surface nodes: the whole bottom slab, then the
+// ring of each interior slab, then the whole top slab, each in (j, i) order
+constexpr int fused_surface_index(int Nq, int i, int j, int k) {
+  const int Nq2 = Nq * Nq, ring = Nq2 - (Nq - 2) * (Nq - 2);
+  if (k == 0) return j * Nq + i;
+  if (k == Nq - 1) return Nq2 + (Nq - 2) * ring + j * Nq + i;
+  if (i != 0 && i != Nq - 1 && j != 0 && j != Nq - 1) return -1;
+  const int rp = (j == 0) ? i : (j == Nq - 1) ? Nq + 2 * (Nq - 2) + i : Nq + 2 * (j - 1) + (i == 0 ? 0 : 1);
+  return Nq2 + (k - 1) * ring + rp;
+}
+// inverses: the element-local node (k * Nq + j) * Nq + i of interior
+// index n, and of surface index n
+constexpr int fused_interior_node(int Nq, int n) {
+  const int m = Nq - 2;
+  return ((n / (m * m) + 1) * Nq + (n / m) % m + 1) * Nq + n % m + 1;
+}
+constexpr int fused_surface_node(int Nq, int n) {
+  const int Nq2 = Nq * Nq, ring = Nq2 - (Nq - 2) * (Nq - 2), top = Nq2 + (Nq - 2) * ring;
+  if (n < Nq2) return n;
+  if (n >= top) return (Nq - 1) * Nq2 + (n - top);
+  const int k = 1 + (n - Nq2) / ring, rp = (n - Nq2) % ring;
+  int i = 0, j = 0;
+  if (rp < Nq) {
+    j = 0;
+    i = rp;
+  } else if (rp >= Nq + 2 * (Nq - 2)) {
+    j = Nq - 1;
+    i = rp - Nq - 2 * (Nq - 2);
+  } else {
+    j = 1 + (rp - Nq) / 2;
+    i = ((rp - Nq) % 2) ? Nq - 1 : 0;
+  }
+  return (k * Nq + j) * Nq + i;
+}
 template <typename T> int launch_bp5_fused(hipStream_t s, const AxArgs<T>& a, T* Ap, double* part);
 int bp5_workgroups(int N, int Nelements);
 template <typename T> int launch_bp1_ax(hipStream_t s, const AxArgs<T>& a);
@@ -186,6 +229,11 @@ int launch_cg_update_xr(hipStream_t s, dlong N, int k, const T* p, const T* Ap, 
                         double* hist_alpha);
 int launch_cg_finish(hipStream_t s, int kend, int maxit, const double* part_rr, int nrr, CgState* st,
                      double* hist_rr);
+
+// Row permutation over N rows, perm[i] = reference row of row i (-1 =
+// padding): to_ref = false: out[i] = in[perm[i]] (0 on padding);
+// to_ref = true: out[perm[i]] = in[i].
+int launch_permute(hipStream_t s, dlong N, const dlong* perm, const double* in, double* out, bool to_ref);
 
 // Mixed precision reliable update: x64 += c32, c32 = 0; then from Ax (FP64)
 // r32 = float(b - Ax) with kRedBlocks partials of (b-Ax).(b-Ax) in

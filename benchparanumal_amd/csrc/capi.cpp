@@ -519,7 +519,7 @@ int bpx_bp_create(bpx_platform p, bpx_mesh m, int bp, double lambda, bpx_bp* out
     delete b;
     return st;
   }
-  b->solver = new cg(*p->p, b->bp.ogs.Ngather, 0, b->bp.GatherPartials());
+  b->solver = new cg(*p->p, b->bp.Ns, 0, b->bp.GatherPartials());
   if (int st = b->solver->status()) {
     delete b;
     return st;
@@ -638,7 +638,7 @@ int bpx_bp_solve(bpx_bp b, double* o_x, double* o_r, double tol, int maxit, int 
   BPX_GUARD_BEGIN
   if (int st = need(b, "bp")) return st;
   int its = 0;
-  const int st = b->solver->Solve(b->bp, o_x, o_r, tol, maxit, verbose, &its);
+  const int st = b->bp.SolveCG(*b->solver, o_x, o_r, tol, maxit, verbose, &its);
   if (iterations) *iterations = its;
   if (rnorm_hist) {
     std::vector<double> rr, al;

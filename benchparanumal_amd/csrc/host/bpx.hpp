@@ -211,10 +211,39 @@ class bp_t : public solver_t {
   // Operator with the gather fused with the q.Aq partial sums
   bool HasOperatorDot() const override { return true; }
   int OperatorDot(const dfloat* o_q, dfloat* o_Aq, double* part, int* npart) override;
-  int OperatorDot32(const float* o_q, float* o_Aq, double* part, int* npart);
   int GatherPartials() const {
-    return std::max({ogs_gather_partials(ogs.NrowBlocksT), ogs_gather_partials(sharedOgs.NrowBlocksT), 1});
+    return std::max({ogs_gather_partials(ogs.NrowBlocksT), ogs_gather_partials(sharedOgs.NrowBlocksT),
+                     ogs_gather_partials(solveOgs.NrowBlocksT), 1});
   }
+
+  // Solve space: the row order the CG solvers run in. For BP5 with one
+  // field the rows are renumbered: rows with two or more local copies
+  // first (in reference order), then, element by element, the element's
+  // interior nodes (in local node order), each run padded to 32 rows (see
+  // fused_interior_pad). The fused operator's Ax then writes whole cache
+  // lines of Ap and the shared-row gather writes the rest; in the
+  // reference order both would write parts of the same lines, and a
+  // partial line costs HBM a read-modify-write. Padding rows belong to no
+  // node and stay zero. Other problems keep the reference order
+  // (permuted = false, Ns = Ngather).
+  bool permuted = false;
+  dlong Ns = 0;
+  deviceBuffer<dlong> o_sperm;  // solve row -> reference row (-1 padding)
+  deviceBuffer<dlong> o_G2Ls;   // local node -> solve row (-1 masked)
+  ogs::ogs_t solveOgs;          // the gather onto all solve rows
+  const dlong* SolveG2L() const { return permuted ? o_G2Ls.ptr() : o_GlobalToLocal.ptr(); }
+  const ogs::ogs_t& SolveOgs() const { return permuted ? solveOgs : ogs; }
+  // reference order <-> solve order (Ngather <-> Ns vectors)
+  int ToSolve(const dfloat* o_ref, dfloat* o_s);
+  int FromSolve(const dfloat* o_s, dfloat* o_ref);
+  // the operators on solve-space vectors: plain (FP64), with the p.Ap
+  // partials (FP64 and FP32; fused or not as tuned)
+  int OperatorS(const dfloat* o_q, dfloat* o_Aq);
+  int OperatorDotS(const dfloat* o_q, dfloat* o_Aq, double* part, int* npart);
+  int OperatorDot32(const float* o_q, float* o_Aq, double* part, int* npart);
+  // cg::Solve on reference-order x and r, run in the solve space
+  int SolveCG(cg& solver, dfloat* o_x, dfloat* o_r, dfloat tol, int maxit, int verbose, int* iterations);
+  deviceBuffer<double> o_sx, o_sr, o_sb;  // solve-space copies (created on first use)
   // times the launch choices that give identical results (gather
   // workgroup order; fused or unfused CG operator) and keeps the fastest
   int Tune();
@@ -234,8 +263,11 @@ class bp_t : public solver_t {
   int PrepareFused();
   template <typename T>
   int OperatorDotFused(const T* o_q, T* o_Aq, double* part, int* npart);
+  int OperatorDotUnfusedS(const dfloat* o_q, dfloat* o_Aq, double* part, int* npart);
   int OperatorDotUnfused32(const float* o_q, float* o_Aq, double* part, int* npart);
-  int AxLocal(const dfloat* o_q, dfloat* o_AqL);
+  // Ax into the local vector; G2L = the GlobalToLocal map to read q
+  // through (default: the reference rows)
+  int AxLocal(const dfloat* o_q, dfloat* o_AqL, const dlong* G2L = nullptr);
   int Operator32(const float* o_q, float* o_Aq);
   int Forcing(dfloat* o_b);
   // K right-hand sides at once (BP5), one vector after another: o_q and
@@ -258,7 +290,7 @@ class bp_t : public solver_t {
   struct MixedWork {
     dlong N = -1;
     int maxit = 0;
-    deviceBuffer<double> Ax64, part_true, part_rr, part_pAp, hist_rr, hist_alpha, norm;
+    deviceBuffer<double> Ax64, part_true, part_rr, part_pAp, hist_rr, hist_alpha, norm, xs, bs;
     deviceBuffer<float> r32, p32, Ap32, c32;
     deviceBuffer<CgState> state;
   } mixed;
@@ -280,10 +312,14 @@ class bp_t : public solver_t {
   deviceBuffer<dlong> o_lattice;
   ogs::ogs_t latticeOgs;
   deviceBuffer<double> o_ggeoK, o_wJK, o_gradL;
-  // fused operator: flagged GlobalToLocal, the shared rows' maps and their
-  // row numbers, the Ax kernel's p.Ap partials
-  deviceBuffer<dlong> o_G2Lf, o_sharedRows;
+  // fused operator (solve space): the surface-node values of each element
+  // (FP64; FP32 in o_slots32), the shared rows' maps over them and their
+  // row numbers, the first interior row, the Ax kernel's p.Ap partials
+  deviceBuffer<double> o_slots;
+  deviceBuffer<float> o_slots32;
+  deviceBuffer<dlong> o_sharedRows;
   ogs::ogs_t sharedOgs;
+  dlong fusedS0 = 0;
   deviceBuffer<double> o_partAx;
   int naxParts = 0;
   // FP32 copies (created on first use)
@@ -296,7 +332,19 @@ class bp_t : public solver_t {
  private:
   template <typename T>
   int ax(const T* q, T* AqL, const T* wJ, const T* ggeo, const T* gllw, const T* D, const T* I,
-         const T* MM, const T* invV);
+         const T* MM, const T* invV, const dlong* G2L = nullptr);
+};
+
+// bp_t seen through the solve space (what cg::Solve applies inside SolveCG)
+class bp_solve_view_t : public solver_t {
+ public:
+  explicit bp_solve_view_t(bp_t& b) : bp(b) {}
+  int Operator(const dfloat* o_q, dfloat* o_Aq) override { return bp.OperatorS(o_q, o_Aq); }
+  bool HasOperatorDot() const override { return true; }
+  int OperatorDot(const dfloat* o_q, dfloat* o_Aq, double* part, int* npart) override {
+    return bp.OperatorDotS(o_q, o_Aq, part, npart);
+  }
+  bp_t& bp;
 };
 
 // Ragged batch of BP5 problems (box meshes of any size and degree), one CG

@@ -15,7 +15,9 @@
 // residual. Unlike restarted defect correction, the Krylov recurrence is
 // kept across the updates (beta continues from the previous r.r, as in
 // reliable-update CG), so the FP32 iteration count stays close to the FP64
-// one while each iteration moves about half the bytes.
+// one while each iteration moves about half the bytes. Only when the FP32
+// recurrence has stopped by itself (r.r = 0, or p.Ap <= 0 once its residual
+// has fallen out of FP32 range) is it restarted from the true residual.
 #include <cmath>
 
 #include "bpx.hpp"
@@ -26,31 +28,42 @@ int bp_t::SolveMixed(dfloat* o_x, const dfloat* o_b, double tol, int interval, i
   if (!(tol > 0) || interval < 1 || maxit < 1)
     return set_error(BPX_ERR_INVALID_ARGUMENT, "mixed solve: tol > 0, interval >= 1 and maxit >= 1 required");
   if (int st = Prepare32()) return st;
-  const dlong N = ogs.Ngather;
+  const dlong N = Ns;  // the recurrence runs in the solve space
   hipStream_t s = platform->stream;
   MixedWork& w = mixed;
   if (w.N != N || w.maxit < maxit) {
     int st;
     if ((st = w.Ax64.alloc(N)) || (st = w.r32.alloc(N)) || (st = w.p32.alloc(N)) || (st = w.Ap32.alloc(N)) ||
+        (st = w.xs.alloc(N)) || (st = w.bs.alloc(N)) ||
         (st = w.c32.alloc(N)) || (st = w.part_true.alloc(kRedBlocks)) || (st = w.part_rr.alloc(kRedBlocks)) ||
         (st = w.part_pAp.alloc(std::max(GatherPartials(), kRedBlocks))) || (st = w.hist_rr.alloc(maxit + 1)) ||
         (st = w.hist_alpha.alloc(maxit + 1)) || (st = w.state.alloc(1)) || (st = w.norm.alloc(1)))
       return st;
+    // padding rows of the solve space are never written by the fused
+    // operator: they must hold zeros from the start
+    if (N > 0) {
+      BPX_HIP(hipMemsetAsync(w.Ap32.ptr(), 0, (size_t)N * sizeof(float), s));
+      BPX_HIP(hipMemsetAsync(w.p32.ptr(), 0, (size_t)N * sizeof(float), s));
+    }
     w.N = N;
     w.maxit = maxit;
   }
   linAlg_t la(*platform);
   double nb = 0;
-  if (int st = la.norm2(N, o_b, &nb)) return st;
+  if (int st = la.norm2(ogs.Ngather, o_b, &nb)) return st;
   if (nb == 0) {
-    if (int st = la.set(N, 0.0, o_x)) return st;
+    if (int st = la.set(ogs.Ngather, 0.0, o_x)) return st;
     if (stats) stats[0] = stats[1] = stats[2] = 0;
     return platform->finish();
   }
+  double* xs = w.xs.ptr();
+  const double* bs = w.bs.ptr();
+  if (int st = ToSolve(o_x, xs)) return st;
+  if (int st = ToSolve(o_b, w.bs.ptr())) return st;
   // true residual of the initial guess
   auto true_residual = [&](double* rn) -> int {
-    if (int st = Operator(o_x, w.Ax64.ptr())) return st;
-    if (int st = launch_true_residual(s, N, o_b, w.Ax64.ptr(), w.r32.ptr(), w.part_true.ptr(), w.part_rr.ptr()))
+    if (int st = OperatorS(xs, w.Ax64.ptr())) return st;
+    if (int st = launch_true_residual(s, N, bs, w.Ax64.ptr(), w.r32.ptr(), w.part_true.ptr(), w.part_rr.ptr()))
       return st;
     if (int st = launch_fold(s, w.part_true.ptr(), kRedBlocks, w.norm.ptr())) return st;
     BPX_HIP(hipMemcpyAsync(platform->h_scalar, w.norm.ptr(), sizeof(double), hipMemcpyDeviceToHost, s));
@@ -61,31 +74,37 @@ int bp_t::SolveMixed(dfloat* o_x, const dfloat* o_b, double tol, int interval, i
   if (int st = launch_set<float>(s, N, 0.0f, w.c32.ptr())) return st;
   double rn = 0;
   if (int st = true_residual(&rn)) return st;
-  int k = 0, updates = 0;
+  if (!std::isfinite(rn) || !std::isfinite(nb)) return set_error(BPX_ERR_BREAKDOWN, "mixed solve: non-finite residual");
+  int k = 0, kseg = 0, updates = 0;  // kseg: iteration within the current recurrence
   CgState* st_d = w.state.ptr();
   while (rn > tol * nb && k < maxit) {
     const int n = std::min(interval, maxit - k);
-    for (int j = 0; j < n; ++j, ++k) {
+    for (int j = 0; j < n; ++j, ++k, ++kseg) {
       // tol = 0: the recurrence never stops itself; the FP64 test decides
-      if (int st = launch_cg_update_p<float>(s, N, k, w.r32.ptr(), w.p32.ptr(), w.part_rr.ptr(), kRedBlocks, st_d,
-                                             w.hist_rr.ptr(), 0.0))
+      if (int st = launch_cg_update_p<float>(s, N, kseg, w.r32.ptr(), w.p32.ptr(), w.part_rr.ptr(), kRedBlocks,
+                                             st_d, w.hist_rr.ptr(), 0.0))
         return st;
       int npart = 0;
       if (int st = OperatorDot32(w.p32.ptr(), w.Ap32.ptr(), w.part_pAp.ptr(), &npart)) return st;
-      if (int st = launch_cg_update_xr<float>(s, N, k, w.p32.ptr(), w.Ap32.ptr(), w.c32.ptr(), w.r32.ptr(),
+      if (int st = launch_cg_update_xr<float>(s, N, kseg, w.p32.ptr(), w.Ap32.ptr(), w.c32.ptr(), w.r32.ptr(),
                                               w.part_pAp.ptr(), npart, w.part_rr.ptr(), st_d, w.hist_rr.ptr(),
                                               w.hist_alpha.ptr()))
         return st;
     }
     // reliable update: x += c, r = b - A x in FP64, r32 = float(r)
-    if (int st = launch_accumulate_f32(s, N, w.c32.ptr(), o_x)) return st;
+    if (int st = launch_accumulate_f32(s, N, w.c32.ptr(), xs)) return st;
     if (int st = true_residual(&rn)) return st;
     ++updates;
+    if (!std::isfinite(rn)) return set_error(BPX_ERR_BREAKDOWN, "mixed solve: non-finite residual");
     CgState hst;
     BPX_HIP(hipMemcpy(&hst, st_d, sizeof(CgState), hipMemcpyDeviceToHost));
-    if (hst.breakdown || !std::isfinite(rn))
-      return set_error(BPX_ERR_BREAKDOWN, "mixed solve: breakdown in the FP32 recurrence");
+    // the FP32 recurrence stopped (r.r = 0 or p.Ap <= 0, e.g. after it has
+    // resolved the residual below FP32 range): restart it from the true
+    // residual unless that has converged
+    if (hst.done || hst.breakdown) kseg = 0;
   }
+  if (int st = FromSolve(xs, o_x)) return st;
+  if (int st = platform->finish()) return st;
   if (stats) {
     stats[0] = updates;
     stats[1] = k;

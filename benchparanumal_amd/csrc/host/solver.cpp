@@ -266,14 +266,14 @@ int bp_t::Setup(platform_t& p, mesh_t& m, int bp_, dfloat lambda_) {
 
 template <typename T>
 int bp_t::ax(const T* q, T* AqL, const T* wJ, const T* ggeo, const T* gllw, const T* D, const T* I, const T* MM,
-             const T* invV) {
+             const T* invV, const dlong* G2L) {
   AxArgs<T> a;
   a.N = mesh->N;
   a.Nelements = mesh->Nelements;
   a.Nfields = Nfields;
   a.affine = haveKappa ? 0 : mesh->opts.affine;  // kappa-scaled factors are per node
   a.elementList = o_elementList.ptr();
-  a.GlobalToLocal = o_GlobalToLocal.ptr();
+  a.GlobalToLocal = G2L ? G2L : o_GlobalToLocal.ptr();
   a.wJ = wJ;
   a.ggeo = ggeo;
   a.gllw = gllw;
@@ -289,12 +289,12 @@ int bp_t::ax(const T* q, T* AqL, const T* wJ, const T* ggeo, const T* gllw, cons
   return launch_bp3_ax<T>(platform->stream, a);
 }
 
-int bp_t::AxLocal(const dfloat* o_q, dfloat* o_AqL) {
+int bp_t::AxLocal(const dfloat* o_q, dfloat* o_AqL, const dlong* G2L) {
   if (haveKappa)
     return ax<double>(o_q, o_AqL, mesh->opts.affine ? o_wJK.ptr() : o_wJ.ptr(), o_ggeoK.ptr(), o_gllw.ptr(),
-                      o_D.ptr(), nullptr, nullptr, nullptr);
+                      o_D.ptr(), nullptr, nullptr, nullptr, G2L);
   return ax<double>(o_q, o_AqL, o_wJ.ptr(), o_ggeo.ptr(), o_gllw.ptr(), o_D.ptr(), o_I.ptr(), o_MM.ptr(),
-                    o_invV.ptr());
+                    o_invV.ptr(), G2L);
 }
 
 hlong bp_t::LatticeNodes() const {
@@ -348,50 +348,132 @@ int bp_t::Operator(const dfloat* o_q, dfloat* o_Aq) {
   return ogs.Gather<double>(*platform, o_Aq, o_AqL.ptr(), 1);
 }
 
+// The operator with the q.Aq partials on reference-order vectors: through
+// the solve space when the rows are renumbered (the Aq rows are the same
+// sums either way, bit for bit).
 int bp_t::OperatorDot(const dfloat* o_q, dfloat* o_Aq, double* part, int* npart) {
+  if (!permuted) return OperatorDotUnfusedS(o_q, o_Aq, part, npart);
+  if (int st = ToSolve(o_q, o_sx.ptr())) return st;
+  if (int st = OperatorDotS(o_sx.ptr(), o_sr.ptr(), part, npart)) return st;
+  return FromSolve(o_sr.ptr(), o_Aq);
+}
+
+int bp_t::ToSolve(const dfloat* o_ref, dfloat* o_s) {
+  if (!permuted) {
+    if (o_s != o_ref)
+      BPX_HIP(hipMemcpyAsync(o_s, o_ref, (size_t)Ns * sizeof(double), hipMemcpyDeviceToDevice, platform->stream));
+    return BPX_OK;
+  }
+  return launch_permute(platform->stream, Ns, o_sperm.ptr(), o_ref, o_s, false);
+}
+
+int bp_t::FromSolve(const dfloat* o_s, dfloat* o_ref) {
+  if (!permuted) {
+    if (o_s != o_ref)
+      BPX_HIP(hipMemcpyAsync(o_ref, o_s, (size_t)Ns * sizeof(double), hipMemcpyDeviceToDevice, platform->stream));
+    return BPX_OK;
+  }
+  return launch_permute(platform->stream, Ns, o_sperm.ptr(), o_s, o_ref, true);
+}
+
+int bp_t::OperatorS(const dfloat* o_q, dfloat* o_Aq) {
+  if (int st = AxLocal(o_q, o_AqL.ptr(), SolveG2L())) return st;
+  const ogs::ogs_t& G = SolveOgs();
+  return launch_ogs_gather<double>(platform->stream, G.NrowBlocksT, 1, G.o_blockRowStartsT.ptr(),
+                                   G.o_rowStartsT.ptr(), G.o_colIdsT.ptr(), o_AqL.ptr(), o_Aq, ogs.gatherMode);
+}
+
+int bp_t::OperatorDotS(const dfloat* o_q, dfloat* o_Aq, double* part, int* npart) {
   if (fused && tuning.fused64) return OperatorDotFused<double>(o_q, o_Aq, part, npart);
-  if (int st = AxLocal(o_q, o_AqL.ptr())) return st;
-  *npart = ogs_gather_partials(ogs.NrowBlocksT);
-  return launch_ogs_gather_dot<double>(platform->stream, ogs.NrowBlocksT, ogs.o_blockRowStartsT.ptr(),
-                                       ogs.o_rowStartsT.ptr(), ogs.o_colIdsT.ptr(), o_AqL.ptr(), o_Aq, o_q, part,
+  return OperatorDotUnfusedS(o_q, o_Aq, part, npart);
+}
+
+int bp_t::OperatorDotUnfusedS(const dfloat* o_q, dfloat* o_Aq, double* part, int* npart) {
+  if (int st = AxLocal(o_q, o_AqL.ptr(), SolveG2L())) return st;
+  const ogs::ogs_t& G = SolveOgs();
+  *npart = ogs_gather_partials(G.NrowBlocksT);
+  return launch_ogs_gather_dot<double>(platform->stream, G.NrowBlocksT, G.o_blockRowStartsT.ptr(),
+                                       G.o_rowStartsT.ptr(), G.o_colIdsT.ptr(), o_AqL.ptr(), o_Aq, o_q, part,
                                        ogs.gatherMode);
 }
 
-// Flagged GlobalToLocal (single-copy nodes marked) and the maps of the rows
-// with two or more copies, in ascending row order with ascending columns:
-// the shared rows are summed exactly as the full gather sums them, and a
-// single-copy row's sum is its one value, so the fused operator's output
-// equals the unfused one bit for bit.
+int bp_t::SolveCG(cg& solver, dfloat* o_x, dfloat* o_r, dfloat tol, int maxit, int verbose, int* iterations) {
+  if (!permuted) return solver.Solve(*this, o_x, o_r, tol, maxit, verbose, iterations);
+  if (int st = ToSolve(o_x, o_sx.ptr())) return st;
+  if (int st = ToSolve(o_r, o_sr.ptr())) return st;
+  bp_solve_view_t view(*this);
+  const int st = solver.Solve(view, o_sx.ptr(), o_sr.ptr(), tol, maxit, verbose, iterations);
+  if (int s2 = FromSolve(o_sx.ptr(), o_x)) return s2;
+  if (int s2 = FromSolve(o_sr.ptr(), o_r)) return s2;
+  return st;
+}
+
+// The solve space and the fused operator's maps (BP5, one field), built
+// when every element is regular: its interior nodes are the only copies of
+// their rows and each surface node is masked or shared (true of the box
+// mesh, all of whose boundary is masked). Every row's gather lists its
+// copies in ascending local order, as the reference maps do, and an
+// interior row's sum is its one value, so each row of Aq is the same sum in
+// either order and the fused operator equals the unfused one bit for bit.
 int bp_t::PrepareFused() {
   fused = false;
+  permuted = false;
+  Ns = ogs.Ngather;
   if (bp != 5 || Nfields != 1) return BPX_OK;
-  const dlong Ng = ogs.Ngather;
-  if (Ng >= kFusedSingle) return BPX_OK;  // row numbers must leave the flag bit free
-  std::vector<dlong> sidx(Ng, -1), sharedRows;
+  const dlong Ng = ogs.Ngather, Nel = mesh->Nelements;
+  const int Nq = mesh->Nq, Np = mesh->Np;
+  const long long NIP = fused_interior_pad(Nq), NSP = fused_surface_pad(Nq);
+  std::vector<char> shared(Ng);
+  for (dlong r = 0; r < Ng; ++r) shared[r] = (ogs.rowStartsT[r + 1] - ogs.rowStartsT[r]) >= 2;
+  for (dlong e = 0; e < Nel; ++e)
+    for (int k = 0; k < Nq; ++k)
+      for (int j = 0; j < Nq; ++j)
+        for (int i = 0; i < Nq; ++i) {
+          const dlong row = GlobalToLocal[(size_t)e * Np + (k * Nq + j) * Nq + i];
+          const bool interior = fused_surface_index(Nq, i, j, k) < 0;
+          if (interior ? (row < 0 || shared[row]) : (row >= 0 && !shared[row])) return BPX_OK;
+        }
+  std::vector<dlong> srow(Ng, -1);
+  dlong Nshared = 0;
   for (dlong r = 0; r < Ng; ++r)
-    if (ogs.rowStartsT[r + 1] - ogs.rowStartsT[r] >= 2) {
-      sidx[r] = (dlong)sharedRows.size();
-      sharedRows.push_back(r);
-    }
-  std::vector<dlong> G2Lf(NLocal), rowOf(NLocal, -1);
-  for (dlong l = 0; l < NLocal; ++l) {
-    const dlong row = GlobalToLocal[l];
-    if (row < 0) {
-      G2Lf[l] = -1;
-    } else if (sidx[row] < 0) {
-      G2Lf[l] = row | kFusedSingle;
-    } else {
-      G2Lf[l] = row;
-      rowOf[l] = sidx[row];
-    }
-  }
-  if (int st = sharedOgs.SetupRows(NLocal, (dlong)sharedRows.size(), rowOf.data())) return st;
+    if (shared[r]) srow[r] = Nshared++;
+  const long long S0 = fused_pad(Nshared);
+  const long long ns = S0 + NIP * Nel, nslots = NSP * Nel;
+  if (ns >= (1LL << 31) - 1 || nslots >= (1LL << 31) - 1) return BPX_OK;  // 32-bit row and slot numbers
+  std::vector<dlong> G2Ls(NLocal), rowOfSlot(nslots, -1), sharedRows(Nshared);
+  for (dlong e = 0; e < Nel; ++e)
+    for (int k = 0; k < Nq; ++k)
+      for (int j = 0; j < Nq; ++j)
+        for (int i = 0; i < Nq; ++i) {
+          const size_t l = (size_t)e * Np + (k * Nq + j) * Nq + i;
+          const dlong row = GlobalToLocal[l];
+          const int sp = fused_surface_index(Nq, i, j, k);
+          if (sp < 0) {
+            srow[row] = (dlong)(S0 + NIP * e + fused_interior_index(Nq, i, j, k));
+          } else if (row >= 0) {
+            rowOfSlot[(size_t)e * NSP + sp] = srow[row];  // a shared row: solve rows 0..Nshared-1
+          }
+        }
+  Ns = (dlong)ns;
+  std::vector<dlong> perm(Ns, -1);
+  for (dlong r = 0; r < Ng; ++r) perm[srow[r]] = r;
+  for (size_t l = 0; l < (size_t)NLocal; ++l) G2Ls[l] = GlobalToLocal[l] < 0 ? -1 : srow[GlobalToLocal[l]];
+  for (dlong r = 0; r < Nshared; ++r) sharedRows[r] = r;
+  if (int st = solveOgs.SetupRows(NLocal, Ns, G2Ls.data())) return st;
+  if (int st = solveOgs.Upload(*platform)) return st;
+  if (int st = sharedOgs.SetupRows((dlong)nslots, Nshared, rowOfSlot.data())) return st;
   if (int st = sharedOgs.Upload(*platform)) return st;
   hipStream_t s = platform->stream;
-  if (int st = o_G2Lf.upload(G2Lf, s)) return st;
+  if (int st = o_sperm.upload(perm, s)) return st;
+  if (int st = o_G2Ls.upload(G2Ls, s)) return st;
   if (int st = o_sharedRows.upload(sharedRows, s)) return st;
-  naxParts = bp5_workgroups(mesh->N, mesh->Nelements);
+  if (int st = o_slots.alloc(nslots)) return st;
+  if (int st = o_sx.alloc(Ns)) return st;
+  if (int st = o_sr.alloc(Ns)) return st;
+  naxParts = bp5_workgroups(mesh->N, Nel);
   if (int st = o_partAx.alloc(std::max(naxParts, 1))) return st;
+  fusedS0 = (dlong)S0;
+  permuted = true;
   fused = true;
   return BPX_OK;
 }
@@ -404,9 +486,10 @@ int bp_t::OperatorDotFused(const T* o_q, T* o_Aq, double* part, int* npart) {
   a.Nelements = mesh->Nelements;
   a.Nfields = 1;
   a.elementList = o_elementList.ptr();
-  a.GlobalToLocal = o_G2Lf.ptr();
+  a.GlobalToLocal = o_G2Ls.ptr();
   a.lambda = (T)lambda;
   a.q = o_q;
+  a.ldq = fusedS0;  // the first interior row
   T* AqL;
   if constexpr (std::is_same<T, double>::value) {
     a.affine = haveKappa ? 0 : mesh->opts.affine;
@@ -414,7 +497,7 @@ int bp_t::OperatorDotFused(const T* o_q, T* o_Aq, double* part, int* npart) {
     a.ggeo = haveKappa ? o_ggeoK.ptr() : o_ggeo.ptr();
     a.gllw = o_gllw.ptr();
     a.D = o_D.ptr();
-    AqL = o_AqL.ptr();
+    AqL = o_slots.ptr();
   } else {
     if (haveKappa) return set_error(BPX_ERR_UNSUPPORTED, "bp: the FP32 operator has no variable coefficient");
     if (int st = Prepare32()) return st;
@@ -423,7 +506,7 @@ int bp_t::OperatorDotFused(const T* o_q, T* o_Aq, double* part, int* npart) {
     a.ggeo = o_ggeo32.ptr();
     a.gllw = o_gllw32.ptr();
     a.D = o_D32.ptr();
-    AqL = o_AqL32.ptr();
+    AqL = o_slots32.ptr();
   }
   a.Aq = AqL;
   if (int st = launch_bp5_fused<T>(s, a, o_Aq, o_partAx.ptr())) return st;
@@ -487,11 +570,12 @@ int bp_t::OperatorDot32(const float* o_q, float* o_Aq, double* part, int* npart)
 
 int bp_t::OperatorDotUnfused32(const float* o_q, float* o_Aq, double* part, int* npart) {
   if (int st = ax<float>(o_q, o_AqL32.ptr(), o_wJ32.ptr(), o_ggeo32.ptr(), o_gllw32.ptr(), o_D32.ptr(),
-                         o_I32.ptr(), o_MM32.ptr(), o_invV32.ptr()))
+                         o_I32.ptr(), o_MM32.ptr(), o_invV32.ptr(), SolveG2L()))
     return st;
-  *npart = ogs_gather_partials(ogs.NrowBlocksT);
-  return launch_ogs_gather_dot<float>(platform->stream, ogs.NrowBlocksT, ogs.o_blockRowStartsT.ptr(),
-                                      ogs.o_rowStartsT.ptr(), ogs.o_colIdsT.ptr(), o_AqL32.ptr(), o_Aq, o_q, part,
+  const ogs::ogs_t& G = SolveOgs();
+  *npart = ogs_gather_partials(G.NrowBlocksT);
+  return launch_ogs_gather_dot<float>(platform->stream, G.NrowBlocksT, G.o_blockRowStartsT.ptr(),
+                                      G.o_rowStartsT.ptr(), G.o_colIdsT.ptr(), o_AqL32.ptr(), o_Aq, o_q, part,
                                       ogs.gatherMode);
 }
 
@@ -549,14 +633,16 @@ int bp_t::Tune() {
   ogs.gatherMode = tuning.gatherMode;
   if (!fused) return BPX_OK;
 
-  // the CG operator: fused (Ax writes single-copy rows, shared rows
-  // gathered) against unfused (Ax, then the full gather); same bits
+  // the CG operator in the solve space: fused (Ax writes single-copy rows,
+  // shared rows gathered) against unfused (Ax, then the full gather); same bits
+  deviceBuffer<double> w;
+  if (int st = w.alloc(2 * (size_t)Ns)) return st;
+  BPX_HIP(hipMemsetAsync(w.ptr(), 0, 2 * (size_t)Ns * sizeof(double), s));
   int np = 0;
-  if (int st = time_best(s, [&] { return OperatorDotFused<double>(v.ptr(), v.ptr() + Ng, part.ptr(), &np); },
+  if (int st = time_best(s, [&] { return OperatorDotFused<double>(w.ptr(), w.ptr() + Ns, part.ptr(), &np); },
                          tuning.fused_us))
     return st;
-  tuning.fused64 = 0;
-  if (int st = time_best(s, [&] { return OperatorDot(v.ptr(), v.ptr() + Ng, part.ptr(), &np); },
+  if (int st = time_best(s, [&] { return OperatorDotUnfusedS(w.ptr(), w.ptr() + Ns, part.ptr(), &np); },
                          tuning.unfused_us))
     return st;
   tuning.fused64 = tuning.fused_us < tuning.unfused_us ? 1 : 0;
@@ -567,18 +653,17 @@ int bp_t::Tune() {
 int bp_t::Tune32() {
   if (!fused) return BPX_OK;
   hipStream_t s = platform->stream;
-  const dlong Ng = ogs.Ngather;
-  if (Ng == 0) return BPX_OK;
+  if (Ns == 0) return BPX_OK;
   deviceBuffer<float> v;
   deviceBuffer<double> part;
-  if (int st = v.alloc(2 * (size_t)Ng)) return st;
+  if (int st = v.alloc(2 * (size_t)Ns)) return st;
   if (int st = part.alloc(GatherPartials())) return st;
-  BPX_HIP(hipMemsetAsync(v.ptr(), 0, 2 * (size_t)Ng * sizeof(float), s));
+  BPX_HIP(hipMemsetAsync(v.ptr(), 0, 2 * (size_t)Ns * sizeof(float), s));
   int np = 0;
-  if (int st = time_best(s, [&] { return OperatorDotFused<float>(v.ptr(), v.ptr() + Ng, part.ptr(), &np); },
+  if (int st = time_best(s, [&] { return OperatorDotFused<float>(v.ptr(), v.ptr() + Ns, part.ptr(), &np); },
                          tuning.fused32_us))
     return st;
-  if (int st = time_best(s, [&] { return OperatorDotUnfused32(v.ptr(), v.ptr() + Ng, part.ptr(), &np); },
+  if (int st = time_best(s, [&] { return OperatorDotUnfused32(v.ptr(), v.ptr() + Ns, part.ptr(), &np); },
                          tuning.unfused32_us))
     return st;
   tuning.fused32 = tuning.fused32_us < tuning.unfused32_us ? 1 : 0;
@@ -609,6 +694,7 @@ int bp_t::Prepare32() {
     if (affine && (st = cvt(o_invV32, m.invV))) return st;
   }
   if ((st = o_AqL32.alloc(NLocal))) return st;
+  if (fused && (st = o_slots32.alloc(o_slots.length()))) return st;
   if ((st = platform->finish())) return st;
   have32 = true;
   if ((st = Tune32())) return st;

@@ -326,6 +326,24 @@ __global__ __launch_bounds__(ST) void k_accumulate_f32(dlong N, float* c32, doub
   }
 }
 
+// Row permutations between the reference row order and a solver's own
+// order: perm[i] is the reference row of row i, or -1 for a padding row.
+__global__ __launch_bounds__(ST) void k_permute_in(dlong N, const dlong* __restrict__ perm,
+                                                   const double* __restrict__ ref, double* __restrict__ out) {
+  for (dlong n = blockIdx.x * ST + threadIdx.x; n < N; n += gridDim.x * ST) {
+    const dlong r = perm[n];
+    out[n] = (r >= 0) ? ref[r] : 0.0;
+  }
+}
+
+__global__ __launch_bounds__(ST) void k_permute_out(dlong N, const dlong* __restrict__ perm,
+                                                    const double* __restrict__ v, double* __restrict__ ref) {
+  for (dlong n = blockIdx.x * ST + threadIdx.x; n < N; n += gridDim.x * ST) {
+    const dlong r = perm[n];
+    if (r >= 0) ref[r] = v[n];
+  }
+}
+
 // Reliable update, part 2: true residual r = b - Ax in FP64, handed to the
 // FP32 recurrence as r32 = float(r), with partials of r.r (FP64, for the
 // stopping test) and of r32.r32 (the recurrence's own r.r).
@@ -469,6 +487,17 @@ int launch_cg_finish(hipStream_t s, int kend, int maxit, const double* part_rr, 
 int launch_accumulate_f32(hipStream_t s, dlong N, float* c32, double* x64) {
   if (N <= 0) return BPX_OK;
   hipLaunchKernelGGL(k_accumulate_f32, dim3(stream_blocks(N)), dim3(ST), 0, s, N, c32, x64);
+  BPX_HIP(hipGetLastError());
+  return BPX_OK;
+}
+
+int launch_permute(hipStream_t s, dlong N, const dlong* perm, const double* in, double* out, bool to_ref) {
+  if (N <= 0) return BPX_OK;
+  if (!perm || !in || !out) return set_error(BPX_ERR_INVALID_ARGUMENT, "permute: null pointer");
+  if (to_ref)
+    hipLaunchKernelGGL(k_permute_out, dim3(stream_blocks(N)), dim3(ST), 0, s, N, perm, in, out);
+  else
+    hipLaunchKernelGGL(k_permute_in, dim3(stream_blocks(N)), dim3(ST), 0, s, N, perm, in, out);
   BPX_HIP(hipGetLastError());
   return BPX_OK;
 }

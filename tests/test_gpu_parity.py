@@ -232,8 +232,9 @@ def test_mixed_precision_reaches_1e10(plat, bp):
     assert np.linalg.norm(host(x) - x_ref) / np.linalg.norm(x_ref) < 1e-7
 
 
-@pytest.mark.parametrize("N,nx,affine", [(1, 3, False), (3, 3, False), (4, 3, False), (5, 2, True), (5, 3, False),
-                                         (7, 4, False), (7, 1, False), (8, 3, False), (11, 2, False)])
+@pytest.mark.parametrize("N,nx,affine", [(N, 2, False) for N in range(1, 16)] +
+                         [(3, 3, False), (5, 2, True), (5, 3, False), (7, 4, False), (7, 1, False), (8, 3, False),
+                          (9, 2, True)])
 def test_fused_operator_bit_exact(plat, N, nx, affine):
     """The operator as CG applies it, fused (BP5: single-copy nodes written
     by the Ax straight into Aq, shared rows gathered) and unfused, equals
@@ -291,6 +292,27 @@ def test_launch_choices_bit_identical(plat, N, affine):
     np.testing.assert_array_equal(res[1][1], res[0][1])
     t = b.tuning()
     assert t["fused32"] in (0, 1)
+
+
+@pytest.mark.parametrize("N", [1, 2, 3, 5, 6, 8, 9, 10, 12, 15])
+def test_fp32_fused_operator_solves_identically(plat, N):
+    """The mixed solve with the FP32 fused operator and with the FP32
+    unfused one: same iterations, same bits (p = 1 with nx = 2 has a single
+    unknown, which the FP32 recurrence resolves exactly)."""
+    m = capi.Mesh(N=N, nx=2, perturb=0.1, seed=4)
+    b = capi.BP(plat, m, 5, lam=1.1)
+    rhs = dev(capi.random_normal(3, b.Ngather))
+    res = []
+    for fz in (0, 1):
+        b.set_tuning(fused32=fz)
+        x = torch.zeros_like(rhs)
+        st = b.solve_mixed(x, rhs, tol=1e-9, interval=16)
+        res.append((st["iterations"], host(x)))
+        Ax = torch.zeros_like(x)
+        b.operator(x, Ax)
+        assert np.linalg.norm(host(rhs) - host(Ax)) <= 1e-9 * np.linalg.norm(host(rhs)) * (1 + 1e-9)
+    assert res[0][0] == res[1][0]
+    np.testing.assert_array_equal(res[0][1], res[1][1])
 
 
 def test_set_tuning_errors(plat):
