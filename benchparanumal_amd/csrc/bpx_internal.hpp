@@ -221,12 +221,16 @@ struct CgState {
 template <typename T>
 int launch_cg_init(hipStream_t s, dlong N, const T* Ap, T* r, double* part_rr);
 template <typename T>
-int launch_cg_update_p(hipStream_t s, dlong N, int k, const T* r, T* p, const double* part_rr, int nrr,
-                       CgState* st, double* hist_rr, double tol);
+int launch_cg_update_p(hipStream_t s, dlong N, int k, const T* r, T* p, T* x, const double* part_rr, int nrr,
+                       CgState* st, double* hist_rr, const double* hist_alpha, double tol, bool pending);
 template <typename T>
-int launch_cg_update_xr(hipStream_t s, dlong N, int k, const T* p, const T* Ap, T* x, T* r,
-                        const double* part_pAp, int npAp, double* part_rr, CgState* st, const double* hist_rr,
-                        double* hist_alpha);
+int launch_cg_update_r(hipStream_t s, dlong N, int k, const T* Ap, T* r, const double* part_pAp, int npAp,
+                       double* part_rr, CgState* st, const double* hist_rr, double* hist_alpha);
+// x <- x + alpha_k p (the deferred update of the last iteration k);
+// when_running: skipped if the recurrence has stopped
+template <typename T>
+int launch_cg_flush_x(hipStream_t s, dlong N, int k, const T* p, T* x, const double* hist_alpha,
+                      const CgState* st, bool when_running);
 int launch_cg_finish(hipStream_t s, int kend, int maxit, const double* part_rr, int nrr, CgState* st,
                      double* hist_rr);
 

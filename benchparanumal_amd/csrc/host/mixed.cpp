@@ -80,17 +80,22 @@ int bp_t::SolveMixed(dfloat* o_x, const dfloat* o_b, double tol, int interval, i
   while (rn > tol * nb && k < maxit) {
     const int n = std::min(interval, maxit - k);
     for (int j = 0; j < n; ++j, ++k, ++kseg) {
-      // tol = 0: the recurrence never stops itself; the FP64 test decides
-      if (int st = launch_cg_update_p<float>(s, N, kseg, w.r32.ptr(), w.p32.ptr(), w.part_rr.ptr(), kRedBlocks,
-                                             st_d, w.hist_rr.ptr(), 0.0))
+      // tol = 0: the recurrence never stops itself; the FP64 test decides.
+      // c += alpha p of the previous iteration rides along (not across a
+      // correction, before which it is flushed)
+      if (int st = launch_cg_update_p<float>(s, N, kseg, w.r32.ptr(), w.p32.ptr(), w.c32.ptr(), w.part_rr.ptr(),
+                                             kRedBlocks, st_d, w.hist_rr.ptr(), w.hist_alpha.ptr(), 0.0, j > 0))
         return st;
       int npart = 0;
       if (int st = OperatorDot32(w.p32.ptr(), w.Ap32.ptr(), w.part_pAp.ptr(), &npart)) return st;
-      if (int st = launch_cg_update_xr<float>(s, N, kseg, w.p32.ptr(), w.Ap32.ptr(), w.c32.ptr(), w.r32.ptr(),
-                                              w.part_pAp.ptr(), npart, w.part_rr.ptr(), st_d, w.hist_rr.ptr(),
-                                              w.hist_alpha.ptr()))
+      if (int st = launch_cg_update_r<float>(s, N, kseg, w.Ap32.ptr(), w.r32.ptr(), w.part_pAp.ptr(), npart,
+                                             w.part_rr.ptr(), st_d, w.hist_rr.ptr(), w.hist_alpha.ptr()))
         return st;
     }
+    // the last iteration's c update (unless the recurrence stopped, which
+    // has applied or dropped it already)
+    if (int st = launch_cg_flush_x<float>(s, N, kseg - 1, w.p32.ptr(), w.c32.ptr(), w.hist_alpha.ptr(), st_d, true))
+      return st;
     // reliable update: x += c, r = b - A x in FP64, r32 = float(r)
     if (int st = launch_accumulate_f32(s, N, w.c32.ptr(), xs)) return st;
     if (int st = true_residual(&rn)) return st;

@@ -138,7 +138,8 @@ int cg_t<T>::Solve(const Apply& apply, const ApplyDot& apply_dot, T* o_x, T* o_r
   if (int st = launch_cg_init<T>(s, N, o_Ap.ptr(), o_r, prr)) return st;
   CgState hst;
   if (MAXIT == 0) {  // only the initial residual and the stopping test
-    if (int st = launch_cg_update_p<T>(s, N, 0, o_r, o_p.ptr(), prr, kRedBlocks, st_d, o_hist_rr.ptr(), tol))
+    if (int st = launch_cg_update_p<T>(s, N, 0, o_r, o_p.ptr(), o_x, prr, kRedBlocks, st_d, o_hist_rr.ptr(),
+                                       o_hist_alpha.ptr(), tol, false))
       return st;
   }
   const int chunk = (tol > 0) ? 16 : std::max(MAXIT, 1);
@@ -146,7 +147,9 @@ int cg_t<T>::Solve(const Apply& apply, const ApplyDot& apply_dot, T* o_x, T* o_r
   while (queued < MAXIT) {
     const int n = std::min(chunk, MAXIT - queued);
     for (int k = queued; k < queued + n; ++k) {
-      if (int st = launch_cg_update_p<T>(s, N, k, o_r, o_p.ptr(), prr, kRedBlocks, st_d, o_hist_rr.ptr(), tol))
+      // also x += alpha_{k-1} p, the x update of iteration k-1
+      if (int st = launch_cg_update_p<T>(s, N, k, o_r, o_p.ptr(), o_x, prr, kRedBlocks, st_d, o_hist_rr.ptr(),
+                                         o_hist_alpha.ptr(), tol, k > 0))
         return st;
       int npart = kRedBlocks;
       if (apply_dot) {
@@ -155,8 +158,8 @@ int cg_t<T>::Solve(const Apply& apply, const ApplyDot& apply_dot, T* o_x, T* o_r
         if (int st = apply(o_p.ptr(), o_Ap.ptr())) return st;
         if (int st = launch_dot_partials<T>(s, N, o_p.ptr(), o_Ap.ptr(), ppAp)) return st;
       }
-      if (int st = launch_cg_update_xr<T>(s, N, k, o_p.ptr(), o_Ap.ptr(), o_x, o_r, ppAp, npart, prr, st_d,
-                                          o_hist_rr.ptr(), o_hist_alpha.ptr()))
+      if (int st = launch_cg_update_r<T>(s, N, k, o_Ap.ptr(), o_r, ppAp, npart, prr, st_d, o_hist_rr.ptr(),
+                                         o_hist_alpha.ptr()))
         return st;
     }
     queued += n;
@@ -169,6 +172,13 @@ int cg_t<T>::Solve(const Apply& apply, const ApplyDot& apply_dot, T* o_x, T* o_r
   }
   BPX_HIP(hipMemcpyAsync(&hst, st_d, sizeof(CgState), hipMemcpyDeviceToHost, s));
   BPX_HIP(hipStreamSynchronize(s));
+  // stopped after its last queued iteration (converged there, or MAXIT):
+  // that iteration's x update is still to be applied
+  if (MAXIT > 0 && hst.done && !hst.breakdown && hst.iter == queued) {
+    if (int st = launch_cg_flush_x<T>(s, N, queued - 1, o_p.ptr(), o_x, o_hist_alpha.ptr(), st_d, false))
+      return st;
+    BPX_HIP(hipStreamSynchronize(s));
+  }
   if (iterations) *iterations = (MAXIT == 0) ? 0 : hst.iter;
   if (MAXIT > 0 && hst.breakdown) return set_error(BPX_ERR_BREAKDOWN, "cg: breakdown (non-finite or non-positive p.Ap / r.r)");
   return BPX_OK;

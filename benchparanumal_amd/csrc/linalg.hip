@@ -184,12 +184,16 @@ __global__ __launch_bounds__(RT) void k_cg_init(dlong N, const T* __restrict__ A
 }
 
 // Start of iteration k: rr_k = r.r from the partials, stopping test,
-// beta = rr_k / rr_{k-1}, p <- r + beta p (p <- r at k = 0). VEC: float4
-// accesses (all pointers 16-byte aligned, float only).
+// beta = rr_k / rr_{k-1}, p <- r + beta p (p <- r at k = 0). With pending
+// (k > 0), first the x update of iteration k-1, x <- x + alpha_{k-1} p,
+// which reads the same p (it is deferred to here from the end of iteration
+// k-1 so that x and p share one pass). VEC: float4 accesses (all pointers
+// 16-byte aligned, float only).
 template <typename T, bool VEC>
 __global__ __launch_bounds__(RT) void k_cg_update_p(dlong N, int k, const T* __restrict__ r, T* __restrict__ p,
-                                                    const double* __restrict__ part_rr, int nrr, CgState* st,
-                                                    double* __restrict__ hist_rr, double tol) {
+                                                    T* __restrict__ x, const double* __restrict__ part_rr, int nrr,
+                                                    CgState* st, double* __restrict__ hist_rr,
+                                                    const double* __restrict__ hist_alpha, double tol, int pending) {
   __shared__ double s_w[33];
   if (k > 0 && st->done) return;
   double rr = fold_partials(part_rr, nrr, s_w);
@@ -215,41 +219,64 @@ __global__ __launch_bounds__(RT) void k_cg_update_p(dlong N, int k, const T* __r
       if (!isfinite(rr)) st->breakdown = 1;
     }
   }
-  if (stop) return;
+  const bool apply = pending && k > 0;
+  const T a = apply ? (T)hist_alpha[k - 1] : T(0);
   const bool first = (k == 0);
+  const bool upd = !stop;
   const T b = first ? T(0) : (T)(rr / hist_rr[k - 1]);
+  if (!apply && !upd) return;
   if (VEC) {
     typedef typename vec4<T>::type V;
     const V* r4 = reinterpret_cast<const V*>(r);
     V* p4 = reinterpret_cast<V*>(p);
+    V* x4 = reinterpret_cast<V*>(x);
     strided_for(N / 4, [&](long long n) {
-      const V rv = r4[n];
-      V pv = first ? rv : p4[n];
-      if (!first) {
-        pv.x = T(1) * rv.x + b * pv.x;
-        pv.y = T(1) * rv.y + b * pv.y;
-        pv.z = T(1) * rv.z + b * pv.z;
-        pv.w = T(1) * rv.w + b * pv.w;
+      V pv = p4[n];
+      if (apply) {
+        V xv = x4[n];
+        xv.x = a * pv.x + T(1) * xv.x; xv.y = a * pv.y + T(1) * xv.y;
+        xv.z = a * pv.z + T(1) * xv.z; xv.w = a * pv.w + T(1) * xv.w;
+        x4[n] = xv;
       }
-      p4[n] = pv;
+      if (upd) {
+        const V rv = r4[n];
+        if (first) {
+          pv = rv;
+        } else {
+          pv.x = T(1) * rv.x + b * pv.x;
+          pv.y = T(1) * rv.y + b * pv.y;
+          pv.z = T(1) * rv.z + b * pv.z;
+          pv.w = T(1) * rv.w + b * pv.w;
+        }
+        p4[n] = pv;
+      }
     });
     const long long tail = (long long)(N / 4) * 4;
     const long long n = tail + (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n < N) p[n] = first ? r[n] : T(1) * r[n] + b * p[n];
+    if (n < N) {
+      const T pv = p[n];
+      if (apply) x[n] = a * pv + T(1) * x[n];
+      if (upd) p[n] = first ? r[n] : T(1) * r[n] + b * pv;
+    }
   } else {
-    strided_for(N, [&](long long n) { p[n] = first ? r[n] : T(1) * r[n] + b * p[n]; });
+    strided_for(N, [&](long long n) {
+      const T pv = p[n];
+      if (apply) x[n] = a * pv + T(1) * x[n];
+      if (upd) p[n] = first ? r[n] : T(1) * r[n] + b * pv;
+    });
   }
 }
 
-// End of iteration k: alpha = rr_k / p.Ap, r <- r - alpha Ap, x <- x + alpha p,
-// r.r partials for iteration k+1. VEC: float4 accesses (float only).
+// End of iteration k: alpha = rr_k / p.Ap, r <- r - alpha Ap, r.r partials
+// for iteration k+1 (x <- x + alpha p is left to the next k_cg_update_p,
+// or to k_cg_flush_x after the last iteration). VEC: float4 accesses
+// (float only).
 template <typename T, bool VEC>
-__global__ __launch_bounds__(RT) void k_cg_update_xr(dlong N, int k, const T* __restrict__ p,
-                                                     const T* __restrict__ Ap, T* __restrict__ x,
-                                                     T* __restrict__ r, const double* __restrict__ part_pAp,
-                                                     int npAp, double* __restrict__ part_rr, CgState* st,
-                                                     const double* __restrict__ hist_rr,
-                                                     double* __restrict__ hist_alpha) {
+__global__ __launch_bounds__(RT) void k_cg_update_r(dlong N, int k, const T* __restrict__ Ap, T* __restrict__ r,
+                                                    const double* __restrict__ part_pAp, int npAp,
+                                                    double* __restrict__ part_rr, CgState* st,
+                                                    const double* __restrict__ hist_rr,
+                                                    double* __restrict__ hist_alpha) {
   __shared__ double s_w[33];
   if (st->done) return;
   const double pAp = fold_partials(part_pAp, npAp, s_w);
@@ -267,19 +294,13 @@ __global__ __launch_bounds__(RT) void k_cg_update_xr(dlong N, int k, const T* __
   double v;
   if (VEC) {
     typedef typename vec4<T>::type V;
-    const V* p4 = reinterpret_cast<const V*>(p);
     const V* A4 = reinterpret_cast<const V*>(Ap);
-    V* x4 = reinterpret_cast<V*>(x);
     V* r4 = reinterpret_cast<V*>(r);
     v = strided_sum(N / 4, [&](long long n) {
       V rv = r4[n];
-      const V Av = A4[n], pv = p4[n];
-      V xv = x4[n];
+      const V Av = A4[n];
       rv.x = rv.x - a * Av.x; rv.y = rv.y - a * Av.y; rv.z = rv.z - a * Av.z; rv.w = rv.w - a * Av.w;
-      xv.x = a * pv.x + T(1) * xv.x; xv.y = a * pv.y + T(1) * xv.y;
-      xv.z = a * pv.z + T(1) * xv.z; xv.w = a * pv.w + T(1) * xv.w;
       r4[n] = rv;
-      x4[n] = xv;
       return ((double)rv.x * (double)rv.x + (double)rv.y * (double)rv.y) +
              ((double)rv.z * (double)rv.z + (double)rv.w * (double)rv.w);
     });
@@ -288,19 +309,29 @@ __global__ __launch_bounds__(RT) void k_cg_update_xr(dlong N, int k, const T* __
     if (n < N) {
       const T rn = r[n] - a * Ap[n];
       r[n] = rn;
-      x[n] = a * p[n] + T(1) * x[n];
       v += (double)rn * (double)rn;
     }
   } else {
     v = strided_sum(N, [&](long long n) {
       const T rn = r[n] - a * Ap[n];
       r[n] = rn;
-      x[n] = a * p[n] + T(1) * x[n];
       return (double)rn * (double)rn;
     });
   }
   const double t = block_allsum(v, s_w);
   if (threadIdx.x == 0) part_rr[blockIdx.x] = t;
+}
+
+// The deferred x update of iteration k: x <- x + alpha_k p. when_running:
+// only if the recurrence has not stopped (a stop inside k_cg_update_p has
+// applied it already, a breakdown leaves its iteration out).
+template <typename T>
+__global__ __launch_bounds__(ST) void k_cg_flush_x(dlong N, int k, const T* __restrict__ p, T* __restrict__ x,
+                                                   const double* __restrict__ hist_alpha, const CgState* st,
+                                                   int when_running) {
+  if (when_running && st->done) return;
+  const T a = (T)hist_alpha[k];
+  for (dlong n = blockIdx.x * ST + threadIdx.x; n < N; n += gridDim.x * ST) x[n] = a * p[n] + T(1) * x[n];
 }
 
 // After the last queued iteration kend: record rr_kend and the state.
@@ -451,28 +482,37 @@ inline bool aligned16(const void* a) { return ((uintptr_t)a & 15) == 0; }
 // float streams use 16-byte accesses when every pointer allows it; FP64
 // keeps the scalar form, whose partial sums the parity tests pin
 template <typename T>
-int launch_cg_update_p(hipStream_t s, dlong N, int k, const T* r, T* p, const double* part_rr, int nrr,
-                       CgState* st, double* hist_rr, double tol) {
-  if (std::is_same<T, float>::value && aligned16(r) && aligned16(p))
-    hipLaunchKernelGGL((k_cg_update_p<T, true>), dim3(RB), dim3(RT), 0, s, N, k, r, p, part_rr, nrr, st, hist_rr,
-                       tol);
+int launch_cg_update_p(hipStream_t s, dlong N, int k, const T* r, T* p, T* x, const double* part_rr, int nrr,
+                       CgState* st, double* hist_rr, const double* hist_alpha, double tol, bool pending) {
+  if (std::is_same<T, float>::value && aligned16(r) && aligned16(p) && aligned16(x))
+    hipLaunchKernelGGL((k_cg_update_p<T, true>), dim3(RB), dim3(RT), 0, s, N, k, r, p, x, part_rr, nrr, st,
+                       hist_rr, hist_alpha, tol, (int)pending);
   else
-    hipLaunchKernelGGL((k_cg_update_p<T, false>), dim3(RB), dim3(RT), 0, s, N, k, r, p, part_rr, nrr, st,
-                       hist_rr, tol);
+    hipLaunchKernelGGL((k_cg_update_p<T, false>), dim3(RB), dim3(RT), 0, s, N, k, r, p, x, part_rr, nrr, st,
+                       hist_rr, hist_alpha, tol, (int)pending);
   BPX_HIP(hipGetLastError());
   return BPX_OK;
 }
 
 template <typename T>
-int launch_cg_update_xr(hipStream_t s, dlong N, int k, const T* p, const T* Ap, T* x, T* r,
-                        const double* part_pAp, int npAp, double* part_rr, CgState* st, const double* hist_rr,
-                        double* hist_alpha) {
-  if (std::is_same<T, float>::value && aligned16(p) && aligned16(Ap) && aligned16(x) && aligned16(r))
-    hipLaunchKernelGGL((k_cg_update_xr<T, true>), dim3(RB), dim3(RT), 0, s, N, k, p, Ap, x, r, part_pAp, npAp,
+int launch_cg_update_r(hipStream_t s, dlong N, int k, const T* Ap, T* r, const double* part_pAp, int npAp,
+                       double* part_rr, CgState* st, const double* hist_rr, double* hist_alpha) {
+  if (std::is_same<T, float>::value && aligned16(Ap) && aligned16(r))
+    hipLaunchKernelGGL((k_cg_update_r<T, true>), dim3(RB), dim3(RT), 0, s, N, k, Ap, r, part_pAp, npAp,
                        part_rr, st, hist_rr, hist_alpha);
   else
-    hipLaunchKernelGGL((k_cg_update_xr<T, false>), dim3(RB), dim3(RT), 0, s, N, k, p, Ap, x, r, part_pAp, npAp,
+    hipLaunchKernelGGL((k_cg_update_r<T, false>), dim3(RB), dim3(RT), 0, s, N, k, Ap, r, part_pAp, npAp,
                        part_rr, st, hist_rr, hist_alpha);
+  BPX_HIP(hipGetLastError());
+  return BPX_OK;
+}
+
+template <typename T>
+int launch_cg_flush_x(hipStream_t s, dlong N, int k, const T* p, T* x, const double* hist_alpha,
+                      const CgState* st, bool when_running) {
+  if (N <= 0 || k < 0) return BPX_OK;
+  hipLaunchKernelGGL(k_cg_flush_x<T>, dim3(stream_blocks(N)), dim3(ST), 0, s, N, k, p, x, hist_alpha, st,
+                     (int)when_running);
   BPX_HIP(hipGetLastError());
   return BPX_OK;
 }
@@ -524,10 +564,11 @@ int launch_update_cg(hipStream_t s, dlong N, double alpha, const double* p, cons
   template int launch_dot_partials<T>(hipStream_t, dlong, const T*, const T*, double*);                       \
   template int launch_dot<T>(hipStream_t, dlong, const T*, const T*, double*, double*);                       \
   template int launch_cg_init<T>(hipStream_t, dlong, const T*, T*, double*);                                  \
-  template int launch_cg_update_p<T>(hipStream_t, dlong, int, const T*, T*, const double*, int, CgState*,     \
-                                     double*, double);                                                        \
-  template int launch_cg_update_xr<T>(hipStream_t, dlong, int, const T*, const T*, T*, T*, const double*, int, \
-                                      double*, CgState*, const double*, double*);
+  template int launch_cg_update_p<T>(hipStream_t, dlong, int, const T*, T*, T*, const double*, int, CgState*, \
+                                     double*, const double*, double, bool);                                   \
+  template int launch_cg_update_r<T>(hipStream_t, dlong, int, const T*, T*, const double*, int, double*,      \
+                                     CgState*, const double*, double*);                                       \
+  template int launch_cg_flush_x<T>(hipStream_t, dlong, int, const T*, T*, const double*, const CgState*, bool);
 BPX_LINALG_INST(double)
 BPX_LINALG_INST(float)
 
