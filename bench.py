@@ -10,9 +10,12 @@ right-hand side at the 167^3 = 4,657,463 masked DOFs):
 
 value = Ndofs * N_it(FP64 CG to 1e-10) / T_step, the FP64-equivalent CG
 rate in GDOF*it/s; the plain FP64 CG (same tolerance) is timed beside it.
-The roofline object covers the dominant kernel, the FP64 BP5 Ax, timed
-with HIP events on the stream it is launched on; its algorithmic bytes are
-the reference model (BK/BK5/src/bk5Run.cpp:116-122). The CPU baseline is the
+The roofline object covers the dominant kernel of the timed solve, the Ax
+of the FP32 CG operator (the fused form when the setup's timing picked it),
+timed kernel by kernel with HIP events on the stream it is launched on; its
+algorithmic bytes are the reference model (BK/BK5/src/bk5Run.cpp:116-122)
+with 4-byte values. The FP64 BK5 Ax (the reference's Ax benchmark) is
+reported beside it with the FP64 model. The CPU baseline is the
 oracle (C++/OpenMP restatement of the reference kernels) running a bounded
 number of FP64 CG iterations of the same problem on the host cores.
 """
@@ -49,13 +52,14 @@ def physical_cores():
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_bp5_ax_fp64_p7_24cube_hbm_traffic.txt")
+TRAFFIC_FILE_F32 = os.path.join(ROOT, "profiles", "k_bp5_ax_fp32_cg_operator_p7_24cube_hbm_traffic.txt")
 
 
-def measured_traffic():
+def measured_traffic(path=TRAFFIC_FILE):
     """HBM bytes per Ax launch from the committed counter summary (FETCH_SIZE and
     WRITE_SIZE scaled by a plain copy of known size), or None."""
     try:
-        for line in open(TRAFFIC_FILE):
+        for line in open(path):
             if line.startswith("traffic_bytes_per_launch:"):
                 return float(line.split(":", 1)[1])
     except OSError:
@@ -63,9 +67,11 @@ def measured_traffic():
     return None
 
 
-def bk5_bytes(Ndofs, Nel, Np):
-    # BK/BK5/src/bk5Run.cpp:116-122 (non-affine hex, FP64, 4-byte dlong)
-    return Ndofs * 8 + Nel * (Np * 7 * 8 + 4 + Np * 4 + Np * 8)
+def bk5_bytes(Ndofs, Nel, Np, value_bytes=8):
+    # BK/BK5/src/bk5Run.cpp:116-122 (non-affine hex, 4-byte dlong); value_bytes
+    # = 4 for the FP32 operator (values float, indices unchanged)
+    w = value_bytes
+    return Ndofs * w + Nel * (Np * 7 * w + 4 + Np * 4 + Np * w)
 
 
 def krhs_bytes(K, Ndofs, Nel, Np):
@@ -323,9 +329,11 @@ def main():
             bp.ax_local(q, AqL)
         for _ in range(args.ax_launches):
             bp.operator(q, Aq)
+        op32 = bp.time_operator(32, args.ax_launches)
         torch.cuda.synchronize()
         log(f"ax-only: {args.ax_launches} BK5 Ax launches, {args.ax_launches} operator (Ax + gather) "
-            f"launches and one 512 MiB copy done")
+            f"launches, {args.ax_launches + 3} FP32 CG operator launches (fused: {op32['fused']}) "
+            f"and one 512 MiB copy done")
         return
 
     b_host = capi.random_normal(7, Ndofs)
@@ -369,7 +377,15 @@ def main():
     log(f"mixed: {st['iterations']} FP32 its, {st['updates']} FP64 corrections, {tmix * 1e3:.1f} ms/solve, "
         f"true rel res {relmix:.2e}")
 
-    # ---- Ax kernel roofline (HIP events on the launch stream = torch's current stream)
+    # ---- roofline: the Ax of the FP32 CG operator, the dominant kernel of the
+    # timed solve, timed per kernel with events on the library's stream
+    op32 = bp.time_operator(32, args.ax_launches)
+    nbytes32 = bk5_bytes(Ndofs, Nel, Np, 4)
+    achieved32 = nbytes32 / (op32["ax_us"] * 1e-6) / 1e9
+    log(f"FP32 CG operator: Ax {op32['ax_us']:.1f} us ({achieved32:.0f} GB/s), gather {op32['gather_us']:.1f} us, "
+        f"fused {op32['fused']}")
+
+    # ---- FP64 BK5 Ax (HIP events on the launch stream = torch's current stream)
     q = torch.from_numpy(capi.random_normal(2, Ndofs)).cuda()
     AqL = torch.empty(bp.NLocal, dtype=torch.float64, device="cuda")
     for _ in range(5):
@@ -455,11 +471,17 @@ def main():
                   "ms_per_solve": tmix * 1e3, "true_rel_res": relmix,
                   "own_rate": Ndofs * st["iterations"] / tmix / 1e9,
                   "speedup_vs_fp64": t64 / tmix},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
-                     "kernel": "k_bp5_ax (FP64, non-affine, Nq=8)", "us_per_launch": t_ax * 1e6,
-                     "algorithmic_bytes": nbytes, "gdof_per_s": Ndofs / t_ax / 1e9,
-                     "device_copy_gbs": copy_gbs},
+        "roofline": {"bound": "hbm", "achieved": achieved32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved32 / HBM_PEAK_GBS, "traffic": measured_traffic(TRAFFIC_FILE_F32),
+                     "kernel": ("k_bp5_ax FP32 CG operator, " +
+                                ("fused (interior rows + surface run)" if op32["fused"] else "plain") +
+                                ", non-affine, Nq=8"),
+                     "us_per_launch": op32["ax_us"], "algorithmic_bytes": nbytes32,
+                     "gather_us_per_launch": op32["gather_us"], "device_copy_gbs": copy_gbs},
+        "bk5_fp64": {"achieved": achieved, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(), "kernel": "k_bp5_ax (FP64, non-affine, Nq=8)",
+                     "us_per_launch": t_ax * 1e6, "algorithmic_bytes": nbytes,
+                     "gdof_per_s": Ndofs / t_ax / 1e9},
         "tuning": bp.tuning(),
         "cpu_baseline": cpu,
         "extensions": ext,

@@ -74,6 +74,7 @@ _SIGS = {
     "bpx_bp_tuning": (_I, [_P, _P, _I]),
     "bpx_bp_operator_dot": (_I, [_P, _P, _P, C.POINTER(_D)]),
     "bpx_bp_set_tuning": (_I, [_P, _I, _I, _I]),
+    "bpx_bp_time_operator": (_I, [_P, _I, _I, _P]),
     "bpx_bp_set_kappa": (_I, [_P, _P]),
     "bpx_batch_create": (_I, [_P, _I, _P, _D, C.POINTER(_P)]),
     "bpx_batch_destroy": (_I, [_P]),
@@ -291,6 +292,13 @@ class BP:
     def set_tuning(self, gather_mode=-1, fused=-1, fused32=-1):
         """Pin launch choices (-1 keeps the timed one)."""
         check(lib().bpx_bp_set_tuning(self.h, gather_mode, fused, fused32))
+
+    def time_operator(self, precision=32, launches=50):
+        """The CG operator (solver row order, tuned form) timed per kernel:
+        {"ax_us", "gather_us", "fused"}."""
+        out = np.zeros(3)
+        check(lib().bpx_bp_time_operator(self.h, precision, launches, ptr(out)))
+        return {"ax_us": float(out[0]), "gather_us": float(out[1]), "fused": bool(out[2])}
 
     def operator_dot(self, q, Aq):
         """Aq = A q as CG applies it (fused for BP5); returns q.Aq."""

@@ -578,6 +578,14 @@ int bpx_bp_set_tuning(bpx_bp b, int gatherMode, int fused, int fused32) {
   BPX_GUARD_END
 }
 
+int bpx_bp_time_operator(bpx_bp b, int precision, int launches, double* out) {
+  BPX_GUARD_BEGIN
+  if (int st = need(b, "bp")) return st;
+  if (int st = need(out, "out")) return st;
+  return b->bp.TimeOperator(precision, launches, out);
+  BPX_GUARD_END
+}
+
 int bpx_bp_operator_dot(bpx_bp b, const double* o_q, double* o_Aq, double* qAq) {
   BPX_GUARD_BEGIN
   if (int st = need(b, "bp")) return st;

@@ -256,6 +256,10 @@ class bp_t : public solver_t {
     double fused32_us = -1, unfused32_us = -1;
   } tuning;
   int Tune32();
+  // the CG operator timed kernel by kernel (see solver.cpp); opMidEvent,
+  // when set, is recorded between the operator's Ax and its gather
+  int TimeOperator(int precision, int launches, double* out);
+  hipEvent_t opMidEvent = nullptr;
   // Fused operator for CG (BP5, one field): single-copy nodes written by
   // the Ax straight into Aq, shared nodes gathered (see launch_bp5_fused).
   // fused = the maps are built; tuning.fused64/fused32 = it is used.

@@ -400,6 +400,7 @@ int bp_t::OperatorDotS(const dfloat* o_q, dfloat* o_Aq, double* part, int* npart
 
 int bp_t::OperatorDotUnfusedS(const dfloat* o_q, dfloat* o_Aq, double* part, int* npart) {
   if (int st = AxLocal(o_q, o_AqL.ptr(), SolveG2L())) return st;
+  if (opMidEvent) BPX_HIP(hipEventRecord(opMidEvent, platform->stream));
   const ogs::ogs_t& G = SolveOgs();
   *npart = ogs_gather_partials(G.NrowBlocksT);
   return launch_ogs_gather_dot<double>(platform->stream, G.NrowBlocksT, G.o_blockRowStartsT.ptr(),
@@ -520,6 +521,7 @@ int bp_t::OperatorDotFused(const T* o_q, T* o_Aq, double* part, int* npart) {
   }
   a.Aq = AqL;
   if (int st = launch_bp5_fused<T>(s, a, o_Aq, o_partAx.ptr())) return st;
+  if (opMidEvent) BPX_HIP(hipEventRecord(opMidEvent, s));
   if (sharedOgs.NrowBlocksT > 0) {
     *npart = ogs_gather_partials(sharedOgs.NrowBlocksT);
     return launch_ogs_gather_dot_rows<T>(s, sharedOgs.NrowBlocksT, sharedOgs.o_blockRowStartsT.ptr(),
@@ -582,6 +584,7 @@ int bp_t::OperatorDotUnfused32(const float* o_q, float* o_Aq, double* part, int*
   if (int st = ax<float>(o_q, o_AqL32.ptr(), o_wJ32.ptr(), o_ggeo32.ptr(), o_gllw32.ptr(), o_D32.ptr(),
                          o_I32.ptr(), o_MM32.ptr(), o_invV32.ptr(), SolveG2L()))
     return st;
+  if (opMidEvent) BPX_HIP(hipEventRecord(opMidEvent, platform->stream));
   const ogs::ogs_t& G = SolveOgs();
   *npart = ogs_gather_partials(G.NrowBlocksT);
   return launch_ogs_gather_dot<float>(platform->stream, G.NrowBlocksT, G.o_blockRowStartsT.ptr(),
@@ -677,6 +680,67 @@ int bp_t::Tune32() {
                          tuning.unfused32_us))
     return st;
   tuning.fused32 = tuning.fused32_us < tuning.unfused32_us ? 1 : 0;
+  return BPX_OK;
+}
+
+// The CG operator as the solvers apply it (solve space, tuned form), timed
+// kernel by kernel with events on the library's stream: out[0] = mean Ax
+// launch (us), out[1] = mean gather launch (us), out[2] = 1 if fused.
+int bp_t::TimeOperator(int precision, int launches, double* out) {
+  if (precision != 32 && precision != 64) return set_error(BPX_ERR_INVALID_ARGUMENT, "time operator: precision 32 or 64");
+  if (launches < 1) return set_error(BPX_ERR_INVALID_ARGUMENT, "time operator: launches < 1");
+  if (precision == 32)
+    if (int st = Prepare32()) return st;
+  hipStream_t s = platform->stream;
+  const size_t n = (size_t)std::max<dlong>(Ns, 1);
+  deviceBuffer<double> v64, part;
+  deviceBuffer<float> v32;
+  if (int st = part.alloc(std::max(GatherPartials(), 1))) return st;
+  if (precision == 64) {
+    if (int st = v64.alloc(2 * n)) return st;
+    BPX_HIP(hipMemsetAsync(v64.ptr(), 0, 2 * n * sizeof(double), s));
+  } else {
+    if (int st = v32.alloc(2 * n)) return st;
+    BPX_HIP(hipMemsetAsync(v32.ptr(), 0, 2 * n * sizeof(float), s));
+  }
+  auto run = [&]() -> int {
+    int np = 0;
+    return precision == 64 ? OperatorDotS(v64.ptr(), v64.ptr() + n, part.ptr(), &np)
+                           : OperatorDot32(v32.ptr(), v32.ptr() + n, part.ptr(), &np);
+  };
+  for (int w = 0; w < 3; ++w)
+    if (int st = run()) return st;
+  std::vector<hipEvent_t> ev(3 * (size_t)launches, nullptr);
+  int st = BPX_OK;
+  for (auto& e : ev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      st = set_error(BPX_ERR_HIP, "time operator: event create failed");
+      break;
+    }
+  for (int i = 0; i < launches && st == BPX_OK; ++i) {
+    if (hipEventRecord(ev[3 * i], s) != hipSuccess) st = set_error(BPX_ERR_HIP, "time operator: event record");
+    opMidEvent = ev[3 * i + 1];
+    if (st == BPX_OK) st = run();
+    opMidEvent = nullptr;
+    if (st == BPX_OK && hipEventRecord(ev[3 * i + 2], s) != hipSuccess)
+      st = set_error(BPX_ERR_HIP, "time operator: event record");
+  }
+  double tax = 0, tg = 0;
+  if (st == BPX_OK && hipStreamSynchronize(s) != hipSuccess) st = set_error(BPX_ERR_HIP, "time operator: sync");
+  for (int i = 0; i < launches && st == BPX_OK; ++i) {
+    float a = 0, b = 0;
+    if (hipEventElapsedTime(&a, ev[3 * i], ev[3 * i + 1]) != hipSuccess ||
+        hipEventElapsedTime(&b, ev[3 * i + 1], ev[3 * i + 2]) != hipSuccess)
+      st = set_error(BPX_ERR_HIP, "time operator: elapsed time");
+    tax += a;
+    tg += b;
+  }
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
+  if (st) return st;
+  out[0] = tax * 1e3 / launches;
+  out[1] = tg * 1e3 / launches;
+  out[2] = fused && (precision == 64 ? tuning.fused64 : tuning.fused32) ? 1.0 : 0.0;
   return BPX_OK;
 }
 

@@ -271,6 +271,13 @@ int bpx_bp_tuning(bpx_bp b, double* out, int capacity);
  * 0/1, FP64 operator fused 0/1, FP32 operator fused 0/1; -1 keeps a
  * choice. */
 int bpx_bp_set_tuning(bpx_bp b, int gatherMode, int fused, int fused32);
+/* No reference counterpart (measurement): the CG operator as the solvers
+ * apply it (their row order, the tuned fused or unfused form) in FP64
+ * (precision 64) or FP32 (32), `launches` times after 3 warm-ups, each
+ * kernel timed with events on the library's stream. out[0] = mean Ax
+ * launch in us, out[1] = mean gather launch in us, out[2] = 1 if the fused
+ * form ran. Synchronises. */
+int bpx_bp_time_operator(bpx_bp b, int precision, int launches, double* out);
 /* bpN_t::Operator(o_q, o_Aq) (BP/BP5/src/bp5Operator.cpp:29-85): Ax over the
  * local elements, then the T gather. Device vectors of length Ngather. */
 int bpx_bp_operator(bpx_bp b, const double* o_q, double* o_Aq);
