@@ -58,11 +58,22 @@ constexpr int bp5_waves() {
   return NF == 1 ? (NQ <= 8 ? 3 : 2) : (NQ <= 8 ? 2 : 1);
 }
 
+// the one-field FP32 Ax at Nq = 8 fits 7 waves per SIMD without spilling
+// (68 VGPRs with D in LDS); measured 52.0 against 53.0 us at 24^3
+template <typename T, int NQ, int NF, int VAR>
+constexpr int bp5_waves_v() {
+  return (sizeof(T) == 4 && NF == 1 && VAR == 0 && NQ == 8) ? 7 : bp5_waves<NQ, NF, VAR == 1>();
+}
+
 // D rows/columns in registers (saves 4*Nq LDS reads per node) for one
-// field per sweep; with several the registers hold the extra pencils
-template <int NQ, int NF>
+// field per sweep in FP64; with several fields the registers hold the
+// extra pencils. Measured at 24^3, p=7: the FP64 plain Ax is faster with
+// them (102.6 against 107.8 us); the FP32 kernels and the FP64 fused one
+// are faster without, at higher occupancy (FP32 fused Ax 55.7 against
+// 59.5 us, FP64 fused 107.6 against 134.8 us, which spilled).
+template <typename T, int NQ, int NF, int VAR>
 constexpr bool bp5_dreg() {
-  return NQ <= 10 && NF == 1;
+  return sizeof(T) == 8 && NQ <= 10 && NF == 1 && VAR != 3;
 }
 
 // KRHS: K right-hand sides stored one after another, r-th at q + r*ldq and
@@ -95,7 +106,7 @@ __global__ __launch_bounds__(NQ* NQ* EPB) __attribute__((amdgpu_waves_per_eu(WAV
   constexpr int NQ2 = NQ * NQ;
   constexpr int NP = NQ2 * NQ;
   constexpr int LD = NQ + 1;  // padded LDS row
-  constexpr bool DREG = bp5_dreg<NQ, NF>();
+  constexpr bool DREG = bp5_dreg<T, NQ, NF, VAR>();
 
   __shared__ T s_D[NQ][LD];
   __shared__ T s_q[NF][EPB][NQ][LD];
@@ -376,7 +387,7 @@ __global__ __launch_bounds__(NQ* NQ* EPB) void k_bp5_kappa_grad(int Nelements, c
 template <typename T, int NQ, int NF, int VAR>
 int launch_bp5(hipStream_t s, const AxArgs<T>& a, T* Ap = nullptr, double* part = nullptr) {
   constexpr int EPB = bp5_epb<NQ>();
-  constexpr int WV = bp5_waves<NQ, NF, VAR == VAR_KRHS>();
+  constexpr int WV = bp5_waves_v<T, NQ, NF, VAR>();
   const int blocks = (a.Nelements + EPB - 1) / EPB;
   if (blocks == 0) return BPX_OK;
   if (a.affine)
