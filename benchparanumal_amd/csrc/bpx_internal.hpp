@@ -174,13 +174,12 @@ int launch_ogs_gather_scatter(hipStream_t s, int Nblocks, int K, const int* bloc
 // gather with a fused p.Ap: one FP64 partial per super-block of row
 // blocks, in part[0..n) with n = ogs_gather_partials(Nblocks)
 int ogs_gather_partials(int Nblocks);
-// the same over a subset of rows: map row i of the maps to rows[i] of gq
-// and p; super-block sb also adds axpart[sb*nax/nsb .. (sb+1)*nax/nsb) to
-// its partial (nsb = ogs_gather_partials(Nblocks))
+// the same, where super-block sb also adds axpart[sb*nax/nsb ..
+// (sb+1)*nax/nsb) to its partial (nsb = ogs_gather_partials(Nblocks))
 template <typename T>
-int launch_ogs_gather_dot_rows(hipStream_t s, int Nblocks, const int* blockStarts, const int* rowStarts,
-                               const int* colIds, const int* rows, const T* q, T* gq, const T* p, double* part,
-                               const double* axpart, int nax, int mode = 0);
+int launch_ogs_gather_dot_fold(hipStream_t s, int Nblocks, const int* blockStarts, const int* rowStarts,
+                               const int* colIds, const T* q, T* gq, const T* p, double* part, const double* axpart,
+                               int nax, int mode = 0);
 template <typename T>
 int launch_ogs_gather_dot(hipStream_t s, int Nblocks, const int* blockStarts, const int* rowStarts,
                           const int* colIds, const T* q, T* gq, const T* p, double* part, int mode = 0);

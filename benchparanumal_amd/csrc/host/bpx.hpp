@@ -317,11 +317,11 @@ class bp_t : public solver_t {
   ogs::ogs_t latticeOgs;
   deviceBuffer<double> o_ggeoK, o_wJK, o_gradL;
   // fused operator (solve space): the surface-node values of each element
-  // (FP64; FP32 in o_slots32), the shared rows' maps over them and their
-  // row numbers, the first interior row, the Ax kernel's p.Ap partials
+  // (FP64; FP32 in o_slots32), the shared rows' maps over them (rows
+  // 0..Nshared-1 of the solve space), the first interior row, the Ax
+  // kernel's p.Ap partials
   deviceBuffer<double> o_slots;
   deviceBuffer<float> o_slots32;
-  deviceBuffer<dlong> o_sharedRows;
   ogs::ogs_t sharedOgs;
   dlong fusedS0 = 0;
   deviceBuffer<double> o_partAx;

@@ -451,7 +451,7 @@ int bp_t::PrepareFused() {
   const long long S0 = fused_pad(Nshared);
   const long long ns = S0 + NIP * Nel, nslots = NSP * Nel;
   if (ns >= (1LL << 31) - 1 || nslots >= (1LL << 31) - 1) return BPX_OK;  // 32-bit row and slot numbers
-  std::vector<dlong> G2Ls(NLocal), rowOfSlot(nslots, -1), sharedRows(Nshared);
+  std::vector<dlong> G2Ls(NLocal), rowOfSlot(nslots, -1);
   for (dlong e = 0; e < Nel; ++e)
     for (int k = 0; k < Nq; ++k)
       for (int j = 0; j < Nq; ++j)
@@ -469,7 +469,6 @@ int bp_t::PrepareFused() {
   std::vector<dlong> perm(Ns, -1);
   for (dlong r = 0; r < Ng; ++r) perm[srow[r]] = r;
   for (size_t l = 0; l < (size_t)NLocal; ++l) G2Ls[l] = GlobalToLocal[l] < 0 ? -1 : srow[GlobalToLocal[l]];
-  for (dlong r = 0; r < Nshared; ++r) sharedRows[r] = r;
   if (int st = solveOgs.SetupRows(NLocal, Ns, G2Ls.data())) return st;
   if (int st = solveOgs.Upload(*platform)) return st;
   if (int st = sharedOgs.SetupRows((dlong)nslots, Nshared, rowOfSlot.data())) return st;
@@ -477,7 +476,6 @@ int bp_t::PrepareFused() {
   hipStream_t s = platform->stream;
   if (int st = o_sperm.upload(perm, s)) return st;
   if (int st = o_G2Ls.upload(G2Ls, s)) return st;
-  if (int st = o_sharedRows.upload(sharedRows, s)) return st;
   if (int st = o_slots.alloc(nslots)) return st;
   if (int st = o_sx.alloc(Ns)) return st;
   if (int st = o_sr.alloc(Ns)) return st;
@@ -524,10 +522,9 @@ int bp_t::OperatorDotFused(const T* o_q, T* o_Aq, double* part, int* npart) {
   if (opMidEvent) BPX_HIP(hipEventRecord(opMidEvent, s));
   if (sharedOgs.NrowBlocksT > 0) {
     *npart = ogs_gather_partials(sharedOgs.NrowBlocksT);
-    return launch_ogs_gather_dot_rows<T>(s, sharedOgs.NrowBlocksT, sharedOgs.o_blockRowStartsT.ptr(),
-                                         sharedOgs.o_rowStartsT.ptr(), sharedOgs.o_colIdsT.ptr(),
-                                         o_sharedRows.ptr(), AqL, o_Aq, o_q, part, o_partAx.ptr(), naxParts,
-                                         ogs.gatherMode);
+    return launch_ogs_gather_dot_fold<T>(s, sharedOgs.NrowBlocksT, sharedOgs.o_blockRowStartsT.ptr(),
+                                         sharedOgs.o_rowStartsT.ptr(), sharedOgs.o_colIdsT.ptr(), AqL, o_Aq, o_q,
+                                         part, o_partAx.ptr(), naxParts, ogs.gatherMode);
   }
   *npart = 1;  // no shared rows (a single element): the Ax partials are the whole dot
   return launch_fold(s, o_partAx.ptr(), naxParts, part);

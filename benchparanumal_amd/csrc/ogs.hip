@@ -49,17 +49,17 @@ __device__ inline int gather_sb(int mode, int w, int it, int G, int nsb) {
 // issues every global load at once (column ids, the gathered values into
 // LDS, and for the lane's own rows their rowStarts and p), phase 2 sums
 // rows out of LDS in column order.
-// ROWS: the maps cover a subset of the rows; map row i is row rows[i] of
-// gq and p, and super-block sb also folds its share of the fused Ax
-// kernel's partials (axpart) into its own.
-template <typename T, bool DOT, bool ROWS>
+// FOLD: super-block sb also folds its share of the fused Ax kernel's
+// partials (axpart) into its own.
+template <typename T, bool DOT, bool FOLD>
 __global__ __launch_bounds__(kThreads) void k_gather(int Nblocks, int mode, const int* __restrict__ blockStarts,
                                                      const int* __restrict__ rowStarts,
                                                      const int* __restrict__ colIds, const T* __restrict__ q,
                                                      T* __restrict__ gq, const T* __restrict__ p,
-                                                     double* __restrict__ part, const int* __restrict__ rows,
+                                                     double* __restrict__ part,
                                                      const double* __restrict__ axpart, int nax) {
   constexpr int RPL = kGatherNnz / kThreads;  // rows per lane when the rows fit the staging
+  constexpr int EPT = kGatherNnz / kThreads;  // entries per lane likewise
   __shared__ T s_v[kGatherNnz];
   __shared__ double s_w[kThreads / 64];
   const int nsb = (Nblocks + kGatherSuper - 1) / kGatherSuper;
@@ -76,16 +76,33 @@ __global__ __launch_bounds__(kThreads) void k_gather(int Nblocks, int mode, cons
     int ra[RPL], re[RPL];
     T pr[RPL];
     if (staged) {
-      for (int g = g0 + threadIdx.x; g < g1; g += kThreads) s_v[g - g0] = q[colIds[g]];
+      // two rounds of independent loads: column ids and row starts; then
+      // the gathered values and p
+      int cid[EPT];
+#pragma unroll
+      for (int m = 0; m < EPT; ++m) {
+        const int g = g0 + threadIdx.x + m * kThreads;
+        cid[m] = (g < g1) ? colIds[g] : -1;
+      }
 #pragma unroll
       for (int m = 0; m < RPL; ++m) {
         const int row = r0 + threadIdx.x + m * kThreads;
         if (row < r1) {
           ra[m] = rowStarts[row];
           re[m] = rowStarts[row + 1];
-          if (DOT) pr[m] = p[ROWS ? rows[row] : row];
         }
       }
+      T val[EPT];
+#pragma unroll
+      for (int m = 0; m < EPT; ++m) val[m] = (cid[m] >= 0) ? q[cid[m]] : T(0);
+      if (DOT) {
+#pragma unroll
+        for (int m = 0; m < RPL; ++m)
+          if (r0 + threadIdx.x + m * kThreads < r1) pr[m] = p[r0 + threadIdx.x + m * kThreads];
+      }
+#pragma unroll
+      for (int m = 0; m < EPT; ++m)
+        if (cid[m] >= 0) s_v[g0 + threadIdx.x + m * kThreads - g0] = val[m];
     }
     __syncthreads();
     double dot = 0;
@@ -96,7 +113,7 @@ __global__ __launch_bounds__(kThreads) void k_gather(int Nblocks, int mode, cons
         if (row < r1) {
           T acc = T(0);
           for (int i = ra[m] - g0; i < re[m] - g0; ++i) acc += s_v[i];
-          gq[ROWS ? rows[row] : row] = acc;
+          gq[row] = acc;
           if (DOT) dot += (double)pr[m] * (double)acc;
         }
       }
@@ -104,9 +121,8 @@ __global__ __launch_bounds__(kThreads) void k_gather(int Nblocks, int mode, cons
       for (int row = r0 + threadIdx.x; row < r1; row += kThreads) {
         T acc = T(0);
         for (int g = rowStarts[row]; g < rowStarts[row + 1]; ++g) acc += q[colIds[g]];
-        const int grow = ROWS ? rows[row] : row;
-        gq[grow] = acc;
-        if (DOT) dot += (double)p[grow] * (double)acc;
+        gq[row] = acc;
+        if (DOT) dot += (double)p[row] * (double)acc;
       }
     }
     if (DOT) {
@@ -118,7 +134,7 @@ __global__ __launch_bounds__(kThreads) void k_gather(int Nblocks, int mode, cons
     if (DOT && threadIdx.x == 0) {
       double t = 0;
       for (int i = 0; i < kThreads / 64; ++i) t += s_w[i];
-      if (ROWS) {
+      if (FOLD) {
         const int a0 = (int)((long long)sb * nax / nsb), a1 = (int)((long long)(sb + 1) * nax / nsb);
         for (int i = a0; i < a1; ++i) t += axpart[i];
       }
@@ -246,7 +262,7 @@ int launch_ogs_gather(hipStream_t s, int Nblocks, int K, const int* blockStarts,
   else
     hipLaunchKernelGGL((k_gather<T, false, false>), dim3(gather_grid(Nblocks)), dim3(kThreads), 0, s, Nblocks,
                        mode, blockStarts, rowStarts, colIds, q, gq, (const T*)nullptr, (double*)nullptr,
-                       (const int*)nullptr, (const double*)nullptr, 0);
+                       (const double*)nullptr, 0);
   BPX_HIP(hipGetLastError());
   return BPX_OK;
 }
@@ -257,20 +273,20 @@ int launch_ogs_gather_dot(hipStream_t s, int Nblocks, const int* blockStarts, co
   if (int st = check(Nblocks, 1, blockStarts, rowStarts, colIds, q, gq)) return st;
   if (Nblocks == 0) return BPX_OK;
   hipLaunchKernelGGL((k_gather<T, true, false>), dim3(gather_grid(Nblocks)), dim3(kThreads), 0, s, Nblocks, mode,
-                     blockStarts, rowStarts, colIds, q, gq, p, part, (const int*)nullptr, (const double*)nullptr, 0);
+                     blockStarts, rowStarts, colIds, q, gq, p, part, (const double*)nullptr, 0);
   BPX_HIP(hipGetLastError());
   return BPX_OK;
 }
 
 template <typename T>
-int launch_ogs_gather_dot_rows(hipStream_t s, int Nblocks, const int* blockStarts, const int* rowStarts,
-                               const int* colIds, const int* rows, const T* q, T* gq, const T* p, double* part,
-                               const double* axpart, int nax, int mode) {
+int launch_ogs_gather_dot_fold(hipStream_t s, int Nblocks, const int* blockStarts, const int* rowStarts,
+                               const int* colIds, const T* q, T* gq, const T* p, double* part, const double* axpart,
+                               int nax, int mode) {
   if (int st = check(Nblocks, 1, blockStarts, rowStarts, colIds, q, gq)) return st;
   if (Nblocks == 0) return BPX_OK;
-  if (!rows || !part || (nax > 0 && !axpart)) return set_error(BPX_ERR_INVALID_ARGUMENT, "ogs rows: null pointer");
+  if (!part || (nax > 0 && !axpart)) return set_error(BPX_ERR_INVALID_ARGUMENT, "ogs fold: null pointer");
   hipLaunchKernelGGL((k_gather<T, true, true>), dim3(gather_grid(Nblocks)), dim3(kThreads), 0, s, Nblocks, mode,
-                     blockStarts, rowStarts, colIds, q, gq, p, part, rows, axpart, nax);
+                     blockStarts, rowStarts, colIds, q, gq, p, part, axpart, nax);
   BPX_HIP(hipGetLastError());
   return BPX_OK;
 }
@@ -309,12 +325,10 @@ template int launch_ogs_gather_dot<double>(hipStream_t, int, const int*, const i
                                            double*, const double*, double*, int);
 template int launch_ogs_gather_dot<float>(hipStream_t, int, const int*, const int*, const int*, const float*,
                                           float*, const float*, double*, int);
-template int launch_ogs_gather_dot_rows<double>(hipStream_t, int, const int*, const int*, const int*, const int*,
-                                                const double*, double*, const double*, double*, const double*,
-                                                int, int);
-template int launch_ogs_gather_dot_rows<float>(hipStream_t, int, const int*, const int*, const int*, const int*,
-                                               const float*, float*, const float*, double*, const double*, int,
-                                               int);
+template int launch_ogs_gather_dot_fold<double>(hipStream_t, int, const int*, const int*, const int*, const double*,
+                                                double*, const double*, double*, const double*, int, int);
+template int launch_ogs_gather_dot_fold<float>(hipStream_t, int, const int*, const int*, const int*, const float*,
+                                               float*, const float*, double*, const double*, int, int);
 template int launch_ogs_scatter<double>(hipStream_t, int, int, const int*, const int*, const int*,
                                         const double*, double*);
 template int launch_ogs_scatter<float>(hipStream_t, int, int, const int*, const int*, const int*,
