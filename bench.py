@@ -288,7 +288,11 @@ def main():
     ap.add_argument("--N", type=int, default=7)
     ap.add_argument("--nx", type=int, default=24)
     ap.add_argument("--tol", type=float, default=1e-10)
-    ap.add_argument("--interval", type=int, default=32,
+    # the solve stops at a correction, so it overshoots by about interval/2
+    # FP32 iterations (~108 us each at 24^3, p=7) against N/interval
+    # corrections (~215 us each): the sum is least near interval ~ 70 for
+    # N ~ 1270; measured 140.2/138.0/136.7/133.7/138.4 ms at 24/32/48/64/96
+    ap.add_argument("--interval", type=int, default=64,
                     help="FP32 iterations between FP64 defect corrections")
     ap.add_argument("--ax-launches", type=int, default=50)
     ap.add_argument("--cpu-its", type=int, default=1000)
