@@ -196,6 +196,23 @@ __global__ __launch_bounds__(RT) void k_cg_update_p(dlong N, int k, const T* __r
                                                     const double* __restrict__ hist_alpha, double tol, int pending) {
   __shared__ double s_w[33];
   if (k > 0 && st->done) return;
+  typedef typename vec4<T>::type V;
+  const V* r4 = reinterpret_cast<const V*>(r);
+  V* p4 = reinterpret_cast<V*>(p);
+  V* x4 = reinterpret_cast<V*>(x);
+  // VEC: the lane's first four items are loaded before the fold of r.r
+  const long long stride = (long long)gridDim.x * blockDim.x, N4 = N / 4;
+  const long long n0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  V rq[VEC ? 4 : 1], pq[VEC ? 4 : 1], xq[VEC ? 4 : 1];
+  if (VEC) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (n0 + j * stride < N4) {
+        rq[j] = r4[n0 + j * stride];
+        pq[j] = p4[n0 + j * stride];
+        xq[j] = x4[n0 + j * stride];
+      }
+  }
   double rr = fold_partials(part_rr, nrr, s_w);
   double tol2;
   if (k == 0) {
@@ -226,20 +243,13 @@ __global__ __launch_bounds__(RT) void k_cg_update_p(dlong N, int k, const T* __r
   const T b = first ? T(0) : (T)(rr / hist_rr[k - 1]);
   if (!apply && !upd) return;
   if (VEC) {
-    typedef typename vec4<T>::type V;
-    const V* r4 = reinterpret_cast<const V*>(r);
-    V* p4 = reinterpret_cast<V*>(p);
-    V* x4 = reinterpret_cast<V*>(x);
-    strided_for(N / 4, [&](long long n) {
-      V pv = p4[n];
+    auto step = [&](const V rv, V pv, V xv, long long n) {
       if (apply) {
-        V xv = x4[n];
         xv.x = a * pv.x + T(1) * xv.x; xv.y = a * pv.y + T(1) * xv.y;
         xv.z = a * pv.z + T(1) * xv.z; xv.w = a * pv.w + T(1) * xv.w;
         x4[n] = xv;
       }
       if (upd) {
-        const V rv = r4[n];
         if (first) {
           pv = rv;
         } else {
@@ -250,7 +260,11 @@ __global__ __launch_bounds__(RT) void k_cg_update_p(dlong N, int k, const T* __r
         }
         p4[n] = pv;
       }
-    });
+    };
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (n0 + j * stride < N4) step(rq[j], pq[j], xq[j], n0 + j * stride);
+    for (long long n = n0 + 4 * stride; n < N4; n += stride) step(r4[n], p4[n], x4[n], n);
     const long long tail = (long long)(N / 4) * 4;
     const long long n = tail + (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (n < N) {
@@ -279,6 +293,22 @@ __global__ __launch_bounds__(RT) void k_cg_update_r(dlong N, int k, const T* __r
                                                     double* __restrict__ hist_alpha) {
   __shared__ double s_w[33];
   if (st->done) return;
+  typedef typename vec4<T>::type V;
+  const V* A4 = reinterpret_cast<const V*>(Ap);
+  V* r4 = reinterpret_cast<V*>(r);
+  // VEC: the lane's first four items are loaded before the fold of p.Ap,
+  // so the stream is in flight while one wavefront folds the partials
+  const long long stride = (long long)gridDim.x * blockDim.x, N4 = N / 4;
+  const long long n0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  V rp[VEC ? 4 : 1], Ap4[VEC ? 4 : 1];
+  if (VEC) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (n0 + j * stride < N4) {
+        rp[j] = r4[n0 + j * stride];
+        Ap4[j] = A4[n0 + j * stride];
+      }
+  }
   const double pAp = fold_partials(part_pAp, npAp, s_w);
   const double alpha = hist_rr[k] / pAp;
   if (!isfinite(alpha) || !(pAp > 0.0)) {
@@ -293,17 +323,18 @@ __global__ __launch_bounds__(RT) void k_cg_update_r(dlong N, int k, const T* __r
   const T a = (T)alpha;
   double v;
   if (VEC) {
-    typedef typename vec4<T>::type V;
-    const V* A4 = reinterpret_cast<const V*>(Ap);
-    V* r4 = reinterpret_cast<V*>(r);
-    v = strided_sum(N / 4, [&](long long n) {
-      V rv = r4[n];
-      const V Av = A4[n];
+    auto step = [&](V rv, const V Av, long long n) {
       rv.x = rv.x - a * Av.x; rv.y = rv.y - a * Av.y; rv.z = rv.z - a * Av.z; rv.w = rv.w - a * Av.w;
       r4[n] = rv;
       return ((double)rv.x * (double)rv.x + (double)rv.y * (double)rv.y) +
              ((double)rv.z * (double)rv.z + (double)rv.w * (double)rv.w);
-    });
+    };
+    double acc[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (n0 + j * stride < N4) acc[j] += step(rp[j], Ap4[j], n0 + j * stride);
+    for (long long n = n0 + 4 * stride; n < N4; n += stride) acc[0] += step(r4[n], A4[n], n);
+    v = (acc[0] + acc[1]) + (acc[2] + acc[3]);
     const long long tail = (long long)(N / 4) * 4;
     const long long n = tail + (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (n < N) {
