@@ -36,8 +36,16 @@ constexpr int cub_waves() {
   return NF == 1 ? (MQ <= 9 ? 2 : 1) : 1;
 }
 
-template <typename T, int NQ, int MQ, int NF, int MODE, int WAVES>
-__global__ __launch_bounds__(MQ* MQ) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_cub_ax(
+// elements per workgroup: an MQ x MQ sheet of lanes each, ~256 lanes (at
+// MQ = 9 one sheet is 81 lanes, which would leave 47 of the second wave's
+// 64 lanes idle; three sheets fill 243 of 256)
+template <int MQ>
+constexpr int cub_epb() {
+  return MQ * MQ >= 128 ? 1 : 256 / (MQ * MQ);
+}
+
+template <typename T, int NQ, int MQ, int NF, int MODE, int WAVES, int EPB>
+__global__ __launch_bounds__(MQ* MQ* EPB) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_cub_ax(
     int Nelements, const int* __restrict__ elementList, const int* __restrict__ G2L,
     const T* __restrict__ W, const T* __restrict__ G, const T* __restrict__ Dq,
     const T* __restrict__ I, T lambda, const T* __restrict__ q, T* __restrict__ Aq) {
@@ -49,23 +57,27 @@ __global__ __launch_bounds__(MQ* MQ) __attribute__((amdgpu_waves_per_eu(WAVES)))
 
   __shared__ T s_I[MQ][NQ + 1];
   __shared__ T s_D[STIFF ? MQ : 1][LD];
-  __shared__ T s_a[NF][MQ][LD];
-  __shared__ T s_b[NF][MQ][LD];
-  __shared__ T s_c[STIFF ? NF : 1][MQ][LD];
+  __shared__ T s_a[NF][EPB][MQ][LD];
+  __shared__ T s_b[NF][EPB][MQ][LD];
+  __shared__ T s_c[STIFF ? NF : 1][EPB][MQ][LD];
 
   const int t = threadIdx.x;
-  const int a = t % MQ;
-  const int b = t / MQ;
-  const int e = elementList[blockIdx.x];
-  const bool in_nodes = (a < NQ) && (b < NQ);
+  const int es = t / (MQ * MQ);
+  const int ab = t - es * MQ * MQ;
+  const int a = ab % MQ;
+  const int b = ab / MQ;
+  const int eo = blockIdx.x * EPB + es;
+  const bool active = eo < Nelements;
+  const int e = active ? elementList[eo] : 0;  // an idle sheet computes element 0 and stores nothing
+  const bool in_nodes = active && (a < NQ) && (b < NQ);
 
-  for (int n = t; n < MQ * NQ; n += MQ * MQ) {
+  for (int n = t; n < MQ * NQ; n += MQ * MQ * EPB) {
     const int g = n / NQ, c = n % NQ;
     // BP1A applies MM^T as okl/bp1AxAffineHex3D.okl does (s_MM[m][i])
     s_I[g][c] = (MODE == MODE_BP1A) ? I[c * NQ + g] : I[g * NQ + c];
   }
   if (STIFF)
-    for (int n = t; n < MQ * MQ; n += MQ * MQ) s_D[n / MQ][n % MQ] = Dq[n];
+    for (int n = t; n < MQ * MQ; n += MQ * MQ * EPB) s_D[n / MQ][n % MQ] = Dq[n];
 
   T u[NF][MQ];
   {
@@ -97,7 +109,7 @@ __global__ __launch_bounds__(MQ* MQ) __attribute__((amdgpu_waves_per_eu(WAVES)))
     for (int f = 0; f < NF; ++f) asm volatile("" : "+v"(u[f][g]));
     if (in_nodes)
 #pragma unroll
-      for (int f = 0; f < NF; ++f) s_a[f][b][a] = u[f][g];
+      for (int f = 0; f < NF; ++f) s_a[f][es][b][a] = u[f][g];
     __syncthreads();
     asm volatile("" ::: "memory");
     if (a < NQ)
@@ -105,8 +117,8 @@ __global__ __launch_bounds__(MQ* MQ) __attribute__((amdgpu_waves_per_eu(WAVES)))
       for (int f = 0; f < NF; ++f) {
         T v = 0;
 #pragma unroll
-        for (int m = 0; m < NQ; ++m) v += s_I[b][m] * s_a[f][m][a];
-        s_b[f][b][a] = v;
+        for (int m = 0; m < NQ; ++m) v += s_I[b][m] * s_a[f][es][m][a];
+        s_b[f][es][b][a] = v;
       }
     __syncthreads();
     asm volatile("" ::: "memory");
@@ -114,7 +126,7 @@ __global__ __launch_bounds__(MQ* MQ) __attribute__((amdgpu_waves_per_eu(WAVES)))
     for (int f = 0; f < NF; ++f) {
       T v = 0;
 #pragma unroll
-      for (int m = 0; m < NQ; ++m) v += s_I[a][m] * s_b[f][b][m];
+      for (int m = 0; m < NQ; ++m) v += s_I[a][m] * s_b[f][es][b][m];
       u[f][g] = v;
     }
   }
@@ -127,11 +139,12 @@ __global__ __launch_bounds__(MQ* MQ) __attribute__((amdgpu_waves_per_eu(WAVES)))
       for (int g = 0; g < MQ; ++g) u[f][g] *= Wp[g * MQ * MQ];
   } else if (MODE == MODE_BP1A) {
     const T J = W[e];
+    if (active)
 #pragma unroll
-    for (int f = 0; f < NF; ++f)
+      for (int f = 0; f < NF; ++f)
 #pragma unroll
-      for (int g = 0; g < NQ; ++g)
-        Aq[(size_t)e * NP * NF + f * NP + g * NQ * NQ + b * NQ + a] = J * u[f][g];
+        for (int g = 0; g < NQ; ++g)
+          Aq[(size_t)e * NP * NF + f * NP + g * NQ * NQ + b * NQ + a] = J * u[f][g];
     return;
   } else {
     // gradient, G, weak divergence on the quadrature grid, g-slab by g-slab
@@ -140,11 +153,18 @@ __global__ __launch_bounds__(MQ* MQ) __attribute__((amdgpu_waves_per_eu(WAVES)))
     for (int f = 0; f < NF; ++f)
 #pragma unroll
       for (int g = 0; g < MQ; ++g) res[f][g] = 0;
+    // BP3A: one set per element; BP3: slab g's factors, slab g+1's loaded
+    // while slab g computes
     T e0 = 0, e1 = 0, e2 = 0, e3 = 0, e4 = 0, e5 = 0, eW = 0;
     if (MODE == MODE_BP3A) {
       const T* ge = G + (size_t)6 * e;
       e0 = ge[0]; e1 = ge[1]; e2 = ge[2]; e3 = ge[3]; e4 = ge[4]; e5 = ge[5];
       eW = W[e];
+    } else {
+      const size_t cb = (size_t)e * MP + b * MQ + a;
+      const T* gp = G + 6 * cb;
+      e0 = gp[0]; e1 = gp[1]; e2 = gp[2]; e3 = gp[3]; e4 = gp[4]; e5 = gp[5];
+      eW = W[cb];
     }
 #pragma unroll
     for (int g = 0; g < MQ; ++g) {
@@ -152,17 +172,15 @@ __global__ __launch_bounds__(MQ* MQ) __attribute__((amdgpu_waves_per_eu(WAVES)))
       for (int f = 0; f < NF; ++f)
 #pragma unroll
         for (int m = 0; m < MQ; ++m) asm volatile("" : "+v"(res[f][m]));
-      T G00, G01, G02, G11, G12, G22, WW;
-      if (MODE == MODE_BP3A) {
-        G00 = e0; G01 = e1; G02 = e2; G11 = e3; G12 = e4; G22 = e5; WW = eW;
-      } else {
-        const size_t cb = (size_t)e * MP + g * MQ * MQ + b * MQ + a;
+      const T G00 = e0, G01 = e1, G02 = e2, G11 = e3, G12 = e4, G22 = e5, WW = eW;
+      if (MODE == MODE_BP3 && g + 1 < MQ) {
+        const size_t cb = (size_t)e * MP + (g + 1) * MQ * MQ + b * MQ + a;
         const T* gp = G + 6 * cb;
-        G00 = gp[0]; G01 = gp[1]; G02 = gp[2]; G11 = gp[3]; G12 = gp[4]; G22 = gp[5];
-        WW = W[cb];
+        e0 = gp[0]; e1 = gp[1]; e2 = gp[2]; e3 = gp[3]; e4 = gp[4]; e5 = gp[5];
+        eW = W[cb];
       }
 #pragma unroll
-      for (int f = 0; f < NF; ++f) s_a[f][b][a] = u[f][g];
+      for (int f = 0; f < NF; ++f) s_a[f][es][b][a] = u[f][g];
       __syncthreads();
       asm volatile("" ::: "memory");
 #pragma unroll
@@ -170,15 +188,15 @@ __global__ __launch_bounds__(MQ* MQ) __attribute__((amdgpu_waves_per_eu(WAVES)))
         T ur = 0, us = 0, ut = 0;
 #pragma unroll
         for (int m = 0; m < MQ; ++m) {
-          ur += s_D[a][m] * s_a[f][b][m];
-          us += s_D[b][m] * s_a[f][m][a];
+          ur += s_D[a][m] * s_a[f][es][b][m];
+          us += s_D[b][m] * s_a[f][es][m][a];
           ut += s_D[g][m] * u[f][m];
         }
         const T Gr = G00 * ur + G01 * us + G02 * ut;
         const T Gs = G01 * ur + G11 * us + G12 * ut;
         const T Gt = G02 * ur + G12 * us + G22 * ut;
-        s_b[f][b][a] = Gr;
-        s_c[f][b][a] = Gs;
+        s_b[f][es][b][a] = Gr;
+        s_c[f][es][b][a] = Gs;
         res[f][g] += lambda * WW * u[f][g];
 #pragma unroll
         for (int m = 0; m < MQ; ++m) res[f][m] += s_D[g][m] * Gt;
@@ -190,8 +208,8 @@ __global__ __launch_bounds__(MQ* MQ) __attribute__((amdgpu_waves_per_eu(WAVES)))
         T acc = 0;
 #pragma unroll
         for (int m = 0; m < MQ; ++m) {
-          acc += s_D[m][a] * s_b[f][b][m];
-          acc += s_D[m][b] * s_c[f][m][a];
+          acc += s_D[m][a] * s_b[f][es][b][m];
+          acc += s_D[m][b] * s_c[f][es][m][a];
         }
         res[f][g] += acc;
       }
@@ -210,7 +228,7 @@ __global__ __launch_bounds__(MQ* MQ) __attribute__((amdgpu_waves_per_eu(WAVES)))
       for (int f = 0; f < NF; ++f) asm volatile("" : "+v"(u[f][g]));
       __syncthreads();
 #pragma unroll
-      for (int f = 0; f < NF; ++f) s_a[f][b][a] = u[f][g];
+      for (int f = 0; f < NF; ++f) s_a[f][es][b][a] = u[f][g];
       __syncthreads();
       asm volatile("" ::: "memory");
       if (a < NQ)
@@ -218,8 +236,8 @@ __global__ __launch_bounds__(MQ* MQ) __attribute__((amdgpu_waves_per_eu(WAVES)))
         for (int f = 0; f < NF; ++f) {
           T v = 0;
 #pragma unroll
-          for (int m = 0; m < MQ; ++m) v += s_I[m][a] * s_a[f][b][m];
-          s_b[f][b][a] = v;
+          for (int m = 0; m < MQ; ++m) v += s_I[m][a] * s_a[f][es][b][m];
+          s_b[f][es][b][a] = v;
         }
       __syncthreads();
       asm volatile("" ::: "memory");
@@ -228,7 +246,7 @@ __global__ __launch_bounds__(MQ* MQ) __attribute__((amdgpu_waves_per_eu(WAVES)))
         for (int f = 0; f < NF; ++f) {
           T v = 0;
 #pragma unroll
-          for (int m = 0; m < MQ; ++m) v += s_I[m][b] * s_b[f][m][a];
+          for (int m = 0; m < MQ; ++m) v += s_I[m][b] * s_b[f][es][m][a];
           u[f][g] = v;
         }
     }
@@ -250,7 +268,9 @@ __global__ __launch_bounds__(MQ* MQ) __attribute__((amdgpu_waves_per_eu(WAVES)))
 template <typename T, int NQ, int MQ, int NF, int MODE>
 int launch_mode(hipStream_t s, const AxArgs<T>& x, const T* W, const T* G, const T* Dq, const T* I) {
   constexpr int WV = cub_waves<MQ, NF>();
-  hipLaunchKernelGGL((k_cub_ax<T, NQ, MQ, NF, MODE, WV>), dim3(x.Nelements), dim3(MQ * MQ), 0, s,
+  constexpr int EPB = cub_epb<MQ>();
+  const int blocks = (x.Nelements + EPB - 1) / EPB;
+  hipLaunchKernelGGL((k_cub_ax<T, NQ, MQ, NF, MODE, WV, EPB>), dim3(blocks), dim3(MQ * MQ * EPB), 0, s,
                      x.Nelements, x.elementList, x.GlobalToLocal, W, G, Dq, I, x.lambda, x.q, x.Aq);
   BPX_HIP(hipGetLastError());
   return BPX_OK;
