@@ -716,9 +716,14 @@ int bp_t::TimeOperator(int precision, int launches, double* out) {
     }
   for (int i = 0; i < launches && st == BPX_OK; ++i) {
     if (hipEventRecord(ev[3 * i], s) != hipSuccess) st = set_error(BPX_ERR_HIP, "time operator: event record");
-    opMidEvent = ev[3 * i + 1];
-    if (st == BPX_OK) st = run();
-    opMidEvent = nullptr;
+    if (st == BPX_OK) {
+      struct Clear {  // the hook must not outlive its event, whatever run() does
+        hipEvent_t& h;
+        ~Clear() { h = nullptr; }
+      } clear{opMidEvent};
+      opMidEvent = ev[3 * i + 1];
+      st = run();
+    }
     if (st == BPX_OK && hipEventRecord(ev[3 * i + 2], s) != hipSuccess)
       st = set_error(BPX_ERR_HIP, "time operator: event record");
   }
