@@ -315,6 +315,29 @@ def test_fp32_fused_operator_solves_identically(plat, N):
     np.testing.assert_array_equal(res[0][1], res[1][1])
 
 
+def test_time_operator(plat):
+    """bpx_bp_time_operator runs the CG operator as tuned and times its two
+    kernels; pinning the form changes which one it reports."""
+    m = capi.Mesh(N=5, nx=4, perturb=0.1, seed=4)
+    b = capi.BP(plat, m, 5, lam=1.0)
+    for prec in (64, 32):
+        for fz in (1, 0):
+            if prec == 64:
+                b.set_tuning(fused=fz)
+            else:
+                b.set_tuning(fused32=fz)
+            t = b.time_operator(prec, 5)
+            assert t["ax_us"] > 0 and t["gather_us"] > 0
+            assert t["fused"] == bool(fz)
+    with pytest.raises(capi.BpxError):
+        b.time_operator(16, 5)
+    with pytest.raises(capi.BpxError):
+        b.time_operator(64, 0)
+    b6 = capi.BP(plat, capi.Mesh(N=3, nx=2), 6, lam=1.0)
+    with pytest.raises(capi.BpxError):
+        b6.time_operator(32, 2)  # the FP32 operator covers one field
+
+
 def test_set_tuning_errors(plat):
     m = capi.Mesh(N=3, nx=2, cubature=True)
     b3 = capi.BP(plat, m, 3, lam=1.0)
