@@ -59,10 +59,15 @@ constexpr int bp5_waves() {
 }
 
 // the one-field FP32 Ax at Nq = 8 fits 7 waves per SIMD without spilling
-// (68 VGPRs with D in LDS); measured 52.0 against 53.0 us at 24^3
+// (68 VGPRs with D in LDS); measured 52.0 against 53.0 us at 24^3. The
+// FP64 one at Nq >= 13 is held to 168 VGPRs (3 waves instead of 2, spills
+// of at most 28 B): more elements, so more bytes in flight, per CU;
+// measured 570 against 682 us at p=12 and 1139 against 1194 us at p=15.
 template <typename T, int NQ, int NF, int VAR>
 constexpr int bp5_waves_v() {
-  return (sizeof(T) == 4 && NF == 1 && VAR == 0 && NQ == 8) ? 7 : bp5_waves<NQ, NF, VAR == 1>();
+  if (sizeof(T) == 4 && NF == 1 && VAR == 0 && NQ == 8) return 7;
+  if (sizeof(T) == 8 && NF == 1 && VAR == 0 && NQ >= 13) return 3;
+  return bp5_waves<NQ, NF, VAR == 1>();
 }
 
 // D rows/columns in registers (saves 4*Nq LDS reads per node) for one
