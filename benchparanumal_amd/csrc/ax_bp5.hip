@@ -71,14 +71,18 @@ constexpr int bp5_waves_v() {
 }
 
 // D rows/columns in registers (saves 4*Nq LDS reads per node) for one
-// field per sweep in FP64; with several fields the registers hold the
-// extra pencils. Measured at 24^3, p=7: the FP64 plain Ax is faster with
-// them (102.6 against 107.8 us); the FP32 kernels and the FP64 fused one
-// are faster without, at higher occupancy (FP32 fused Ax 55.7 against
-// 59.5 us, FP64 fused 107.6 against 134.8 us, which spilled).
+// field per sweep in FP64 up to Nq = 8; with several fields the registers
+// hold the extra pencils. Measured at 24^3, p=7: the FP64 plain Ax is
+// faster with them (102.6 against 107.8 us); the FP32 kernels and the FP64
+// fused one are faster without, at higher occupancy (FP32 fused Ax 55.7
+// against 59.5 us, FP64 fused 107.6 against 134.8 us, which spilled), and
+// so is the FP64 plain one at Nq = 9 (3 waves instead of 2: 100.5 against
+// 110.7 us at 20^3, p=8; even at Nq = 10). The batch and k-RHS variants
+// keep D in registers up to Nq = 10 (the ragged batch ran 4% slower
+// without).
 template <typename T, int NQ, int NF, int VAR>
 constexpr bool bp5_dreg() {
-  return sizeof(T) == 8 && NQ <= 10 && NF == 1 && VAR != 3;
+  return sizeof(T) == 8 && NF == 1 && VAR != 3 && (NQ <= 8 || (NQ <= 10 && VAR != 0));
 }
 
 // KRHS: K right-hand sides stored one after another, r-th at q + r*ldq and
