@@ -301,6 +301,9 @@ def main():
                     help="skip the k-RHS, batch and adjoint measurements")
     ap.add_argument("--ax-only", action="store_true",
                     help="only the Ax launches and a 512 MiB device copy (for counter runs)")
+    ap.add_argument("--counter-kernels", action="store_true",
+                    help="only 3 launches each of the FP64 Ax of BK5 p=15 (vector-ALU and matrix-core "
+                         "kernels), BP3 p=7 and BK5 p=7 on the 24^3 perturbed box (for counter runs)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -315,6 +318,22 @@ def main():
 
     torch.cuda.set_device(0)
     plat = capi.Platform(0)
+    if args.counter_kernels:
+        for bp, N, cub, engines in ((5, 15, False, (1, 2)), (3, 7, True, (0,)), (5, 7, False, (0,))):
+            mk = capi.Mesh(N=N, nx=args.nx, cubature=cub, perturb=0.1, seed=1)
+            bk = capi.BP(plat, mk, bp, lam=1.0)
+            qk = torch.from_numpy(capi.random_normal(2, bk.Ngather)).cuda()
+            AqLk = torch.empty(bk.NLocal, dtype=torch.float64, device="cuda")
+            for eng in engines:
+                bk.set_ax_engine(eng)
+                for _ in range(3):
+                    bk.ax_local(qk, AqLk)
+                torch.cuda.synchronize()
+                log(f"counter kernels: BP{bp} p={N} engine {eng}, 3 Ax launches")
+            del bk, mk, qk, AqLk
+            torch.cuda.empty_cache()
+        return
+
     t0 = time.perf_counter()
     mesh = capi.Mesh(N=args.N, nx=args.nx, perturb=0.1, seed=1)
     bp = capi.BP(plat, mesh, 5, lam=1.0)

@@ -75,6 +75,7 @@ _SIGS = {
     "bpx_bp_operator_dot": (_I, [_P, _P, _P, C.POINTER(_D)]),
     "bpx_bp_set_tuning": (_I, [_P, _I, _I, _I]),
     "bpx_bp_time_operator": (_I, [_P, _I, _I, _P]),
+    "bpx_bp_set_ax_engine": (_I, [_P, _I]),
     "bpx_bp_set_kappa": (_I, [_P, _P]),
     "bpx_batch_create": (_I, [_P, _I, _P, _D, C.POINTER(_P)]),
     "bpx_batch_destroy": (_I, [_P]),
@@ -292,6 +293,11 @@ class BP:
     def set_tuning(self, gather_mode=-1, fused=-1, fused32=-1):
         """Pin launch choices (-1 keeps the timed one)."""
         check(lib().bpx_bp_set_tuning(self.h, gather_mode, fused, fused32))
+
+    def set_ax_engine(self, engine):
+        """BP5 at N = 15, FP64: 0 = auto (matrix cores), 1 = vector-ALU Ax,
+        2 = matrix-core Ax (the two agree to rounding, not bit for bit)."""
+        check(lib().bpx_bp_set_ax_engine(self.h, int(engine)))
 
     def time_operator(self, precision=32, launches=50):
         """The CG operator (solver row order, tuned form) timed per kernel:

@@ -44,6 +44,11 @@ int launch_bp5_ax(hipStream_t s, const AxArgs<T>& a) {
   if (a.Nelements == 0) return BPX_OK;
   if (!a.wJ || !a.ggeo || !a.D || (a.affine && !a.gllw))
     return set_error(BPX_ERR_INVALID_ARGUMENT, "bp5 Ax: null geometry/operator pointer");
+  if (a.engine == AX_ENGINE_MATRIX && !bp5_uses_matrix_cores(a))
+    return set_error(BPX_ERR_UNSUPPORTED, "bp5 Ax: the matrix-core kernel covers FP64, one field, N = 15");
+  if constexpr (std::is_same<T, double>::value) {
+    if (bp5_uses_matrix_cores(a)) return launch_bp5_ax_mfma16(s, a);
+  }
   BPX_AX_SWITCH(launch_bp5_nq)
 }
 
@@ -102,6 +107,11 @@ int launch_bp5_fused(hipStream_t s, const AxArgs<T>& a, T* Ap, double* part) {
   if (int st = check("bp5 fused Ax", a.Nelements, a.q, a.Aq, a.GlobalToLocal, a.elementList)) return st;
   if (a.Nelements == 0) return BPX_OK;
   if (a.Nfields != 1 || !Ap || !part) return set_error(BPX_ERR_INVALID_ARGUMENT, "bp5 fused Ax: bad arguments");
+  if (a.engine == AX_ENGINE_MATRIX && !bp5_uses_matrix_cores(a))
+    return set_error(BPX_ERR_UNSUPPORTED, "bp5 fused Ax: the matrix-core kernel covers FP64, N = 15");
+  if constexpr (std::is_same<T, double>::value) {
+    if (bp5_uses_matrix_cores(a)) return launch_bp5_fused_mfma16(s, a, Ap, part);
+  }
   switch (a.N + 1) {
     BPX_FUSED_CASE(2) BPX_FUSED_CASE(3) BPX_FUSED_CASE(4) BPX_FUSED_CASE(5) BPX_FUSED_CASE(6)
     BPX_FUSED_CASE(7) BPX_FUSED_CASE(8) BPX_FUSED_CASE(9) BPX_FUSED_CASE(10) BPX_FUSED_CASE(11)

@@ -50,10 +50,25 @@ struct AxArgs {
   // of Aq at Aq + r*ldAq (one vector after another)
   int K = 1;
   long long ldq = 0, ldAq = 0;
+  // BP5, FP64, one field, Nq = 16: the r and s contractions on the matrix
+  // cores (launch_bp5_ax_mfma16) or on the vector ALU; the two agree to
+  // rounding, not bit for bit. AX_ENGINE_AUTO = matrix cores where they
+  // apply (the faster: 814 against 1084 us at 24^3, p = 15).
+  int engine = 0;
 };
+enum { AX_ENGINE_AUTO = 0, AX_ENGINE_VECTOR = 1, AX_ENGINE_MATRIX = 2 };
+// whether the BP5 Ax of these arguments runs on the matrix cores
+template <typename T>
+inline bool bp5_uses_matrix_cores(const AxArgs<T>& a) {
+  return sizeof(T) == 8 && a.N == 15 && a.Nfields == 1 && a.engine != AX_ENGINE_VECTOR;
+}
 
 // Launchers (defined in the .hip files). Return BPX_OK or an error code.
 template <typename T> int launch_bp5_ax(hipStream_t s, const AxArgs<T>& a);
+// one wavefront per element, r and s contractions on the matrix cores
+// (ax_bp5_mfma.hip); BP5 at N = 15, FP64, one field
+int launch_bp5_ax_mfma16(hipStream_t s, const AxArgs<double>& a);
+int launch_bp5_fused_mfma16(hipStream_t s, const AxArgs<double>& a, double* Ap, double* part);
 // a.K right-hand sides in one launch; chunk = right-hand sides held in
 // registers per sweep (1, 2 or 4, dividing K; 0 picks the default)
 template <typename T> int launch_bp5_ax_krhs(hipStream_t s, const AxArgs<T>& a, int chunk);

@@ -578,6 +578,19 @@ int bpx_bp_set_tuning(bpx_bp b, int gatherMode, int fused, int fused32) {
   BPX_GUARD_END
 }
 
+int bpx_bp_set_ax_engine(bpx_bp b, int engine) {
+  BPX_GUARD_BEGIN
+  if (int st = need(b, "bp")) return st;
+  if (engine != AX_ENGINE_AUTO && engine != AX_ENGINE_VECTOR && engine != AX_ENGINE_MATRIX)
+    return set_error(BPX_ERR_INVALID_ARGUMENT, "bp: Ax engine must be 0 (auto), 1 (vector ALU) or 2 (matrix cores)");
+  auto& bp = b->bp;
+  if (engine == AX_ENGINE_MATRIX && (bp.bp != 5 || bp.Nfields != 1 || bp.mesh->N != 15))
+    return set_error(BPX_ERR_UNSUPPORTED, "bp: the matrix-core Ax covers BP5 at N = 15 only");
+  bp.axEngine = engine;
+  return BPX_OK;
+  BPX_GUARD_END
+}
+
 int bpx_bp_time_operator(bpx_bp b, int precision, int launches, double* out) {
   BPX_GUARD_BEGIN
   if (int st = need(b, "bp")) return st;

@@ -256,6 +256,9 @@ class bp_t : public solver_t {
     double fused32_us = -1, unfused32_us = -1;
   } tuning;
   int Tune32();
+  // BP5, one field, N = 15, FP64: where the r and s contractions run
+  // (AX_ENGINE_AUTO: the matrix cores; see AxArgs::engine)
+  int axEngine = AX_ENGINE_AUTO;
   // the CG operator timed kernel by kernel (see solver.cpp); opMidEvent,
   // when set, is recorded between the operator's Ax and its gather
   int TimeOperator(int precision, int launches, double* out);

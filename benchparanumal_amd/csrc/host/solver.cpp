@@ -294,6 +294,7 @@ int bp_t::ax(const T* q, T* AqL, const T* wJ, const T* ggeo, const T* gllw, cons
   a.lambda = (T)lambda;
   a.q = q;
   a.Aq = AqL;
+  if constexpr (std::is_same<T, double>::value) a.engine = axEngine;
   if (bp == 5 || bp == 6) return launch_bp5_ax<T>(platform->stream, a);
   if (bp == 1 || bp == 2) return launch_bp1_ax<T>(platform->stream, a);
   return launch_bp3_ax<T>(platform->stream, a);
@@ -506,6 +507,7 @@ int bp_t::OperatorDotFused(const T* o_q, T* o_Aq, double* part, int* npart) {
     a.ggeo = haveKappa ? o_ggeoK.ptr() : o_ggeo.ptr();
     a.gllw = o_gllw.ptr();
     a.D = o_D.ptr();
+    a.engine = axEngine;
     AqL = o_slots.ptr();
   } else {
     if (haveKappa) return set_error(BPX_ERR_UNSUPPORTED, "bp: the FP32 operator has no variable coefficient");

@@ -271,6 +271,12 @@ int bpx_bp_tuning(bpx_bp b, double* out, int capacity);
  * 0/1, FP64 operator fused 0/1, FP32 operator fused 0/1; -1 keeps a
  * choice. */
 int bpx_bp_set_tuning(bpx_bp b, int gatherMode, int fused, int fused32);
+/* No reference counterpart: where the BP5 Ax at N = 15 in FP64 (one field)
+ * runs its r and s contractions: 0 = auto (the matrix cores; default), 1 =
+ * vector ALU, 2 = matrix cores (BPX_ERR_UNSUPPORTED for other problems).
+ * Covers the plain and the fused (CG) form; the two engines agree to
+ * rounding, not bit for bit. Other problems run on the vector ALU. */
+int bpx_bp_set_ax_engine(bpx_bp b, int engine);
 /* No reference counterpart (measurement): the CG operator as the solvers
  * apply it (their row order, the tuned fused or unfused form) in FP64
  * (precision 64) or FP32 (32), `launches` times after 3 warm-ups, each

@@ -78,7 +78,9 @@ def single_kernel(plat, P, arr, qk, K, affine, lam):
                                        (3, 4, 2), (4, 6, 2), (10, 4, 4), (15, 2, 2)])
 def test_krhs_ax_local_equals_single(plat, N, K, chunk, affine):
     """Kernel entry: every right-hand side equals the single-vector kernel on
-    the same arrays (both run the same fused multiply-adds in the same order)."""
+    the same arrays to rounding (the same fused multiply-adds in the same
+    order, except at N = 15, where the single-vector kernel runs its r and s
+    contractions on the matrix cores)."""
     nx = 2 if N >= 10 else 3
     lam = 1.3
     P = oracle_lib.Problem(bp=5, N=N, nx=nx, affine=affine, perturb=0.0 if affine else 0.1, seed=3, lam=lam)

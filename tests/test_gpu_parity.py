@@ -130,7 +130,7 @@ def test_dot_products_deterministic(plat):
         assert abs(nrm.value - np.linalg.norm(x)) <= 1e-14 * np.linalg.norm(x)
 
 
-@pytest.mark.parametrize("N,nx", [(7, 8), (4, 10)])
+@pytest.mark.parametrize("N,nx", [(7, 8), (4, 10), (15, 2)])
 def test_cg_history_matches_oracle(plat, N, nx):
     """100 CG iterations, N(0,1) right-hand side on the perturbed box."""
     m, P = make_pair(5, N, nx, lam=1.0, seed=1)
@@ -347,3 +347,47 @@ def test_set_tuning_errors(plat):
     b5 = capi.BP(plat, capi.Mesh(N=3, nx=2), 5, lam=1.0)
     with pytest.raises(capi.BpxError):
         b5.set_tuning(gather_mode=2)
+
+
+@pytest.mark.parametrize("affine,lam,nx", [(False, 1.3, 2), (True, 0.7, 2), (False, 0.0, 1)])
+def test_matrix_core_ax_matches_vector_and_oracle(plat, affine, lam, nx):
+    """BP5 at N = 15 in FP64: the Ax with its r and s contractions on the
+    matrix cores (the default there) agrees with the vector-ALU kernel to
+    rounding and with the oracle to the FP64 bar, as the local Ax and as the
+    CG operator; under either engine the fused and unfused operator give the
+    same bits. nx = 1: one element, no shared rows."""
+    m, P = make_pair(5, 15, nx, affine=affine, lam=lam)
+    b = capi.BP(plat, m, 5, lam=lam)
+    q = np.random.default_rng(15 + nx).standard_normal(b.Ngather)
+    outs = {}
+    for eng in (2, 1, 0):
+        b.set_ax_engine(eng)
+        AL = torch.full((b.NLocal,), float("nan"), dtype=torch.float64, device="cuda")
+        b.ax_local(dev(q), AL)
+        res = []
+        for fz in (0, 1):
+            b.set_tuning(fused=fz)
+            A = torch.full((b.Ngather,), float("nan"), dtype=torch.float64, device="cuda")
+            qAq = b.operator_dot(dev(q), A)
+            res.append(host(A))
+            assert abs(qAq - float(np.dot(q, res[-1]))) <= 1e-13 * np.abs(q * res[-1]).sum()
+        np.testing.assert_array_equal(res[1], res[0])
+        outs[eng] = (host(AL), res[0])
+    np.testing.assert_array_equal(outs[0][0], outs[2][0])  # auto = matrix cores
+    np.testing.assert_array_equal(outs[0][1], outs[2][1])
+    assert 0 < rel_err(outs[2][0], outs[1][0]) <= 1e-14
+    assert rel_err(outs[2][0], P.ax_local(q)) <= AX_RTOL
+    assert rel_err(outs[2][1], P.operator(q)) <= AX_RTOL
+
+
+def test_ax_engine_errors(plat):
+    b = capi.BP(plat, capi.Mesh(N=3, nx=2), 5, lam=1.0)
+    with pytest.raises(capi.BpxError):
+        b.set_ax_engine(2)  # the matrix-core kernel covers N = 15 only
+    with pytest.raises(capi.BpxError):
+        b.set_ax_engine(3)
+    b.set_ax_engine(1)
+    b.set_ax_engine(0)
+    b6 = capi.BP(plat, capi.Mesh(N=15, nx=1), 6, lam=1.0)
+    with pytest.raises(capi.BpxError):
+        b6.set_ax_engine(2)  # one field only
