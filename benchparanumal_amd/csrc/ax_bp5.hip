@@ -148,24 +148,33 @@ __global__ __launch_bounds__(NQ* NQ* EPB) __attribute__((amdgpu_waves_per_eu(WAV
   for (int c = 0; c < nsweep; ++c) {
   T rq[NF][NQ];
   T rA[NF][NQ];
+  // Every load below is issued unconditionally from a valid address (a
+  // masked node reads wJ[0], never empty, and stores 0): a load under a
+  // branch makes the compiler wait out each G2L -> q round trip in turn.
   if (KRHS) {
 #pragma unroll
     for (int k = 0; k < NQ; ++k) {
       const T* src = q + (size_t)(c * NF) * ldq + max(gid[k], 0);
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
-        rq[f][k] = (gid[k] >= 0) ? src[(size_t)f * ldq] : T(0);
+        const T v = *(gid[k] >= 0 ? src + (size_t)f * ldq : wJ);
+        rq[f][k] = (gid[k] >= 0) ? v : T(0);
         rA[f][k] = T(0);
       }
     }
   } else {
-    const size_t lbase = (size_t)e * NP * NF + ij;
+    const size_t lbase = (size_t)e * NP * NF + ij;  // e = 0 on an idle sheet: in bounds
+    int id[NF][NQ];
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int k = 0; k < NQ; ++k) id[f][k] = active ? G2L[lbase + f * NP + k * NQ2] : -1;
 #pragma unroll
     for (int f = 0; f < NF; ++f)
 #pragma unroll
       for (int k = 0; k < NQ; ++k) {
-        const int id = active ? G2L[lbase + f * NP + k * NQ2] : -1;
-        rq[f][k] = (id >= 0) ? q[id] : T(0);
+        const T v = *(id[f][k] >= 0 ? q + id[f][k] : wJ);
+        rq[f][k] = (id[f][k] >= 0) ? v : T(0);
         rA[f][k] = T(0);
       }
   }
@@ -351,12 +360,18 @@ __global__ __launch_bounds__(NQ* NQ* EPB) void k_bp5_kappa_grad(int Nelements, c
   const bool active = e < Nelements;
   for (int n = t; n < NQ2; n += NQ2 * EPB) s_D[n / NQ][n % NQ] = D[n];
 
+  // loads from valid addresses without branches (masked: ggeo[0], stored 0)
   double ru[NQ], rl[NQ];
+  int id[NQ];
+#pragma unroll
+  for (int k = 0; k < NQ; ++k) id[k] = G2L[(size_t)(active ? e : 0) * NP + k * NQ2 + ij];
 #pragma unroll
   for (int k = 0; k < NQ; ++k) {
-    const int id = active ? G2L[(size_t)e * NP + k * NQ2 + ij] : -1;
-    ru[k] = id >= 0 ? u[id] : 0.0;
-    rl[k] = id >= 0 ? lam[id] : 0.0;
+    const bool on = active && id[k] >= 0;
+    const double vu = *(on ? u + id[k] : ggeo);
+    const double vl = *(on ? lam + id[k] : ggeo);
+    ru[k] = on ? vu : 0.0;
+    rl[k] = on ? vl : 0.0;
   }
   __syncthreads();
 #pragma unroll

@@ -31,8 +31,12 @@ namespace {
 
 enum { MODE_BP1 = 0, MODE_BP1A = 1, MODE_BP3 = 2, MODE_BP3A = 3 };
 
-template <int MQ, int NF>
+// The FP64 BP1 kernel on the 9-point grid spills some 56 registers at 2
+// waves per SIMD; at 1 it keeps everything in registers (measured 175
+// against 182 us at 24^3, p=7). BP3 is faster at 2 despite its spills.
+template <typename T, int MQ, int NF, int MODE>
 constexpr int cub_waves() {
+  if (sizeof(T) == 8 && MODE == MODE_BP1 && NF == 1 && MQ == 9) return 1;
   return NF == 1 ? (MQ <= 9 ? 2 : 1) : 1;
 }
 
@@ -81,13 +85,23 @@ __global__ __launch_bounds__(MQ* MQ* EPB) __attribute__((amdgpu_waves_per_eu(WAV
 
   T u[NF][MQ];
   {
+    // loads from valid addresses without branches (lanes off the node grid
+    // read node (min(a), min(b)); masked nodes read W[0], never empty, and
+    // store 0): a load under a branch makes the compiler wait out each
+    // G2L -> q round trip in turn
     T r[NF][NQ];
+    int id[NF][NQ];
+    const size_t nb = (size_t)e * NP * NF + min(b, NQ - 1) * NQ + min(a, NQ - 1);
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int c = 0; c < NQ; ++c) id[f][c] = in_nodes ? G2L[nb + f * NP + c * NQ * NQ] : -1;
 #pragma unroll
     for (int f = 0; f < NF; ++f)
 #pragma unroll
       for (int c = 0; c < NQ; ++c) {
-        const int id = in_nodes ? G2L[(size_t)e * NP * NF + f * NP + c * NQ * NQ + b * NQ + a] : -1;
-        r[f][c] = (id >= 0) ? q[id] : T(0);
+        const T v = *(id[f][c] >= 0 ? q + id[f][c] : W);
+        r[f][c] = (id[f][c] >= 0) ? v : T(0);
       }
     __syncthreads();
     // interpolate along c in registers
@@ -267,7 +281,7 @@ __global__ __launch_bounds__(MQ* MQ* EPB) __attribute__((amdgpu_waves_per_eu(WAV
 
 template <typename T, int NQ, int MQ, int NF, int MODE>
 int launch_mode(hipStream_t s, const AxArgs<T>& x, const T* W, const T* G, const T* Dq, const T* I) {
-  constexpr int WV = cub_waves<MQ, NF>();
+  constexpr int WV = cub_waves<T, MQ, NF, MODE>();
   constexpr int EPB = cub_epb<MQ>();
   const int blocks = (x.Nelements + EPB - 1) / EPB;
   hipLaunchKernelGGL((k_cub_ax<T, NQ, MQ, NF, MODE, WV, EPB>), dim3(blocks), dim3(MQ * MQ * EPB), 0, s,
