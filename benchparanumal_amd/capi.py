@@ -76,6 +76,7 @@ _SIGS = {
     "bpx_bp_set_tuning": (_I, [_P, _I, _I, _I]),
     "bpx_bp_time_operator": (_I, [_P, _I, _I, _P]),
     "bpx_bp_set_ax_engine": (_I, [_P, _I]),
+    "bpx_bp_time_operator_in_solve": (_I, [_P, _P, _P, _D, _I, _I, _P]),
     "bpx_bp_set_kappa": (_I, [_P, _P]),
     "bpx_batch_create": (_I, [_P, _I, _P, _D, C.POINTER(_P)]),
     "bpx_batch_destroy": (_I, [_P]),
@@ -293,6 +294,14 @@ class BP:
     def set_tuning(self, gather_mode=-1, fused=-1, fused32=-1):
         """Pin launch choices (-1 keeps the timed one)."""
         check(lib().bpx_bp_set_tuning(self.h, gather_mode, fused, fused32))
+
+    def time_operator_in_solve(self, x, rhs, tol=1e-10, interval=64, maxit=100000):
+        """solve_mixed with the FP32 CG operator timed per application inside
+        the solve: {"ax_us", "gather_us", "launches", "fused"}."""
+        out = np.zeros(4)
+        check(lib().bpx_bp_time_operator_in_solve(self.h, ptr(x), ptr(rhs), tol, interval, maxit, ptr(out)))
+        return {"ax_us": float(out[0]), "gather_us": float(out[1]), "launches": int(out[2]),
+                "fused": bool(out[3])}
 
     def set_ax_engine(self, engine):
         """BP5 at N = 15, FP64: 0 = auto (matrix cores), 1 = vector-ALU Ax,

@@ -414,14 +414,16 @@ int launch_bp5(hipStream_t s, const AxArgs<T>& a, T* Ap = nullptr, double* part 
   constexpr int WV = bp5_waves_v<T, NQ, NF, VAR>();
   const int blocks = (a.Nelements + EPB - 1) / EPB;
   if (blocks == 0) return BPX_OK;
+  const int* noPid = nullptr;
+  const BatchState* noState = nullptr;
   if (a.affine)
-    hipLaunchKernelGGL((k_bp5_ax<T, NQ, NF, EPB, true, WV, VAR>), dim3(blocks), dim3(NQ * NQ * EPB), 0, s,
-                       a.Nelements, a.K, a.elementList, a.GlobalToLocal, a.wJ, a.ggeo, a.gllw, a.D, a.lambda, a.q,
-                       a.Aq, a.ldq, a.ldAq, nullptr, nullptr, Ap, part);
+    launch_stamped(k_bp5_ax<T, NQ, NF, EPB, true, WV, VAR>, dim3(blocks), dim3(NQ * NQ * EPB), s, a.evStart,
+                   a.evStop, a.Nelements, a.K, a.elementList, a.GlobalToLocal, a.wJ, a.ggeo, a.gllw, a.D,
+                   a.lambda, a.q, a.Aq, a.ldq, a.ldAq, noPid, noState, Ap, part);
   else
-    hipLaunchKernelGGL((k_bp5_ax<T, NQ, NF, EPB, false, WV, VAR>), dim3(blocks), dim3(NQ * NQ * EPB), 0, s,
-                       a.Nelements, a.K, a.elementList, a.GlobalToLocal, a.wJ, a.ggeo, a.gllw, a.D, a.lambda, a.q,
-                       a.Aq, a.ldq, a.ldAq, nullptr, nullptr, Ap, part);
+    launch_stamped(k_bp5_ax<T, NQ, NF, EPB, false, WV, VAR>, dim3(blocks), dim3(NQ * NQ * EPB), s, a.evStart,
+                   a.evStop, a.Nelements, a.K, a.elementList, a.GlobalToLocal, a.wJ, a.ggeo, a.gllw, a.D,
+                   a.lambda, a.q, a.Aq, a.ldq, a.ldAq, noPid, noState, Ap, part);
   BPX_HIP(hipGetLastError());
   return BPX_OK;
 }

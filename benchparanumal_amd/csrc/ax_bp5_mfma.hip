@@ -271,11 +271,13 @@ int launch_mfma16(hipStream_t s, const AxArgs<double>& a, double* Ap, double* pa
     return set_error(BPX_ERR_UNSUPPORTED, "bp5 matrix-core Ax: degree 15, one field only");
   if (a.Nelements == 0) return BPX_OK;
   if (a.affine)
-    hipLaunchKernelGGL((k_bp5_ax_mfma16<true, FUSED>), dim3(a.Nelements), dim3(64), 0, s, a.elementList,
-                       a.GlobalToLocal, a.wJ, a.ggeo, a.gllw, a.D, a.lambda, a.q, a.Aq, a.ldq, Ap, part);
+    launch_stamped(k_bp5_ax_mfma16<true, FUSED>, dim3(a.Nelements), dim3(64), s, a.evStart, a.evStop,
+                   a.elementList, a.GlobalToLocal, a.wJ, a.ggeo, a.gllw, a.D, a.lambda, a.q, a.Aq, a.ldq, Ap,
+                   part);
   else
-    hipLaunchKernelGGL((k_bp5_ax_mfma16<false, FUSED>), dim3(a.Nelements), dim3(64), 0, s, a.elementList,
-                       a.GlobalToLocal, a.wJ, a.ggeo, a.gllw, a.D, a.lambda, a.q, a.Aq, a.ldq, Ap, part);
+    launch_stamped(k_bp5_ax_mfma16<false, FUSED>, dim3(a.Nelements), dim3(64), s, a.evStart, a.evStop,
+                   a.elementList, a.GlobalToLocal, a.wJ, a.ggeo, a.gllw, a.D, a.lambda, a.q, a.Aq, a.ldq, Ap,
+                   part);
   BPX_HIP(hipGetLastError());
   return BPX_OK;
 }

@@ -55,8 +55,25 @@ struct AxArgs {
   // rounding, not bit for bit. AX_ENGINE_AUTO = matrix cores where they
   // apply (the faster: 814 against 1084 us at 24^3, p = 15).
   int engine = 0;
+  // when set (BP5 Ax), the kernel's dispatch stamps its start and end into
+  // these events (hipExtLaunchKernelGGL: no barrier packets around it)
+  hipEvent_t evStart = nullptr, evStop = nullptr;
 };
 enum { AX_ENGINE_AUTO = 0, AX_ENGINE_VECTOR = 1, AX_ENGINE_MATRIX = 2 };
+#if defined(__HIPCC__)
+}  // namespace bpx
+#include <hip/hip_ext.h>
+namespace bpx {
+// a launch whose dispatch stamps its own start and end into e0 / e1 when
+// they are set (kernel time without the gaps event packets would add)
+template <typename K, typename... A>
+void launch_stamped(K kernel, dim3 grid, dim3 block, hipStream_t s, hipEvent_t e0, hipEvent_t e1, A... args) {
+  if (e0)
+    hipExtLaunchKernelGGL(kernel, grid, block, 0, s, e0, e1, 0, args...);
+  else
+    hipLaunchKernelGGL(kernel, grid, block, 0, s, args...);
+}
+#endif
 // whether the BP5 Ax of these arguments runs on the matrix cores
 template <typename T>
 inline bool bp5_uses_matrix_cores(const AxArgs<T>& a) {

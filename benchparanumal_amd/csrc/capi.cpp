@@ -678,6 +678,15 @@ int bpx_bp_solve_mixed(bpx_bp b, double* o_x, const double* o_b, double tol, int
   BPX_GUARD_END
 }
 
+int bpx_bp_time_operator_in_solve(bpx_bp b, double* o_x, const double* o_b, double tol, int interval, int maxit,
+                                  double* out) {
+  BPX_GUARD_BEGIN
+  if (int st = need(b, "bp")) return st;
+  if (int st = need(out, "out")) return st;
+  return b->bp.TimeOperatorInSolve(o_x, o_b, tol, interval, maxit, out);
+  BPX_GUARD_END
+}
+
 // ----------------------------------------------------------- batch ----
 int bpx_batch_create(bpx_platform p, int nprob, const bpx_mesh_opts* opts, double lambda, bpx_batch* out) {
   BPX_GUARD_BEGIN

@@ -259,10 +259,24 @@ class bp_t : public solver_t {
   // BP5, one field, N = 15, FP64: where the r and s contractions run
   // (AX_ENGINE_AUTO: the matrix cores; see AxArgs::engine)
   int axEngine = AX_ENGINE_AUTO;
-  // the CG operator timed kernel by kernel (see solver.cpp); opMidEvent,
-  // when set, is recorded between the operator's Ax and its gather
+  // the CG operator launched back to back and timed kernel by kernel: the
+  // Ax by its own dispatch stamps, the gather from the Ax's end to an event
+  // after it
   int TimeOperator(int precision, int launches, double* out);
-  hipEvent_t opMidEvent = nullptr;
+  // the FP32 operator timed inside a mixed solve (SolveMixed's arguments):
+  // out = {mean Ax us, mean gather us, applications timed, fused}
+  int TimeOperatorInSolve(dfloat* o_x, const dfloat* o_b, double tol, int interval, int maxit, double* out);
+  // when set, each FP32 CG-operator application takes the next three
+  // events: its Ax's start and end (stamped by the dispatch) and one after
+  // its gather
+  struct OpTimer {
+    std::vector<hipEvent_t> ev;
+    size_t used = 0;
+  };
+  OpTimer* opTimer32 = nullptr;
+  // when set, the next BP5 Ax launch stamps its start and end into these
+  // (AxArgs::evStart / evStop)
+  hipEvent_t axEvStart = nullptr, axEvStop = nullptr;
   // Fused operator for CG (BP5, one field): single-copy nodes written by
   // the Ax straight into Aq, shared nodes gathered (see launch_bp5_fused).
   // fused = the maps are built; tuning.fused64/fused32 = it is used.

@@ -295,6 +295,8 @@ int bp_t::ax(const T* q, T* AqL, const T* wJ, const T* ggeo, const T* gllw, cons
   a.q = q;
   a.Aq = AqL;
   if constexpr (std::is_same<T, double>::value) a.engine = axEngine;
+  a.evStart = axEvStart;
+  a.evStop = axEvStop;
   if (bp == 5 || bp == 6) return launch_bp5_ax<T>(platform->stream, a);
   if (bp == 1 || bp == 2) return launch_bp1_ax<T>(platform->stream, a);
   return launch_bp3_ax<T>(platform->stream, a);
@@ -401,7 +403,6 @@ int bp_t::OperatorDotS(const dfloat* o_q, dfloat* o_Aq, double* part, int* npart
 
 int bp_t::OperatorDotUnfusedS(const dfloat* o_q, dfloat* o_Aq, double* part, int* npart) {
   if (int st = AxLocal(o_q, o_AqL.ptr(), SolveG2L())) return st;
-  if (opMidEvent) BPX_HIP(hipEventRecord(opMidEvent, platform->stream));
   const ogs::ogs_t& G = SolveOgs();
   *npart = ogs_gather_partials(G.NrowBlocksT);
   return launch_ogs_gather_dot<double>(platform->stream, G.NrowBlocksT, G.o_blockRowStartsT.ptr(),
@@ -520,8 +521,9 @@ int bp_t::OperatorDotFused(const T* o_q, T* o_Aq, double* part, int* npart) {
     AqL = o_slots32.ptr();
   }
   a.Aq = AqL;
+  a.evStart = axEvStart;
+  a.evStop = axEvStop;
   if (int st = launch_bp5_fused<T>(s, a, o_Aq, o_partAx.ptr())) return st;
-  if (opMidEvent) BPX_HIP(hipEventRecord(opMidEvent, s));
   if (sharedOgs.NrowBlocksT > 0) {
     *npart = ogs_gather_partials(sharedOgs.NrowBlocksT);
     return launch_ogs_gather_dot_fold<T>(s, sharedOgs.NrowBlocksT, sharedOgs.o_blockRowStartsT.ptr(),
@@ -575,15 +577,74 @@ int bp_t::OperatorDot32(const float* o_q, float* o_Aq, double* part, int* npart)
   if (Nfields != 1) return set_error(BPX_ERR_UNSUPPORTED, "bp: the FP32 operator covers one field (BP1/3/5)");
   if (haveKappa) return set_error(BPX_ERR_UNSUPPORTED, "bp: the FP32 operator has no variable coefficient");
   if (int st = Prepare32()) return st;
-  if (fused && tuning.fused32) return OperatorDotFused<float>(o_q, o_Aq, part, npart);
-  return OperatorDotUnfused32(o_q, o_Aq, part, npart);
+  auto apply = [&]() {
+    return (fused && tuning.fused32) ? OperatorDotFused<float>(o_q, o_Aq, part, npart)
+                                     : OperatorDotUnfused32(o_q, o_Aq, part, npart);
+  };
+  if (!opTimer32 || opTimer32->used + 3 > opTimer32->ev.size()) return apply();
+  hipEvent_t* e = &opTimer32->ev[opTimer32->used];
+  opTimer32->used += 3;
+  int st;
+  {
+    struct Clear {  // the hooks must not outlive this application
+      hipEvent_t &a, &b;
+      ~Clear() { a = b = nullptr; }
+    } clear{axEvStart, axEvStop};
+    axEvStart = e[0];  // stamped by the Ax dispatch itself
+    axEvStop = e[1];
+    st = apply();
+  }
+  if (st) return st;
+  BPX_HIP(hipEventRecord(e[2], platform->stream));
+  return BPX_OK;
+}
+
+int bp_t::TimeOperatorInSolve(dfloat* o_x, const dfloat* o_b, double tol, int interval, int maxit, double* out) {
+  if (maxit < 1) return set_error(BPX_ERR_INVALID_ARGUMENT, "time operator in solve: maxit < 1");
+  OpTimer t;  // the first 8192 applications are timed
+  t.ev.assign(3 * (size_t)std::min(maxit, 8192), nullptr);
+  int st = BPX_OK;
+  for (auto& e : t.ev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      st = set_error(BPX_ERR_HIP, "time operator in solve: event create failed");
+      break;
+    }
+  if (st == BPX_OK) {
+    struct Clear {
+      OpTimer*& h;
+      ~Clear() { h = nullptr; }
+    } clear{opTimer32};
+    opTimer32 = &t;
+    double stats[3];
+    st = SolveMixed(o_x, o_b, tol, interval, maxit, stats);
+  }
+  const size_t n = t.used / 3;
+  if (st == BPX_OK && n == 0) st = set_error(BPX_ERR_INVALID_ARGUMENT, "time operator in solve: no FP32 iteration ran");
+  if (st == BPX_OK && hipStreamSynchronize(platform->stream) != hipSuccess)
+    st = set_error(BPX_ERR_HIP, "time operator in solve: sync");
+  double tax = 0, tg = 0;
+  for (size_t i = 0; i < n && st == BPX_OK; ++i) {
+    float a = 0, b = 0;
+    if (hipEventElapsedTime(&a, t.ev[3 * i], t.ev[3 * i + 1]) != hipSuccess ||
+        hipEventElapsedTime(&b, t.ev[3 * i + 1], t.ev[3 * i + 2]) != hipSuccess)
+      st = set_error(BPX_ERR_HIP, "time operator in solve: elapsed time");
+    tax += a;
+    tg += b;
+  }
+  for (auto& e : t.ev)
+    if (e) (void)hipEventDestroy(e);
+  if (st) return st;
+  out[0] = tax * 1e3 / n;
+  out[1] = tg * 1e3 / n;
+  out[2] = (double)n;
+  out[3] = fused && tuning.fused32 ? 1.0 : 0.0;
+  return BPX_OK;
 }
 
 int bp_t::OperatorDotUnfused32(const float* o_q, float* o_Aq, double* part, int* npart) {
   if (int st = ax<float>(o_q, o_AqL32.ptr(), o_wJ32.ptr(), o_ggeo32.ptr(), o_gllw32.ptr(), o_D32.ptr(),
                          o_I32.ptr(), o_MM32.ptr(), o_invV32.ptr(), SolveG2L()))
     return st;
-  if (opMidEvent) BPX_HIP(hipEventRecord(opMidEvent, platform->stream));
   const ogs::ogs_t& G = SolveOgs();
   *npart = ogs_gather_partials(G.NrowBlocksT);
   return launch_ogs_gather_dot<float>(platform->stream, G.NrowBlocksT, G.o_blockRowStartsT.ptr(),
@@ -717,13 +778,13 @@ int bp_t::TimeOperator(int precision, int launches, double* out) {
       break;
     }
   for (int i = 0; i < launches && st == BPX_OK; ++i) {
-    if (hipEventRecord(ev[3 * i], s) != hipSuccess) st = set_error(BPX_ERR_HIP, "time operator: event record");
-    if (st == BPX_OK) {
-      struct Clear {  // the hook must not outlive its event, whatever run() does
-        hipEvent_t& h;
-        ~Clear() { h = nullptr; }
-      } clear{opMidEvent};
-      opMidEvent = ev[3 * i + 1];
+    {
+      struct Clear {  // the hooks must not outlive their events, whatever run() does
+        hipEvent_t &a, &b;
+        ~Clear() { a = b = nullptr; }
+      } clear{axEvStart, axEvStop};
+      axEvStart = ev[3 * i];  // stamped by the Ax dispatch itself
+      axEvStop = ev[3 * i + 1];
       st = run();
     }
     if (st == BPX_OK && hipEventRecord(ev[3 * i + 2], s) != hipSuccess)

@@ -279,10 +279,11 @@ int bpx_bp_set_tuning(bpx_bp b, int gatherMode, int fused, int fused32);
 int bpx_bp_set_ax_engine(bpx_bp b, int engine);
 /* No reference counterpart (measurement): the CG operator as the solvers
  * apply it (their row order, the tuned fused or unfused form) in FP64
- * (precision 64) or FP32 (32), `launches` times after 3 warm-ups, each
- * kernel timed with events on the library's stream. out[0] = mean Ax
- * launch in us, out[1] = mean gather launch in us, out[2] = 1 if the fused
- * form ran. Synchronises. */
+ * (precision 64) or FP32 (32), `launches` times back to back after 3
+ * warm-ups: the Ax timed by HIP events its own dispatch stamps, the gather
+ * from the Ax's end to an event after it. out[0] = mean Ax launch in us,
+ * out[1] = mean gather launch in us, out[2] = 1 if the fused form ran.
+ * Synchronises. */
 int bpx_bp_time_operator(bpx_bp b, int precision, int launches, double* out);
 /* bpN_t::Operator(o_q, o_Aq) (BP/BP5/src/bp5Operator.cpp:29-85): Ax over the
  * local elements, then the T gather. Device vectors of length Ngather. */
@@ -331,6 +332,13 @@ int bpx_bp_solve(bpx_bp b, double* o_x, double* o_r, double tol, int maxit, int 
  * FP32 iterations, final FP64 relative residual]. */
 int bpx_bp_solve_mixed(bpx_bp b, double* o_x, const double* o_b, double tol, int interval,
                        int maxit, double* stats);
+/* No reference counterpart (measurement): bpx_bp_solve_mixed with the FP32
+ * CG operator timed inside the solve: each application's Ax by HIP events
+ * its own dispatch stamps (start and end), its gather from that end to an
+ * event after it (the first 8192 applications). out (host): [mean Ax us, mean gather us, applications
+ * timed, fused 0/1]. */
+int bpx_bp_time_operator_in_solve(bpx_bp b, double* o_x, const double* o_b, double tol, int interval,
+                                  int maxit, double* out);
 
 /* ---------------------------------------------------------------- batch --
  * No reference counterpart. A ragged batch of BP5 problems (box meshes of

@@ -338,6 +338,30 @@ def test_time_operator(plat):
         b6.time_operator(32, 2)  # the FP32 operator covers one field
 
 
+def test_time_operator_in_solve(plat):
+    """bpx_bp_time_operator_in_solve is bpx_bp_solve_mixed with events
+    around each FP32 operator application: the same solution bits, one timed
+    application per FP32 iteration (up to its cap of 8192)."""
+    m = capi.Mesh(N=5, nx=4, perturb=0.1, seed=4)
+    b = capi.BP(plat, m, 5, lam=1.0)
+    rhs = dev(capi.random_normal(3, b.Ngather))
+    for fz in (1, 0):
+        b.set_tuning(fused32=fz)
+        x1 = torch.zeros_like(rhs)
+        st = b.solve_mixed(x1, rhs, tol=1e-10, interval=16)
+        x2 = torch.zeros_like(rhs)
+        t = b.time_operator_in_solve(x2, rhs, tol=1e-10, interval=16)
+        np.testing.assert_array_equal(host(x2), host(x1))
+        assert t["launches"] == st["iterations"] > 0
+        assert t["ax_us"] > 0 and t["gather_us"] > 0
+        assert t["fused"] == bool(fz)
+    x3 = torch.zeros_like(rhs)
+    t = b.time_operator_in_solve(x3, rhs, tol=1e-10, interval=16, maxit=5)
+    assert t["launches"] == 5
+    with pytest.raises(capi.BpxError):
+        b.time_operator_in_solve(x3, rhs, tol=1e-10, interval=16, maxit=0)
+
+
 def test_set_tuning_errors(plat):
     m = capi.Mesh(N=3, nx=2, cubature=True)
     b3 = capi.BP(plat, m, 3, lam=1.0)
