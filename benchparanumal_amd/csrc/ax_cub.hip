@@ -284,8 +284,8 @@ int launch_mode(hipStream_t s, const AxArgs<T>& x, const T* W, const T* G, const
   constexpr int WV = cub_waves<T, MQ, NF, MODE>();
   constexpr int EPB = cub_epb<MQ>();
   const int blocks = (x.Nelements + EPB - 1) / EPB;
-  hipLaunchKernelGGL((k_cub_ax<T, NQ, MQ, NF, MODE, WV, EPB>), dim3(blocks), dim3(MQ * MQ * EPB), 0, s,
-                     x.Nelements, x.elementList, x.GlobalToLocal, W, G, Dq, I, x.lambda, x.q, x.Aq);
+  launch_stamped(k_cub_ax<T, NQ, MQ, NF, MODE, WV, EPB>, dim3(blocks), dim3(MQ * MQ * EPB), s, x.evStart, x.evStop,
+                 x.Nelements, x.elementList, x.GlobalToLocal, W, G, Dq, I, x.lambda, x.q, x.Aq);
   BPX_HIP(hipGetLastError());
   return BPX_OK;
 }

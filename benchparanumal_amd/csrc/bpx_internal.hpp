@@ -55,8 +55,8 @@ struct AxArgs {
   // rounding, not bit for bit. AX_ENGINE_AUTO = matrix cores where they
   // apply (the faster: 814 against 1084 us at 24^3, p = 15).
   int engine = 0;
-  // when set (BP5 Ax), the kernel's dispatch stamps its start and end into
-  // these events (hipExtLaunchKernelGGL: no barrier packets around it)
+  // when set (BP1-BP6 Ax), the kernel's dispatch stamps its start and end
+  // into these events (hipExtLaunchKernelGGL: no barrier packets around it)
   hipEvent_t evStart = nullptr, evStop = nullptr;
 };
 enum { AX_ENGINE_AUTO = 0, AX_ENGINE_VECTOR = 1, AX_ENGINE_MATRIX = 2 };

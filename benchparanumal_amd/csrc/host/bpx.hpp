@@ -274,8 +274,8 @@ class bp_t : public solver_t {
     size_t used = 0;
   };
   OpTimer* opTimer32 = nullptr;
-  // when set, the next BP5 Ax launch stamps its start and end into these
-  // (AxArgs::evStart / evStop)
+  // when set, the next Ax launch (BP1-BP6) stamps its start and end into
+  // these (AxArgs::evStart / evStop)
   hipEvent_t axEvStart = nullptr, axEvStop = nullptr;
   // Fused operator for CG (BP5, one field): single-copy nodes written by
   // the Ax straight into Aq, shared nodes gathered (see launch_bp5_fused).

@@ -626,8 +626,10 @@ int bp_t::TimeOperatorInSolve(dfloat* o_x, const dfloat* o_b, double tol, int in
   for (size_t i = 0; i < n && st == BPX_OK; ++i) {
     float a = 0, b = 0;
     if (hipEventElapsedTime(&a, t.ev[3 * i], t.ev[3 * i + 1]) != hipSuccess ||
-        hipEventElapsedTime(&b, t.ev[3 * i + 1], t.ev[3 * i + 2]) != hipSuccess)
+        hipEventElapsedTime(&b, t.ev[3 * i + 1], t.ev[3 * i + 2]) != hipSuccess) {
+      (void)hipGetLastError();  // do not leave the failure for the next launch check
       st = set_error(BPX_ERR_HIP, "time operator in solve: elapsed time");
+    }
     tax += a;
     tg += b;
   }
@@ -795,8 +797,10 @@ int bp_t::TimeOperator(int precision, int launches, double* out) {
   for (int i = 0; i < launches && st == BPX_OK; ++i) {
     float a = 0, b = 0;
     if (hipEventElapsedTime(&a, ev[3 * i], ev[3 * i + 1]) != hipSuccess ||
-        hipEventElapsedTime(&b, ev[3 * i + 1], ev[3 * i + 2]) != hipSuccess)
+        hipEventElapsedTime(&b, ev[3 * i + 1], ev[3 * i + 2]) != hipSuccess) {
+      (void)hipGetLastError();  // do not leave the failure for the next launch check
       st = set_error(BPX_ERR_HIP, "time operator: elapsed time");
+    }
     tax += a;
     tg += b;
   }

@@ -77,28 +77,32 @@ __global__ __launch_bounds__(kThreads) void k_gather(int Nblocks, int mode, cons
     T pr[RPL];
     if (staged) {
       // two rounds of independent loads: column ids and row starts; then
-      // the gathered values and p
+      // the gathered values and p. Every load is unconditional, from an
+      // index clamped into the block (a load under a branch makes the
+      // compiler wait for it, and for every earlier load, in place); lanes
+      // past the block discard what they read.
       int cid[EPT];
 #pragma unroll
       for (int m = 0; m < EPT; ++m) {
         const int g = g0 + threadIdx.x + m * kThreads;
-        cid[m] = (g < g1) ? colIds[g] : -1;
+        const int v = colIds[max(min(g, g1 - 1), 0)];
+        cid[m] = (g < g1) ? v : -1;
       }
 #pragma unroll
       for (int m = 0; m < RPL; ++m) {
-        const int row = r0 + threadIdx.x + m * kThreads;
-        if (row < r1) {
-          ra[m] = rowStarts[row];
-          re[m] = rowStarts[row + 1];
-        }
+        const int rc = max(min(r0 + (int)threadIdx.x + m * kThreads, r1 - 1), 0);
+        ra[m] = rowStarts[rc];
+        re[m] = rowStarts[rc + 1];
       }
       T val[EPT];
 #pragma unroll
-      for (int m = 0; m < EPT; ++m) val[m] = (cid[m] >= 0) ? q[cid[m]] : T(0);
+      for (int m = 0; m < EPT; ++m) {
+        const T v = q[max(cid[m], 0)];
+        val[m] = (cid[m] >= 0) ? v : T(0);
+      }
       if (DOT) {
 #pragma unroll
-        for (int m = 0; m < RPL; ++m)
-          if (r0 + threadIdx.x + m * kThreads < r1) pr[m] = p[r0 + threadIdx.x + m * kThreads];
+        for (int m = 0; m < RPL; ++m) pr[m] = p[max(min(r0 + (int)threadIdx.x + m * kThreads, r1 - 1), 0)];
       }
 #pragma unroll
       for (int m = 0; m < EPT; ++m)

@@ -336,6 +336,14 @@ def test_time_operator(plat):
     b6 = capi.BP(plat, capi.Mesh(N=3, nx=2), 6, lam=1.0)
     with pytest.raises(capi.BpxError):
         b6.time_operator(32, 2)  # the FP32 operator covers one field
+    t6 = b6.time_operator(64, 2)
+    assert t6["ax_us"] > 0 and t6["gather_us"] > 0
+    # the cubature kernels stamp their launches as well
+    for bp, affine in ((1, False), (1, True), (3, False), (3, True)):
+        bc = capi.BP(plat, capi.Mesh(N=4, nx=3, cubature=True, affine=affine), bp, lam=1.0)
+        for prec in (64, 32):
+            t = bc.time_operator(prec, 3)
+            assert t["ax_us"] > 0 and t["gather_us"] > 0 and not t["fused"]
 
 
 def test_time_operator_in_solve(plat):
