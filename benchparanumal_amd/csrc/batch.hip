@@ -184,20 +184,48 @@ __global__ __launch_bounds__(GT) void k_batch_gather(const BatchSuper* __restric
   if (st[sb.pid].done) return;
   const int r0 = sb.r0, r1 = sb.r1;
   const int g0 = rowStarts[r0], g1 = rowStarts[r1];
-  const bool staged = (g1 - g0) <= GMAX;
-  if (staged)
-    for (int g = g0 + threadIdx.x; g < g1; g += GT) s_v[g - g0] = q[colIds[g]];
-  __syncthreads();
+  const bool staged = (g1 - g0) <= GMAX && (r1 - r0) <= GMAX;
+  constexpr int PL = GMAX / GT;  // entries (and rows) per lane when staged
   double dot = 0;
-  for (int row = r0 + threadIdx.x; row < r1; row += GT) {
-    const int a = rowStarts[row], e = rowStarts[row + 1];
-    double acc = 0;
-    if (staged)
-      for (int g = a; g < e; ++g) acc += s_v[g - g0];
-    else
+  if (staged) {
+    // all of a lane's loads are issued before any is used, from indices
+    // clamped into the super-block (lanes past it discard what they read):
+    // a load under a branch makes the compiler wait for it in place
+    int cid[PL], ra[PL], re[PL];
+    double v[PL], pr[PL];
+#pragma unroll
+    for (int m = 0; m < PL; ++m) {
+      cid[m] = colIds[max(min(g0 + (int)threadIdx.x + m * GT, g1 - 1), 0)];
+      const int rc = max(min(r0 + (int)threadIdx.x + m * GT, r1 - 1), 0);
+      ra[m] = rowStarts[rc];
+      re[m] = rowStarts[rc + 1];
+      pr[m] = p[rc];
+    }
+#pragma unroll
+    for (int m = 0; m < PL; ++m) v[m] = q[cid[m]];
+#pragma unroll
+    for (int m = 0; m < PL; ++m)
+      if (g0 + (int)threadIdx.x + m * GT < g1) s_v[threadIdx.x + m * GT] = v[m];
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < PL; ++m) {
+      const int row = r0 + threadIdx.x + m * GT;
+      if (row < r1) {
+        double acc = 0;
+        for (int g = ra[m]; g < re[m]; ++g) acc += s_v[g - g0];
+        gq[row] = acc;
+        dot += pr[m] * acc;
+      }
+    }
+  } else {
+    __syncthreads();  // the same barrier count on every path (uniform: staged is per block)
+    for (int row = r0 + threadIdx.x; row < r1; row += GT) {
+      const int a = rowStarts[row], e = rowStarts[row + 1];
+      double acc = 0;
       for (int g = a; g < e; ++g) acc += q[colIds[g]];
-    gq[row] = acc;
-    dot += p[row] * acc;
+      gq[row] = acc;
+      dot += p[row] * acc;
+    }
   }
   dot = wave_sum(dot);
   if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = dot;
