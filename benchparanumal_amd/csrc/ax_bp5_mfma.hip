@@ -100,7 +100,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void
   // gather q into LDS, 8 slabs per batch: every load of a batch is issued
   // before the first is used (a load under a branch, stored straight to
   // LDS, would wait out one memory round trip per node). Masked nodes
-  // (id < 0) read row 0 and store 0; q holds interior rows at N = 15.
+  // (id < 0) read wJ[0], which is never empty (q may be), and store 0.
 #pragma unroll
   for (int k0 = 0; k0 < NQ; k0 += 8) {
     int id[8][4];
@@ -112,7 +112,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void
 #pragma unroll
     for (int k = 0; k < 8; ++k)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) qv[k][v] = q[max(id[k][v], 0)];
+      for (int v = 0; v < 4; ++v) qv[k][v] = *(id[k][v] >= 0 ? q + id[k][v] : wJ);
 #pragma unroll
     for (int k = 0; k < 8; ++k)
 #pragma unroll
