@@ -187,7 +187,8 @@ def bench_batch(capi, plat):
 
 
 def bench_p15(capi, plat):
-    """BK5 Ax at 24^3, p = 15 (Np 4096, ggeo 2.7 GB)."""
+    """BK5 Ax at 24^3, p = 15 (Np 4096, ggeo 2.7 GB), and 100 iterations of
+    the FP64 BP5 CG on the same mesh (N(0,1) right-hand side)."""
     import torch
 
     m = capi.Mesh(N=15, nx=24, perturb=0.1, seed=1)
@@ -196,8 +197,21 @@ def bench_p15(capi, plat):
     AqL = torch.empty(bp.NLocal, dtype=torch.float64, device="cuda")
     t = event_time(lambda: bp.ax_local(q, AqL), 20)
     nb = bk5_bytes(bp.Ngather, m.Nelements, m.Np)
+    del AqL
+    b = torch.from_numpy(capi.random_normal(7, bp.Ngather)).cuda()
+    x = torch.zeros_like(b)
+    r = torch.empty_like(b)
+
+    def cg100():
+        x.zero_()
+        r.copy_(b)
+        return bp.solve(x, r, tol=0.0, maxit=100, history=False)[0]
+
+    t_cg = event_time(cg100, 1)
     return {"workload": "BK5 Ax, 24^3 hex p=15 perturbed", "Ndofs": bp.Ngather, "us": t * 1e6,
-            "gdof_per_s": bp.Ngather / t / 1e9, "algorithmic_bytes": nb, "gbs": nb / t / 1e9}
+            "gdof_per_s": bp.Ngather / t / 1e9, "algorithmic_bytes": nb, "gbs": nb / t / 1e9,
+            "cg100_ms": t_cg * 1e3, "cg_gdof_it_per_s": bp.Ngather * 100 / t_cg / 1e9,
+            "cg_operator": "fused" if bp.tuning()["fused"] == 1 else "unfused"}
 
 
 def bench_bp3(capi, plat, args):

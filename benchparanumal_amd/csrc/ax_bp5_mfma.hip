@@ -230,6 +230,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void
   constexpr int NS = NP - NI;
   constexpr int NSP = fused_surface_pad(NQ);
   static_assert(NP <= NQ * NQ * LW, "the node-order copy fits in the q tile");
+  // p of the lane's interior rows (indices l + 64w + 256t), all requested
+  // before the staging so the loads overlap it (clamped: past the run the
+  // value is not used)
+  constexpr int NT = (NIP + 255) / 256;
+  const double* pin = q + ldq + (size_t)e * NIP;  // interior rows of p
+  double pv[4][NT];
+#pragma unroll
+  for (int w = 0; w < 4; ++w)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) pv[w][t] = pin[min(l + 64 * w + 256 * t, NI - 1)];
+  __syncthreads();  // every lane has read its last q from the tile
   double* s_out = &s_q[0][0][0];
 #pragma unroll
   for (int k = 0; k < NQ; ++k)
@@ -237,17 +248,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void
     for (int v = 0; v < 4; ++v) s_out[k * NQ2 + 64 * v + l] = acc[k][v];
   __syncthreads();
   double* inner = Ap + ldq + (size_t)e * NIP;
-  const double* pin = q + ldq + (size_t)e * NIP;  // interior rows of p
   double pd[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
   for (int w = 0; w < 4; ++w)
-    for (int n = l + 64 * w; n < NIP; n += 256) {
-      double val = 0.0;
-      if (n < NI) {
-        val = s_out[fused_interior_node(NQ, n)];
-        pd[w] = fma(pin[n], val, pd[w]);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const int n = l + 64 * w + 256 * t;
+      if (n < NIP) {
+        double val = 0.0;
+        if (n < NI) {
+          val = s_out[fused_interior_node(NQ, n)];
+          pd[w] = fma(pv[w][t], val, pd[w]);
+        }
+        inner[n] = val;
       }
-      inner[n] = val;
     }
   double* surf = Aq + (size_t)e * NSP;
   for (int n = l; n < NSP; n += 64) surf[n] = (n < NS) ? s_out[fused_surface_node(NQ, n)] : 0.0;
