@@ -75,7 +75,9 @@ constexpr int bp5_waves_v() {
 // hold the extra pencils. Measured at 24^3, p=7: the FP64 plain Ax is
 // faster with them (102.6 against 107.8 us); the FP32 kernels and the FP64
 // fused one are faster without, at higher occupancy (FP32 fused Ax 55.7
-// against 59.5 us, FP64 fused 107.6 against 134.8 us, which spilled), and
+// against 59.5 us, FP64 fused 107.6 against 134.8 us, which spilled; with
+// the branch-free gather it spills 5 registers, and the FP64 CG still ran
+// 232.5 against 226.5 ms with D in registers), and
 // so is the FP64 plain one at Nq = 9 (3 waves instead of 2: 100.5 against
 // 110.7 us at 20^3, p=8; even at Nq = 10). The batch and k-RHS variants
 // keep D in registers up to Nq = 10 (the ragged batch ran 4% slower
