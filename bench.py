@@ -214,6 +214,31 @@ def bench_p15(capi, plat):
             "cg_operator": "fused" if bp.tuning()["fused"] == 1 else "unfused"}
 
 
+def bench_other_bps(capi, plat):
+    """BP1, BP2, BP4 and BP6 Ax at 24^3, p = 7 (the perturbed box; three
+    fields for BP2/4/6), against each one's own byte model: BP1/BP2
+    BP/BP1/src/bp1Run.cpp:107-120 (cubature weights), BP4
+    BP/BP4/src/bp4Run.cpp:106-119 (cubature factors), BP6
+    BP/BP6/src/bp6Run.cpp:106-134 (node factors)."""
+    import torch
+
+    rows = []
+    for bpn in (1, 2, 4, 6):
+        m = capi.Mesh(N=7, nx=24, cubature=bpn <= 4, perturb=0.1, seed=1)
+        bp = capi.BP(plat, m, bpn, lam=1.0)
+        Nd, Nel, Np, nf = bp.Ngather, m.Nelements, m.Np, bp.Nfields
+        q = torch.from_numpy(capi.random_normal(2, Nd)).cuda()
+        AqL = torch.empty(bp.NLocal, dtype=torch.float64, device="cuda")
+        t = event_time(lambda: bp.ax_local(q, AqL), 20)
+        geo = m.cubNp * 8 if bpn <= 2 else (m.cubNp if bpn == 4 else Np) * 7 * 8
+        nb = Nd * 8 + Nel * (geo + 4 + Np * nf * (4 + 8))
+        rows.append({"bp": bpn, "fields": nf, "Ndofs": Nd, "us": t * 1e6, "gdof_per_s": Nd / t / 1e9,
+                     "algorithmic_bytes": nb, "gbs": nb / t / 1e9, "frac": nb / t / 1e9 / HBM_PEAK_GBS})
+        del bp, m, q, AqL
+        torch.cuda.empty_cache()
+    return {"workload": "BP1/BP2/BP4/BP6 Ax, 24^3 hex p=7 perturbed, FP64", "rows": rows}
+
+
 def bench_bp3(capi, plat, args):
     """BP3 (over-integrated stiffness, cubNq = p+2) at 24^3, p = 7: Ax, and
     the FP64 CG against the FP32 CG with FP64 defect correction, both to tol."""
@@ -482,7 +507,7 @@ def main():
     if not args.no_extensions:
         ext = {"krhs_ax": bench_krhs(capi, plat), "ragged_batch_cg": bench_batch(capi, plat),
                "adjoint": bench_adjoint(capi, plat), "bp3": bench_bp3(capi, plat, args),
-               "bk5_p15": bench_p15(capi, plat)}
+               "bk5_p15": bench_p15(capi, plat), "other_bps": bench_other_bps(capi, plat)}
         log("extensions: " + json.dumps({k: {kk: vv for kk, vv in v.items() if kk in
                                                  ("ms_per_solve", "speedup_vs_sequential",
                                                   "ms_forward_adjoint_gradient", "single_us",
