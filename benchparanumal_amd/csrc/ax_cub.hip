@@ -293,6 +293,152 @@ __global__ __launch_bounds__(MQ* MQ* EPB) __attribute__((amdgpu_waves_per_eu(WAV
   }
 }
 
+// BP1/BP2 non-affine (Aq = I^T (cubwJ .* (I q)), okl/bp1AxHex3D.okl:61-325)
+// for NQ <= 8, one wavefront per element. The element passes through two
+// LDS tiles, x[k][j][a] (after interpolating along i) and y[k][b][a] (after
+// j); each 1D pass gives every lane whole rows or columns, with their NQ or
+// MQ inputs in registers:
+//   rows (k,j):    x = I q along i          (the q gather reads these rows)
+//   columns (k,a): y = I x along j
+//   columns (b,a): along k, times cubwJ, and back along k, in registers
+//   columns (k,a): x = I^T y along j
+//   rows (k,j):    Aq = I^T x along i       (written to the local vector)
+// I (MQ x NQ) is uniform over the wavefront and read through scalar loads.
+// A lane holds at most 2 of the MQ^2 cubature columns, whose cubwJ values
+// are requested at the start. The tiles take 9.8 KB at NQ = 8 (16
+// wavefronts per CU). This replaces the sheet kernel, whose FP64 form
+// spilled at 2 waves/SIMD and held 3 elements per CU at 1.
+template <typename T, int NQ, int NF>
+__global__ __launch_bounds__(64) void k_bp1_wave(const int* __restrict__ elementList, const int* __restrict__ G2L,
+                                                 const T* __restrict__ W, const T* __restrict__ I,
+                                                 const T* __restrict__ q, T* __restrict__ Aq) {
+  constexpr int MQ = NQ + 1;
+  constexpr int NQ2 = NQ * NQ;
+  constexpr int NP = NQ2 * NQ;
+  constexpr int MQ2 = MQ * MQ;
+  constexpr int MP = MQ2 * MQ;
+  constexpr int LX = MQ | 1;  // odd row stride of x: conflict-free row reads
+  constexpr int C3 = (MQ2 + 63) / 64;  // cubature columns per lane
+  static_assert(NQ2 <= 64 && NQ * MQ <= 128 && C3 <= 2, "one or two rows/columns per lane");
+  __shared__ T s_x[NQ * NQ * LX];
+  __shared__ T s_y[NQ * MQ * MQ];
+  typedef const __attribute__((address_space(4))) T* cptr;
+  const cptr Ic = (cptr)I;  // I[g][c] = Ic[g * NQ + c]
+  const int l = threadIdx.x;
+  const int e = elementList[blockIdx.x];
+
+  // cubwJ of the lane's cubature columns (b, a) = col, all g
+  T wv[C3][MQ];
+#pragma unroll
+  for (int c = 0; c < C3; ++c) {
+    const int col = min(l + 64 * c, MQ2 - 1);
+#pragma unroll
+    for (int g = 0; g < MQ; ++g) wv[c][g] = W[(size_t)e * MP + g * MQ2 + col];
+  }
+
+  for (int f = 0; f < NF; ++f) {
+    const size_t lb = (size_t)e * NP * NF + (size_t)f * NP;
+    // rows (k, j) = row: gather q, interpolate along i
+    if (l < NQ2) {
+      int id[NQ];
+#pragma unroll
+      for (int i = 0; i < NQ; ++i) id[i] = G2L[lb + l * NQ + i];
+      T v[NQ];
+#pragma unroll
+      for (int i = 0; i < NQ; ++i) {
+        const T t = *(id[i] >= 0 ? q + id[i] : W);  // masked: a valid address, value 0
+        v[i] = id[i] >= 0 ? t : T(0);
+      }
+#pragma unroll
+      for (int a = 0; a < MQ; ++a) {
+        T acc = 0;
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) acc += Ic[a * NQ + i] * v[i];
+        s_x[l * LX + a] = acc;
+      }
+    }
+    __syncthreads();
+    // columns (k, a): along j
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int col = l + 64 * c;
+      if (col < NQ * MQ) {
+        const int k = col / MQ, a = col - k * MQ;
+        T v[NQ];
+#pragma unroll
+        for (int j = 0; j < NQ; ++j) v[j] = s_x[(k * NQ + j) * LX + a];
+#pragma unroll
+        for (int b = 0; b < MQ; ++b) {
+          T acc = 0;
+#pragma unroll
+          for (int j = 0; j < NQ; ++j) acc += Ic[b * NQ + j] * v[j];
+          s_y[(k * MQ + b) * MQ + a] = acc;
+        }
+      }
+    }
+    __syncthreads();
+    // columns (b, a): along k, times cubwJ, back along k (in place)
+#pragma unroll
+    for (int c = 0; c < C3; ++c) {
+      const int col = l + 64 * c;
+      if (col < MQ2) {
+        T v[NQ];
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) v[k] = s_y[k * MQ2 + col];
+        T z[MQ];
+#pragma unroll
+        for (int g = 0; g < MQ; ++g) {
+          T acc = 0;
+#pragma unroll
+          for (int k = 0; k < NQ; ++k) acc += Ic[g * NQ + k] * v[k];
+          z[g] = acc * wv[c][g];
+        }
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) {
+          T acc = 0;
+#pragma unroll
+          for (int g = 0; g < MQ; ++g) acc += Ic[g * NQ + k] * z[g];
+          s_y[k * MQ2 + col] = acc;
+        }
+      }
+    }
+    __syncthreads();
+    // columns (k, a): back along j
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int col = l + 64 * c;
+      if (col < NQ * MQ) {
+        const int k = col / MQ, a = col - k * MQ;
+        T v[MQ];
+#pragma unroll
+        for (int b = 0; b < MQ; ++b) v[b] = s_y[(k * MQ + b) * MQ + a];
+#pragma unroll
+        for (int j = 0; j < NQ; ++j) {
+          T acc = 0;
+#pragma unroll
+          for (int b = 0; b < MQ; ++b) acc += Ic[b * NQ + j] * v[b];
+          s_x[(k * NQ + j) * LX + a] = acc;
+        }
+      }
+    }
+    __syncthreads();
+    // rows (k, j): back along i, to the local vector
+    if (l < NQ2) {
+      T v[MQ];
+#pragma unroll
+      for (int a = 0; a < MQ; ++a) v[a] = s_x[l * LX + a];
+#pragma unroll
+      for (int i = 0; i < NQ; ++i) {
+        T acc = 0;
+#pragma unroll
+        for (int a = 0; a < MQ; ++a) acc += Ic[a * NQ + i] * v[a];
+        Aq[lb + l * NQ + i] = acc;
+      }
+    }
+    __syncthreads();  // the next field rewrites the tiles
+  }
+}
+
 template <typename T, int NQ, int MQ, int NF, int MODE>
 int launch_mode(hipStream_t s, const AxArgs<T>& x, const T* W, const T* G, const T* Dq, const T* I) {
   constexpr int WV = cub_waves<T, MQ, NF, MODE>();
@@ -309,6 +455,12 @@ int launch_mode(hipStream_t s, const AxArgs<T>& x, const T* W, const T* G, const
 template <typename T, int NQ, int NF>
 int launch_bp1_nq(hipStream_t s, const AxArgs<T>& a) {
   if (a.affine) return launch_mode<T, NQ, NQ, NF, MODE_BP1A>(s, a, a.wJ, nullptr, nullptr, a.MM);
+  if constexpr (NQ <= 8) {
+    launch_stamped(k_bp1_wave<T, NQ, NF>, dim3(a.Nelements), dim3(64), s, a.evStart, a.evStop, a.elementList,
+                   a.GlobalToLocal, a.wJ, a.I, a.q, a.Aq);
+    BPX_HIP(hipGetLastError());
+    return BPX_OK;
+  }
   return launch_mode<T, NQ, NQ + 1, NF, MODE_BP1>(s, a, a.wJ, nullptr, nullptr, a.I);
 }
 
