@@ -439,6 +439,222 @@ __global__ __launch_bounds__(64) void k_bp1_wave(const int* __restrict__ element
   }
 }
 
+// BP3/BP4 non-affine (Aq = I^T (cubD^T G cubD + lambda cubwJ) I q,
+// okl/bp3AxHex3D.okl:369-712) for NQ <= 8, one wavefront per element, the
+// BP1 scheme with the stiffness on the cubature grid in between. Three LDS
+// tiles of MQ^3: tu holds y[k][b][a] and then u[g][b][a]; tr holds the
+// r-fluxes Gr (and x[k][j][a] before and after); ts the s-fluxes Gs.
+//   rows (k,j):      x = I q along i                      -> tr
+//   columns (k,a):   y = I x along j                      -> tu
+//   columns (b,a):   u = I y along k, in registers and    -> tu
+//   columns (b,a):   ur, us from tu, ut in registers; G;
+//                    Gr -> tr, Gs -> ts; res = lambda W u + D^T Gt (registers)
+//   columns (b,a):   res += D^T Gr (from tr) + D^T Gs (from ts); back along k -> tu
+//   columns (k,a):   back along j                         -> tr
+//   rows (k,j):      back along i                         -> Aq
+// I and the t-direction D rows are uniform over the wavefront (scalar
+// loads); the lane's own D rows and columns come from global memory.
+template <typename T, int NQ, int NF>
+__global__ __launch_bounds__(64) void k_bp3_wave(const int* __restrict__ elementList, const int* __restrict__ G2L,
+                                                 const T* __restrict__ W, const T* __restrict__ G,
+                                                 const T* __restrict__ Dq, const T* __restrict__ I, T lambda,
+                                                 const T* __restrict__ q, T* __restrict__ Aq) {
+  constexpr int MQ = NQ + 1;
+  constexpr int NQ2 = NQ * NQ;
+  constexpr int NP = NQ2 * NQ;
+  constexpr int MQ2 = MQ * MQ;
+  constexpr int MP = MQ2 * MQ;
+  constexpr int LX = MQ | 1;
+  static_assert(NQ2 <= 64 && NQ * MQ <= 128 && MQ2 <= 128, "one or two rows/columns per lane");
+  static_assert(NQ * NQ * LX <= MP, "x fits in an MQ^3 tile");
+  __shared__ T tu[MP], tr[MP], ts[MP];
+  typedef const __attribute__((address_space(4))) T* cptr;
+  const cptr Ic = (cptr)I;   // I[g][c] = Ic[g * NQ + c]
+  const cptr Dc = (cptr)Dq;  // D[r][c] = Dc[r * MQ + c]
+  const int l = threadIdx.x;
+  const int e = elementList[blockIdx.x];
+
+  for (int f = 0; f < NF; ++f) {
+    const size_t lb = (size_t)e * NP * NF + (size_t)f * NP;
+    // rows (k, j): gather q, along i -> x in tr
+    if (l < NQ2) {
+      int id[NQ];
+#pragma unroll
+      for (int i = 0; i < NQ; ++i) id[i] = G2L[lb + l * NQ + i];
+      T v[NQ];
+#pragma unroll
+      for (int i = 0; i < NQ; ++i) {
+        const T t = *(id[i] >= 0 ? q + id[i] : W);  // masked: a valid address, value 0
+        v[i] = id[i] >= 0 ? t : T(0);
+      }
+#pragma unroll
+      for (int a = 0; a < MQ; ++a) {
+        T acc = 0;
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) acc += Ic[a * NQ + i] * v[i];
+        tr[l * LX + a] = acc;
+      }
+    }
+    __syncthreads();
+    // columns (k, a): along j -> y in tu
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int col = l + 64 * c;
+      if (col < NQ * MQ) {
+        const int k = col / MQ, a = col - k * MQ;
+        T v[NQ];
+#pragma unroll
+        for (int j = 0; j < NQ; ++j) v[j] = tr[(k * NQ + j) * LX + a];
+#pragma unroll
+        for (int b = 0; b < MQ; ++b) {
+          T acc = 0;
+#pragma unroll
+          for (int j = 0; j < NQ; ++j) acc += Ic[b * NQ + j] * v[j];
+          tu[k * MQ2 + b * MQ + a] = acc;
+        }
+      }
+    }
+    __syncthreads();
+    // columns (b, a): along k -> u, kept and written in place
+    T u[2][MQ];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int col = l + 64 * c;
+      if (col < MQ2) {
+        T v[NQ];
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) v[k] = tu[k * MQ2 + col];
+#pragma unroll
+        for (int g = 0; g < MQ; ++g) {
+          T acc = 0;
+#pragma unroll
+          for (int k = 0; k < NQ; ++k) acc += Ic[g * NQ + k] * v[k];
+          u[c][g] = acc;
+        }
+      }
+    }
+    __syncthreads();  // every column has read its y before u overwrites it
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int col = l + 64 * c;
+      if (col < MQ2)
+#pragma unroll
+        for (int g = 0; g < MQ; ++g) tu[g * MQ2 + col] = u[c][g];
+    }
+    __syncthreads();
+    // columns (b, a): gradient, G, fluxes; res = lambda W u + D^T Gt
+    T res[2][MQ];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int col = l + 64 * c;
+      if (col < MQ2) {
+        const int b = col / MQ, a = col - b * MQ;
+        T Da[MQ], Db[MQ];
+#pragma unroll
+        for (int m = 0; m < MQ; ++m) {
+          Da[m] = Dq[a * MQ + m];
+          Db[m] = Dq[b * MQ + m];
+        }
+        T Gt[MQ];
+#pragma unroll
+        for (int g = 0; g < MQ; ++g) {
+          T ur = 0, us = 0, ut = 0;
+#pragma unroll
+          for (int m = 0; m < MQ; ++m) {
+            ur += Da[m] * tu[g * MQ2 + b * MQ + m];
+            us += Db[m] * tu[g * MQ2 + m * MQ + a];
+            ut += Dc[g * MQ + m] * u[c][m];
+          }
+          // FP32 (the mixed solve's operator): non-temporal, as in k_cub_ax
+          constexpr bool SG = sizeof(T) == 4;
+          const size_t cb = (size_t)e * MP + g * MQ2 + col;
+          const T* gp = G + 6 * cb;
+          const T G00 = geo_load<SG>(gp + 0), G01 = geo_load<SG>(gp + 1), G02 = geo_load<SG>(gp + 2),
+                  G11 = geo_load<SG>(gp + 3), G12 = geo_load<SG>(gp + 4), G22 = geo_load<SG>(gp + 5);
+          const T WW = geo_load<SG>(W + cb);
+          tr[g * MQ2 + col] = G00 * ur + G01 * us + G02 * ut;
+          ts[g * MQ2 + col] = G01 * ur + G11 * us + G12 * ut;
+          Gt[g] = G02 * ur + G12 * us + G22 * ut;
+          res[c][g] = lambda * WW * u[c][g];
+        }
+#pragma unroll
+        for (int g = 0; g < MQ; ++g) {
+          T acc = 0;
+#pragma unroll
+          for (int m = 0; m < MQ; ++m) acc += Dc[m * MQ + g] * Gt[m];
+          res[c][g] += acc;
+        }
+      }
+    }
+    __syncthreads();
+    // columns (b, a): r and s parts of the weak divergence; back along k
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int col = l + 64 * c;
+      if (col < MQ2) {
+        const int b = col / MQ, a = col - b * MQ;
+        T DTa[MQ], DTb[MQ];
+#pragma unroll
+        for (int m = 0; m < MQ; ++m) {
+          DTa[m] = Dq[m * MQ + a];
+          DTb[m] = Dq[m * MQ + b];
+        }
+#pragma unroll
+        for (int g = 0; g < MQ; ++g) {
+          T acc = 0;
+#pragma unroll
+          for (int m = 0; m < MQ; ++m) {
+            acc += DTa[m] * tr[g * MQ2 + b * MQ + m];
+            acc += DTb[m] * ts[g * MQ2 + m * MQ + a];
+          }
+          res[c][g] += acc;
+        }
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) {
+          T acc = 0;
+#pragma unroll
+          for (int g = 0; g < MQ; ++g) acc += Ic[g * NQ + k] * res[c][g];
+          tu[k * MQ2 + col] = acc;
+        }
+      }
+    }
+    __syncthreads();
+    // columns (k, a): back along j -> x in tr
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int col = l + 64 * c;
+      if (col < NQ * MQ) {
+        const int k = col / MQ, a = col - k * MQ;
+        T v[MQ];
+#pragma unroll
+        for (int b = 0; b < MQ; ++b) v[b] = tu[k * MQ2 + b * MQ + a];
+#pragma unroll
+        for (int j = 0; j < NQ; ++j) {
+          T acc = 0;
+#pragma unroll
+          for (int b = 0; b < MQ; ++b) acc += Ic[b * NQ + j] * v[b];
+          tr[(k * NQ + j) * LX + a] = acc;
+        }
+      }
+    }
+    __syncthreads();
+    // rows (k, j): back along i -> Aq
+    if (l < NQ2) {
+      T v[MQ];
+#pragma unroll
+      for (int a = 0; a < MQ; ++a) v[a] = tr[l * LX + a];
+#pragma unroll
+      for (int i = 0; i < NQ; ++i) {
+        T acc = 0;
+#pragma unroll
+        for (int a = 0; a < MQ; ++a) acc += Ic[a * NQ + i] * v[a];
+        Aq[lb + l * NQ + i] = acc;
+      }
+    }
+    __syncthreads();  // the next field rewrites the tiles
+  }
+}
+
 template <typename T, int NQ, int MQ, int NF, int MODE>
 int launch_mode(hipStream_t s, const AxArgs<T>& x, const T* W, const T* G, const T* Dq, const T* I) {
   constexpr int WV = cub_waves<T, MQ, NF, MODE>();
@@ -467,6 +683,12 @@ int launch_bp1_nq(hipStream_t s, const AxArgs<T>& a) {
 template <typename T, int NQ, int NF>
 int launch_bp3_nq(hipStream_t s, const AxArgs<T>& a) {
   if (a.affine) return launch_mode<T, NQ, NQ, NF, MODE_BP3A>(s, a, a.wJ, a.ggeo, a.D, a.invV);
+  if constexpr (NQ <= 8) {
+    launch_stamped(k_bp3_wave<T, NQ, NF>, dim3(a.Nelements), dim3(64), s, a.evStart, a.evStop, a.elementList,
+                   a.GlobalToLocal, a.wJ, a.ggeo, a.D, a.I, a.lambda, a.q, a.Aq);
+    BPX_HIP(hipGetLastError());
+    return BPX_OK;
+  }
   return launch_mode<T, NQ, NQ + 1, NF, MODE_BP3>(s, a, a.wJ, a.ggeo, a.D, a.I);
 }
 
