@@ -440,8 +440,10 @@ __global__ __launch_bounds__(64) void k_bp1_wave(const int* __restrict__ element
 }
 
 // BP3/BP4 non-affine (Aq = I^T (cubD^T G cubD + lambda cubwJ) I q,
-// okl/bp3AxHex3D.okl:369-712) for NQ <= 8, one wavefront per element, the
-// BP1 scheme with the stiffness on the cubature grid in between. Three LDS
+// okl/bp3AxHex3D.okl:369-712) for NQ <= 8, two wavefronts per element (one
+// row or column per lane: 64 wide, the MQ^2 = 81 columns fitted in one pass
+// of 128 lanes instead of two of 64, the second 17 wide), the BP1 scheme
+// with the stiffness on the cubature grid in between. Three LDS
 // tiles of MQ^3: tu holds y[k][b][a] and then u[g][b][a]; tr holds the
 // r-fluxes Gr (and x[k][j][a] before and after); ts the s-fluxes Gs.
 //   rows (k,j):      x = I q along i                      -> tr
@@ -455,7 +457,7 @@ __global__ __launch_bounds__(64) void k_bp1_wave(const int* __restrict__ element
 // I and the t-direction D rows are uniform over the wavefront (scalar
 // loads); the lane's own D rows and columns come from global memory.
 template <typename T, int NQ, int NF>
-__global__ __launch_bounds__(64) void k_bp3_wave(const int* __restrict__ elementList, const int* __restrict__ G2L,
+__global__ __launch_bounds__(128) void k_bp3_wave(const int* __restrict__ elementList, const int* __restrict__ G2L,
                                                  const T* __restrict__ W, const T* __restrict__ G,
                                                  const T* __restrict__ Dq, const T* __restrict__ I, T lambda,
                                                  const T* __restrict__ q, T* __restrict__ Aq) {
@@ -465,7 +467,7 @@ __global__ __launch_bounds__(64) void k_bp3_wave(const int* __restrict__ element
   constexpr int MQ2 = MQ * MQ;
   constexpr int MP = MQ2 * MQ;
   constexpr int LX = MQ | 1;
-  static_assert(NQ2 <= 64 && NQ * MQ <= 128 && MQ2 <= 128, "one or two rows/columns per lane");
+  static_assert(NQ2 <= 128 && NQ * MQ <= 128 && MQ2 <= 128, "one row/column per lane");
   static_assert(NQ * NQ * LX <= MP, "x fits in an MQ^3 tile");
   __shared__ T tu[MP], tr[MP], ts[MP];
   typedef const __attribute__((address_space(4))) T* cptr;
@@ -497,9 +499,9 @@ __global__ __launch_bounds__(64) void k_bp3_wave(const int* __restrict__ element
     }
     __syncthreads();
     // columns (k, a): along j -> y in tu
-#pragma unroll
-    for (int c = 0; c < 2; ++c) {
-      const int col = l + 64 * c;
+    {
+      constexpr int c = 0;
+      const int col = l;
       if (col < NQ * MQ) {
         const int k = col / MQ, a = col - k * MQ;
         T v[NQ];
@@ -516,10 +518,10 @@ __global__ __launch_bounds__(64) void k_bp3_wave(const int* __restrict__ element
     }
     __syncthreads();
     // columns (b, a): along k -> u, kept and written in place
-    T u[2][MQ];
-#pragma unroll
-    for (int c = 0; c < 2; ++c) {
-      const int col = l + 64 * c;
+    T u[1][MQ];  // [column of the lane]
+    {
+      constexpr int c = 0;
+      const int col = l;
       if (col < MQ2) {
         T v[NQ];
 #pragma unroll
@@ -534,19 +536,19 @@ __global__ __launch_bounds__(64) void k_bp3_wave(const int* __restrict__ element
       }
     }
     __syncthreads();  // every column has read its y before u overwrites it
-#pragma unroll
-    for (int c = 0; c < 2; ++c) {
-      const int col = l + 64 * c;
+    {
+      constexpr int c = 0;
+      const int col = l;
       if (col < MQ2)
 #pragma unroll
         for (int g = 0; g < MQ; ++g) tu[g * MQ2 + col] = u[c][g];
     }
     __syncthreads();
     // columns (b, a): gradient, G, fluxes; res = lambda W u + D^T Gt
-    T res[2][MQ];
-#pragma unroll
-    for (int c = 0; c < 2; ++c) {
-      const int col = l + 64 * c;
+    T res[1][MQ];  // [column of the lane]
+    {
+      constexpr int c = 0;
+      const int col = l;
       if (col < MQ2) {
         const int b = col / MQ, a = col - b * MQ;
         T Da[MQ], Db[MQ];
@@ -588,9 +590,9 @@ __global__ __launch_bounds__(64) void k_bp3_wave(const int* __restrict__ element
     }
     __syncthreads();
     // columns (b, a): r and s parts of the weak divergence; back along k
-#pragma unroll
-    for (int c = 0; c < 2; ++c) {
-      const int col = l + 64 * c;
+    {
+      constexpr int c = 0;
+      const int col = l;
       if (col < MQ2) {
         const int b = col / MQ, a = col - b * MQ;
         T DTa[MQ], DTb[MQ];
@@ -620,9 +622,9 @@ __global__ __launch_bounds__(64) void k_bp3_wave(const int* __restrict__ element
     }
     __syncthreads();
     // columns (k, a): back along j -> x in tr
-#pragma unroll
-    for (int c = 0; c < 2; ++c) {
-      const int col = l + 64 * c;
+    {
+      constexpr int c = 0;
+      const int col = l;
       if (col < NQ * MQ) {
         const int k = col / MQ, a = col - k * MQ;
         T v[MQ];
@@ -684,7 +686,7 @@ template <typename T, int NQ, int NF>
 int launch_bp3_nq(hipStream_t s, const AxArgs<T>& a) {
   if (a.affine) return launch_mode<T, NQ, NQ, NF, MODE_BP3A>(s, a, a.wJ, a.ggeo, a.D, a.invV);
   if constexpr (NQ <= 8) {
-    launch_stamped(k_bp3_wave<T, NQ, NF>, dim3(a.Nelements), dim3(64), s, a.evStart, a.evStop, a.elementList,
+    launch_stamped(k_bp3_wave<T, NQ, NF>, dim3(a.Nelements), dim3(128), s, a.evStart, a.evStop, a.elementList,
                    a.GlobalToLocal, a.wJ, a.ggeo, a.D, a.I, a.lambda, a.q, a.Aq);
     BPX_HIP(hipGetLastError());
     return BPX_OK;
