@@ -44,8 +44,9 @@ def make_pair(bp, N, nx, ny=None, nz=None, affine=False, perturb=0.1, seed=4, la
 
 
 @pytest.mark.parametrize("affine", [False, True])
-@pytest.mark.parametrize("bp,N", [(1, 3), (1, 7), (2, 3), (3, 3), (3, 7), (4, 2), (5, 1), (5, 3), (5, 4), (5, 5), (5, 7),
-                                  (5, 8), (5, 11), (5, 15), (6, 3), (6, 7)])
+@pytest.mark.parametrize("bp,N", [(1, 1), (1, 3), (1, 5), (1, 7), (1, 8), (2, 3), (2, 7), (3, 1), (3, 3), (3, 5), (3, 7),
+                                  (3, 8), (4, 2), (4, 7), (5, 1), (5, 3), (5, 4), (5, 5), (5, 7), (5, 8), (5, 11), (5, 15),
+                                  (6, 3), (6, 7)])
 def test_bp_operator_matches_oracle(plat, bp, N, affine):
     nx = 2 if N >= 11 else 3
     m, P = make_pair(bp, N, nx, 2, 2, affine=affine)
@@ -200,7 +201,7 @@ def test_cg_zero_rhs_and_nan_breakdown(plat):
     assert ei.value.code == -7
 
 
-@pytest.mark.parametrize("bp", [3, 5])
+@pytest.mark.parametrize("bp", [1, 3, 5])
 def test_fp32_operator_close_to_fp64(plat, bp):
     m, P = make_pair(bp, 7, 4, lam=1.0)
     b = capi.BP(plat, m, bp, lam=1.0)
@@ -212,7 +213,7 @@ def test_fp32_operator_close_to_fp64(plat, bp):
     assert rel_err(host(A32).astype(np.float64), host(A64)) <= 2e-5
 
 
-@pytest.mark.parametrize("bp", [3, 5])
+@pytest.mark.parametrize("bp", [1, 3, 5])
 def test_mixed_precision_reaches_1e10(plat, bp):
     m, P = make_pair(bp, 7, 6, lam=1.0)
     b = capi.BP(plat, m, bp, lam=1.0)
