@@ -70,6 +70,13 @@ constexpr int bp5_waves_v() {
   return bp5_waves<NQ, NF, VAR == 1>();
 }
 
+// occupancy target of the FP32 kernels with the trilinear geometry (the
+// plain one at Nq = 8 spills 35 registers at 7 waves; 5 hold it)
+template <int NQ, int VAR>
+constexpr int bp5_waves_tri() {
+  return (VAR == 0 && NQ == 8) ? 5 : bp5_waves_v<float, NQ, 1, VAR>();
+}
+
 // D rows/columns in registers (saves 4*Nq LDS reads per node) for one
 // field per sweep in FP64 up to Nq = 8; with several fields the registers
 // hold the extra pencils. Measured at 24^3, p=7: the FP64 plain Ax is
@@ -116,7 +123,17 @@ __device__ inline T geo_load(const T* p) {
   return *p;
 }
 
-template <typename T, int NQ, int NF, int EPB, bool AFFINE, int WAVES, int VAR>
+// TRI (FP32, one field, non-affine; plain or fused): the geometric factors
+// are computed in the kernel from the element's trilinear map instead of
+// read from memory. ggeo then holds 24 values per element, for x, y and z
+// in turn the coefficients of r, s, t, rs, rt, st, rst of
+// x(r,s,t) = sum_v (1+r_v r)(1+s_v s)(1+t_v t) X_v / 8 and one pad, and wJ
+// holds the Nq GLL nodes. Lane (i,j) keeps the derivatives' pencil
+// constants (x_r and x_s are linear in t along the pencil, x_t is
+// constant); each node then forms the adjugate of the Jacobian, J and
+// G = adj adj^T w / J (the formula of the stored factors), ~55 VALU
+// operations in place of 28 B of HBM traffic per node.
+template <typename T, int NQ, int NF, int EPB, bool AFFINE, int WAVES, int VAR, bool TRI = false>
 __global__ __launch_bounds__(NQ* NQ* EPB) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_bp5_ax(
     int Nelements, int K, const int* __restrict__ elementList, const int* __restrict__ G2L,
     const T* __restrict__ wJ, const T* __restrict__ ggeo, const T* __restrict__ gllw,
@@ -127,10 +144,24 @@ __global__ __launch_bounds__(NQ* NQ* EPB) __attribute__((amdgpu_waves_per_eu(WAV
   constexpr bool BATCH = VAR == VAR_BATCH;
   constexpr bool FUSED = VAR == VAR_FUSED;
   static_assert(!FUSED || NF == 1, "the fused operator covers one field");
+  static_assert(!TRI || (sizeof(T) == 4 && NF == 1 && !AFFINE && (VAR == VAR_PLAIN || FUSED)),
+                "TRI: FP32, one field, non-affine");
   constexpr int NQ2 = NQ * NQ;
   constexpr int NP = NQ2 * NQ;
   constexpr int LD = NQ + 1;  // padded LDS row
-  constexpr bool DREG = bp5_dreg<T, NQ, NF, VAR>();
+  // TRI moves ~77 MB per launch at 24^3, p = 7 instead of ~274 MB, and
+  // the LDS reads become the bound: D rows/columns in registers and the
+  // broadcast D[k][m] from scalar (constant address space) loads
+  // (measured Ax 44.1 -> 38.2 -> 36.5 us, the mixed solve 111.1 -> 101.6 ms).
+  // Both only at Nq = 8 and 16, where the FP32 operator was measured
+  // against the FP64 one at FP32 rounding. With D in registers the FP32
+  // operator came out 17-47% off at Nq = 5, 6, 7, 9, 10 and 12 (right at
+  // 2, 3, 4, 8, 16); the cause is not established, so those Nq keep D in
+  // LDS, the form that is right at every Nq.
+  constexpr bool DREG = (TRI && NQ % 8 == 0) || bp5_dreg<T, NQ, NF, VAR>();
+  constexpr bool SDK = TRI && NQ % 8 == 0;
+  typedef const __attribute__((address_space(4))) T* cptr;
+  const cptr Dc = (cptr)D;
 
   __shared__ T s_D[NQ][LD];
   __shared__ T s_q[NF][EPB][NQ][LD];
@@ -197,7 +228,23 @@ __global__ __launch_bounds__(NQ* NQ* EPB) __attribute__((amdgpu_waves_per_eu(WAV
   // geometry of the current slab (non-affine: prefetched one slab ahead)
   T g0 = 0, g1 = 0, g2 = 0, g3 = 0, g4 = 0, g5 = 0, gw = 0;
   T wij = 0;
-  if (AFFINE) {
+  // TRI: per coordinate, d/dr = Ar + Br t, d/ds = As + Bs t, d/dt = Ct
+  T Ar[TRI ? 3 : 1], Br[TRI ? 3 : 1], As[TRI ? 3 : 1], Bs[TRI ? 3 : 1], Ct[TRI ? 3 : 1];
+  if (TRI) {
+    const T r = wJ[i], sn = wJ[j];
+    const T* ce = ggeo + (size_t)24 * e;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const T c_r = ce[8 * d + 0], c_s = ce[8 * d + 1], c_t = ce[8 * d + 2], c_rs = ce[8 * d + 3];
+      const T c_rt = ce[8 * d + 4], c_st = ce[8 * d + 5], c_rst = ce[8 * d + 6];
+      Ar[d] = fmad(c_rs, sn, c_r);
+      Br[d] = fmad(c_rst, sn, c_rt);
+      As[d] = fmad(c_rs, r, c_s);
+      Bs[d] = fmad(c_rst, r, c_st);
+      Ct[d] = fmad(Br[d], r, fmad(c_st, sn, c_t));
+    }
+    wij = gllw[i] * gllw[j];
+  } else if (AFFINE) {
     const T* ge = ggeo + (size_t)6 * e;
     g0 = ge[0]; g1 = ge[1]; g2 = ge[2]; g3 = ge[3]; g4 = ge[4]; g5 = ge[5];
     gw = wJ[e];
@@ -231,7 +278,28 @@ __global__ __launch_bounds__(NQ* NQ* EPB) __attribute__((amdgpu_waves_per_eu(WAV
       for (int m = 0; m < NQ; ++m) asm volatile("" : "+v"(rA[f][m]));
 
     T G00, G01, G02, G11, G12, G22, W;
-    if (AFFINE) {
+    if (TRI) {
+      const T tk = wJ[k];
+      const T xr = fmad(Br[0], tk, Ar[0]), yr = fmad(Br[1], tk, Ar[1]), zr = fmad(Br[2], tk, Ar[2]);
+      const T xs = fmad(Bs[0], tk, As[0]), ys = fmad(Bs[1], tk, As[1]), zs = fmad(Bs[2], tk, As[2]);
+      const T xt = Ct[0], yt = Ct[1], zt = Ct[2];
+      // rows of the adjugate: J grad r, J grad s, J grad t
+      const T arx = fmad(ys, zt, -(zs * yt)), ary = fmad(zs, xt, -(xs * zt)), arz = fmad(xs, yt, -(ys * xt));
+      const T asx = fmad(zr, yt, -(yr * zt)), asy = fmad(xr, zt, -(zr * xt)), asz = fmad(yr, xt, -(xr * yt));
+      const T atx = fmad(yr, zs, -(zr * ys)), aty = fmad(zr, xs, -(xr * zs)), atz = fmad(xr, ys, -(yr * xs));
+      const T J = fmad(xr, arx, fmad(yr, ary, zr * arz));
+      const T w = wij * gllw[k];
+      // hardware reciprocal (1 ulp) in place of the correctly rounded
+      // division, which takes ~10 instructions (Ax 44.8 -> 44.1 us)
+      const T sc = w * __builtin_amdgcn_rcpf(J);
+      G00 = sc * fmad(arx, arx, fmad(ary, ary, arz * arz));
+      G01 = sc * fmad(arx, asx, fmad(ary, asy, arz * asz));
+      G02 = sc * fmad(arx, atx, fmad(ary, aty, arz * atz));
+      G11 = sc * fmad(asx, asx, fmad(asy, asy, asz * asz));
+      G12 = sc * fmad(asx, atx, fmad(asy, aty, asz * atz));
+      G22 = sc * fmad(atx, atx, fmad(aty, aty, atz * atz));
+      W = J * w;
+    } else if (AFFINE) {
       const T w = wij * gllw[k];
       G00 = g0 * w; G01 = g1 * w; G02 = g2 * w; G11 = g3 * w; G12 = g4 * w; G22 = g5 * w;
       W = gw * w;
@@ -257,7 +325,7 @@ __global__ __launch_bounds__(NQ* NQ* EPB) __attribute__((amdgpu_waves_per_eu(WAV
       for (int m = 0; m < NQ; ++m) {
         qr = fmad((DREG ? Di[m] : s_D[i][m]), s_q[f][es][j][m], qr);
         qs = fmad((DREG ? Dj[m] : s_D[j][m]), s_q[f][es][m][i], qs);
-        qt = fmad(s_D[k][m], rq[f][m], qt);
+        qt = fmad((SDK ? Dc[k * NQ + m] : s_D[k][m]), rq[f][m], qt);
       }
       const T Gr = fmad(G02, qt, fmad(G01, qs, G00 * qr));
       const T Gs = fmad(G12, qt, fmad(G11, qs, G01 * qr));
@@ -266,7 +334,7 @@ __global__ __launch_bounds__(NQ* NQ* EPB) __attribute__((amdgpu_waves_per_eu(WAV
       s_s[f][es][j][i] = Gs;
       rA[f][k] = fmad(lambda * W, rq[f][k], rA[f][k]);
 #pragma unroll
-      for (int m = 0; m < NQ; ++m) rA[f][m] = fmad(s_D[k][m], Gt, rA[f][m]);
+      for (int m = 0; m < NQ; ++m) rA[f][m] = fmad((SDK ? Dc[k * NQ + m] : s_D[k][m]), Gt, rA[f][m]);
     }
     __syncthreads();
     asm volatile("" ::: "memory");
@@ -315,9 +383,18 @@ __global__ __launch_bounds__(NQ* NQ* EPB) __attribute__((amdgpu_waves_per_eu(WAV
       T* surf = Aq + (size_t)e * NSP;
       for (int n = ij; n < NSP; n += NQ2) surf[n] = (n < NS) ? s_out[es][fused_surface_node(NQ, n)] : T(0);
     }
-    // workgroup sum of the p.Ap terms (fixed order)
+    // workgroup sum of the p.Ap terms (fixed order). When NQ^2*EPB is not a
+    // multiple of 64 the last wavefront is partial, and the registers of
+    // its missing lanes hold whatever an earlier wave left there: a lane
+    // adds its butterfly partner's value only where that lane exists (full
+    // wavefronts add every value, as a plain butterfly does)
+    const int lane = t & 63;
+    const int nlive = min(64, NQ2 * EPB - (t & ~63));
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) pd += __shfl_xor(pd, off, 64);
+    for (int off = 32; off > 0; off >>= 1) {
+      const double o = __shfl_xor(pd, off, 64);
+      if ((lane ^ off) < nlive) pd += o;
+    }
     if (EPB * NQ2 <= 64) {
       if (t == 0) part[blockIdx.x] = pd;
     } else {
@@ -433,6 +510,16 @@ int launch_bp5(hipStream_t s, const AxArgs<T>& a, T* Ap = nullptr, double* part 
   if (blocks == 0) return BPX_OK;
   const int* noPid = nullptr;
   const BatchState* noState = nullptr;
+  if constexpr (std::is_same<T, float>::value && NF == 1 && (VAR == VAR_PLAIN || VAR == VAR_FUSED)) {
+    if (a.tri && !a.affine) {
+      constexpr int WT = bp5_waves_tri<NQ, VAR>();
+      launch_stamped(k_bp5_ax<T, NQ, 1, EPB, false, WT, VAR, true>, dim3(blocks), dim3(NQ * NQ * EPB), s,
+                     a.evStart, a.evStop, a.Nelements, a.K, a.elementList, a.GlobalToLocal, a.wJ, a.ggeo, a.gllw,
+                     a.D, a.lambda, a.q, a.Aq, a.ldq, a.ldAq, noPid, noState, Ap, part);
+      BPX_HIP(hipGetLastError());
+      return BPX_OK;
+    }
+  }
   if (a.affine)
     launch_stamped(k_bp5_ax<T, NQ, NF, EPB, true, WV, VAR>, dim3(blocks), dim3(NQ * NQ * EPB), s, a.evStart,
                    a.evStop, a.Nelements, a.K, a.elementList, a.GlobalToLocal, a.wJ, a.ggeo, a.gllw, a.D,

@@ -55,6 +55,10 @@ struct AxArgs {
   // rounding, not bit for bit. AX_ENGINE_AUTO = matrix cores where they
   // apply (the faster: 814 against 1084 us at 24^3, p = 15).
   int engine = 0;
+  // FP32 BP5, non-affine: ggeo holds the coefficients of each element's
+  // trilinear map (24 per element) and wJ the GLL nodes; the kernel forms
+  // the geometric factors itself
+  int tri = 0;
   // when set (BP1-BP6 Ax), the kernel's dispatch stamps its start and end
   // into these events (hipExtLaunchKernelGGL: no barrier packets around it)
   hipEvent_t evStart = nullptr, evStop = nullptr;

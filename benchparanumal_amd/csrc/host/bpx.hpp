@@ -107,6 +107,10 @@ class mesh_t {
   int Setup(const MeshOpts& o);
   // mesh_t::MaskedGatherScatterSetup ids (meshMaskedGatherScatterSetup.cpp:31-77)
   void MaskedGlobalIds(int Nfields, std::vector<hlong>& ids) const;
+  // per element, for x, y and z in turn, the coefficients of r, s, t, rs,
+  // rt, st, rst of the trilinear map through its 8 vertices, and one pad
+  // (24 values): the nodal coordinates are this map at the GLL nodes
+  void TrilinearCoefficients(std::vector<double>& c) const;
 
   MeshOpts opts;
   int N = 0, Nq = 0, Np = 0, cubNq = 0, cubNp = 0;
@@ -346,6 +350,7 @@ class bp_t : public solver_t {
   // FP32 copies (created on first use)
   int Prepare32();
   bool have32 = false;
+  int tri32 = 0;  // the FP32 BP5 operator forms its geometric factors from the trilinear maps
   deviceBuffer<float> o_wJ32, o_ggeo32, o_gllw32, o_D32, o_I32, o_MM32, o_invV32, o_AqL32;
   // nodal coordinates for the forcing (created on first use)
   deviceBuffer<double> o_x, o_y, o_z, o_gllwRhs;

@@ -28,6 +28,10 @@ namespace bpx {
 
 namespace {
 
+// reference coordinates of the 8 vertices of an element, in vertex order
+const double kVr[8] = {-1, 1, 1, -1, -1, 1, 1, -1}, kVs[8] = {-1, -1, 1, 1, -1, -1, 1, 1},
+             kVt[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
+
 struct Geo9 {
   double xr, xs, xt, yr, ys, yt, zr, zs, zt;
 };
@@ -52,6 +56,25 @@ inline bool metric(const Geo9& d, double& J, double g[6]) {
 }
 
 }  // namespace
+
+void mesh_t::TrilinearCoefficients(std::vector<double>& c) const {
+  c.assign((size_t)Nelements * 24, 0.0);
+  for (dlong e = 0; e < Nelements; ++e)
+    for (int d = 0; d < 3; ++d) {
+      const double* V = (d == 0 ? EX.data() : d == 1 ? EY.data() : EZ.data()) + (size_t)e * 8;
+      double* ce = &c[(size_t)e * 24 + 8 * d];
+      for (int v = 0; v < 8; ++v) {
+        const double X = 0.125 * V[v];
+        ce[0] += kVr[v] * X;
+        ce[1] += kVs[v] * X;
+        ce[2] += kVt[v] * X;
+        ce[3] += kVr[v] * kVs[v] * X;
+        ce[4] += kVr[v] * kVt[v] * X;
+        ce[5] += kVs[v] * kVt[v] * X;
+        ce[6] += kVr[v] * kVs[v] * kVt[v] * X;
+      }
+    }
+}
 
 int mesh_t::Setup(const MeshOpts& o) {
   if (o.N < 1 || o.N > 15) return set_error(BPX_ERR_INVALID_ARGUMENT, "mesh: degree N must be in 1..15");
@@ -110,8 +133,7 @@ int mesh_t::Setup(const MeshOpts& o) {
   x.assign(NL, 0.0);
   y.assign(NL, 0.0);
   z.assign(NL, 0.0);
-  static const double sr[8] = {-1, 1, 1, -1, -1, 1, 1, -1}, ss[8] = {-1, -1, 1, 1, -1, -1, 1, 1},
-                      st[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
+  const double *sr = kVr, *ss = kVs, *st = kVt;
 #pragma omp parallel for
   for (dlong e = 0; e < Nelements; ++e)
     for (int n = 0; n < Np; ++n) {

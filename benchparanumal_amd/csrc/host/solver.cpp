@@ -295,6 +295,7 @@ int bp_t::ax(const T* q, T* AqL, const T* wJ, const T* ggeo, const T* gllw, cons
   a.q = q;
   a.Aq = AqL;
   if constexpr (std::is_same<T, double>::value) a.engine = axEngine;
+  if constexpr (std::is_same<T, float>::value) a.tri = tri32;
   a.evStart = axEvStart;
   a.evStop = axEvStop;
   if (bp == 5 || bp == 6) return launch_bp5_ax<T>(platform->stream, a);
@@ -514,6 +515,7 @@ int bp_t::OperatorDotFused(const T* o_q, T* o_Aq, double* part, int* npart) {
     if (haveKappa) return set_error(BPX_ERR_UNSUPPORTED, "bp: the FP32 operator has no variable coefficient");
     if (int st = Prepare32()) return st;
     a.affine = mesh->opts.affine;
+    a.tri = tri32;
     a.wJ = o_wJ32.ptr();
     a.ggeo = o_ggeo32.ptr();
     a.gllw = o_gllw32.ptr();
@@ -823,7 +825,16 @@ int bp_t::Prepare32() {
   const mesh_t& m = *mesh;
   const int affine = m.opts.affine;
   int st = BPX_OK;
-  if (bp == 5 || bp == 6) {
+  if (bp == 5 && !affine) {
+    // the FP32 operator forms its geometric factors from the elements'
+    // trilinear maps: 96 B per element instead of 28 B per node
+    std::vector<double> tri;
+    m.TrilinearCoefficients(tri);
+    if ((st = cvt(o_wJ32, m.gllz)) || (st = cvt(o_ggeo32, tri)) || (st = cvt(o_gllw32, m.gllw)) ||
+        (st = cvt(o_D32, m.D)))
+      return st;
+    tri32 = 1;
+  } else if (bp == 5 || bp == 6) {
     if ((st = cvt(o_wJ32, m.wJ)) || (st = cvt(o_ggeo32, m.ggeo)) || (st = cvt(o_gllw32, m.gllw)) ||
         (st = cvt(o_D32, m.D)))
       return st;

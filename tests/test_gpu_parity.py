@@ -201,9 +201,13 @@ def test_cg_zero_rhs_and_nan_breakdown(plat):
     assert ei.value.code == -7
 
 
-@pytest.mark.parametrize("bp", [1, 3, 5])
-def test_fp32_operator_close_to_fp64(plat, bp):
-    m, P = make_pair(bp, 7, 4, lam=1.0)
+@pytest.mark.parametrize("bp,N", [(1, 7), (3, 7), (5, 7), (5, 1), (5, 2), (5, 4), (5, 5), (5, 6), (5, 8),
+                                  (5, 9), (5, 11), (5, 15)])
+def test_fp32_operator_close_to_fp64(plat, bp, N):
+    """The FP32 operator (BP5 non-affine: geometric factors formed in the
+    kernel from the elements' trilinear maps) against the FP64 one, at
+    degrees whose rows of D do and do not fill whole 16-byte units."""
+    m, P = make_pair(bp, N, 4 if N <= 8 else 2, lam=1.0)
     b = capi.BP(plat, m, bp, lam=1.0)
     q = np.random.default_rng(5).standard_normal(b.Ngather)
     A64 = torch.zeros(b.Ngather, dtype=torch.float64, device="cuda")
