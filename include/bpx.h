@@ -296,7 +296,9 @@ int bpx_bp_operator(bpx_bp b, const double* o_q, double* o_Aq);
 int bpx_bp_operator_dot(bpx_bp b, const double* o_q, double* o_Aq, double* qAq);
 /* Ax alone into the local vector (BK/BK5/src/bk5Run.cpp:59-71). */
 int bpx_bp_ax_local(bpx_bp b, const double* o_q, double* o_AqL);
-/* FP32 operator on the same mesh (factors rounded to float). */
+/* FP32 operator on the same mesh (D, I and factors rounded to float; BP5
+ * non-affine forms its geometric factors in the kernel from each element's
+ * trilinear map, agreeing with the FP64 factors to float rounding). */
 int bpx_bp_operator_f32(bpx_bp b, const float* o_q, float* o_Aq);
 /* No reference counterpart: bpN_t::Operator (BP/BP5/src/bp5Operator.cpp:29-85)
  * on K right-hand sides (BP5 only), stored one after another: o_q and o_Aq
