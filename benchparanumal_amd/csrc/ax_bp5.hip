@@ -71,10 +71,10 @@ constexpr int bp5_waves_v() {
 }
 
 // occupancy target of the FP32 kernels with the trilinear geometry (the
-// plain one at Nq = 8 spills 35 registers at 7 waves; 5 hold it)
+// plain one at Nq = 8 spills at 7 and at 5 waves; 4 hold it)
 template <int NQ, int VAR>
 constexpr int bp5_waves_tri() {
-  return (VAR == 0 && NQ == 8) ? 5 : bp5_waves_v<float, NQ, 1, VAR>();
+  return (VAR == 0 && NQ == 8) ? 4 : bp5_waves_v<float, NQ, 1, VAR>();
 }
 
 // D rows/columns in registers (saves 4*Nq LDS reads per node) for one
@@ -228,11 +228,17 @@ __global__ __launch_bounds__(NQ* NQ* EPB) __attribute__((amdgpu_waves_per_eu(WAV
   // geometry of the current slab (non-affine: prefetched one slab ahead)
   T g0 = 0, g1 = 0, g2 = 0, g3 = 0, g4 = 0, g5 = 0, gw = 0;
   T wij = 0;
-  // TRI: per coordinate, d/dr = Ar + Br t, d/ds = As + Bs t, d/dt = Ct
-  T Ar[TRI ? 3 : 1], Br[TRI ? 3 : 1], As[TRI ? 3 : 1], Bs[TRI ? 3 : 1], Ct[TRI ? 3 : 1];
+  // TRI: along the pencil, x_r = Ar + Br t, x_s = As + Bs t, x_t = Ct, so
+  // the adjugate rows J grad r = x_s x x_t = Pr + Qr t and J grad s =
+  // x_t x x_r = Ps + Qs t are linear in t, J grad t = x_r x x_s = T0 + T1 t
+  // + T2 t^2 is quadratic, and so is J = x_t . (J grad t) = j0 + j1 t + j2 t^2
+  // (14 operations per node instead of 27 for the derivatives, adjugate and J)
+  T Pr[TRI ? 3 : 1], Qr[TRI ? 3 : 1], Ps[TRI ? 3 : 1], Qs[TRI ? 3 : 1];
+  T T0[TRI ? 3 : 1], T1[TRI ? 3 : 1], T2[TRI ? 3 : 1], jt[TRI ? 3 : 1];
   if (TRI) {
     const T r = wJ[i], sn = wJ[j];
     const T* ce = ggeo + (size_t)24 * e;
+    T Ar[3], Br[3], As[3], Bs[3], Ct[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       const T c_r = ce[8 * d + 0], c_s = ce[8 * d + 1], c_t = ce[8 * d + 2], c_rs = ce[8 * d + 3];
@@ -243,6 +249,24 @@ __global__ __launch_bounds__(NQ* NQ* EPB) __attribute__((amdgpu_waves_per_eu(WAV
       Bs[d] = fmad(c_rst, r, c_st);
       Ct[d] = fmad(Br[d], r, fmad(c_st, sn, c_t));
     }
+    // (a x b)_d = a_{d+1} b_{d+2} - a_{d+2} b_{d+1}
+    auto cross = [](const T* a, const T* b, int d) {
+      const int d1 = (d + 1) % 3, d2 = (d + 2) % 3;
+      return fmad(a[d1], b[d2], -(a[d2] * b[d1]));
+    };
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      Pr[d] = cross(As, Ct, d);
+      Qr[d] = cross(Bs, Ct, d);
+      Ps[d] = cross(Ct, Ar, d);
+      Qs[d] = cross(Ct, Br, d);
+      T0[d] = cross(Ar, As, d);
+      T1[d] = cross(Ar, Bs, d) + cross(Br, As, d);
+      T2[d] = cross(Br, Bs, d);
+    }
+    jt[0] = fmad(Ct[0], T0[0], fmad(Ct[1], T0[1], Ct[2] * T0[2]));
+    jt[1] = fmad(Ct[0], T1[0], fmad(Ct[1], T1[1], Ct[2] * T1[2]));
+    jt[2] = fmad(Ct[0], T2[0], fmad(Ct[1], T2[1], Ct[2] * T2[2]));
     wij = gllw[i] * gllw[j];
   } else if (AFFINE) {
     const T* ge = ggeo + (size_t)6 * e;
@@ -277,27 +301,23 @@ __global__ __launch_bounds__(NQ* NQ* EPB) __attribute__((amdgpu_waves_per_eu(WAV
 #pragma unroll
       for (int m = 0; m < NQ; ++m) asm volatile("" : "+v"(rA[f][m]));
 
-    T G00, G01, G02, G11, G12, G22, W;
+    T G00 = 0, G01 = 0, G02 = 0, G11 = 0, G12 = 0, G22 = 0, W;
+    // TRI: rows of the adjugate (J grad r, J grad s, J grad t) and w/J; G is
+    // applied as adj (w/J) adj^T without forming it (21 operations per node
+    // instead of 24 to form G and 9 to apply it)
+    T arx = 0, ary = 0, arz = 0, asx = 0, asy = 0, asz = 0, atx = 0, aty = 0, atz = 0, sc = 0;
     if (TRI) {
       const T tk = wJ[k];
-      const T xr = fmad(Br[0], tk, Ar[0]), yr = fmad(Br[1], tk, Ar[1]), zr = fmad(Br[2], tk, Ar[2]);
-      const T xs = fmad(Bs[0], tk, As[0]), ys = fmad(Bs[1], tk, As[1]), zs = fmad(Bs[2], tk, As[2]);
-      const T xt = Ct[0], yt = Ct[1], zt = Ct[2];
-      // rows of the adjugate: J grad r, J grad s, J grad t
-      const T arx = fmad(ys, zt, -(zs * yt)), ary = fmad(zs, xt, -(xs * zt)), arz = fmad(xs, yt, -(ys * xt));
-      const T asx = fmad(zr, yt, -(yr * zt)), asy = fmad(xr, zt, -(zr * xt)), asz = fmad(yr, xt, -(xr * yt));
-      const T atx = fmad(yr, zs, -(zr * ys)), aty = fmad(zr, xs, -(xr * zs)), atz = fmad(xr, ys, -(yr * xs));
-      const T J = fmad(xr, arx, fmad(yr, ary, zr * arz));
+      arx = fmad(Qr[0], tk, Pr[0]); ary = fmad(Qr[1], tk, Pr[1]); arz = fmad(Qr[2], tk, Pr[2]);
+      asx = fmad(Qs[0], tk, Ps[0]); asy = fmad(Qs[1], tk, Ps[1]); asz = fmad(Qs[2], tk, Ps[2]);
+      atx = fmad(fmad(T2[0], tk, T1[0]), tk, T0[0]);
+      aty = fmad(fmad(T2[1], tk, T1[1]), tk, T0[1]);
+      atz = fmad(fmad(T2[2], tk, T1[2]), tk, T0[2]);
+      const T J = fmad(fmad(jt[2], tk, jt[1]), tk, jt[0]);
       const T w = wij * gllw[k];
       // hardware reciprocal (1 ulp) in place of the correctly rounded
       // division, which takes ~10 instructions (Ax 44.8 -> 44.1 us)
-      const T sc = w * __builtin_amdgcn_rcpf(J);
-      G00 = sc * fmad(arx, arx, fmad(ary, ary, arz * arz));
-      G01 = sc * fmad(arx, asx, fmad(ary, asy, arz * asz));
-      G02 = sc * fmad(arx, atx, fmad(ary, aty, arz * atz));
-      G11 = sc * fmad(asx, asx, fmad(asy, asy, asz * asz));
-      G12 = sc * fmad(asx, atx, fmad(asy, aty, asz * atz));
-      G22 = sc * fmad(atx, atx, fmad(aty, aty, atz * atz));
+      sc = w * __builtin_amdgcn_rcpf(J);
       W = J * w;
     } else if (AFFINE) {
       const T w = wij * gllw[k];
@@ -327,9 +347,19 @@ __global__ __launch_bounds__(NQ* NQ* EPB) __attribute__((amdgpu_waves_per_eu(WAV
         qs = fmad((DREG ? Dj[m] : s_D[j][m]), s_q[f][es][m][i], qs);
         qt = fmad((SDK ? Dc[k * NQ + m] : s_D[k][m]), rq[f][m], qt);
       }
-      const T Gr = fmad(G02, qt, fmad(G01, qs, G00 * qr));
-      const T Gs = fmad(G12, qt, fmad(G11, qs, G01 * qr));
-      const T Gt = fmad(G22, qt, fmad(G12, qs, G02 * qr));
+      T Gr, Gs, Gt;
+      if (TRI) {
+        const T gx = sc * fmad(atx, qt, fmad(asx, qs, arx * qr));
+        const T gy = sc * fmad(aty, qt, fmad(asy, qs, ary * qr));
+        const T gz = sc * fmad(atz, qt, fmad(asz, qs, arz * qr));
+        Gr = fmad(arz, gz, fmad(ary, gy, arx * gx));
+        Gs = fmad(asz, gz, fmad(asy, gy, asx * gx));
+        Gt = fmad(atz, gz, fmad(aty, gy, atx * gx));
+      } else {
+        Gr = fmad(G02, qt, fmad(G01, qs, G00 * qr));
+        Gs = fmad(G12, qt, fmad(G11, qs, G01 * qr));
+        Gt = fmad(G22, qt, fmad(G12, qs, G02 * qr));
+      }
       s_r[f][es][j][i] = Gr;
       s_s[f][es][j][i] = Gs;
       rA[f][k] = fmad(lambda * W, rq[f][k], rA[f][k]);
