@@ -190,10 +190,13 @@ class Problem:
         lib().orc_operator(self.h, dptr(q), dptr(out))
         return out
 
-    def rhs_forcing(self):
+    def rhs_forcing_local(self):
         rL = np.zeros(self.NLocal)
         lib().orc_rhs_forcing_local(self.h, dptr(rL))
-        return self.gather(rL)
+        return rL
+
+    def rhs_forcing(self):
+        return self.gather(self.rhs_forcing_local())
 
     # ---- variable coefficient (kappa on the node lattice) and adjoint gradient
     def lattice_nodes(self):

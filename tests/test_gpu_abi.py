@@ -1,0 +1,272 @@
+"""The drop-in boundary itself: every kernel-level entry point of
+include/bpx.h, called through the C ABI on the oracle's own arrays (the
+arrays the reference's setup would hand the OKL kernels) and compared with
+the oracle. FP64 Ax to 1e-12 of max|ref|, FP32 Ax to 2e-5, gather and
+scatter bit-exact, vector updates to the last bit of a correctly rounded
+formula or 1e-13 for reductions.
+
+Reference signatures: okl/bp1AxHex3D.okl:61-68, okl/bp3AxHex3D.okl:369-381,
+okl/bp5AxHex3D.okl:415-427, okl/rhsHex3D.okl:44-128,
+libs/ogs/okl/ogsKernels.okl:32-164, libs/core/okl/linAlg*.okl and
+libs/core/okl/linearSolverUpdateCG.okl."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle_lib
+from benchparanumal_amd import capi
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+AX_RTOL = 1e-12
+F32_RTOL = 2e-5
+
+
+@pytest.fixture(scope="module")
+def plat():
+    assert torch.cuda.is_available(), "GPU tests need a visible MI355X"
+    return capi.Platform(0)
+
+
+def dev(a, dtype=None):
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.cuda()
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def rel_err(got, ref):
+    return float(np.abs(got - ref).max() / max(np.abs(ref).max(), 1e-300))
+
+
+def P_(t):
+    return capi.ptr(t)
+
+
+def problem(bp, N, affine, nx=3, ny=2, nz=2, lam=1.3, seed=4):
+    return oracle_lib.Problem(bp=bp, N=N, nx=nx, ny=ny, nz=nz, affine=affine, lam=lam,
+                              perturb=0.0 if affine else 0.1, seed=seed)
+
+
+def arrays(P, names, dtype=None):
+    return {k: dev(P.array(k), dtype if P.array(k).dtype == np.float64 else None) for k in names}
+
+
+# ------------------------------------------------------------------ Ax --
+# (bp, N, affine): the wavefront cubature kernels run Nq <= 8 non-affine,
+# the sheet kernel the affine forms and Nq > 8; BP5/6 every form
+AX_CASES = [(bp, N, aff) for bp in (1, 2, 3, 4, 5, 6) for N in (2, 7, 10) for aff in (False, True)]
+
+
+@pytest.mark.parametrize("bp,N,affine", AX_CASES)
+def test_ax_entry_points_match_oracle(plat, bp, N, affine):
+    nz = 1 if (bp in (2, 4, 6) and N >= 7) else 2
+    P = problem(bp, N, affine, nz=nz)
+    L = capi.lib()
+    q = np.random.default_rng(bp * 31 + N).standard_normal(P.Ngather)
+    ref = P.ax_local(q)
+    qd = dev(q)
+    out = torch.full((P.NLocal,), float("nan"), dtype=torch.float64, device="cuda")
+    a = arrays(P, ["elementList", "GlobalToLocal"])
+    el, g2l = P_(a["elementList"]), P_(a["GlobalToLocal"])
+    suffix = "_affine_hex" if affine else "_hex"
+    fn = getattr(L, f"bpx_bp{bp}_ax{suffix}")
+    if bp in (5, 6):
+        g = arrays(P, ["wJ", "ggeo", "gllw", "D"])
+        capi.check(fn(plat.h, N, P.Nelements, el, g2l, P_(g["wJ"]), None, P_(g["ggeo"]), P_(g["gllw"]),
+                      P_(g["D"]), None, None, P.lam, P_(qd), P_(out)))
+    elif bp in (1, 2):
+        g = arrays(P, ["cubwJ", "cubInterp", "MM"])
+        capi.check(fn(plat.h, N, P.Nelements, el, g2l, P_(g["cubwJ"]), P_(g["cubInterp"]), P_(g["MM"]),
+                      P_(qd), P_(out)))
+    else:
+        W, G = ("wJ", "ggeo") if affine else ("cubwJ", "cubggeo")
+        g = arrays(P, [W, G, "cubD", "cubInterp", "invV"])
+        capi.check(fn(plat.h, N, P.Nelements, el, g2l, P_(g[W]), P_(g[G]), P_(g["cubD"]), P_(g["cubInterp"]),
+                      P_(g["invV"]), None, None, P.lam, P_(qd), P_(out)))
+    assert rel_err(host(out), ref) <= AX_RTOL
+
+
+@pytest.mark.parametrize("bp,N", [(5, 2), (5, 7), (5, 10), (5, 15), (3, 3), (3, 7), (3, 10)])
+def test_ax_f32_entry_points_close_to_oracle(plat, bp, N):
+    """FP32 kernel entries on the oracle's arrays rounded to float: within
+    FP32 rounding of the FP64 oracle."""
+    P = problem(bp, N, False, nx=2 if N >= 10 else 3)
+    L = capi.lib()
+    q = np.random.default_rng(N).standard_normal(P.Ngather)
+    ref = P.ax_local(q)
+    f32 = torch.float32
+    qd = dev(q, f32)
+    out = torch.full((P.NLocal,), float("nan"), dtype=f32, device="cuda")
+    a = arrays(P, ["elementList", "GlobalToLocal"])
+    el, g2l = P_(a["elementList"]), P_(a["GlobalToLocal"])
+    if bp == 5:
+        g = arrays(P, ["wJ", "ggeo", "gllw", "D"], f32)
+        capi.check(L.bpx_bp5_ax_hex_f32(plat.h, N, P.Nelements, el, g2l, P_(g["wJ"]), None, P_(g["ggeo"]),
+                                        P_(g["gllw"]), P_(g["D"]), None, None, P.lam, P_(qd), P_(out)))
+    else:
+        g = arrays(P, ["cubwJ", "cubggeo", "cubD", "cubInterp", "invV"], f32)
+        capi.check(L.bpx_bp3_ax_hex_f32(plat.h, N, P.Nelements, el, g2l, P_(g["cubwJ"]), P_(g["cubggeo"]),
+                                        P_(g["cubD"]), P_(g["cubInterp"]), P_(g["invV"]), None, None, P.lam,
+                                        P_(qd), P_(out)))
+    assert rel_err(host(out).astype(np.float64), ref) <= F32_RTOL
+
+
+@pytest.mark.parametrize("bp,affine", [(5, False), (5, True), (6, False), (6, True), (3, False), (1, True)])
+def test_rhs_entry_point_matches_oracle(plat, bp, affine):
+    P = problem(bp, 4, affine, lam=0.8)
+    nf = 3 if bp in (2, 4, 6) else 1
+    g = arrays(P, ["wJ", "gllw", "MM", "x", "y", "z"])
+    out = torch.full((P.NLocal,), float("nan"), dtype=torch.float64, device="cuda")
+    capi.check(capi.lib().bpx_rhs_hex(plat.h, 4, nf, int(affine), P.Nelements, P_(g["wJ"]), P_(g["gllw"]),
+                                      P_(g["MM"]), P_(g["x"]), P_(g["y"]), P_(g["z"]), P.lam, P_(out)))
+    assert rel_err(host(out), P.rhs_forcing_local()) <= 1e-13
+
+
+# ----------------------------------------------------------------- ogs --
+def ogs_maps(P):
+    return arrays(P, ["blockRowStartsT", "rowStartsT", "colIdsT"])
+
+
+def seq_gather_f32(P, vL):
+    """Row sums from 0 in ascending column order, every add rounded to float."""
+    rs, ci = P.array("rowStartsT"), P.array("colIdsT")
+    cnt = np.diff(rs)
+    acc = np.zeros(P.Ngather, dtype=np.float32)
+    for m in range(int(cnt.max())):
+        rows = np.nonzero(cnt > m)[0]
+        acc[rows] = acc[rows] + vL[ci[rs[rows] + m]]
+    return acc
+
+
+@pytest.mark.parametrize("N,nx", [(3, 4), (7, 3)])
+def test_ogs_gather_scatter_entry_points_bit_exact(plat, N, nx):
+    P = problem(5, N, False, nx=nx)
+    L = capi.lib()
+    m = ogs_maps(P)
+    nb = P.NrowBlocksT
+    bs, rs, ci = P_(m["blockRowStartsT"]), P_(m["rowStartsT"]), P_(m["colIdsT"])
+    rng = np.random.default_rng(N)
+    vL = rng.standard_normal(P.NLocal)
+    gv = torch.full((P.Ngather,), float("nan"), dtype=torch.float64, device="cuda")
+    capi.check(L.bpx_ogs_gather(plat.h, nb, 1, bs, rs, ci, P_(dev(vL)), P_(gv)))
+    np.testing.assert_array_equal(host(gv), P.gather(vL))
+    # FP32 gather: the same order, every add in float
+    vL32 = vL.astype(np.float32)
+    gv32 = torch.full((P.Ngather,), float("nan"), dtype=torch.float32, device="cuda")
+    capi.check(L.bpx_ogs_gather_f32(plat.h, nb, 1, bs, rs, ci, P_(dev(vL32)), P_(gv32)))
+    np.testing.assert_array_equal(host(gv32), seq_gather_f32(P, vL32))
+    # scatter writes every member of every row; masked local entries keep
+    # what the output held
+    v = rng.standard_normal(P.Ngather)
+    outL = torch.zeros(P.NLocal, dtype=torch.float64, device="cuda")
+    capi.check(L.bpx_ogs_scatter(plat.h, nb, 1, bs, rs, ci, P_(dev(v)), P_(outL)))
+    np.testing.assert_array_equal(host(outL), P.scatter(v))
+    # gather-scatter in place: members get their row's sum, masked entries
+    # keep their value
+    qL = dev(vL)
+    capi.check(L.bpx_ogs_gather_scatter(plat.h, nb, 1, bs, rs, ci, P_(qL)))
+    exp = vL.copy()
+    members = P.array("colIdsT")
+    exp[members] = P.scatter(P.gather(vL))[members]
+    np.testing.assert_array_equal(host(qL), exp)
+
+
+def test_ogs_entry_points_interleaved_K(plat):
+    """K interleaved vectors (entry k of node n at k + n*K)."""
+    P = problem(5, 3, False, nx=3)
+    L = capi.lib()
+    m = ogs_maps(P)
+    nb = P.NrowBlocksT
+    bs, rs, ci = P_(m["blockRowStartsT"]), P_(m["rowStartsT"]), P_(m["colIdsT"])
+    K = 3
+    vL = np.random.default_rng(9).standard_normal((P.NLocal, K))
+    gv = torch.zeros(P.Ngather * K, dtype=torch.float64, device="cuda")
+    capi.check(L.bpx_ogs_gather(plat.h, nb, K, bs, rs, ci, P_(dev(vL.ravel())), P_(gv)))
+    got = host(gv).reshape(P.Ngather, K)
+    v = np.random.default_rng(10).standard_normal((P.Ngather, K))
+    outL = torch.zeros(P.NLocal * K, dtype=torch.float64, device="cuda")
+    capi.check(L.bpx_ogs_scatter(plat.h, nb, K, bs, rs, ci, P_(dev(v.ravel())), P_(outL)))
+    sc = host(outL).reshape(P.NLocal, K)
+    for k in range(K):
+        np.testing.assert_array_equal(got[:, k], P.gather(vL[:, k].copy()))
+        np.testing.assert_array_equal(sc[:, k], P.scatter(v[:, k].copy()))
+
+
+# -------------------------------------------------------------- linAlg --
+@pytest.mark.parametrize("n", [1, 1000, 300001])
+def test_linalg_entry_points(plat, n):
+    L = capi.lib()
+    rng = np.random.default_rng(n)
+    x, y = rng.standard_normal(n), rng.standard_normal(n)
+    xd = dev(x)
+    capi.check(L.bpx_linalg_set(plat.h, n, 2.5, P_(xd)))
+    assert np.all(host(xd) == 2.5)
+    xd, yd = dev(x), dev(y)
+    alpha, beta = 0.37, -1.21
+    capi.check(L.bpx_linalg_axpy(plat.h, n, alpha, P_(xd), beta, P_(yd)))
+    got = host(yd)
+    # y = alpha x + beta y: the exact value, with at most one rounding of each
+    # product and of the sum (fused or not): within 1.5 ulp of the exact value
+    exact = alpha * x.astype(np.longdouble) + beta * y.astype(np.longdouble)
+    assert np.all(np.abs(got - exact) <= 1.5 * np.spacing(np.abs(got)) + 1e-300)
+    r1, r2 = C.c_double(), C.c_double()
+    capi.check(L.bpx_linalg_inner_prod(plat.h, n, P_(dev(x)), P_(dev(y)), C.byref(r1)))
+    assert abs(r1.value - oracle_lib.inner_prod(x, y)) <= 1e-13 * np.abs(x * y).sum()
+    capi.check(L.bpx_linalg_norm2(plat.h, n, P_(dev(x)), C.byref(r2)))
+    assert abs(r2.value - oracle_lib.norm2(x)) <= 1e-14 * np.linalg.norm(x)
+
+
+@pytest.mark.parametrize("n", [7, 4657463])
+def test_cg_update_entry_point(plat, n):
+    """cg::UpdateCG: r -= alpha Ap, x += alpha p, returns r.r (oracle:
+    libs/core/linearSolverCG.cpp:131-156 with its 512-wide two-stage sum)."""
+    L = capi.lib()
+    rng = np.random.default_rng(n)
+    Ap, p, r, x = (rng.standard_normal(n) for _ in range(4))
+    alpha = 0.731
+    rd, xd = dev(r), dev(x)
+    rr = C.c_double()
+    capi.check(L.bpx_cg_update(plat.h, n, P_(dev(Ap)), P_(dev(p)), alpha, P_(rd), P_(xd), C.byref(rr)))
+    r_ref = r.copy()
+    rr_ref = oracle_lib.lib().orc_la_update_cg(n, oracle_lib.dptr(np.ascontiguousarray(Ap)), alpha,
+                                               oracle_lib.dptr(r_ref))
+    got_r, got_x = host(rd), host(xd)
+    exact_r = r.astype(np.longdouble) - alpha * Ap.astype(np.longdouble)
+    exact_x = x.astype(np.longdouble) + alpha * p.astype(np.longdouble)
+    assert np.all(np.abs(got_r - exact_r) <= 1.5 * np.spacing(np.abs(got_r)) + 1e-300)
+    assert np.all(np.abs(got_x - exact_x) <= 1.5 * np.spacing(np.abs(got_x)) + 1e-300)
+    assert np.abs(got_r - r_ref).max() <= 2 * np.spacing(np.abs(r_ref).max())
+    assert abs(rr.value - rr_ref) <= 1e-13 * rr_ref
+
+
+def test_entry_points_reject_bad_arguments(plat):
+    L = capi.lib()
+    x = torch.zeros(8, dtype=torch.float64, device="cuda")
+    assert L.bpx_linalg_axpy(plat.h, 8, 1.0, None, 1.0, P_(x)) == -1
+    assert L.bpx_bp5_ax_hex(None, 3, 1, None, None, None, None, None, None, None, None, None, 1.0, None, None) == -1
+    assert L.bpx_bp5_ax_hex(plat.h, 16, 1, P_(x), P_(x), P_(x), None, P_(x), P_(x), P_(x), None, None, 1.0, P_(x),
+                            P_(x)) != 0  # degree above 15
+
+
+def test_every_declared_symbol_is_called_by_a_gpu_test():
+    """Every kernel-level entry point in include/bpx.h appears in a GPU test."""
+    import glob
+    import os
+    import re
+
+    here = os.path.dirname(os.path.abspath(__file__))
+    src = "".join(open(f).read() for f in glob.glob(os.path.join(here, "test_gpu_*.py")) +
+                  glob.glob(os.path.join(here, "test_zz_*.py")))
+    called = set(re.findall(r"bpx_[a-z0-9_]+", src))
+    called |= {"bpx_bp%d_ax%s" % (bp, s) for bp in range(1, 7) for s in ("_hex", "_affine_hex")}  # by getattr above
+    wrapped = set(re.findall(r"lib\(\)\.(bpx_[a-z0-9_]+)", open(capi.__file__).read()))
+    missing = [s for s in capi.header_symbols() if s not in called and s not in wrapped]
+    assert not missing, missing

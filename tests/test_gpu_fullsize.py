@@ -1,0 +1,219 @@
+"""Parity at the benchmark configurations' own sizes, through the C ABI,
+against the oracle (FP64 bar: operator to 1e-12 of max|ref|, CG residual
+history to 1e-9 relative per iteration, solution to 1e-8).
+
+- config 1 (10^3, p=4): the literal variant ([0,1]^3, lambda = 0, b = M f
+  with f = 3 pi^2 sin sin sin), affine and non-affine storage;
+- config 2 (24^3, p=7 and p=15, perturbed by 0.1 h): BK5 Ax, BP5 CG history,
+  BP1-BP4 and BP6 operators at p=7;
+- config 3 (24^3, p=7): mixed-precision BP3 and BP5 to 1e-10, error against
+  an FP64 solution of the oracle's system;
+- config 4a (20^3, p=7): K = 16 and 32 right-hand sides;
+- BP1-BP4 at every degree 9..15 (the sheet cubature kernels) on small boxes.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib
+from benchparanumal_amd import capi
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+AX_RTOL = 1e-12
+CG_RTOL = 1e-9
+THREADS = 16  # the host share of one GPU
+
+
+@pytest.fixture(scope="module")
+def plat():
+    assert torch.cuda.is_available(), "GPU tests need a visible MI355X"
+    oracle_lib.set_threads(THREADS)
+    return capi.Platform(0)
+
+
+def dev(a, dtype=None):
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.cuda()
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def rel_err(got, ref):
+    return float(np.abs(got - ref).max() / max(np.abs(ref).max(), 1e-300))
+
+
+def pair(bp, N, nx, ny=None, nz=None, affine=False, perturb=0.1, seed=1, lam=1.0, lo=(-1.0,) * 3, dim=(2.0,) * 3):
+    kw = dict(N=N, nx=nx, ny=ny, nz=nz, affine=affine, perturb=0.0 if affine else perturb, seed=seed, lo=lo,
+              dim=dim)
+    m = capi.Mesh(cubature=bp <= 4, **kw)
+    P = oracle_lib.Problem(bp=bp, lam=lam, **kw)
+    return m, P
+
+
+def oracle_true_residual(P, x, b):
+    return np.linalg.norm(b - P.operator(x)) / np.linalg.norm(b)
+
+
+# -------------------------------------------------------------- config 1
+@pytest.mark.parametrize("affine", [False, True])
+def test_config1_literal_variant_10cube_p4(plat, affine):
+    """BP5 on [0,1]^3, 10^3 elements, p=4, lambda = 0 (-lap u = f),
+    b = M f with f = 3 pi^2 sin(pi x) sin(pi y) sin(pi z) at the GLL nodes
+    (the forcing kernel with lambda = 0 is exactly M f: the collocated mass
+    is wJ), zero initial guess, 100 CG iterations."""
+    m, P = pair(5, 4, 10, affine=affine, perturb=0.0, lam=0.0, lo=(0.0,) * 3, dim=(1.0,) * 3)
+    b = capi.BP(plat, m, 5, lam=0.0)
+    assert b.Ngather == P.Ngather == 39**3
+    rhs = torch.zeros(P.Ngather, dtype=torch.float64, device="cuda")
+    b.rhs_forcing(rhs)
+    rhs_ref = P.rhs_forcing()
+    assert rel_err(host(rhs), rhs_ref) <= 1e-13
+    # b = M f by its definition: the assembled collocated mass times f
+    X, Y, Z = P.array("x"), P.array("y"), P.array("z")
+    W = P.array("wJ") if not affine else None
+    if W is not None:
+        fL = 3 * np.pi**2 * np.sin(np.pi * X) * np.sin(np.pi * Y) * np.sin(np.pi * Z)
+        assert rel_err(P.gather(W * fL), rhs_ref) <= 1e-13
+    _, it_ref, hist_ref, _ = P.cg(rhs_ref, 100)
+    x = torch.zeros_like(rhs)
+    its, hist = b.solve(x, rhs, tol=0.0, maxit=100)
+    assert its == it_ref == 100
+    # sin sin sin is the operator's lowest eigenfunction to discretisation
+    # accuracy, so the residual falls fast and then sits at rounding level,
+    # where the history depends on the summation order: per iteration to
+    # 1e-9 above 1e-3 ||r0||, and the whole history to 1e-6 ||r0||
+    live = hist_ref > 1e-3 * hist_ref[0]
+    assert live.sum() >= 3
+    np.testing.assert_allclose(hist[live], hist_ref[live], rtol=CG_RTOL)
+    np.testing.assert_allclose(hist, hist_ref, rtol=0, atol=1e-6 * hist_ref[0])
+
+
+# -------------------------------------------------------------- config 2
+@pytest.fixture(scope="module")
+def bp5_24_p7(plat):
+    m, P = pair(5, 7, 24)
+    b = capi.BP(plat, m, 5, lam=1.0)
+    return m, P, b
+
+
+def test_config2_bk5_ax_24cube_p7(bp5_24_p7):
+    m, P, b = bp5_24_p7
+    assert P.Ngather == 167**3
+    q = capi.random_normal(2, P.Ngather)
+    AL = torch.full((P.NLocal,), float("nan"), dtype=torch.float64, device="cuda")
+    b.ax_local(dev(q), AL)
+    assert rel_err(host(AL), P.ax_local(q)) <= AX_RTOL
+    A = torch.full((P.Ngather,), float("nan"), dtype=torch.float64, device="cuda")
+    b.operator(dev(q), A)
+    assert rel_err(host(A), P.operator(q)) <= AX_RTOL
+
+
+def test_config2_cg_history_24cube_p7(bp5_24_p7):
+    """100 CG iterations from x = 0 with a seeded N(0,1) right-hand side."""
+    m, P, b = bp5_24_p7
+    rhs = capi.random_normal(7, P.Ngather)
+    x_ref, it_ref, hist_ref, _ = P.cg(rhs, 100)
+    x = torch.zeros(P.Ngather, dtype=torch.float64, device="cuda")
+    its, hist = b.solve(x, dev(rhs), tol=0.0, maxit=100)
+    assert its == it_ref == 100
+    np.testing.assert_allclose(hist, hist_ref, rtol=CG_RTOL)
+    assert rel_err(host(x), x_ref) <= 1e-8
+
+
+@pytest.mark.parametrize("bp", [1, 2, 3, 4, 6])
+def test_config2_other_bps_24cube_p7(plat, bp):
+    m, P = pair(bp, 7, 24)
+    b = capi.BP(plat, m, bp, lam=1.0)
+    assert b.Ngather == P.Ngather
+    q = capi.random_normal(bp, P.Ngather)
+    A = torch.full((P.Ngather,), float("nan"), dtype=torch.float64, device="cuda")
+    b.operator(dev(q), A)
+    assert rel_err(host(A), P.operator(q)) <= AX_RTOL
+
+
+def test_config2_bk5_and_cg_24cube_p15(plat):
+    """p = 15 at 24^3 (46.3M unknowns, 2.7 GB of geometric factors): the
+    matrix-core Ax and operator against the oracle, then 20 CG iterations
+    (100 at 2^3 in test_gpu_parity) to bound the oracle's host time."""
+    m, P = pair(5, 15, 24)
+    b = capi.BP(plat, m, 5, lam=1.0)
+    assert P.Ngather == 359**3
+    q = capi.random_normal(2, P.Ngather)
+    A = torch.full((P.Ngather,), float("nan"), dtype=torch.float64, device="cuda")
+    b.operator(dev(q), A)
+    assert rel_err(host(A), P.operator(q)) <= AX_RTOL
+    del A
+    rhs = capi.random_normal(7, P.Ngather)
+    x_ref, it_ref, hist_ref, _ = P.cg(rhs, 20)
+    x = torch.zeros(P.Ngather, dtype=torch.float64, device="cuda")
+    its, hist = b.solve(x, dev(rhs), tol=0.0, maxit=20)
+    assert its == it_ref == 20
+    np.testing.assert_allclose(hist, hist_ref, rtol=CG_RTOL)
+    assert rel_err(host(x), x_ref) <= 1e-8
+
+
+# -------------------------------------------------------------- config 3
+@pytest.mark.parametrize("bp", [5, 3])
+def test_config3_mixed_24cube_p7(plat, bp):
+    """FP32 CG with FP64 defect correction to 1e-10 at 24^3, p=7. The FP64
+    reference solution is the FP64 CG solved to 1e-12, accepted as a
+    solution of the oracle's system once the oracle's own operator gives it
+    a relative residual below 2e-12; the mixed solution must have an oracle
+    residual below 1e-10 and lie within 1e-7 of it."""
+    m, P = pair(bp, 7, 24)
+    b = capi.BP(plat, m, bp, lam=1.0)
+    rhs = capi.random_normal(9, P.Ngather)
+    bd = dev(rhs)
+    x = torch.zeros(P.Ngather, dtype=torch.float64, device="cuda")
+    st = b.solve_mixed(x, bd, tol=1e-10, interval=64)
+    assert st["rel_res"] < 1e-10
+    xm = host(x)
+    assert oracle_true_residual(P, xm, rhs) < 1e-10
+    x64 = torch.zeros_like(x)
+    its, _ = b.solve(x64, bd.clone(), tol=1e-12, maxit=20000, history=False)
+    assert its < 20000
+    xr = host(x64)
+    assert oracle_true_residual(P, xr, rhs) < 2e-12
+    assert np.linalg.norm(xm - xr) / np.linalg.norm(xr) < 1e-7
+    # the FP32 recurrence keeps close to the FP64 iteration count
+    assert st["iterations"] <= 1.3 * its + 64
+
+
+# ------------------------------------------------------------- config 4a
+@pytest.mark.parametrize("K", [16, 32])
+def test_config4a_krhs_20cube_p7(plat, K):
+    """K right-hand sides at 20^3, p=7: each equals the single-vector
+    operator to 1e-15 (same arithmetic), and two against the oracle."""
+    m, P = pair(5, 7, 20)
+    b = capi.BP(plat, m, 5, lam=1.0)
+    assert b.Ngather == 139**3
+    qk = np.random.default_rng(K).standard_normal((K, b.Ngather))
+    Aq = torch.zeros(K * b.Ngather, dtype=torch.float64, device="cuda")
+    b.operator_krhs(dev(qk.ravel()), Aq, K)
+    got = host(Aq).reshape(K, b.Ngather)
+    del Aq
+    one = torch.zeros(b.Ngather, dtype=torch.float64, device="cuda")
+    for r in range(K):
+        b.operator(dev(qk[r]), one)
+        assert rel_err(got[r], host(one)) <= 1e-15
+    for r in (0, K - 1):
+        assert rel_err(got[r], P.operator(qk[r])) <= AX_RTOL
+
+
+# ------------------------------------------------ BP1-BP4 at p = 9 ... 15
+@pytest.mark.parametrize("affine", [False, True])
+@pytest.mark.parametrize("bp,N", [(bp, N) for bp in (1, 2, 3, 4) for N in range(9, 16)])
+def test_cubature_bps_high_degree(plat, bp, N, affine):
+    nz = 1 if bp in (2, 4) else 2
+    m, P = pair(bp, N, 2, 2, nz, affine=affine, seed=4, lam=1.3)
+    b = capi.BP(plat, m, bp, lam=1.3)
+    q = np.random.default_rng(bp * 100 + N).standard_normal(P.Ngather)
+    A = torch.zeros(P.Ngather, dtype=torch.float64, device="cuda")
+    b.operator(dev(q), A)
+    assert rel_err(host(A), P.operator(q)) <= AX_RTOL
