@@ -56,6 +56,29 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_bp5_ax_fp64_p7_24cube_hbm_traff
 TRAFFIC_FILE_F32 = os.path.join(ROOT, "profiles", "k_bp5_ax_fp32_cg_operator_p7_24cube_hbm_traffic.txt")
 
 
+LIMITER_FILE_F32 = os.path.join(ROOT, "profiles", "k_bp5_ax_fp32_cg_operator_p7_24cube_sq_counters.txt")
+
+
+def kernel_limiter(path):
+    """What bounds the kernel, from the committed SQ counter summary
+    (lines 'limiter_<key>: <value>'), or None."""
+    out = {}
+    try:
+        for line in open(path):
+            if line.startswith("limiter_"):
+                k, v = line[len("limiter_"):].split(":", 1)
+                v = v.strip()
+                try:
+                    out[k.strip()] = float(v)
+                except ValueError:
+                    out[k.strip()] = v
+    except OSError:
+        return None
+    if out:
+        out["source"] = "committed counter summary " + os.path.relpath(path, ROOT)
+    return out or None
+
+
 def measured_traffic(path=TRAFFIC_FILE):
     """HBM bytes per Ax launch from the committed counter summary (FETCH_SIZE and
     WRITE_SIZE scaled by a plain copy of known size), or None."""
@@ -80,6 +103,28 @@ def bk5_tri_bytes(Ndofs, Nel, Np):
     # trilinear map: bk5_bytes with 4-byte values and the 7 factors per node
     # replaced by the map's 24 coefficients per element
     return Ndofs * 4 + Nel * (24 * 4 + 4 + Np * 4 + Np * 4)
+
+
+def bp5_ax_flops(Nq, Nel, affine=False, nfields=1):
+    # BP/BP5/src/bp5Run.cpp:159-180 (hex): (12 Nq^4 + 18 Nq^3) per element,
+    # (12 Nq^4 + 21 Nq^3) affine
+    return (12 * Nq**4 + (21 if affine else 18) * Nq**3) * nfields * Nel
+
+
+def gather_bytes(Ndofs, NLocal):
+    # BP/BP5/src/bp5Run.cpp:136-139
+    return (Ndofs + 1) * 4 + NLocal * 4 + NLocal * 8 + Ndofs * 8
+
+
+def cg_solve_model(Ndofs, NLocal, ax_bytes, ax_flops, iterations):
+    """Bytes and flops of a CG solve of `iterations` iterations, the
+    reference's model (BP/BP5/src/bp5Run.cpp:141-142, 185-188): a first
+    iteration of 4 vector passes and 5 Ndofs flops, then per iteration 11
+    vector passes, the Ax and the gather."""
+    gb = gather_bytes(Ndofs, NLocal)
+    nbytes = (4 * Ndofs * 8 + ax_bytes + gb) + (11 * Ndofs * 8 + ax_bytes + gb) * iterations
+    nflops = (5 * Ndofs + ax_flops + NLocal) + (11 * Ndofs + ax_flops + NLocal) * iterations
+    return nbytes, nflops
 
 
 def krhs_bytes(K, Ndofs, Nel, Np):
@@ -327,6 +372,85 @@ def bench_adjoint(capi, plat):
             "Ndofs": bp.Ngather, "ms_forward_adjoint_gradient": t * 1e3, "gradient_kernel_us": tg * 1e6}
 
 
+def reference_protocol(bp, b, Ndofs, NLocal, Nel, Nq):
+    """bp5_t::Run's timed protocol (BP/BP5/src/bp5Run.cpp:66-95): a
+    1000-iteration warm-up solve with tol = 0, then a timed 100-iteration
+    solve from x = 0, reported with the reference's byte and flop models
+    (:106-188). The timing is host wall time around the solve with the
+    device synchronised on both sides (GlobalPlatformTime)."""
+    import torch
+
+    x = torch.zeros_like(b)
+    r = b.clone()
+    bp.solve(x, r, tol=0.0, maxit=1000, history=False)
+    x.zero_()
+    r.copy_(b)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    its, _ = bp.solve(x, r, tol=0.0, maxit=100, history=False)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    Np = Nq**3
+    nbytes, nflops = cg_solve_model(Ndofs, NLocal, bk5_bytes(Ndofs, Nel, Np), bp5_ax_flops(Nq, Nel), its)
+    return {"protocol": "1000-iteration warm-up, then 100 timed iterations from x = 0 (tol 0)",
+            "iterations": its, "elapsed_s": t, "time_per_dof_s": t / Ndofs, "gbs": nbytes / t / 1e9,
+            "gflops": nflops / t / 1e9, "gdof_it_per_s": Ndofs * its / t / 1e9}
+
+
+def cpu_baselines(args, b_host, Ndofs):
+    """The oracle (the C++/OpenMP restatement of the reference's OKL loop
+    nests, OCCA Serial/OpenMP mode: @outer -> omp parallel for) on the
+    host: the benchmark's own problem (24^3, p=7) with a bounded number of
+    FP64 CG iterations on all physical cores and on one, and the reference's
+    sample problem (10^3, p=4) with the reference's protocol, both
+    variants."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+
+    phys = physical_cores()
+    omp = min(phys, int(os.environ.get("OMP_NUM_THREADS", "1000")))
+    out = {"physical_cores": phys, "openmp_threads": omp}
+
+    P = oracle_lib.Problem(bp=5, N=args.N, nx=args.nx, lam=1.0, perturb=0.1, seed=1)
+    assert P.Ngather == Ndofs
+    runs = {}
+    for name, threads, its in (("openmp", omp, args.cpu_its), ("serial", 1, args.cpu_serial_its)):
+        got = oracle_lib.set_threads(threads)
+        tc = time.perf_counter()
+        P.cg(b_host, its)
+        tcpu = time.perf_counter() - tc
+        runs[name] = {"value": Ndofs * its / tcpu / 1e9, "unit": "GDOF·it/s", "threads": got,
+                      "iterations": its, "seconds": tcpu}
+        log(f"cpu baseline ({name}, {got} threads): {runs[name]['value']:.4f} GDOF*it/s, {its} its in {tcpu:.1f}s")
+    del P
+    out["problem_24cube_p7"] = runs
+
+    # the reference's sample problem (README.md:109-115): BP5, 10^3, p=4
+    rows = []
+    for variant, kw in (("reference semantics: [-1,1]^3, lambda 1, forcing rhs", dict(lam=1.0)),
+                        ("literal: [0,1]^3, lambda 0, b = M f",
+                         dict(lam=0.0, lo=(0.0, 0.0, 0.0), dim=(1.0, 1.0, 1.0)))):
+        Pc = oracle_lib.Problem(bp=5, N=4, nx=10, **kw)
+        rhs = Pc.rhs_forcing()
+        Nd, NL, Nel = Pc.Ngather, Pc.NLocal, Pc.Nelements
+        for name, threads in (("openmp", omp), ("serial", 1)):
+            got = oracle_lib.set_threads(threads)
+            Pc.cg(rhs, 1000)
+            tc = time.perf_counter()
+            _, its, _, _ = Pc.cg(rhs, 100)
+            tcpu = time.perf_counter() - tc
+            nbytes, nflops = cg_solve_model(Nd, NL, bk5_bytes(Nd, Nel, 125), bp5_ax_flops(5, Nel), its)
+            rows.append({"problem": "BP5 10^3 hex p=4 (" + variant + ")", "Ndofs": Nd, "mode": name,
+                         "threads": got, "protocol": "1000-iteration warm-up, 100 timed iterations",
+                         "elapsed_s": tcpu, "gdof_it_per_s": Nd * its / tcpu / 1e9,
+                         "gbs": nbytes / tcpu / 1e9, "gflops": nflops / tcpu / 1e9})
+            log(f"cpu config 1 ({variant}, {name}): {rows[-1]['elapsed_s']:.4f}s for 100 its")
+        del Pc
+    out["problem_10cube_p4"] = rows
+    oracle_lib.set_threads(omp)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -343,6 +467,7 @@ def main():
                     help="FP32 iterations between FP64 defect corrections")
     ap.add_argument("--ax-launches", type=int, default=50)
     ap.add_argument("--cpu-its", type=int, default=1000)
+    ap.add_argument("--cpu-serial-its", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extensions", action="store_true",
                     help="skip the k-RHS, batch and adjoint measurements")
@@ -446,6 +571,37 @@ def main():
     relmix = float(torch.linalg.norm(b - Ax) / torch.linalg.norm(b))
     log(f"mixed: {st['iterations']} FP32 its, {st['updates']} FP64 corrections, {tmix * 1e3:.1f} ms/solve, "
         f"true rel res {relmix:.2e}")
+    x_mixed = x.clone()
+
+    # ---- the same mixed solve with the FP32 operator streaming the stored
+    # geometric factors rounded to FP32 (the reference kernel with dfloat =
+    # float), timed the same way; the default forms them from the elements'
+    # trilinear maps
+    bp.set_fp32_geometry(1)
+    for _ in range(args.warmup):
+        mixed_solve()
+    tmix_s, st_s = timed(mixed_solve, args.steps)
+    bp.operator(x, Ax)
+    relmix_s = float(torch.linalg.norm(b - Ax) / torch.linalg.norm(b))
+    x.zero_()
+    ins_s = bp.time_operator_in_solve(x, b, tol=args.tol, interval=args.interval)
+    stored32 = bk5_bytes(Ndofs, Nel, Np, 4)
+    mixed_stored = {"fp32_geometry": "stored factors rounded to FP32", "fp32_iterations": st_s["iterations"],
+                    "fp64_corrections": st_s["updates"], "ms_per_solve": tmix_s * 1e3, "true_rel_res": relmix_s,
+                    "fp64_equivalent_gdof_it_per_s": Ndofs * its64 / tmix_s / 1e9,
+                    "speedup_vs_fp64": t64 / tmix_s,
+                    "ax_us_in_solve": ins_s["ax_us"], "ax_algorithmic_bytes": stored32,
+                    "ax_gbs": stored32 / (ins_s["ax_us"] * 1e-6) / 1e9,
+                    "ax_hbm_frac": stored32 / (ins_s["ax_us"] * 1e-6) / 1e9 / HBM_PEAK_GBS}
+    log(f"mixed, stored FP32 factors: {st_s['iterations']} FP32 its, {tmix_s * 1e3:.1f} ms/solve, "
+        f"Ax {ins_s['ax_us']:.1f} us in the solve")
+    bp.set_fp32_geometry(0)
+    mixed_solve()  # rebuild the default FP32 operator before it is timed below
+
+    # ---- the reference's own timed protocol on the FP64 CG (100 iterations)
+    refp = reference_protocol(bp, b, Ndofs, bp.NLocal, Nel, args.N + 1)
+    log(f"reference protocol (FP64 CG, 100 its): {refp['elapsed_s'] * 1e3:.1f} ms, {refp['gbs']:.0f} GB/s, "
+        f"{refp['gflops']:.0f} GFLOP/s, {refp['gdof_it_per_s']:.2f} GDOF*it/s")
 
     # ---- roofline: the Ax of the FP32 CG operator, the dominant kernel of the
     # timed solve, timed inside one more solve identical to the timed ones
@@ -456,7 +612,6 @@ def main():
     ins = bp.time_operator_in_solve(x, b, tol=args.tol, interval=args.interval)
     op32 = bp.time_operator(32, args.ax_launches)
     nbytes32 = bk5_tri_bytes(Ndofs, Nel, Np)
-    stored32 = bk5_bytes(Ndofs, Nel, Np, 4)  # the same operator with stored factors
     achieved32 = nbytes32 / (ins["ax_us"] * 1e-6) / 1e9
     log(f"FP32 CG operator in the solve ({ins['launches']} applications): Ax {ins['ax_us']:.1f} us "
         f"({achieved32:.0f} GB/s), gather {ins['gather_us']:.1f} us, fused {ins['fused']}; back to back: "
@@ -495,21 +650,14 @@ def main():
     # ---- CPU baseline: the oracle's FP64 CG on the host cores, bounded sample
     cpu = None
     if not args.no_cpu_baseline:
-        sys.path.insert(0, os.path.join(ROOT, "tests"))
-        import oracle_lib
-
-        cores = min(physical_cores(), int(os.environ.get("OMP_NUM_THREADS", "1000")))
-        threads = oracle_lib.set_threads(cores)
-        P = oracle_lib.Problem(bp=5, N=args.N, nx=args.nx, lam=1.0, perturb=0.1, seed=1)
-        assert P.Ngather == Ndofs
-        tc = time.perf_counter()
-        P.cg(b_host, args.cpu_its)
-        tcpu = time.perf_counter() - tc
-        cpu = {"value": Ndofs * args.cpu_its / tcpu / 1e9, "unit": "GDOF·it/s", "cores": threads,
-               "kind": "port",
-               "sample": f"oracle FP64 CG (restated OKL loop nests, OpenMP), same 24^3 p=7 BP5 problem, "
-                         f"{args.cpu_its} iterations in {tcpu:.1f}s"}
-        log(f"cpu baseline: {cpu['value']:.4f} GDOF*it/s on {threads} threads")
+        cb = cpu_baselines(args, b_host, Ndofs)
+        omp_run = cb["problem_24cube_p7"]["openmp"]
+        cpu = {"value": omp_run["value"], "unit": "GDOF·it/s", "cores": omp_run["threads"], "kind": "port",
+               "sample": f"oracle FP64 CG (restated OKL loop nests, OpenMP over elements), same {args.nx}^3 "
+                         f"p={args.N} BP5 problem and right-hand side, {omp_run['iterations']} iterations in "
+                         f"{omp_run['seconds']:.1f}s",
+               "physical_cores": cb["physical_cores"], "serial": cb["problem_24cube_p7"]["serial"],
+               "config1_10cube_p4": cb["problem_10cube_p4"]}
 
     ext = None
     if not args.no_extensions:
@@ -543,13 +691,23 @@ def main():
             "N": args.N, "nx": args.nx, "Ndofs": Ndofs, "tol": args.tol, "interval": args.interval,
         },
         "fp64_cg": {"value": Ndofs * its64 / t64 / 1e9, "unit": "GDOF·it/s", "iterations": its64,
-                    "ms_per_solve": t64 * 1e3, "true_rel_res": rel64},
+                    "ms_per_solve": t64 * 1e3, "true_rel_res": rel64,
+                    "gflops": cg_solve_model(Ndofs, bp.NLocal, nbytes, bp5_ax_flops(args.N + 1, Nel), its64)[1]
+                    / t64 / 1e9,
+                    "gbs": cg_solve_model(Ndofs, bp.NLocal, nbytes, bp5_ax_flops(args.N + 1, Nel), its64)[0]
+                    / t64 / 1e9,
+                    "reference_protocol": refp},
+        "mixed_stored_factors": mixed_stored,
         "mixed": {"fp32_iterations": st["iterations"], "fp64_corrections": st["updates"],
                   "ms_per_solve": tmix * 1e3, "true_rel_res": relmix,
                   "own_rate": Ndofs * st["iterations"] / tmix / 1e9,
                   "speedup_vs_fp64": t64 / tmix},
         "roofline": {"bound": "hbm", "achieved": achieved32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved32 / HBM_PEAK_GBS, "traffic": measured_traffic(TRAFFIC_FILE_F32),
+                     "traffic_source": "committed counter summary " + os.path.relpath(TRAFFIC_FILE_F32, ROOT) +
+                                       " (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a 512 MiB copy); "
+                                       "not measured in this run",
+                     "limiter": kernel_limiter(LIMITER_FILE_F32),
                      "kernel": ("k_bp5_ax FP32 CG operator, " +
                                 ("fused (interior rows + surface run)" if ins["fused"] else "plain") +
                                 ", geometric factors from the trilinear map, non-affine, Nq=8"),
@@ -557,12 +715,12 @@ def main():
                      "timed_in": "one more mixed solve identical to the timed ones",
                      "algorithmic_bytes": nbytes32, "gather_us_per_launch": ins["gather_us"],
                      "us_per_launch_back_to_back": op32["ax_us"], "device_copy_gbs": copy_gbs,
-                     "stored_geometry_algorithmic_bytes": stored32,
-                     "stored_geometry_equivalent_gbs": stored32 / (ins["ax_us"] * 1e-6) / 1e9},
+                     "gflops": bp5_ax_flops(args.N + 1, Nel) / (ins["ax_us"] * 1e-6) / 1e9},
         "bk5_fp64": {"achieved": achieved, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(), "kernel": "k_bp5_ax (FP64, non-affine, Nq=8)",
                      "us_per_launch": t_ax * 1e6, "algorithmic_bytes": nbytes,
-                     "gdof_per_s": Ndofs / t_ax / 1e9},
+                     "gdof_per_s": Ndofs / t_ax / 1e9,
+                     "gflops": bp5_ax_flops(args.N + 1, Nel) / t_ax / 1e9},
         "tuning": bp.tuning(),
         "cpu_baseline": cpu,
         "extensions": ext,

@@ -47,6 +47,21 @@ def smooth_lattice_field(shape, seed, passes=3, std=0.5):
     return g * (std / g.std())
 
 
+def _solve_relative(bp, rhs, tol, maxit):
+    """x = A^{-1} rhs by bp's CG with the right-hand side scaled to unit
+    norm, so that tol is the relative residual: the CG's stopping test,
+    ||r|| <= max(tol ||r0||, tol) (libs/core/linearSolverCG.cpp:84-88), has
+    an absolute floor that a small right-hand side would hit first.
+    Returns (x, iterations)."""
+    nrm = float(torch.linalg.vector_norm(rhs))
+    x = torch.zeros_like(rhs)
+    if nrm == 0.0:
+        return x, 0
+    r = rhs / nrm
+    its, _ = bp.solve(x, r, tol=tol, maxit=maxit, history=False)
+    return x.mul_(nrm), its
+
+
 class KappaSolve(torch.autograd.Function):
     """u = A(kappa)^{-1} b on a BP5 capi.BP; differentiable in kappa and b."""
 
@@ -56,9 +71,7 @@ class KappaSolve(torch.autograd.Function):
             raise TypeError("KappaSolve works in float64")
         kappa = kappa.contiguous()
         bp.set_kappa(kappa)
-        u = torch.zeros_like(b)
-        r = b.detach().clone()
-        its, _ = bp.solve(u, r, tol=tol, maxit=maxit, history=False)
+        u, its = _solve_relative(bp, b.detach(), tol, maxit)
         if its >= maxit:
             raise RuntimeError(f"KappaSolve: CG did not reach {tol:g} in {maxit} iterations")
         ctx.bp, ctx.tol, ctx.maxit = bp, tol, maxit
@@ -71,9 +84,7 @@ class KappaSolve(torch.autograd.Function):
         kappa, u = ctx.saved_tensors
         bp = ctx.bp
         bp.set_kappa(kappa)  # the operator may have been reset since forward
-        lam = torch.zeros_like(u)
-        r = grad_u.detach().contiguous().clone()
-        its, _ = bp.solve(lam, r, tol=ctx.tol, maxit=ctx.maxit, history=False)
+        lam, its = _solve_relative(bp, grad_u.detach().contiguous(), ctx.tol, ctx.maxit)
         if its >= ctx.maxit:
             raise RuntimeError(f"KappaSolve: adjoint CG did not reach {ctx.tol:g} in {ctx.maxit} iterations")
         grad_kappa = torch.empty_like(kappa)

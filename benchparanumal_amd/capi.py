@@ -64,6 +64,10 @@ _SIGS = {
     "bpx_ogs_array": (_I, [_P, C.c_char_p, _P, _I64P, C.POINTER(_I)]),
     "bpx_ogs_gather_dev": (_I, [_P, _P, _I, _P, _P]),
     "bpx_ogs_scatter_dev": (_I, [_P, _P, _I, _P, _P]),
+    "bpx_ogs_gather_scatter_dev": (_I, [_P, _P, _I, _P]),
+    "bpx_cg_create": (_I, [_P, _I, _I, C.POINTER(_P)]),
+    "bpx_cg_destroy": (_I, [_P]),
+    "bpx_cg_solve": (_I, [_P, _P, _P, _P, _P, _D, _I, _I, C.POINTER(_I), _P]),
     "bpx_bp_create": (_I, [_P, _P, _I, _D, C.POINTER(_P)]),
     "bpx_bp_destroy": (_I, [_P]),
     "bpx_bp_info": (_I, [_P, _I64P, _I]),
@@ -76,6 +80,7 @@ _SIGS = {
     "bpx_bp_set_tuning": (_I, [_P, _I, _I, _I]),
     "bpx_bp_time_operator": (_I, [_P, _I, _I, _P]),
     "bpx_bp_set_ax_engine": (_I, [_P, _I]),
+    "bpx_bp_set_fp32_geometry": (_I, [_P, _I]),
     "bpx_bp_time_operator_in_solve": (_I, [_P, _P, _P, _D, _I, _I, _P]),
     "bpx_bp_set_kappa": (_I, [_P, _P]),
     "bpx_batch_create": (_I, [_P, _I, _P, _D, C.POINTER(_P)]),
@@ -86,6 +91,9 @@ _SIGS = {
     "bpx_batch_solve": (_I, [_P, _P, _P, _D, _I, _P, _P]),
     "bpx_bp_kappa_gradient": (_I, [_P, _P, _P, _P]),
     "bpx_bp_rhs_forcing": (_I, [_P, _P]),
+    "bpx_mesh_plot_fields": (_I, [_P, _I, _P, C.c_char_p]),
+    "bpx_bp_plot_solution": (_I, [_P, _P, C.c_char_p]),
+    "bpx_bp_error_norms": (_I, [_P, _P, _P]),
     "bpx_bp_solve": (_I, [_P, _P, _P, _D, _I, _I, C.POINTER(_I), _P]),
     "bpx_bp_solve_mixed": (_I, [_P, _P, _P, _D, _I, _I, _P]),
 }
@@ -216,6 +224,11 @@ class Mesh:
     def array(self, name):
         return _read_array(lib().bpx_mesh_array, self.h, name)
 
+    def plot_fields(self, QL, nfields, filename):
+        """VTU file of host data QL ([element][field][node]) at the local nodes."""
+        QL = np.ascontiguousarray(QL, dtype=np.float64)
+        check(lib().bpx_mesh_plot_fields(self.h, int(nfields), QL.ctypes.data, os.fsencode(filename)))
+
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
             _lib.bpx_mesh_destroy(self.h)
@@ -245,6 +258,10 @@ class Ogs:
 
     def scatter(self, platform, gv, v, K=1):
         check(lib().bpx_ogs_scatter_dev(platform.h, self.h, K, ptr(gv), ptr(v)))
+
+    def gather_scatter(self, platform, v, K=1):
+        """In place: every copy of a node gets the sum over its copies."""
+        check(lib().bpx_ogs_gather_scatter_dev(platform.h, self.h, K, ptr(v)))
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
@@ -285,11 +302,17 @@ class BP:
 
     def tuning(self):
         """Launch choices timed at setup (identical results whichever is chosen)."""
-        out = np.zeros(9)
-        lib().bpx_bp_tuning(self.h, ptr(out), 9)
+        out = np.zeros(10)
+        lib().bpx_bp_tuning(self.h, ptr(out), 10)
         f = [float(v) for v in out]
         return {"gather_mode": int(f[0]), "gather_us": f[1:3], "fused": int(f[3]), "fused_us": f[4],
-                "unfused_us": f[5], "fused32": int(f[6]), "fused32_us": f[7], "unfused32_us": f[8]}
+                "unfused_us": f[5], "fused32": int(f[6]), "fused32_us": f[7], "unfused32_us": f[8],
+                "fp32_trilinear": int(f[9])}
+
+    def set_fp32_geometry(self, mode):
+        """FP32 BP5 geometry: 0 = from the elements' trilinear maps when the
+        mesh's nodes are those maps (default), 1 = stored factors rounded to FP32."""
+        check(lib().bpx_bp_set_fp32_geometry(self.h, int(mode)))
 
     def set_tuning(self, gather_mode=-1, fused=-1, fused32=-1):
         """Pin launch choices (-1 keeps the timed one)."""
@@ -327,6 +350,17 @@ class BP:
 
     def rhs_forcing(self, b):
         check(lib().bpx_bp_rhs_forcing(self.h, ptr(b)))
+
+    def plot_solution(self, u, filename):
+        """VTU file of the solution u (device, Ngather) at the local nodes."""
+        check(lib().bpx_bp_plot_solution(self.h, ptr(u), os.fsencode(filename)))
+
+    def error_norms(self, u):
+        """u (device, Ngather) against the forcing's manufactured solution:
+        {"max", "l2", "rel_l2"}."""
+        out = np.zeros(3)
+        check(lib().bpx_bp_error_norms(self.h, ptr(u), ptr(out)))
+        return {"max": float(out[0]), "l2": float(out[1]), "rel_l2": float(out[2])}
 
     def solve(self, x, r, tol=0.0, maxit=100, verbose=False, history=True):
         its = _I()

@@ -190,6 +190,11 @@ int launch_batch_gather(hipStream_t s, int nsupers, const BatchSuper* supers, co
                         const BatchState* st);
 int launch_batch_count_done(hipStream_t s, int nprob, const BatchState* st, int* count);
 
+// per-workgroup partials (max |diff|, sum wJ diff^2, sum wJ (c u)^2) of a
+// one-field solution u (read through G2L) against c sin(pi x) sin(pi y)
+// sin(pi z) at the local nodes; part holds 3 * kRedBlocks doubles
+int launch_error_norms(hipStream_t s, long long NLocal, const int* G2L, const double* u, const double* x,
+                       const double* y, const double* z, const double* wJ, double c, double* part);
 int launch_rhs(hipStream_t s, int N, int Nfields, int affine, int Nelements, const double* wJ,
                const double* gllw, const double* x, const double* y, const double* z,
                double lambda, double* rhs);

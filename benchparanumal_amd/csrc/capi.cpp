@@ -29,6 +29,19 @@ struct bpx_bp_s {
   deviceBuffer<double> dotPart;  // partials of bpx_bp_operator_dot
   ~bpx_bp_s() { delete solver; }
 };
+// solver_t whose Operator is a caller's callback (bpx_cg_solve)
+struct callback_solver_t : public solver_t {
+  bpx_operator_fn fn = nullptr;
+  void* ctx = nullptr;
+  int Operator(const dfloat* o_q, dfloat* o_Aq) override {
+    const int st = fn(ctx, o_q, o_Aq);
+    return st == 0 ? BPX_OK : set_error(st < 0 ? st : BPX_ERR_INVALID_ARGUMENT, "cg: the operator callback failed");
+  }
+};
+struct bpx_cg_s {
+  cg* solver = nullptr;
+  ~bpx_cg_s() { delete solver; }
+};
 struct bpx_batch_s {
   bpx_platform_s* plat = nullptr;
   bp_batch_t b;
@@ -381,6 +394,48 @@ int bpx_cg_update(bpx_platform p, int N, const double* Ap, const double* o_p, do
   return BPX_OK;
 }
 
+// ------------------------------------------------------------------ cg --
+int bpx_cg_create(bpx_platform p, int N, int Nhalo, bpx_cg* out) {
+  BPX_GUARD_BEGIN
+  if (int st = need(p, "platform")) return st;
+  if (int st = need(out, "out")) return st;
+  if (N < 0 || Nhalo < 0) return set_error(BPX_ERR_INVALID_ARGUMENT, "cg: N and Nhalo must be >= 0");
+  bpx_cg_s* c = new bpx_cg_s;
+  c->solver = new cg(*p->p, N, Nhalo);
+  if (int st = c->solver->status()) {
+    delete c;
+    return st;
+  }
+  *out = c;
+  return BPX_OK;
+  BPX_GUARD_END
+}
+
+int bpx_cg_destroy(bpx_cg c) {
+  delete c;
+  return BPX_OK;
+}
+
+int bpx_cg_solve(bpx_cg c, bpx_operator_fn op, void* ctx, double* o_x, double* o_r, double tol, int maxit,
+                 int verbose, int* iterations, double* rnorm_hist) {
+  BPX_GUARD_BEGIN
+  if (int st = need(c, "cg")) return st;
+  if (!op) return set_error(BPX_ERR_INVALID_ARGUMENT, "cg: operator callback is NULL");
+  callback_solver_t s;
+  s.fn = op;
+  s.ctx = ctx;
+  int its = 0;
+  const int st = c->solver->Solve(s, o_x, o_r, tol, maxit, verbose, &its);
+  if (iterations) *iterations = its;
+  if (rnorm_hist && st == BPX_OK) {
+    std::vector<double> rr, al;
+    if (int s2 = c->solver->impl.History(rr, al)) return s2;
+    for (int k = 0; k <= maxit; ++k) rnorm_hist[k] = (k < (int)rr.size()) ? std::sqrt(rr[k]) : 0.0;
+  }
+  return st;
+  BPX_GUARD_END
+}
+
 // ---------------------------------------------------------------- mesh --
 int bpx_mesh_create(const bpx_mesh_opts* o, bpx_mesh* out) {
   BPX_GUARD_BEGIN
@@ -506,6 +561,14 @@ int bpx_ogs_scatter_dev(bpx_platform p, bpx_ogs o, int K, const double* gv, doub
   return o->o.Scatter<double>(*p->p, v, gv, K);
 }
 
+int bpx_ogs_gather_scatter_dev(bpx_platform p, bpx_ogs o, int K, double* v) {
+  if (int st = need(p, "platform")) return st;
+  if (int st = need(o, "ogs")) return st;
+  if (!o->o.o_rowStartsT.ptr() && o->o.Ngather > 0)
+    if (int st = o->o.Upload(*p->p)) return st;
+  return o->o.GatherScatter(*p->p, v, K);
+}
+
 // ------------------------------------------------------------------ BP --
 int bpx_bp_create(bpx_platform p, bpx_mesh m, int bp, double lambda, bpx_bp* out) {
   BPX_GUARD_BEGIN
@@ -548,8 +611,9 @@ int bpx_bp_tuning(bpx_bp b, double* out, int capacity) {
   const auto& t = b->bp.tuning;
   const double v[] = {(double)t.gatherMode, t.gather_us[0], t.gather_us[1],
                       b->bp.fused ? (double)t.fused64 : -1.0, t.fused_us, t.unfused_us,
-                      b->bp.fused ? (double)t.fused32 : -1.0, t.fused32_us, t.unfused32_us};
-  const int n = 9;
+                      b->bp.fused ? (double)t.fused32 : -1.0, t.fused32_us, t.unfused32_us,
+                      b->bp.have32 ? (double)b->bp.tri32 : -1.0};
+  const int n = 10;
   for (int i = 0; i < n && i < capacity; ++i) out[i] = v[i];
   return n;
 }
@@ -575,6 +639,13 @@ int bpx_bp_set_tuning(bpx_bp b, int gatherMode, int fused, int fused32) {
     t.fused32 = fused32;
   }
   return BPX_OK;
+  BPX_GUARD_END
+}
+
+int bpx_bp_set_fp32_geometry(bpx_bp b, int mode) {
+  BPX_GUARD_BEGIN
+  if (int st = need(b, "bp")) return st;
+  return b->bp.SetFp32Geometry(mode);
   BPX_GUARD_END
 }
 
@@ -646,6 +717,32 @@ int bpx_bp_kappa_gradient(bpx_bp b, const double* o_u, const double* o_lam, doub
   if (int st = need(b, "bp")) return st;
   if (!o_u || !o_lam || !o_grad) return set_error(BPX_ERR_INVALID_ARGUMENT, "kappa gradient: null pointer");
   return b->bp.KappaGradient(o_u, o_lam, o_grad);
+  BPX_GUARD_END
+}
+
+int bpx_mesh_plot_fields(bpx_mesh m, int Nfields, const double* QL, const char* filename) {
+  BPX_GUARD_BEGIN
+  if (int st = need(m, "mesh")) return st;
+  if (int st = need(filename, "filename")) return st;
+  return m->m.PlotFields(QL, Nfields, filename);
+  BPX_GUARD_END
+}
+
+int bpx_bp_plot_solution(bpx_bp b, const double* o_u, const char* filename) {
+  BPX_GUARD_BEGIN
+  if (int st = need(b, "bp")) return st;
+  if (int st = need(o_u, "o_u")) return st;
+  if (int st = need(filename, "filename")) return st;
+  return b->bp.PlotSolution(o_u, filename);
+  BPX_GUARD_END
+}
+
+int bpx_bp_error_norms(bpx_bp b, const double* o_u, double* out) {
+  BPX_GUARD_BEGIN
+  if (int st = need(b, "bp")) return st;
+  if (int st = need(o_u, "o_u")) return st;
+  if (int st = need(out, "out")) return st;
+  return b->bp.ErrorNorms(o_u, out);
   BPX_GUARD_END
 }
 

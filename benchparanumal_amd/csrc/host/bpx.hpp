@@ -111,6 +111,14 @@ class mesh_t {
   // rt, st, rst of the trilinear map through its 8 vertices, and one pad
   // (24 values): the nodal coordinates are this map at the GLL nodes
   void TrilinearCoefficients(std::vector<double>& c) const;
+  // whether every local node lies on its element's trilinear map of the 8
+  // vertices at the GLL point (to 1e-12 of the box size): then the
+  // geometric factors are the map's exact derivatives
+  bool NodesAreTrilinear() const;
+  // mesh_t::PlotFields (libs/mesh/meshPlotFields3D.cpp:31-121): VTU file of
+  // the local nodes with Nfields values each (QL: host, [e][f][n]); each
+  // element is drawn as N^3 sub-cubes of 6 tetrahedra
+  int PlotFields(const double* QL, int Nfields, const std::string& fileName) const;
 
   MeshOpts opts;
   int N = 0, Nq = 0, Np = 0, cubNq = 0, cubNp = 0;
@@ -142,6 +150,38 @@ class ogs_t {
   int gatherMode = 0;  // workgroup order of the gather (set by the tuner; same results either way)
   std::vector<dlong> rowStartsT, colIdsT, blockRowStartsT;
   deviceBuffer<dlong> o_rowStartsT, o_colIdsT, o_blockRowStartsT;
+};
+
+// halo_t (include/ogs.hpp:355-379): the exchange of ghost values of q
+// between ranks before the elements on a rank boundary apply Ax
+// (BP/BP5/src/bp5Operator.cpp:31-50). On one GPU every node is owned, so
+// the halo is empty (Nhalo = 0) and each exchange moves nothing, as
+// libs/ogs/ogsHalo.cpp:46-145 posts no messages on a single rank. The calls
+// stay in the operator so the multi-GPU exchange slots in at the same
+// points.
+class halo_t {
+ public:
+  int SetupFromGather(const ogs_t& o) {
+    Nlocal = o.N;
+    Nhalo = 0;
+    NhaloP = 0;
+    return BPX_OK;
+  }
+  template <typename T>
+  int Exchange(T* o_v, int k) const {
+    (void)o_v;
+    (void)k;
+    return BPX_OK;
+  }
+  template <typename T>
+  int ExchangeStart(T* o_v, int k) const {
+    return Exchange(o_v, k);
+  }
+  template <typename T>
+  int ExchangeFinish(T* o_v, int k) const {
+    return Exchange(o_v, k);
+  }
+  dlong Nlocal = 0, Nhalo = 0, NhaloP = 0;
 };
 }  // namespace ogs
 
@@ -295,6 +335,12 @@ class bp_t : public solver_t {
   int AxLocal(const dfloat* o_q, dfloat* o_AqL, const dlong* G2L = nullptr);
   int Operator32(const float* o_q, float* o_Aq);
   int Forcing(dfloat* o_b);
+  // the solution (reference row order) scattered to the local nodes and
+  // written as a VTU file (BP/BP5/src/bp5Run.cpp:220-238)
+  int PlotSolution(const dfloat* o_u, const std::string& fileName);
+  // error against the forcing's manufactured solution (one field):
+  // out = {max |u_h - u|, ||u_h - u||_L2, ||u_h - u||_L2 / ||u||_L2}
+  int ErrorNorms(const dfloat* o_u, double* out);
   // K right-hand sides at once (BP5), one vector after another: o_q and
   // o_Aq hold K vectors of Ngather, o_AqL K of NLocal; chunk as for
   // launch_bp5_ax_krhs (0 = default)
@@ -325,6 +371,7 @@ class bp_t : public solver_t {
   int bp = 5, Nfields = 1;
   dfloat lambda = 1.0;
   ogs::ogs_t ogs;
+  ogs::halo_t gHalo;  // ghost exchange of q (empty on one GPU)
   dlong NLocal = 0;
   std::vector<dlong> GlobalToLocal;
   deviceBuffer<dlong> o_GlobalToLocal, o_elementList;
@@ -351,6 +398,12 @@ class bp_t : public solver_t {
   int Prepare32();
   bool have32 = false;
   int tri32 = 0;  // the FP32 BP5 operator forms its geometric factors from the trilinear maps
+  // FP32 BP5 geometry: 0 = from the elements' trilinear maps when every
+  // node is its element's trilinear map of the vertices (checked on the
+  // host), else stored factors rounded to FP32; 1 = stored factors rounded
+  // to FP32 always. Changing it rebuilds the FP32 operator on next use.
+  int fp32Geometry = 0;
+  int SetFp32Geometry(int mode);
   deviceBuffer<float> o_wJ32, o_ggeo32, o_gllw32, o_D32, o_I32, o_MM32, o_invV32, o_AqL32;
   // nodal coordinates for the forcing (created on first use)
   deviceBuffer<double> o_x, o_y, o_z, o_gllwRhs;

@@ -76,6 +76,36 @@ void mesh_t::TrilinearCoefficients(std::vector<double>& c) const {
     }
 }
 
+bool mesh_t::NodesAreTrilinear() const {
+  std::vector<double> c;
+  TrilinearCoefficients(c);
+  const double scale = std::max(opts.dim[0], std::max(opts.dim[1], opts.dim[2]));
+  const double tol = 1e-12 * scale;
+  bool ok = true;
+#pragma omp parallel for reduction(&& : ok)
+  for (dlong e = 0; e < Nelements; ++e) {
+    const double* ce = &c[(size_t)e * 24];
+    for (int k = 0; k < Nq && ok; ++k)
+      for (int j = 0; j < Nq; ++j)
+        for (int i = 0; i < Nq; ++i) {
+          const double r = gllz[i], s = gllz[j], t = gllz[k];
+          const size_t id = (size_t)e * Np + i + j * Nq + k * Nq * Nq;
+          const double* X[3] = {x.data(), y.data(), z.data()};
+          for (int d = 0; d < 3; ++d) {
+            const double* cd = ce + 8 * d;
+            // the constant term: the mean of the 8 vertices
+            const double* V = (d == 0 ? EX.data() : d == 1 ? EY.data() : EZ.data()) + (size_t)e * 8;
+            double c0 = 0;
+            for (int v = 0; v < 8; ++v) c0 += 0.125 * V[v];
+            const double m = c0 + cd[0] * r + cd[1] * s + cd[2] * t + cd[3] * r * s + cd[4] * r * t +
+                             cd[5] * s * t + cd[6] * r * s * t;
+            if (std::fabs(m - X[d][id]) > tol) ok = false;
+          }
+        }
+  }
+  return ok;
+}
+
 int mesh_t::Setup(const MeshOpts& o) {
   if (o.N < 1 || o.N > 15) return set_error(BPX_ERR_INVALID_ARGUMENT, "mesh: degree N must be in 1..15");
   if (o.nx < 1 || o.ny < 1 || o.nz < 1) return set_error(BPX_ERR_INVALID_ARGUMENT, "mesh: nx, ny, nz must be >= 1");

@@ -244,6 +244,7 @@ int bp_t::Setup(platform_t& p, mesh_t& m, int bp_, dfloat lambda_) {
   GlobalToLocal.resize(NLocal);
   ogs.SetupGlobalToLocalMapping(GlobalToLocal.data());
   if (int st = ogs.Upload(p)) return st;
+  if (int st = gHalo.SetupFromGather(ogs)) return st;
   if (int st = o_GlobalToLocal.upload(GlobalToLocal, s)) return st;
   std::vector<dlong> list(m.Nelements);
   for (dlong e = 0; e < m.Nelements; ++e) list[e] = e;
@@ -358,6 +359,11 @@ int bp_t::KappaGradient(const dfloat* o_u, const dfloat* o_lam, dfloat* o_grad) 
 // bpN_t::Operator (BP/BP5/src/bp5Operator.cpp:29-85). On one GPU the halo
 // exchanges move nothing, so the two element halves become one launch.
 int bp_t::Operator(const dfloat* o_q, dfloat* o_Aq) {
+  // the reference splits Ax into the elements that need no ghost values
+  // and those that do, around the halo exchange (bp5Operator.cpp:31-84);
+  // with no halo one launch covers every element
+  if (int st = gHalo.ExchangeStart(o_q, 1)) return st;
+  if (int st = gHalo.ExchangeFinish(o_q, 1)) return st;
   if (int st = AxLocal(o_q, o_AqL.ptr())) return st;
   return ogs.Gather<double>(*platform, o_Aq, o_AqL.ptr(), 1);
 }
@@ -825,7 +831,8 @@ int bp_t::Prepare32() {
   const mesh_t& m = *mesh;
   const int affine = m.opts.affine;
   int st = BPX_OK;
-  if (bp == 5 && !affine) {
+  tri32 = 0;
+  if (bp == 5 && !affine && fp32Geometry == 0 && m.NodesAreTrilinear()) {
     // the FP32 operator forms its geometric factors from the elements'
     // trilinear maps: 96 B per element instead of 28 B per node
     std::vector<double> tri;
@@ -853,6 +860,16 @@ int bp_t::Prepare32() {
   have32 = true;
   if ((st = Tune32())) return st;
   return platform->finish();
+}
+
+int bp_t::SetFp32Geometry(int mode) {
+  if (mode != 0 && mode != 1) return set_error(BPX_ERR_INVALID_ARGUMENT, "bp: FP32 geometry mode must be 0 or 1");
+  if (mode == fp32Geometry) return BPX_OK;
+  if (int st = platform->finish()) return st;  // the FP32 buffers may still be in use
+  fp32Geometry = mode;
+  have32 = false;  // rebuilt (and its fused form re-timed) on next use
+  tri32 = 0;
+  return BPX_OK;
 }
 
 int bp_t::Operator32(const float* o_q, float* o_Aq) {

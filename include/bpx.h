@@ -50,6 +50,7 @@ typedef struct bpx_mesh_s* bpx_mesh;         /* replaces mesh_t (include/mesh.hp
 typedef struct bpx_ogs_s* bpx_ogs;           /* replaces ogs::ogs_t (include/ogs.hpp:216-341) */
 typedef struct bpx_bp_s* bpx_bp;             /* replaces bpN_t (BP/BP5/bp5.hpp:43-74) */
 typedef struct bpx_batch_s* bpx_batch;       /* no reference counterpart: many BP5 problems */
+typedef struct bpx_cg_s* bpx_cg;             /* replaces cg (include/linearSolver.hpp:69-81) */
 
 const char* bpx_last_error(void);
 int bpx_version(void);
@@ -204,6 +205,25 @@ int bpx_linalg_norm2(bpx_platform p, int N, const double* x, double* result);
 int bpx_cg_update(bpx_platform p, int N, const double* Ap, const double* o_p, double alpha,
                   double* r, double* x, double* rdotr);
 
+/* ------------------------------------------------------------------ cg --
+ * cg(platform, N, Nhalo) (libs/core/linearSolverCG.cpp:33-60) and
+ * linearSolver_t::Solve(solver_t&, o_x, o_rhs, tol, MAXIT, verbose)
+ * (include/linearSolver.hpp:49-54) on any operator: solver_t::Operator
+ * (include/solver.hpp:47-49) is the callback `op`, called on the host
+ * thread with device vectors of N + Nhalo entries; it must queue its work
+ * on the platform's stream (bpx_platform_stream) and return 0, or a
+ * nonzero code that ends the solve with that code. The cg object owns its
+ * work vectors (allocated here, as the reference's constructor does).
+ * Scalars stay on the device; an iteration never waits for the host. o_r
+ * holds the right-hand side on entry and the residual on exit, o_x the
+ * initial guess and the solution. rnorm_hist (host, may be NULL) gets
+ * MAXIT + 1 residual norms (entry 0 = initial). */
+typedef int (*bpx_operator_fn)(void* ctx, const double* o_q, double* o_Aq);
+int bpx_cg_create(bpx_platform p, int N, int Nhalo, bpx_cg* out);
+int bpx_cg_destroy(bpx_cg c);
+int bpx_cg_solve(bpx_cg c, bpx_operator_fn op, void* ctx, double* o_x, double* o_r, double tol,
+                 int maxit, int verbose, int* iterations, double* rnorm_hist);
+
 /* ---------------------------------------------------------------- mesh --
  * mesh_t(platform, settings, comm) -> Setup for a hex box
  * (libs/mesh/meshSetup.cpp:31-85, meshSetupBoxHex3D.cpp:31-131).
@@ -248,6 +268,10 @@ int bpx_ogs_array(bpx_ogs o, const char* name, void* dst, int64_t* count, int* d
  * (libs/ogs/ogs.cpp:205-301, :394-492) on device vectors. */
 int bpx_ogs_gather_dev(bpx_platform p, bpx_ogs o, int K, const double* v, double* gv);
 int bpx_ogs_scatter_dev(bpx_platform p, bpx_ogs o, int K, const double* gv, double* v);
+/* ogs_t::GatherScatter(o_v, K, Add, Sym) (libs/ogs/ogs.cpp:494-581) in
+ * place: every copy of a node gets the sum over its copies; local entries
+ * in no row (masked) keep their value. */
+int bpx_ogs_gather_scatter_dev(bpx_platform p, bpx_ogs o, int K, double* v);
 
 /* ------------------------------------------------------------------ BP --
  * bpN_t(platform, settings, mesh, lambda) (BP/BP5/src/bp5Setup.cpp:29-95)
@@ -264,13 +288,24 @@ int bpx_bp_info(bpx_bp b, int64_t* out, int capacity);
  * single-copy rows, the gather sums the shared rows) or unfused (0: Ax,
  * then the full gather; -1 = no fused form for this problem), time fused,
  * time unfused; out[6..8]: the same for the FP32 operator (timed when it is
- * first used). Returns the number of values. */
+ * first used); out[9]: 1 if the FP32 operator forms its geometric factors
+ * from the elements' trilinear maps, 0 if it reads stored factors rounded
+ * to FP32, -1 if it has not been built yet. Returns the number of values. */
 int bpx_bp_tuning(bpx_bp b, double* out, int capacity);
 /* No reference counterpart: pin launch choices instead of the timed ones
  * (for reproducible timings; results do not depend on them): gather order
  * 0/1, FP64 operator fused 0/1, FP32 operator fused 0/1; -1 keeps a
  * choice. */
 int bpx_bp_set_tuning(bpx_bp b, int gatherMode, int fused, int fused32);
+/* No reference counterpart: the geometry of the FP32 BP5 operator (the
+ * inner operator of bpx_bp_solve_mixed). mode 0 (default): formed in the
+ * kernel from each element's trilinear map when every node of the mesh is
+ * its element's trilinear map of the 8 vertices (checked on the host; the
+ * box meshes of bpx_mesh_create are), else stored factors; mode 1: the FP64
+ * geometric factors rounded to FP32 and streamed (the operator of the
+ * reference's kernels with dfloat = float). Other operators always use
+ * stored factors. The FP32 operator is rebuilt on its next use. */
+int bpx_bp_set_fp32_geometry(bpx_bp b, int mode);
 /* No reference counterpart: where the BP5 Ax at N = 15 in FP64 (one field)
  * runs its r and s contractions: 0 = auto (the matrix cores; default), 1 =
  * vector ALU, 2 = matrix cores (BPX_ERR_UNSUPPORTED for other problems).
@@ -317,6 +352,24 @@ int bpx_bp_set_kappa(bpx_bp b, const double* o_kappa);
  * o_u, o_lam: device, Ngather; o_grad: device, Nnodes. Requires
  * bpx_bp_set_kappa first. */
 int bpx_bp_kappa_gradient(bpx_bp b, const double* o_u, const double* o_lam, double* o_grad);
+/* mesh_t::PlotFields (libs/mesh/meshPlotFields3D.cpp:31-121): a VTU file
+ * (ascii) of the mesh's local nodes with Nfields values each, QL (host,
+ * layout [element][field][node]); each element is N^3 sub-cubes of 6
+ * tetrahedra. Host only. */
+int bpx_mesh_plot_fields(bpx_mesh m, int Nfields, const double* QL, const char* filename);
+/* The solution written as bpN_t::Run does with an output file name
+ * (BP/BP5/src/bp5Run.cpp:220-238): o_u (device, Ngather, reference row
+ * order) scattered to the local nodes (Dirichlet nodes 0), then
+ * bpx_mesh_plot_fields. Synchronises. */
+int bpx_bp_plot_solution(bpx_bp b, const double* o_u, const char* filename);
+/* No reference counterpart (verification): o_u (device, Ngather) against
+ * the manufactured solution of the forcing, u = sin(pi x) sin(pi y)
+ * sin(pi z) for BP3/BP5 (-lap u + lambda u = (3 pi^2 + lambda) u) and
+ * 3 pi^2 u for BP1 (forced with lambda = 0), over the local nodes with the
+ * GLL quadrature: out[0] = max |u_h - u|, out[1] = ||u_h - u||_L2,
+ * out[2] = ||u_h - u||_L2 / ||u||_L2. One-field problems only.
+ * Synchronises. */
+int bpx_bp_error_norms(bpx_bp b, const double* o_u, double* out);
 /* b = gather(forcing) as bpN_t::Run does (BP/BP5/src/bp5Run.cpp:51-64). */
 int bpx_bp_rhs_forcing(bpx_bp b, double* o_b);
 /* cg::Solve (libs/core/linearSolverCG.cpp:62-129) with device-resident

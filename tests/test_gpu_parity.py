@@ -427,3 +427,35 @@ def test_ax_engine_errors(plat):
     b6 = capi.BP(plat, capi.Mesh(N=15, nx=1), 6, lam=1.0)
     with pytest.raises(capi.BpxError):
         b6.set_ax_engine(2)  # one field only
+
+
+@pytest.mark.parametrize("N", [3, 7, 8, 12])
+def test_fp32_stored_factor_operator(plat, N):
+    """The FP32 BP5 operator with the stored geometric factors rounded to
+    FP32 (bpx_bp_set_fp32_geometry(1): the reference kernel with dfloat =
+    float) against the FP64 operator, and the mixed solve on it to 1e-10;
+    the default on the box mesh forms the factors from the trilinear maps."""
+    m, P = make_pair(5, N, 3 if N <= 8 else 2, lam=1.0)
+    b = capi.BP(plat, m, 5, lam=1.0)
+    q = np.random.default_rng(N).standard_normal(b.Ngather)
+    A64 = torch.zeros(b.Ngather, dtype=torch.float64, device="cuda")
+    b.operator(dev(q), A64)
+    assert b.tuning()["fp32_trilinear"] == -1  # not built yet
+    A32 = torch.zeros(b.Ngather, dtype=torch.float32, device="cuda")
+    b.operator_f32(dev(q, torch.float32), A32)
+    assert b.tuning()["fp32_trilinear"] == 1
+    b.set_fp32_geometry(1)
+    A32s = torch.zeros_like(A32)
+    b.operator_f32(dev(q, torch.float32), A32s)
+    assert b.tuning()["fp32_trilinear"] == 0
+    assert rel_err(host(A32s).astype(np.float64), host(A64)) <= 2e-5
+    assert not np.array_equal(host(A32s), host(A32))
+    rhs = capi.random_normal(9, b.Ngather)
+    x = torch.zeros(b.Ngather, dtype=torch.float64, device="cuda")
+    st = b.solve_mixed(x, dev(rhs), tol=1e-10, interval=32)
+    assert st["rel_res"] < 1e-10
+    Ax = torch.zeros_like(x)
+    b.operator(x, Ax)
+    assert np.linalg.norm(rhs - host(Ax)) / np.linalg.norm(rhs) < 1e-10
+    with pytest.raises(capi.BpxError):
+        b.set_fp32_geometry(2)
