@@ -173,19 +173,22 @@ def bench_krhs(capi, plat):
         q = torch.from_numpy(capi.random_normal(4, Ndofs * K)).cuda()
         A = torch.empty(NL * K, dtype=torch.float64, device="cuda")
         best = None
-        for chunk in (1, 2, 4):
+        per_chunk = {}
+        for chunk in (0, 1, 2, 4):
             def run():
                 capi.check(L.bpx_bp5_ax_hex_krhs(plat.h, 7, K, chunk, Nel, capi.ptr(elist), capi.ptr(G2L),
                                                  capi.ptr(arr["wJ"]), None, capi.ptr(arr["ggeo"]),
                                                  capi.ptr(arr["gllw"]), capi.ptr(arr["D"]), None, None, 1.0,
                                                  capi.ptr(q), Ndofs, capi.ptr(A), NL))
             t = event_time(run, 10)
+            per_chunk[chunk] = t * 1e6
             if best is None or t < best[1]:
                 best = (chunk, t)
         chunk, t = best
         nb = krhs_bytes(K, Ndofs, Nel, Np)
         rows.append({"K": K, "chunk": chunk, "us": t * 1e6, "gdof_per_s": K * Ndofs / t / 1e9,
-                     "gbs": nb / t / 1e9, "algorithmic_bytes": nb, "speedup_vs_K_single": K * t1 / t})
+                     "gbs": nb / t / 1e9, "algorithmic_bytes": nb, "speedup_vs_K_single": K * t1 / t,
+                     "us_by_chunk": per_chunk})
         del q, A
     out["rows"] = rows
     return out
@@ -451,6 +454,48 @@ def cpu_baselines(args, b_host, Ndofs):
     return out
 
 
+def roofline_object(ins, op32, nbytes32, achieved32, copy_gbs, Nel, args):
+    """The roofline of the dominant kernel of the timed solve, the FP32 CG
+    operator's Ax. Its bound is VALU issue (it forms the geometric factors
+    from the trilinear maps instead of streaming them): the committed SQ
+    counter summary gives its VALU instructions per launch, each wave64 FP32
+    instruction takes 4 cycles of one of the 1,024 SIMDs, and the launch's
+    own duration, timed in this run inside one more mixed solve, gives the
+    issue rate it reaches. The HBM figure (the reference byte model with
+    4-byte values and the map coefficients in place of the stored factors)
+    is reported beside it."""
+    t = ins["ax_us"] * 1e-6
+    hbm = {"achieved": achieved32, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved32 / HBM_PEAK_GBS,
+           "algorithmic_bytes": nbytes32, "traffic": measured_traffic(TRAFFIC_FILE_F32),
+           "traffic_source": "committed counter summary " + os.path.relpath(TRAFFIC_FILE_F32, ROOT) +
+                             " (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a 512 MiB copy); not measured in "
+                             "this run"}
+    out = {"kernel": ("k_bp5_ax FP32 CG operator, " +
+                      ("fused (interior rows + surface run)" if ins["fused"] else "plain") +
+                      ", geometric factors from the trilinear map, non-affine, Nq=8"),
+           "us_per_launch": ins["ax_us"], "launches_timed": ins["launches"],
+           "timed_in": "one more mixed solve identical to the timed ones (dispatch-stamped HIP events)",
+           "gather_us_per_launch": ins["gather_us"], "us_per_launch_back_to_back": op32["ax_us"],
+           "device_copy_gbs": copy_gbs,
+           "model_gflops": bp5_ax_flops(args.N + 1, Nel) / t / 1e9}
+    lim = kernel_limiter(LIMITER_FILE_F32)
+    if lim and "valu_instructions_per_launch" in lim:
+        instr = lim["valu_instructions_per_launch"]
+        lanes_per_clk = 64 / lim["valu_cycles_per_instruction"]
+        peak = lim["simds"] * lanes_per_clk * lim["clock_ghz"] * 1e9  # VALU lane-operations per second
+        achieved = instr * 64 / t
+        out.update({"bound": "valu-issue", "achieved": achieved / 1e12, "peak": peak / 1e12,
+                    "unit": "T VALU lane-ops/s", "frac": achieved / peak, "traffic": hbm["traffic"],
+                    "valu_instructions_per_launch": instr,
+                    "valu_issue_us": instr * lim["valu_cycles_per_instruction"] / (lim["simds"] * lim["clock_ghz"] * 1e9) * 1e6,
+                    "counter_tflops": lim.get("counter_flops_per_launch", 0) / t / 1e12,
+                    "limiter_source": lim["source"], "hbm": hbm})
+    else:
+        out.update({"bound": "hbm", "achieved": achieved32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": achieved32 / HBM_PEAK_GBS, "traffic": hbm["traffic"], "hbm": hbm})
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -466,6 +511,9 @@ def main():
     ap.add_argument("--interval", type=int, default=64,
                     help="FP32 iterations between FP64 defect corrections")
     ap.add_argument("--ax-launches", type=int, default=50)
+    ap.add_argument("--cg-fold", type=int, default=1, choices=(0, 1),
+                    help="1: the fused CG operator also does the interior rows' direction update "
+                         "(the default); 0: one update kernel over all rows (same bits)")
     ap.add_argument("--cpu-its", type=int, default=1000)
     ap.add_argument("--cpu-serial-its", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -509,6 +557,7 @@ def main():
     t0 = time.perf_counter()
     mesh = capi.Mesh(N=args.N, nx=args.nx, perturb=0.1, seed=1)
     bp = capi.BP(plat, mesh, 5, lam=1.0)
+    bp.set_cg_fold(args.cg_fold)
     Ndofs, Nel, Np = bp.Ngather, mesh.Nelements, mesh.Np
     log(f"setup: {time.perf_counter() - t0:.1f}s  Ndofs={Ndofs}  Nel={Nel}  p={args.N}")
 
@@ -702,20 +751,7 @@ def main():
                   "ms_per_solve": tmix * 1e3, "true_rel_res": relmix,
                   "own_rate": Ndofs * st["iterations"] / tmix / 1e9,
                   "speedup_vs_fp64": t64 / tmix},
-        "roofline": {"bound": "hbm", "achieved": achieved32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved32 / HBM_PEAK_GBS, "traffic": measured_traffic(TRAFFIC_FILE_F32),
-                     "traffic_source": "committed counter summary " + os.path.relpath(TRAFFIC_FILE_F32, ROOT) +
-                                       " (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a 512 MiB copy); "
-                                       "not measured in this run",
-                     "limiter": kernel_limiter(LIMITER_FILE_F32),
-                     "kernel": ("k_bp5_ax FP32 CG operator, " +
-                                ("fused (interior rows + surface run)" if ins["fused"] else "plain") +
-                                ", geometric factors from the trilinear map, non-affine, Nq=8"),
-                     "us_per_launch": ins["ax_us"], "launches_timed": ins["launches"],
-                     "timed_in": "one more mixed solve identical to the timed ones",
-                     "algorithmic_bytes": nbytes32, "gather_us_per_launch": ins["gather_us"],
-                     "us_per_launch_back_to_back": op32["ax_us"], "device_copy_gbs": copy_gbs,
-                     "gflops": bp5_ax_flops(args.N + 1, Nel) / (ins["ax_us"] * 1e-6) / 1e9},
+        "roofline": roofline_object(ins, op32, nbytes32, achieved32, copy_gbs, Nel, args),
         "bk5_fp64": {"achieved": achieved, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(), "kernel": "k_bp5_ax (FP64, non-affine, Nq=8)",
                      "us_per_launch": t_ax * 1e6, "algorithmic_bytes": nbytes,

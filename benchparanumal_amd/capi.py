@@ -80,6 +80,7 @@ _SIGS = {
     "bpx_bp_set_tuning": (_I, [_P, _I, _I, _I]),
     "bpx_bp_time_operator": (_I, [_P, _I, _I, _P]),
     "bpx_bp_set_ax_engine": (_I, [_P, _I]),
+    "bpx_bp_set_cg_fold": (_I, [_P, _I]),
     "bpx_bp_set_fp32_geometry": (_I, [_P, _I]),
     "bpx_bp_time_operator_in_solve": (_I, [_P, _P, _P, _D, _I, _I, _P]),
     "bpx_bp_set_kappa": (_I, [_P, _P]),
@@ -325,6 +326,11 @@ class BP:
         check(lib().bpx_bp_time_operator_in_solve(self.h, ptr(x), ptr(rhs), tol, interval, maxit, ptr(out)))
         return {"ax_us": float(out[0]), "gather_us": float(out[1]), "launches": int(out[2]),
                 "fused": bool(out[3])}
+
+    def set_cg_fold(self, on):
+        """1 (default): the fused CG operator also does the interior rows'
+        direction update; 0: one update kernel over all rows (same bits)."""
+        check(lib().bpx_bp_set_cg_fold(self.h, int(on)))
 
     def set_ax_engine(self, engine):
         """BP5 at N = 15, FP64: 0 = auto (matrix cores), 1 = vector-ALU Ax,

@@ -117,6 +117,25 @@ enum { VAR_PLAIN = 0, VAR_KRHS = 1, VAR_BATCH = 2, VAR_FUSED = 3 };
 // (measured at 24^3, p=7: mixed solve 128.8 -> 122.7 ms, FP64 CG 236.0 ->
 // 226.1 ms; the Ax itself unchanged). The plain kernel keeps cached loads:
 // launched back to back (the Ax benchmark) it reuses geometry from the cache.
+// The fused epilogue's run index -> element-local node maps as tables (one
+// 16-bit entry per run position; 0xFFFF = padding): a lane looks its node
+// up with one load instead of forming it with integer divisions and the
+// surface run's case analysis (about 250 of the FP32 operator's 1,194 VALU
+// instructions per element at Nq = 8 went to that).
+template <int NQ>
+struct FusedNodeTables {
+  unsigned short inner[fused_interior_pad(NQ) > 0 ? fused_interior_pad(NQ) : 1];
+  unsigned short surf[fused_surface_pad(NQ)];
+  constexpr FusedNodeTables() : inner(), surf() {
+    for (int n = 0; n < fused_interior_pad(NQ); ++n)
+      inner[n] = n < fused_interior(NQ) ? (unsigned short)fused_interior_node(NQ, n) : (unsigned short)0xFFFF;
+    for (int n = 0; n < fused_surface_pad(NQ); ++n)
+      surf[n] = n < NQ * NQ * NQ - fused_interior(NQ) ? (unsigned short)fused_surface_node(NQ, n)
+                                                       : (unsigned short)0xFFFF;
+  }
+};
+__constant__ FusedNodeTables<BPX_NQ> kFusedNodes = FusedNodeTables<BPX_NQ>();
+
 template <bool STREAM, typename T>
 __device__ inline T geo_load(const T* p) {
   if constexpr (STREAM) return __builtin_nontemporal_load(p);
@@ -133,17 +152,25 @@ __device__ inline T geo_load(const T* p) {
 // constant); each node then forms the adjugate of the Jacobian, J and
 // G = adj adj^T w / J (the formula of the stored factors), ~55 VALU
 // operations in place of 28 B of HBM traffic per node.
-template <typename T, int NQ, int NF, int EPB, bool AFFINE, int WAVES, int VAR, bool TRI = false>
+// FOLD (FUSED only): the CG direction update of the element's interior rows
+// (rows >= S0 = ldq, each the only copy of its row) is done here, as
+// k_cg_update_p does it for the rows below S0: x <- alpha_{k-1} p + x and
+// p <- r + beta p (p <- r at k = 0), the same fused multiply-adds, so the
+// solve gives the bits it gives with the update in its own kernel. q is
+// then the direction p itself (fold.p, updated in place: no other lane
+// reads an interior row).
+template <typename T, int NQ, int NF, int EPB, bool AFFINE, int WAVES, int VAR, bool TRI = false, bool FOLD = false>
 __global__ __launch_bounds__(NQ* NQ* EPB) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_bp5_ax(
     int Nelements, int K, const int* __restrict__ elementList, const int* __restrict__ G2L,
     const T* __restrict__ wJ, const T* __restrict__ ggeo, const T* __restrict__ gllw,
-    const T* __restrict__ D, T lambda, const T* __restrict__ q, T* __restrict__ Aq, long long ldq,
+    const T* __restrict__ D, T lambda, const T* q, T* __restrict__ Aq, long long ldq,
     long long ldA, const int* __restrict__ elemPid, const BatchState* __restrict__ bst, T* __restrict__ Ap,
-    double* __restrict__ part) {
+    double* __restrict__ part, CgFold<T> fold) {
   constexpr bool KRHS = VAR == VAR_KRHS;
   constexpr bool BATCH = VAR == VAR_BATCH;
   constexpr bool FUSED = VAR == VAR_FUSED;
   static_assert(!FUSED || NF == 1, "the fused operator covers one field");
+  static_assert(!FOLD || FUSED, "the folded direction update is a form of the fused operator");
   static_assert(!TRI || (sizeof(T) == 4 && NF == 1 && !AFFINE && (VAR == VAR_PLAIN || FUSED)),
                 "TRI: FP32, one field, non-affine");
   constexpr int NQ2 = NQ * NQ;
@@ -183,6 +210,19 @@ __global__ __launch_bounds__(NQ* NQ* EPB) __attribute__((amdgpu_waves_per_eu(WAV
 
   for (int n = t; n < NQ2; n += NQ2 * EPB) s_D[n / NQ][n % NQ] = D[n];
 
+  // FUSED: the epilogue's node tables, loaded now so their latency hides
+  // behind the sweep (positions past a run's end read its last entry)
+  constexpr int FUI = (FUSED && fused_interior_pad(NQ) > 0) ? (fused_interior_pad(NQ) + NQ2 - 1) / NQ2 : 1;
+  constexpr int FUS = FUSED ? (fused_surface_pad(NQ) + NQ2 - 1) / NQ2 : 1;
+  int fni[FUI], fns[FUS];
+  if constexpr (FUSED) {
+#pragma unroll
+    for (int u = 0; u < FUI; ++u)
+      fni[u] = fused_interior_pad(NQ) > 0 ? kFusedNodes.inner[min(ij + u * NQ2, fused_interior_pad(NQ) - 1)] : 0xFFFF;
+#pragma unroll
+    for (int u = 0; u < FUS; ++u) fns[u] = kFusedNodes.surf[min(ij + u * NQ2, fused_surface_pad(NQ) - 1)];
+  }
+
   // KRHS: the pencil's gather rows, read once for all sweeps
   int gid[KRHS ? NQ : 1];
   if (KRHS) {
@@ -215,6 +255,33 @@ __global__ __launch_bounds__(NQ* NQ* EPB) __attribute__((amdgpu_waves_per_eu(WAV
     for (int f = 0; f < NF; ++f)
 #pragma unroll
       for (int k = 0; k < NQ; ++k) id[f][k] = active ? G2L[lbase + f * NP + k * NQ2] : -1;
+    // FOLD: which of the CG updates of iteration fold.k apply (the stopping
+    // test and its flags are k_cg_update_p's, written before this launch)
+    bool fx = false, fp = false, ffirst = false;
+    T fa = T(0), fb = T(0);
+    T fr[FOLD ? NQ : 1], fxv[FOLD ? NQ : 1];
+    // the lane's pencil crosses interior rows at k = 1 .. NQ-2 when it is an
+    // interior column of the element
+    const bool icol = FOLD && active && i >= 1 && i <= NQ - 2 && j >= 1 && j <= NQ - 2;
+    if constexpr (FOLD) {
+      const int kk = fold.k;
+      const int stop_at = fold.st->stop_at;
+      const bool done_before = kk > 0 && (fold.st->done || (stop_at != 0 && stop_at <= kk));
+      if (!done_before) {
+        fx = fold.pending && kk > 0;
+        fp = stop_at != kk + 1;
+        ffirst = kk == 0;
+        if (fx) fa = (T)fold.hist_alpha[kk - 1];
+        if (fp && !ffirst) fb = (T)(fold.hist_rr[kk] / fold.hist_rr[kk - 1]);
+      }
+      if (icol && (fx || fp)) {
+#pragma unroll
+        for (int k = 1; k < NQ - 1; ++k) {
+          fr[k] = fold.r[id[0][k]];
+          fxv[k] = fold.x[id[0][k]];
+        }
+      }
+    }
 #pragma unroll
     for (int f = 0; f < NF; ++f)
 #pragma unroll
@@ -223,6 +290,20 @@ __global__ __launch_bounds__(NQ* NQ* EPB) __attribute__((amdgpu_waves_per_eu(WAV
         rq[f][k] = (id[f][k] >= 0) ? v : T(0);
         rA[f][k] = T(0);
       }
+    if constexpr (FOLD) {
+      if (icol && (fx || fp)) {
+#pragma unroll
+        for (int k = 1; k < NQ - 1; ++k) {
+          const T pv = rq[0][k];
+          if (fx) fold.x[id[0][k]] = fmad(fa, pv, fxv[k]);
+          if (fp) {
+            const T pn = ffirst ? fr[k] : fmad(fb, pv, fr[k]);
+            rq[0][k] = pn;
+            fold.p[id[0][k]] = pn;
+          }
+        }
+      }
+    }
   }
 
   // geometry of the current slab (non-affine: prefetched one slab ahead)
@@ -401,17 +482,24 @@ __global__ __launch_bounds__(NQ* NQ* EPB) __attribute__((amdgpu_waves_per_eu(WAV
     double pd = 0;
     if (active) {
       T* inner = Ap + ldq + (size_t)e * NIP;
-      for (int n = ij; n < NIP; n += NQ2) {
-        T v = T(0);
-        if (n < NI) {
-          const int node = fused_interior_node(NQ, n);
-          v = s_out[es][node];
-          pd = fma((double)s_pin[es][node], (double)v, pd);
+#pragma unroll
+      for (int u = 0; u < FUI; ++u) {
+        const int n = ij + u * NQ2;
+        if (n < NIP) {
+          T v = T(0);
+          if (fni[u] != 0xFFFF) {
+            v = s_out[es][fni[u]];
+            pd = fma((double)s_pin[es][fni[u]], (double)v, pd);
+          }
+          inner[n] = v;
         }
-        inner[n] = v;
       }
       T* surf = Aq + (size_t)e * NSP;
-      for (int n = ij; n < NSP; n += NQ2) surf[n] = (n < NS) ? s_out[es][fused_surface_node(NQ, n)] : T(0);
+#pragma unroll
+      for (int u = 0; u < FUS; ++u) {
+        const int n = ij + u * NQ2;
+        if (n < NSP) surf[n] = (fns[u] != 0xFFFF) ? s_out[es][fns[u]] : T(0);
+      }
     }
     // workgroup sum of the p.Ap terms (fixed order). When NQ^2*EPB is not a
     // multiple of 64 the last wavefront is partial, and the registers of
@@ -454,6 +542,179 @@ __global__ __launch_bounds__(NQ* NQ* EPB) __attribute__((amdgpu_waves_per_eu(WAV
     }
   }
   }  // sweeps
+}
+
+// k right-hand sides with the geometric factors read once per workgroup
+// (non-affine): a workgroup holds one element as W lane sheets, sheet s
+// applying the operator to NF right-hand sides of its own; the 7 factors of
+// slab k+1 are loaded by all W*Nq^2 lanes together (about 7/W loads per
+// lane), staged in an LDS double buffer (component-major, so each node's
+// reads are conflict-free) and read from there by every sheet. A pass
+// covers W*NF right-hand sides; K needs ceil(K / (W*NF)) passes per element,
+// and the blocks of one element's passes are placed on the same XCD (block
+// b runs on XCD b mod 8), so a pass after the first finds the factors in
+// that XCD's L2. The arithmetic is the single-vector kernel's, term by term
+// (D from LDS, the same fma order), so each right-hand side gets the bits
+// k_bp5_ax gives it.
+template <int NQ>
+constexpr int krhs_wg_max_sheets() {
+  return (1024 / (NQ * NQ)) < 16 ? (1024 / (NQ * NQ)) : 16;
+}
+
+template <typename T, int NQ, int NF, int W, bool DR>
+__global__ __launch_bounds__(NQ* NQ* W) __attribute__((amdgpu_waves_per_eu(DR ? 2 : 4))) void k_bp5_krhs_wg(
+    int Nelements, int K, int npass, const int* __restrict__ elementList, const int* __restrict__ G2L,
+    const T* __restrict__ wJ, const T* __restrict__ ggeo, const T* __restrict__ D, T lambda,
+    const T* __restrict__ q, T* __restrict__ Aq, long long ldq, long long ldA) {
+  constexpr int NQ2 = NQ * NQ;
+  constexpr int NP = NQ2 * NQ;
+  constexpr int LD = NQ + 1;
+  constexpr int NG = 7 * NQ2;  // factors of one slab
+  constexpr int NT = W * NQ2;  // lanes
+  // D[k][m] is the same for every lane: scalar loads (constant address space)
+  typedef const __attribute__((address_space(4))) T* cptr;
+  const cptr Dc = (cptr)D;
+  // block -> (element, pass): the passes of element eo share its XCD
+  const int xcd = blockIdx.x & 7;
+  const int m = blockIdx.x >> 3;
+  const int pass = m % npass;
+  const int eo = (m / npass) * 8 + xcd;
+  if (eo >= Nelements) return;  // uniform over the workgroup, before any barrier
+  const int e = elementList[eo];
+
+  extern __shared__ __align__(16) unsigned char krhs_smem[];
+  T* s_D = reinterpret_cast<T*>(krhs_smem);  // [NQ][LD]
+  T* s_geo = s_D + NQ * LD;                  // [2][7][NQ2]
+  T* s_work = s_geo + 2 * NG;                // per sheet: s_q, s_r, s_s [NF][NQ][LD] each
+
+  const int t = threadIdx.x;
+  const int sh = t / NQ2;
+  const int ij = t - sh * NQ2;
+  const int i = ij % NQ;
+  const int j = ij / NQ;
+  const int r0 = (pass * W + sh) * NF;  // first right-hand side of this sheet
+  const bool live = r0 < K;
+  T* s_q = s_work + (size_t)sh * 3 * NF * NQ * LD;
+  T* s_r = s_q + NF * NQ * LD;
+  T* s_s = s_r + NF * NQ * LD;
+  auto SQ = [&](T* base, int f, int a, int b) -> T& { return base[(f * NQ + a) * LD + b]; };
+
+  for (int n = t; n < NQ2; n += NT) s_D[(n / NQ) * LD + n % NQ] = D[n];
+
+  // cooperative load of the factors of slab k into registers, then LDS
+  constexpr int GMAX = (NG + NT - 1) / NT;  // loads per lane
+  T gv[GMAX];
+  auto load_slab = [&](int k) {
+    const T* gs = ggeo + ((size_t)e * NP + (size_t)k * NQ2) * 6;
+    const T* ws = wJ + (size_t)e * NP + (size_t)k * NQ2;
+#pragma unroll
+    for (int u = 0; u < GMAX; ++u) {
+      const int f = t + u * NT;
+      gv[u] = (f < 6 * NQ2) ? __builtin_nontemporal_load(gs + f) : (f < NG ? __builtin_nontemporal_load(ws + (f - 6 * NQ2)) : T(0));
+    }
+  };
+  auto store_slab = [&](int buf) {
+    T* g = s_geo + buf * NG;
+#pragma unroll
+    for (int u = 0; u < GMAX; ++u) {
+      const int f = t + u * NT;
+      if (f < 6 * NQ2) g[(f % 6) * NQ2 + f / 6] = gv[u];
+      else if (f < NG) g[6 * NQ2 + (f - 6 * NQ2)] = gv[u];
+    }
+  };
+
+  // the pencil's gather rows and the sheet's right-hand sides (branch-free:
+  // a masked node, or an idle sheet, reads wJ[0] and keeps 0)
+  int gid[NQ];
+#pragma unroll
+  for (int k = 0; k < NQ; ++k) gid[k] = G2L[(size_t)e * NP + k * NQ2 + ij];
+  load_slab(0);
+  T rq[NF][NQ], rA[NF][NQ];
+#pragma unroll
+  for (int k = 0; k < NQ; ++k) {
+    const bool on = live && gid[k] >= 0;
+    const T* src = q + (size_t)(live ? r0 : 0) * ldq + max(gid[k], 0);
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      const T v = *(on ? src + (size_t)f * ldq : wJ);
+      rq[f][k] = on ? v : T(0);
+      rA[f][k] = T(0);
+    }
+  }
+  store_slab(0);
+  __syncthreads();
+  // DR: row and column i and j of D in registers (4 Nq LDS reads fewer per
+  // node and right-hand side), at fewer waves per SIMD
+  T Di[DR ? NQ : 1], Dj[DR ? NQ : 1], DTi[DR ? NQ : 1], DTj[DR ? NQ : 1];
+  if constexpr (DR) {
+#pragma unroll
+    for (int mm = 0; mm < NQ; ++mm) {
+      Di[mm] = s_D[i * LD + mm];
+      Dj[mm] = s_D[j * LD + mm];
+      DTi[mm] = s_D[mm * LD + i];
+      DTj[mm] = s_D[mm * LD + j];
+    }
+  }
+
+#pragma unroll
+  for (int k = 0; k < NQ; ++k) {
+#pragma unroll
+    for (int f = 0; f < NF; ++f)
+#pragma unroll
+      for (int mm = 0; mm < NQ; ++mm) asm volatile("" : "+v"(rA[f][mm]));
+    if (k + 1 < NQ) load_slab(k + 1);
+#pragma unroll
+    for (int f = 0; f < NF; ++f) SQ(s_q, f, j, i) = rq[f][k];
+    __syncthreads();
+    asm volatile("" ::: "memory");
+
+    const T* g = s_geo + (k & 1) * NG;
+    const T G00 = g[0 * NQ2 + ij], G01 = g[1 * NQ2 + ij], G02 = g[2 * NQ2 + ij];
+    const T G11 = g[3 * NQ2 + ij], G12 = g[4 * NQ2 + ij], G22 = g[5 * NQ2 + ij];
+    const T Wk = g[6 * NQ2 + ij];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      T qr = 0, qs = 0, qt = 0;
+#pragma unroll
+      for (int mm = 0; mm < NQ; ++mm) {
+        qr = fmad(DR ? Di[mm] : s_D[i * LD + mm], SQ(s_q, f, j, mm), qr);
+        qs = fmad(DR ? Dj[mm] : s_D[j * LD + mm], SQ(s_q, f, mm, i), qs);
+        qt = fmad(Dc[k * NQ + mm], rq[f][mm], qt);
+      }
+      const T Gr = fmad(G02, qt, fmad(G01, qs, G00 * qr));
+      const T Gs = fmad(G12, qt, fmad(G11, qs, G01 * qr));
+      const T Gt = fmad(G22, qt, fmad(G12, qs, G02 * qr));
+      SQ(s_r, f, j, i) = Gr;
+      SQ(s_s, f, j, i) = Gs;
+      rA[f][k] = fmad(lambda * Wk, rq[f][k], rA[f][k]);
+#pragma unroll
+      for (int mm = 0; mm < NQ; ++mm) rA[f][mm] = fmad(Dc[k * NQ + mm], Gt, rA[f][mm]);
+    }
+    // slab k+1's factors into the other buffer (its last readers, in slab
+    // k-1, finished before the barrier above)
+    if (k + 1 < NQ) store_slab((k + 1) & 1);
+    __syncthreads();
+    asm volatile("" ::: "memory");
+
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      T acc = 0;
+#pragma unroll
+      for (int mm = 0; mm < NQ; ++mm) {
+        acc = fmad(DR ? DTi[mm] : s_D[mm * LD + i], SQ(s_r, f, j, mm), acc);
+        acc = fmad(DR ? DTj[mm] : s_D[mm * LD + j], SQ(s_s, f, mm, i), acc);
+      }
+      rA[f][k] += acc;
+    }
+  }
+  if (live) {
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) {
+      T* out = Aq + (size_t)r0 * ldA + (size_t)e * NP + k * NQ2 + ij;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) out[(size_t)f * ldA] = rA[f][k];
+    }
+  }
 }
 
 // Adjoint gradient of the variable-coefficient BP5 (an extension):
@@ -532,8 +793,8 @@ __global__ __launch_bounds__(NQ* NQ* EPB) void k_bp5_kappa_grad(int Nelements, c
   }
 }
 
-template <typename T, int NQ, int NF, int VAR>
-int launch_bp5(hipStream_t s, const AxArgs<T>& a, T* Ap = nullptr, double* part = nullptr) {
+template <typename T, int NQ, int NF, int VAR, bool FOLD>
+int launch_bp5_v(hipStream_t s, const AxArgs<T>& a, T* Ap, double* part) {
   constexpr int EPB = bp5_epb<NQ>();
   constexpr int WV = bp5_waves_v<T, NQ, NF, VAR>();
   const int blocks = (a.Nelements + EPB - 1) / EPB;
@@ -543,23 +804,36 @@ int launch_bp5(hipStream_t s, const AxArgs<T>& a, T* Ap = nullptr, double* part 
   if constexpr (std::is_same<T, float>::value && NF == 1 && (VAR == VAR_PLAIN || VAR == VAR_FUSED)) {
     if (a.tri && !a.affine) {
       constexpr int WT = bp5_waves_tri<NQ, VAR>();
-      launch_stamped(k_bp5_ax<T, NQ, 1, EPB, false, WT, VAR, true>, dim3(blocks), dim3(NQ * NQ * EPB), s,
+      launch_stamped(k_bp5_ax<T, NQ, 1, EPB, false, WT, VAR, true, FOLD>, dim3(blocks), dim3(NQ * NQ * EPB), s,
                      a.evStart, a.evStop, a.Nelements, a.K, a.elementList, a.GlobalToLocal, a.wJ, a.ggeo, a.gllw,
-                     a.D, a.lambda, a.q, a.Aq, a.ldq, a.ldAq, noPid, noState, Ap, part);
+                     a.D, a.lambda, a.q, a.Aq, a.ldq, a.ldAq, noPid, noState, Ap, part, a.fold);
       BPX_HIP(hipGetLastError());
       return BPX_OK;
     }
   }
   if (a.affine)
-    launch_stamped(k_bp5_ax<T, NQ, NF, EPB, true, WV, VAR>, dim3(blocks), dim3(NQ * NQ * EPB), s, a.evStart,
-                   a.evStop, a.Nelements, a.K, a.elementList, a.GlobalToLocal, a.wJ, a.ggeo, a.gllw, a.D,
-                   a.lambda, a.q, a.Aq, a.ldq, a.ldAq, noPid, noState, Ap, part);
+    launch_stamped(k_bp5_ax<T, NQ, NF, EPB, true, WV, VAR, false, FOLD>, dim3(blocks), dim3(NQ * NQ * EPB), s,
+                   a.evStart, a.evStop, a.Nelements, a.K, a.elementList, a.GlobalToLocal, a.wJ, a.ggeo, a.gllw,
+                   a.D, a.lambda, a.q, a.Aq, a.ldq, a.ldAq, noPid, noState, Ap, part, a.fold);
   else
-    launch_stamped(k_bp5_ax<T, NQ, NF, EPB, false, WV, VAR>, dim3(blocks), dim3(NQ * NQ * EPB), s, a.evStart,
-                   a.evStop, a.Nelements, a.K, a.elementList, a.GlobalToLocal, a.wJ, a.ggeo, a.gllw, a.D,
-                   a.lambda, a.q, a.Aq, a.ldq, a.ldAq, noPid, noState, Ap, part);
+    launch_stamped(k_bp5_ax<T, NQ, NF, EPB, false, WV, VAR, false, FOLD>, dim3(blocks), dim3(NQ * NQ * EPB), s,
+                   a.evStart, a.evStop, a.Nelements, a.K, a.elementList, a.GlobalToLocal, a.wJ, a.ggeo, a.gllw,
+                   a.D, a.lambda, a.q, a.Aq, a.ldq, a.ldAq, noPid, noState, Ap, part, a.fold);
   BPX_HIP(hipGetLastError());
   return BPX_OK;
+}
+
+template <typename T, int NQ, int NF, int VAR>
+int launch_bp5(hipStream_t s, const AxArgs<T>& a, T* Ap = nullptr, double* part = nullptr) {
+  if constexpr (VAR == VAR_FUSED) {
+    if (a.fold.on) {
+      if (!a.fold.p || !a.fold.r || !a.fold.x || !a.fold.hist_rr || !a.fold.hist_alpha || !a.fold.st ||
+          a.fold.p != a.q)
+        return set_error(BPX_ERR_INVALID_ARGUMENT, "bp5 fused Ax: incomplete CG fold arguments");
+      return launch_bp5_v<T, NQ, NF, VAR, true>(s, a, Ap, part);
+    }
+  }
+  return launch_bp5_v<T, NQ, NF, VAR, false>(s, a, Ap, part);
 }
 
 }  // namespace
@@ -569,8 +843,106 @@ int launch_bp5_nq(hipStream_t s, const AxArgs<T>& a) {
   return launch_bp5<T, NQ, NF, VAR_PLAIN>(s, a);
 }
 
+// k_bp5_krhs_wg: W sheets (at most 16, 1024 lanes and 64 KB of LDS) with NF
+// right-hand sides each, W no larger than the K/NF the element needs
+template <typename T, int NQ, int NF>
+constexpr int krhs_wg_sheets() {
+  constexpr size_t fixed = (size_t)NQ * (NQ + 1) + 2 * 7 * NQ * NQ;
+  constexpr size_t per_sheet = (size_t)3 * NF * NQ * (NQ + 1);
+  constexpr int w_lds = (int)(((65536 / sizeof(T)) - fixed) / per_sheet);
+  return w_lds < krhs_wg_max_sheets<NQ>() ? w_lds : krhs_wg_max_sheets<NQ>();
+}
+// the form is used where a sheet fits and its registers fit four waves
+// per SIMD (FP64 with two right-hand sides per sheet spills at Nq >= 8)
+template <typename T, int NQ, int NF>
+constexpr bool krhs_wg_fits() {
+  return krhs_wg_sheets<T, NQ, NF>() >= 1 && !(NF == 2 && sizeof(T) == 8 && NQ >= 8);
+}
+// four sheets (measured at 20^3, p = 7, FP64: 1.42x over K single launches
+// at K = 32 with D in registers, against 1.31x with two sheets and 1.18x
+// with eight), fewer when K needs fewer
+inline int krhs_wg_default_sheets(int need) { return need <= 1 ? 1 : need <= 2 ? 2 : 4; }
+
+template <typename T, int NQ, int NF, int W, bool DR>
+int launch_bp5_krhs_wg_w(hipStream_t s, const AxArgs<T>& a) {
+  constexpr int NQ2 = NQ * NQ;
+  constexpr int LD = NQ + 1;
+  constexpr size_t fixed = (size_t)NQ * LD + 2 * 7 * NQ2;
+  constexpr size_t per_sheet = (size_t)3 * NF * NQ * LD;
+  static_assert(W <= krhs_wg_sheets<T, NQ, NF>(), "k-RHS workgroup: sheets beyond the LDS or lane limit");
+  const int need = (a.K + NF - 1) / NF;
+  const int npass = (need + W - 1) / W;
+  const long long groups = (a.Nelements + 7) / 8;
+  if (groups * 8 * npass > 0x7fffffffLL) return set_error(BPX_ERR_UNSUPPORTED, "bp5 k-RHS Ax: grid too large");
+  const size_t smem = (fixed + W * per_sheet) * sizeof(T);
+  hipLaunchKernelGGL((k_bp5_krhs_wg<T, NQ, NF, W, DR>), dim3((unsigned)(groups * 8 * npass)), dim3(W * NQ2), smem, s,
+                     a.Nelements, a.K, npass, a.elementList, a.GlobalToLocal, a.wJ, a.ggeo, a.D, a.lambda, a.q, a.Aq,
+                     (long long)a.ldq, (long long)a.ldAq);
+  BPX_HIP(hipGetLastError());
+  return BPX_OK;
+}
+
+// sheets per workgroup: a.krhs_sheets when set (1, 2, 4, 8 or 16, within
+// the LDS and lane limits), else the default below
+template <typename T, int NQ, int NF>
+int launch_bp5_krhs_wg(hipStream_t s, const AxArgs<T>& a) {
+  constexpr int w_max = krhs_wg_sheets<T, NQ, NF>();
+  const int need = (a.K + NF - 1) / NF;
+  int W = a.krhs_sheets;
+  if (W == 0) W = krhs_wg_default_sheets(need);
+  while (W > w_max || W > 16) W /= 2;
+  // D in registers: FP64, one right-hand side per sheet, Nq <= 10 (default)
+  constexpr bool dr_fits = sizeof(T) == 8 && NF == 1 && NQ <= 10;
+  const bool dr = dr_fits && a.krhs_dreg >= 0;
+  switch (W) {
+    case 16:
+      if constexpr (w_max >= 16) return launch_bp5_krhs_wg_w<T, NQ, NF, 16, false>(s, a);
+      break;
+    case 8:
+      if constexpr (w_max >= 8) {
+        if constexpr (dr_fits) {
+          if (dr) return launch_bp5_krhs_wg_w<T, NQ, NF, 8, true>(s, a);
+        }
+        return launch_bp5_krhs_wg_w<T, NQ, NF, 8, false>(s, a);
+      }
+      break;
+    case 4:
+      if constexpr (w_max >= 4) {
+        if constexpr (dr_fits) {
+          if (dr) return launch_bp5_krhs_wg_w<T, NQ, NF, 4, true>(s, a);
+        }
+        return launch_bp5_krhs_wg_w<T, NQ, NF, 4, false>(s, a);
+      }
+      break;
+    case 2:
+      if constexpr (w_max >= 2) {
+        if constexpr (dr_fits) {
+          if (dr) return launch_bp5_krhs_wg_w<T, NQ, NF, 2, true>(s, a);
+        }
+        return launch_bp5_krhs_wg_w<T, NQ, NF, 2, false>(s, a);
+      }
+      break;
+    case 1:
+      if constexpr (dr_fits) {
+        if (dr) return launch_bp5_krhs_wg_w<T, NQ, NF, 1, true>(s, a);
+      }
+      return launch_bp5_krhs_wg_w<T, NQ, NF, 1, false>(s, a);
+    default:
+      break;
+  }
+  return set_error(BPX_ERR_INVALID_ARGUMENT, "bp5 k-RHS Ax: sheets per workgroup must be 1, 2, 4, 8 or 16");
+}
+
 template <typename T, int NQ>
 int launch_bp5_krhs_nq(hipStream_t s, const AxArgs<T>& a, int chunk) {
+  // non-affine: the factors are read once per workgroup (shared by its
+  // sheets); affine factors are per element, so the sweep kernel serves
+  if (chunk == 0) {
+    if constexpr (krhs_wg_fits<T, NQ, 1>()) {
+      if (!a.affine) return launch_bp5_krhs_wg<T, NQ, 1>(s, a);
+    }
+    chunk = (a.K % 2 == 0) ? 2 : 1;  // the sweep form's default
+  }
   switch (chunk) {
     case 1: return launch_bp5<T, NQ, 1, VAR_KRHS>(s, a);
     case 2: return launch_bp5<T, NQ, 2, VAR_KRHS>(s, a);
@@ -615,7 +987,7 @@ int launch_batch_v(hipStream_t s, int Nelements, const int* elementList, const i
   if (blocks == 0) return BPX_OK;
   hipLaunchKernelGGL((k_bp5_ax<double, NQ, 1, EPB, false, WV, VAR_BATCH>), dim3(blocks), dim3(NQ * NQ * EPB), 0,
                      s, Nelements, 1, elementList, G2L, wJ, ggeo, (const double*)nullptr, D, lambda, q, Aq, 0LL,
-                     0LL, elemPid, done, (double*)nullptr, (double*)nullptr);
+                     0LL, elemPid, done, (double*)nullptr, (double*)nullptr, CgFold<double>());
   BPX_HIP(hipGetLastError());
   return BPX_OK;
 }

@@ -81,9 +81,8 @@ int launch_bp5_ax_krhs(hipStream_t s, const AxArgs<T>& a, int chunk) {
   if (a.K > 1 && (a.ldq < 1 || a.ldAq < 1))
     return set_error(BPX_ERR_INVALID_ARGUMENT, "bp5 k-RHS Ax: leading dimensions must be positive");
   if (a.Nfields != 1) return set_error(BPX_ERR_UNSUPPORTED, "bp5 k-RHS Ax: one field per right-hand side");
-  if (chunk == 0) chunk = a.K % 2 == 0 ? 2 : 1;
-  if ((chunk != 1 && chunk != 2 && chunk != 4) || a.K % chunk != 0)
-    return set_error(BPX_ERR_INVALID_ARGUMENT, "bp5 k-RHS Ax: chunk must be 1, 2 or 4 and divide K");
+  if (chunk != 0 && ((chunk != 1 && chunk != 2 && chunk != 4) || a.K % chunk != 0))
+    return set_error(BPX_ERR_INVALID_ARGUMENT, "bp5 k-RHS Ax: chunk must be 0, or 1, 2 or 4 dividing K");
   if (a.Nelements == 0) return BPX_OK;
   if (!a.wJ || !a.ggeo || !a.D || (a.affine && !a.gllw))
     return set_error(BPX_ERR_INVALID_ARGUMENT, "bp5 k-RHS Ax: null geometry/operator pointer");
@@ -110,7 +109,11 @@ int launch_bp5_fused(hipStream_t s, const AxArgs<T>& a, T* Ap, double* part) {
   if (a.engine == AX_ENGINE_MATRIX && !bp5_uses_matrix_cores(a))
     return set_error(BPX_ERR_UNSUPPORTED, "bp5 fused Ax: the matrix-core kernel covers FP64, N = 15");
   if constexpr (std::is_same<T, double>::value) {
-    if (bp5_uses_matrix_cores(a)) return launch_bp5_fused_mfma16(s, a, Ap, part);
+    if (bp5_uses_matrix_cores(a)) {
+      if (a.fold.on)
+        return set_error(BPX_ERR_UNSUPPORTED, "bp5 fused Ax: the matrix-core kernel has no folded CG update");
+      return launch_bp5_fused_mfma16(s, a, Ap, part);
+    }
   }
   switch (a.N + 1) {
     BPX_FUSED_CASE(2) BPX_FUSED_CASE(3) BPX_FUSED_CASE(4) BPX_FUSED_CASE(5) BPX_FUSED_CASE(6)

@@ -26,6 +26,26 @@ int hip_error(hipError_t e, const char* what);
   } while (0)
 
 // ------------------------------------------------- kernel parameter packs --
+struct CgState;
+
+// The CG direction update folded into the fused BP5 operator (one field):
+// for the element's interior rows, the only copies of their rows, the Ax
+// kernel forms x <- x + alpha_{k-1} p and p <- r + beta p itself, with the
+// arithmetic and the stopping logic of k_cg_update_p, which then runs on
+// the rows before the first interior row only.
+template <typename T>
+struct CgFold {
+  int on = 0;
+  int k = 0;        // CG iteration
+  int pending = 0;  // the x update of iteration k-1 is still to be applied
+  T* p = nullptr;   // the direction (the operator's q, updated in place)
+  const T* r = nullptr;
+  T* x = nullptr;
+  const double* hist_rr = nullptr;     // r.r before iteration k (k_cg_update_p wrote entry k)
+  const double* hist_alpha = nullptr;  // alpha of iteration k-1
+  const CgState* st = nullptr;
+};
+
 // Ax kernels: positional arguments of the OKL kernels, templated on the
 // value type. Fields that a BP does not use stay null.
 template <typename T>
@@ -50,6 +70,13 @@ struct AxArgs {
   // of Aq at Aq + r*ldAq (one vector after another)
   int K = 1;
   long long ldq = 0, ldAq = 0;
+  // k_bp5_krhs_wg (chunk 0, non-affine): lane sheets per workgroup (1, 2,
+  // 4, 8, 16; 0 = default) and D's rows and columns in registers where
+  // they fit (>= 0, the default) or read from LDS (-1); the same bits
+  int krhs_sheets = 0;
+  int krhs_dreg = 0;
+  // fused BP5 operator only
+  CgFold<T> fold;
   // BP5, FP64, one field, Nq = 16: the r and s contractions on the matrix
   // cores (launch_bp5_ax_mfma16) or on the vector ALU; the two agree to
   // rounding, not bit for bit. AX_ENGINE_AUTO = matrix cores where they
@@ -252,7 +279,15 @@ struct CgState {
   int done;       // 1 once r.r <= tol2, the iteration cap is reached, or breakdown
   int breakdown;  // 1 when p.Ap or r.r stopped being finite/positive
   int maxit;
+  // k + 1 when the stopping test of k_cg_update_p fired at iteration k, else
+  // 0. That kernel records its stop here, never in `done`: its other
+  // workgroups test `done` and `stop_at <= k` on entry, and a stop written
+  // by its own first workgroup must not make them skip the x update of
+  // iteration k - 1, which every row still needs. The kernels after it
+  // treat stop_at != 0 as done, and k_cg_finish folds it into `done`.
+  int stop_at;
 };
+
 
 // CG steps, templated on the vector type (double: plain CG; float: the
 // inner solve of the mixed-precision path). Iteration k is passed by the

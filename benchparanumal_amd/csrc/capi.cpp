@@ -649,6 +649,13 @@ int bpx_bp_set_fp32_geometry(bpx_bp b, int mode) {
   BPX_GUARD_END
 }
 
+int bpx_bp_set_cg_fold(bpx_bp b, int on) {
+  if (int st = need(b, "bp")) return st;
+  if (on != 0 && on != 1) return set_error(BPX_ERR_INVALID_ARGUMENT, "bp: cg fold must be 0 or 1");
+  b->bp.cgFold = on;
+  return BPX_OK;
+}
+
 int bpx_bp_set_ax_engine(bpx_bp b, int engine) {
   BPX_GUARD_BEGIN
   if (int st = need(b, "bp")) return st;

@@ -197,6 +197,13 @@ class solver_t {
     (void)o_q; (void)o_Aq; (void)part; (void)npart;
     return set_error(BPX_ERR_UNSUPPORTED, "solver: no fused operator + dot");
   }
+  // Optional: OperatorDot that also does the CG direction update of the
+  // rows from FoldRows() on (see CgFold); 0 rows = not offered.
+  virtual dlong FoldRows() const { return 0; }
+  virtual int OperatorDotFold(const CgFold<dfloat>& fold, dfloat* o_Aq, double* part, int* npart) {
+    (void)fold; (void)o_Aq; (void)part; (void)npart;
+    return set_error(BPX_ERR_UNSUPPORTED, "solver: no folded CG operator");
+  }
 };
 
 // linearSolver_t (include/linearSolver.hpp:37-81)
@@ -205,6 +212,21 @@ class linearSolver_t {
   virtual ~linearSolver_t() = default;
   virtual int Solve(solver_t& solver, dfloat* o_x, dfloat* o_r, dfloat tol, int MAXIT, int verbose,
                     int* iterations) = 0;
+};
+
+// Device work buffers of one CG solve: p and Ap (N entries), the state,
+// kRedBlocks partials of r.r, the p.Ap partials (as many as the operator
+// writes, at least kRedBlocks) and MAXIT + 1 entries each of the r.r and
+// alpha histories.
+template <typename T>
+struct CgWork {
+  T* p = nullptr;
+  T* Ap = nullptr;
+  CgState* state = nullptr;
+  double* part_rr = nullptr;
+  double* part_pAp = nullptr;
+  double* hist_rr = nullptr;
+  double* hist_alpha = nullptr;
 };
 
 // cg (libs/core/linearSolverCG.cpp:33-156) with device-resident scalars.
@@ -217,11 +239,15 @@ class cg_t {
   // apply_dot(o_q, o_Aq, part, &npart): the operator fused with the p.Ap
   // partial sums (npart FP64 partials written to part); may be empty.
   typedef std::function<int(const T*, T*, double*, int*)> ApplyDot;
+  // apply_fold(fold, o_Ap, part, &npart): apply_dot on fold.p, with the
+  // direction update of iteration fold.k folded in for the rows from
+  // fold_rows on (k_cg_update_p then covers the rows below fold_rows)
+  typedef std::function<int(const CgFold<T>&, T*, double*, int*)> ApplyFold;
 
   // npart_max: most partials apply_dot writes (at least kRedBlocks is kept)
   int Setup(platform_t& p, dlong N, int npart_max = 0);
   int Solve(const Apply& apply, const ApplyDot& apply_dot, T* o_x, T* o_r, double tol, int MAXIT,
-            int* iterations);
+            int* iterations, const ApplyFold& apply_fold = ApplyFold(), dlong fold_rows = 0);
   // squared residual norms r.r of the last Solve (entry 0 = initial)
   int History(std::vector<double>& rr, std::vector<double>& alpha);
 
@@ -233,6 +259,13 @@ class cg_t {
   deviceBuffer<double> o_hist_rr, o_hist_alpha;
   int hist_len = 0;
 };
+
+// The CG iteration of cg_t::Solve on caller-provided work buffers (queued
+// on stream s; allocates nothing).
+template <typename T>
+int cg_run(hipStream_t s, dlong N, const CgWork<T>& w, const typename cg_t<T>::Apply& apply,
+           const typename cg_t<T>::ApplyDot& apply_dot, T* o_x, T* o_r, double tol, int MAXIT, int* iterations,
+           const typename cg_t<T>::ApplyFold& apply_fold = typename cg_t<T>::ApplyFold(), dlong fold_rows = 0);
 
 class cg : public linearSolver_t {
  public:
@@ -284,7 +317,7 @@ class bp_t : public solver_t {
   // partials (FP64 and FP32; fused or not as tuned)
   int OperatorS(const dfloat* o_q, dfloat* o_Aq);
   int OperatorDotS(const dfloat* o_q, dfloat* o_Aq, double* part, int* npart);
-  int OperatorDot32(const float* o_q, float* o_Aq, double* part, int* npart);
+  int OperatorDot32(const float* o_q, float* o_Aq, double* part, int* npart, const CgFold<float>* fold = nullptr);
   // cg::Solve on reference-order x and r, run in the solve space
   int SolveCG(cg& solver, dfloat* o_x, dfloat* o_r, dfloat tol, int maxit, int verbose, int* iterations);
   deviceBuffer<double> o_sx, o_sr, o_sb;  // solve-space copies (created on first use)
@@ -327,7 +360,12 @@ class bp_t : public solver_t {
   bool fused = false;
   int PrepareFused();
   template <typename T>
-  int OperatorDotFused(const T* o_q, T* o_Aq, double* part, int* npart);
+  int OperatorDotFused(const T* o_q, T* o_Aq, double* part, int* npart, const CgFold<T>* fold = nullptr);
+  // the rows a folded CG operator leaves to k_cg_update_p (the solve space's
+  // rows before the first interior run), or 0 where no fold applies:
+  // precision 64 or 32, the fused operator in use, and cgFold set
+  dlong FoldRows(int precision) const;
+  int cgFold = 1;  // fold the CG direction update into the fused operator (same bits)
   int OperatorDotUnfusedS(const dfloat* o_q, dfloat* o_Aq, double* part, int* npart);
   int OperatorDotUnfused32(const float* o_q, float* o_Aq, double* part, int* npart);
   // Ax into the local vector; G2L = the GlobalToLocal map to read q
@@ -422,6 +460,10 @@ class bp_solve_view_t : public solver_t {
   bool HasOperatorDot() const override { return true; }
   int OperatorDot(const dfloat* o_q, dfloat* o_Aq, double* part, int* npart) override {
     return bp.OperatorDotS(o_q, o_Aq, part, npart);
+  }
+  dlong FoldRows() const override { return bp.FoldRows(64); }
+  int OperatorDotFold(const CgFold<dfloat>& fold, dfloat* o_Aq, double* part, int* npart) override {
+    return bp.OperatorDotFused<double>(fold.p, o_Aq, part, npart, &fold);
   }
   bp_t& bp;
 };

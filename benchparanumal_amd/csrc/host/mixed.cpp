@@ -77,17 +77,35 @@ int bp_t::SolveMixed(dfloat* o_x, const dfloat* o_b, double tol, int interval, i
   if (!std::isfinite(rn) || !std::isfinite(nb)) return set_error(BPX_ERR_BREAKDOWN, "mixed solve: non-finite residual");
   int k = 0, kseg = 0, updates = 0;  // kseg: iteration within the current recurrence
   CgState* st_d = w.state.ptr();
+  // with the fused FP32 operator the direction update of the interior rows
+  // is done inside it (CgFold); k_cg_update_p covers the rows below
+  const dlong frows = FoldRows(32);
   while (rn > tol * nb && k < maxit) {
     const int n = std::min(interval, maxit - k);
     for (int j = 0; j < n; ++j, ++k, ++kseg) {
       // tol = 0: the recurrence never stops itself; the FP64 test decides.
       // c += alpha p of the previous iteration rides along (not across a
       // correction, before which it is flushed)
-      if (int st = launch_cg_update_p<float>(s, N, kseg, w.r32.ptr(), w.p32.ptr(), w.c32.ptr(), w.part_rr.ptr(),
-                                             kRedBlocks, st_d, w.hist_rr.ptr(), w.hist_alpha.ptr(), 0.0, j > 0))
+      if (int st = launch_cg_update_p<float>(s, frows > 0 ? frows : N, kseg, w.r32.ptr(), w.p32.ptr(), w.c32.ptr(),
+                                             w.part_rr.ptr(), kRedBlocks, st_d, w.hist_rr.ptr(), w.hist_alpha.ptr(),
+                                             0.0, j > 0))
         return st;
       int npart = 0;
-      if (int st = OperatorDot32(w.p32.ptr(), w.Ap32.ptr(), w.part_pAp.ptr(), &npart)) return st;
+      if (frows > 0) {
+        CgFold<float> f;
+        f.on = 1;
+        f.k = kseg;
+        f.pending = j > 0;
+        f.p = w.p32.ptr();
+        f.r = w.r32.ptr();
+        f.x = w.c32.ptr();
+        f.hist_rr = w.hist_rr.ptr();
+        f.hist_alpha = w.hist_alpha.ptr();
+        f.st = st_d;
+        if (int st = OperatorDot32(w.p32.ptr(), w.Ap32.ptr(), w.part_pAp.ptr(), &npart, &f)) return st;
+      } else if (int st = OperatorDot32(w.p32.ptr(), w.Ap32.ptr(), w.part_pAp.ptr(), &npart)) {
+        return st;
+      }
       if (int st = launch_cg_update_r<float>(s, N, kseg, w.Ap32.ptr(), w.r32.ptr(), w.part_pAp.ptr(), npart,
                                              w.part_rr.ptr(), st_d, w.hist_rr.ptr(), w.hist_alpha.ptr()))
         return st;
@@ -106,7 +124,7 @@ int bp_t::SolveMixed(dfloat* o_x, const dfloat* o_b, double tol, int interval, i
     // the FP32 recurrence stopped (r.r = 0 or p.Ap <= 0, e.g. after it has
     // resolved the residual below FP32 range): restart it from the true
     // residual unless that has converged
-    if (hst.done || hst.breakdown) kseg = 0;
+    if (hst.done || hst.stop_at || hst.breakdown) kseg = 0;
   }
   if (int st = FromSolve(xs, o_x)) return st;
   if (int st = platform->finish()) return st;

@@ -122,24 +122,40 @@ int cg_t<T>::Setup(platform_t& p, dlong N_, int npart_max) {
 // after every chunk of 16 iterations, so a converged solve stops early.
 template <typename T>
 int cg_t<T>::Solve(const Apply& apply, const ApplyDot& apply_dot, T* o_x, T* o_r, double tol, int MAXIT,
-                   int* iterations) {
+                   int* iterations, const ApplyFold& apply_fold, dlong fold_rows) {
   if (MAXIT < 0) return set_error(BPX_ERR_INVALID_ARGUMENT, "cg: MAXIT < 0");
-  hipStream_t s = platform->stream;
   if (hist_len < MAXIT + 1) {
     if (int st = o_hist_rr.alloc(MAXIT + 1)) return st;
     if (int st = o_hist_alpha.alloc(MAXIT + 1)) return st;
     hist_len = MAXIT + 1;
   }
-  CgState* st_d = o_state.ptr();
-  double* prr = o_part_rr.ptr();
-  double* ppAp = o_part_pAp.ptr();
+  CgWork<T> w;
+  w.p = o_p.ptr();
+  w.Ap = o_Ap.ptr();
+  w.state = o_state.ptr();
+  w.part_rr = o_part_rr.ptr();
+  w.part_pAp = o_part_pAp.ptr();
+  w.hist_rr = o_hist_rr.ptr();
+  w.hist_alpha = o_hist_alpha.ptr();
+  return cg_run<T>(platform->stream, N, w, apply, apply_dot, o_x, o_r, tol, MAXIT, iterations, apply_fold,
+                   fold_rows);
+}
+
+template <typename T>
+int cg_run(hipStream_t s, dlong N, const CgWork<T>& w, const typename cg_t<T>::Apply& apply,
+           const typename cg_t<T>::ApplyDot& apply_dot, T* o_x, T* o_r, double tol, int MAXIT, int* iterations,
+           const typename cg_t<T>::ApplyFold& apply_fold, dlong fold_rows) {
+  if (MAXIT < 0) return set_error(BPX_ERR_INVALID_ARGUMENT, "cg: MAXIT < 0");
+  CgState* st_d = w.state;
+  double* prr = w.part_rr;
+  double* ppAp = w.part_pAp;
   // r <- b - A x (linearSolverCG.cpp:79-82)
-  if (int st = apply(o_x, o_Ap.ptr())) return st;
-  if (int st = launch_cg_init<T>(s, N, o_Ap.ptr(), o_r, prr)) return st;
+  if (int st = apply(o_x, w.Ap)) return st;
+  if (int st = launch_cg_init<T>(s, N, w.Ap, o_r, prr)) return st;
   CgState hst;
   if (MAXIT == 0) {  // only the initial residual and the stopping test
-    if (int st = launch_cg_update_p<T>(s, N, 0, o_r, o_p.ptr(), o_x, prr, kRedBlocks, st_d, o_hist_rr.ptr(),
-                                       o_hist_alpha.ptr(), tol, false))
+    if (int st = launch_cg_update_p<T>(s, N, 0, o_r, w.p, o_x, prr, kRedBlocks, st_d, w.hist_rr, w.hist_alpha,
+                                       tol, false))
       return st;
   }
   const int chunk = (tol > 0) ? 16 : std::max(MAXIT, 1);
@@ -147,23 +163,43 @@ int cg_t<T>::Solve(const Apply& apply, const ApplyDot& apply_dot, T* o_x, T* o_r
   while (queued < MAXIT) {
     const int n = std::min(chunk, MAXIT - queued);
     for (int k = queued; k < queued + n; ++k) {
-      // also x += alpha_{k-1} p, the x update of iteration k-1
-      if (int st = launch_cg_update_p<T>(s, N, k, o_r, o_p.ptr(), o_x, prr, kRedBlocks, st_d, o_hist_rr.ptr(),
-                                         o_hist_alpha.ptr(), tol, k > 0))
-        return st;
       int npart = kRedBlocks;
-      if (apply_dot) {
-        if (int st = apply_dot(o_p.ptr(), o_Ap.ptr(), ppAp, &npart)) return st;
-      } else {
-        if (int st = apply(o_p.ptr(), o_Ap.ptr())) return st;
-        if (int st = launch_dot_partials<T>(s, N, o_p.ptr(), o_Ap.ptr(), ppAp)) return st;
+      if (apply_fold) {
+        // the direction update of the rows below fold_rows here, of the
+        // others inside the operator
+        if (int st = launch_cg_update_p<T>(s, fold_rows, k, o_r, w.p, o_x, prr, kRedBlocks, st_d, w.hist_rr,
+                                           w.hist_alpha, tol, k > 0))
+          return st;
+        CgFold<T> f;
+        f.on = 1;
+        f.k = k;
+        f.pending = k > 0;
+        f.p = w.p;
+        f.r = o_r;
+        f.x = o_x;
+        f.hist_rr = w.hist_rr;
+        f.hist_alpha = w.hist_alpha;
+        f.st = st_d;
+        if (int st = apply_fold(f, w.Ap, ppAp, &npart)) return st;
+        if (int st = launch_cg_update_r<T>(s, N, k, w.Ap, o_r, ppAp, npart, prr, st_d, w.hist_rr, w.hist_alpha))
+          return st;
+        continue;
       }
-      if (int st = launch_cg_update_r<T>(s, N, k, o_Ap.ptr(), o_r, ppAp, npart, prr, st_d, o_hist_rr.ptr(),
-                                         o_hist_alpha.ptr()))
+      // also x += alpha_{k-1} p, the x update of iteration k-1
+      if (int st = launch_cg_update_p<T>(s, N, k, o_r, w.p, o_x, prr, kRedBlocks, st_d, w.hist_rr, w.hist_alpha,
+                                         tol, k > 0))
+        return st;
+      if (apply_dot) {
+        if (int st = apply_dot(w.p, w.Ap, ppAp, &npart)) return st;
+      } else {
+        if (int st = apply(w.p, w.Ap)) return st;
+        if (int st = launch_dot_partials<T>(s, N, w.p, w.Ap, ppAp)) return st;
+      }
+      if (int st = launch_cg_update_r<T>(s, N, k, w.Ap, o_r, ppAp, npart, prr, st_d, w.hist_rr, w.hist_alpha))
         return st;
     }
     queued += n;
-    if (int st = launch_cg_finish(s, queued, MAXIT, prr, kRedBlocks, st_d, o_hist_rr.ptr())) return st;
+    if (int st = launch_cg_finish(s, queued, MAXIT, prr, kRedBlocks, st_d, w.hist_rr)) return st;
     if (queued < MAXIT) {
       BPX_HIP(hipMemcpyAsync(&hst, st_d, sizeof(CgState), hipMemcpyDeviceToHost, s));
       BPX_HIP(hipStreamSynchronize(s));
@@ -175,8 +211,7 @@ int cg_t<T>::Solve(const Apply& apply, const ApplyDot& apply_dot, T* o_x, T* o_r
   // stopped after its last queued iteration (converged there, or MAXIT):
   // that iteration's x update is still to be applied
   if (MAXIT > 0 && hst.done && !hst.breakdown && hst.iter == queued) {
-    if (int st = launch_cg_flush_x<T>(s, N, queued - 1, o_p.ptr(), o_x, o_hist_alpha.ptr(), st_d, false))
-      return st;
+    if (int st = launch_cg_flush_x<T>(s, N, queued - 1, w.p, o_x, w.hist_alpha, st_d, false)) return st;
     BPX_HIP(hipStreamSynchronize(s));
   }
   if (iterations) *iterations = (MAXIT == 0) ? 0 : hst.iter;
@@ -199,6 +234,12 @@ int cg_t<T>::History(std::vector<double>& rr, std::vector<double>& alpha) {
 
 template class cg_t<double>;
 template class cg_t<float>;
+template int cg_run<double>(hipStream_t, dlong, const CgWork<double>&, const cg_t<double>::Apply&,
+                            const cg_t<double>::ApplyDot&, double*, double*, double, int, int*,
+                            const cg_t<double>::ApplyFold&, dlong);
+template int cg_run<float>(hipStream_t, dlong, const CgWork<float>&, const cg_t<float>::Apply&,
+                           const cg_t<float>::ApplyDot&, float*, float*, double, int, int*,
+                           const cg_t<float>::ApplyFold&, dlong);
 
 cg::cg(platform_t& p, dlong N, dlong Nhalo, int npart_max) { status_ = impl.Setup(p, N + Nhalo, npart_max); }
 
@@ -208,8 +249,14 @@ int cg::Solve(solver_t& solver, dfloat* o_x, dfloat* o_r, dfloat tol, int MAXIT,
   cg_t<double>::ApplyDot fused;
   if (solver.HasOperatorDot())
     fused = [&](const double* q, double* Aq, double* part, int* n) { return solver.OperatorDot(q, Aq, part, n); };
+  cg_t<double>::ApplyFold fold;
+  const dlong foldRows = solver.FoldRows();
+  if (foldRows > 0)
+    fold = [&](const CgFold<double>& f, double* Aq, double* part, int* n) {
+      return solver.OperatorDotFold(f, Aq, part, n);
+    };
   const int st = impl.Solve([&](const double* q, double* Aq) { return solver.Operator(q, Aq); }, fused, o_x, o_r,
-                            tol, MAXIT, &its);
+                            tol, MAXIT, &its, fold, foldRows);
   if (iterations) *iterations = its;
   if (verbose) {
     std::vector<double> rr, al;
@@ -496,10 +543,20 @@ int bp_t::PrepareFused() {
   return BPX_OK;
 }
 
+dlong bp_t::FoldRows(int precision) const {
+  if (!cgFold || !fused || !permuted) return 0;
+  if (precision == 64 ? !tuning.fused64 : !tuning.fused32) return 0;
+  // the FP64 operator at N = 15 runs on the matrix-core kernel, which keeps
+  // the separate update kernel
+  if (precision == 64 && mesh->N == 15 && axEngine != AX_ENGINE_VECTOR) return 0;
+  return fusedS0;
+}
+
 template <typename T>
-int bp_t::OperatorDotFused(const T* o_q, T* o_Aq, double* part, int* npart) {
+int bp_t::OperatorDotFused(const T* o_q, T* o_Aq, double* part, int* npart, const CgFold<T>* fold) {
   hipStream_t s = platform->stream;
   AxArgs<T> a;
+  if (fold) a.fold = *fold;
   a.N = mesh->N;
   a.Nelements = mesh->Nelements;
   a.Nfields = 1;
@@ -542,8 +599,8 @@ int bp_t::OperatorDotFused(const T* o_q, T* o_Aq, double* part, int* npart) {
   return launch_fold(s, o_partAx.ptr(), naxParts, part);
 }
 
-template int bp_t::OperatorDotFused<double>(const double*, double*, double*, int*);
-template int bp_t::OperatorDotFused<float>(const float*, float*, double*, int*);
+template int bp_t::OperatorDotFused<double>(const double*, double*, double*, int*, const CgFold<double>*);
+template int bp_t::OperatorDotFused<float>(const float*, float*, double*, int*, const CgFold<float>*);
 
 int bp_t::AxLocalKrhs(int K, const dfloat* o_q, dfloat* o_AqL, int chunk) {
   if (bp != 5) return set_error(BPX_ERR_UNSUPPORTED, "bp: k right-hand sides are implemented for BP5");
@@ -581,12 +638,14 @@ int bp_t::OperatorKrhs(int K, const dfloat* o_q, dfloat* o_Aq, int chunk) {
   return BPX_OK;
 }
 
-int bp_t::OperatorDot32(const float* o_q, float* o_Aq, double* part, int* npart) {
+int bp_t::OperatorDot32(const float* o_q, float* o_Aq, double* part, int* npart, const CgFold<float>* fold) {
   if (Nfields != 1) return set_error(BPX_ERR_UNSUPPORTED, "bp: the FP32 operator covers one field (BP1/3/5)");
   if (haveKappa) return set_error(BPX_ERR_UNSUPPORTED, "bp: the FP32 operator has no variable coefficient");
   if (int st = Prepare32()) return st;
+  if (fold && !(fused && tuning.fused32))
+    return set_error(BPX_ERR_INVALID_ARGUMENT, "bp: the folded CG update needs the fused FP32 operator");
   auto apply = [&]() {
-    return (fused && tuning.fused32) ? OperatorDotFused<float>(o_q, o_Aq, part, npart)
+    return (fused && tuning.fused32) ? OperatorDotFused<float>(o_q, o_Aq, part, npart, fold)
                                      : OperatorDotUnfused32(o_q, o_Aq, part, npart);
   };
   if (!opTimer32 || opTimer32->used + 3 > opTimer32->ev.size()) return apply();

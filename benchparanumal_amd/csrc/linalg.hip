@@ -195,7 +195,7 @@ __global__ __launch_bounds__(RT) void k_cg_update_p(dlong N, int k, const T* __r
                                                     CgState* st, double* __restrict__ hist_rr,
                                                     const double* __restrict__ hist_alpha, double tol, int pending) {
   __shared__ double s_w[33];
-  if (k > 0 && st->done) return;
+  if (k > 0 && (st->done || (st->stop_at != 0 && st->stop_at <= k))) return;
   typedef typename vec4<T>::type V;
   const V* r4 = reinterpret_cast<const V*>(r);
   V* p4 = reinterpret_cast<V*>(p);
@@ -229,11 +229,12 @@ __global__ __launch_bounds__(RT) void k_cg_update_p(dlong N, int k, const T* __r
       st->tol2 = tol2;
       st->breakdown = 0;
       st->done = 0;
+      st->stop_at = 0;
     }
     if (stop) {
-      st->done = 1;
       st->iter = k;
       if (!isfinite(rr)) st->breakdown = 1;
+      st->stop_at = k + 1;
     }
   }
   const bool apply = pending && k > 0;
@@ -292,7 +293,7 @@ __global__ __launch_bounds__(RT) void k_cg_update_r(dlong N, int k, const T* __r
                                                     const double* __restrict__ hist_rr,
                                                     double* __restrict__ hist_alpha) {
   __shared__ double s_w[33];
-  if (st->done) return;
+  if (st->done || st->stop_at) return;
   typedef typename vec4<T>::type V;
   const V* A4 = reinterpret_cast<const V*>(Ap);
   V* r4 = reinterpret_cast<V*>(r);
@@ -360,7 +361,7 @@ template <typename T>
 __global__ __launch_bounds__(ST) void k_cg_flush_x(dlong N, int k, const T* __restrict__ p, T* __restrict__ x,
                                                    const double* __restrict__ hist_alpha, const CgState* st,
                                                    int when_running) {
-  if (when_running && st->done) return;
+  if (when_running && (st->done || st->stop_at)) return;
   const T a = (T)hist_alpha[k];
   for (dlong n = blockIdx.x * ST + threadIdx.x; n < N; n += gridDim.x * ST) x[n] = a * p[n] + T(1) * x[n];
 }
@@ -370,6 +371,10 @@ __global__ __launch_bounds__(RT) void k_cg_finish(int kend, int maxit, const dou
                                                   int nrr, CgState* st, double* __restrict__ hist_rr) {
   __shared__ double s_w[33];
   if (st->done) return;
+  if (st->stop_at) {  // stopped inside the chunk: iter is already set
+    if (threadIdx.x == 0) st->done = 1;
+    return;
+  }
   const double rr = fold_partials(part_rr, nrr, s_w);
   if (threadIdx.x == 0) {
     hist_rr[kend] = rr;

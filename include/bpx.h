@@ -90,12 +90,15 @@ int bpx_bp5_ax_affine_hex(bpx_platform p, int N, int Nelements, const int* eleme
                           double* Aq);
 /* No reference counterpart (the reference applies Ax to one vector per
  * launch; okl/bp5AxHex3D.okl:415-427 is the single-vector form): BP5 Ax on K
- * right-hand sides at once, geometry streamed once. The right-hand sides are
- * stored one after another: q + r*ldq (global, indexed through
- * GlobalToLocal) and Aq + r*ldAq (local, e*Np + n), r = 0..K-1. chunk =
- * right-hand sides per register sweep (1, 2 or 4, dividing K; 0 =
- * default). Each right-hand side equals the single-vector bpx_bp5_ax_hex
- * bit for bit. */
+ * right-hand sides at once. The right-hand sides are stored one after
+ * another: q + r*ldq (global, indexed through GlobalToLocal) and Aq +
+ * r*ldAq (local, e*Np + n), r = 0..K-1. chunk = 0 (default): one workgroup
+ * per element and group of right-hand sides, whose sheets share each slab
+ * of geometric factors through LDS (non-affine; affine meshes use the
+ * sweep form with its default chunk); chunk = 1, 2 or 4 (dividing K): each
+ * element swept K/chunk times with chunk right-hand sides in registers.
+ * Each right-hand side equals the single-vector bpx_bp5_ax_hex bit for bit
+ * (to rounding at N = 15, where that one runs on the matrix cores). */
 int bpx_bp5_ax_hex_krhs(bpx_platform p, int N, int K, int chunk, int Nelements,
                         const int* elementList, const int* GlobalToLocal, const double* wJ,
                         const double* vgeo, const double* ggeo, const double* gllw,
@@ -224,6 +227,7 @@ int bpx_cg_destroy(bpx_cg c);
 int bpx_cg_solve(bpx_cg c, bpx_operator_fn op, void* ctx, double* o_x, double* o_r, double tol,
                  int maxit, int verbose, int* iterations, double* rnorm_hist);
 
+
 /* ---------------------------------------------------------------- mesh --
  * mesh_t(platform, settings, comm) -> Setup for a hex box
  * (libs/mesh/meshSetup.cpp:31-85, meshSetupBoxHex3D.cpp:31-131).
@@ -312,6 +316,12 @@ int bpx_bp_set_fp32_geometry(bpx_bp b, int mode);
  * Covers the plain and the fused (CG) form; the two engines agree to
  * rounding, not bit for bit. Other problems run on the vector ALU. */
 int bpx_bp_set_ax_engine(bpx_bp b, int engine);
+/* No reference counterpart (launch choice, same results): with on = 1
+ * (the default) the CG solves that use the fused BP5 operator do the
+ * direction update of the elements' interior rows (x += alpha p, p = r +
+ * beta p) inside that operator's Ax, and the separate update kernel covers
+ * the shared rows only; on = 0 keeps one update kernel over all rows. */
+int bpx_bp_set_cg_fold(bpx_bp b, int on);
 /* No reference counterpart (measurement): the CG operator as the solvers
  * apply it (their row order, the tuned fused or unfused form) in FP64
  * (precision 64) or FP32 (32), `launches` times back to back after 3

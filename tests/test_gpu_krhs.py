@@ -75,7 +75,10 @@ def single_kernel(plat, P, arr, qk, K, affine, lam):
 
 @pytest.mark.parametrize("affine", [False, True])
 @pytest.mark.parametrize("N,K,chunk", [(7, 1, 1), (7, 2, 2), (7, 4, 4), (7, 8, 2), (7, 8, 4), (7, 3, 1),
-                                       (3, 4, 2), (4, 6, 2), (10, 4, 4), (15, 2, 2)])
+                                       (3, 4, 2), (4, 6, 2), (10, 4, 4), (15, 2, 2),
+                                       # chunk 0: the workgroup form (factors shared by its sheets)
+                                       (7, 1, 0), (7, 3, 0), (7, 9, 0), (1, 5, 0), (4, 6, 0), (9, 4, 0),
+                                       (10, 8, 0), (12, 3, 0), (15, 2, 0), (15, 5, 0)])
 def test_krhs_ax_local_equals_single(plat, N, K, chunk, affine):
     """Kernel entry: every right-hand side equals the single-vector kernel on
     the same arrays to rounding (the same fused multiply-adds in the same

@@ -319,6 +319,38 @@ def test_fp32_fused_operator_solves_identically(plat, N):
     np.testing.assert_array_equal(res[0][1], res[1][1])
 
 
+@pytest.mark.parametrize("N,nx", [(2, 4), (3, 3), (7, 3), (8, 2), (15, 2)])
+def test_cg_fold_bit_identical(plat, N, nx):
+    """The CG solves with the interior rows' direction update folded into
+    the fused operator (the default) and with one update kernel over all
+    rows: the same iterations, residual histories and solutions, bit for
+    bit, for the FP64 CG stopped by its tolerance and for the mixed solve.
+    The solution of a solve stopped by its tolerance carries every row's
+    last update: its true residual is the recurrence's last one."""
+    m = capi.Mesh(N=N, nx=nx, perturb=0.1, seed=6)
+    b = capi.BP(plat, m, 5, lam=1.0)
+    b.set_tuning(fused=1, fused32=1)
+    rhs = capi.random_normal(12, b.Ngather)
+    out = {}
+    for fold in (1, 0):
+        b.set_cg_fold(fold)
+        x = torch.zeros(b.Ngather, dtype=torch.float64, device="cuda")
+        its, hist = b.solve(x, dev(rhs), tol=1e-8, maxit=5000)
+        assert 0 < its < 5000
+        Ax = torch.zeros_like(x)
+        b.operator(x, Ax)
+        true_res = float(np.linalg.norm(rhs - host(Ax)))
+        assert abs(true_res - hist[its]) <= 1e-4 * hist[its], (true_res, hist[its])
+        xm = torch.zeros_like(x)
+        st = b.solve_mixed(xm, dev(rhs), tol=1e-10, interval=16)
+        out[fold] = (its, hist, host(x), st["iterations"], st["updates"], host(xm))
+    b.set_cg_fold(1)
+    assert out[1][0] == out[0][0] and out[1][3] == out[0][3] and out[1][4] == out[0][4]
+    np.testing.assert_array_equal(out[1][1], out[0][1])
+    np.testing.assert_array_equal(out[1][2], out[0][2])
+    np.testing.assert_array_equal(out[1][5], out[0][5])
+
+
 def test_time_operator(plat):
     """bpx_bp_time_operator runs the CG operator as tuned and times its two
     kernels; pinning the form changes which one it reports."""
