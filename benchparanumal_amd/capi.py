@@ -68,6 +68,7 @@ _SIGS = {
     "bpx_cg_create": (_I, [_P, _I, _I, C.POINTER(_P)]),
     "bpx_cg_destroy": (_I, [_P]),
     "bpx_cg_solve": (_I, [_P, _P, _P, _P, _P, _D, _I, _I, C.POINTER(_I), _P]),
+    "bpx_cg_solve_noalloc": (_I, [_P, _I, _P, _P, _P, _P, _D, _I, C.POINTER(_I), _P]),
     "bpx_bp_create": (_I, [_P, _P, _I, _D, C.POINTER(_P)]),
     "bpx_bp_destroy": (_I, [_P]),
     "bpx_bp_info": (_I, [_P, _I64P, _I]),

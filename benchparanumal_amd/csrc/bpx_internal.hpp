@@ -5,6 +5,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <mutex>
 #include <string>
 
 #include "bpx.h"
@@ -288,6 +289,26 @@ struct CgState {
   int stop_at;
 };
 
+// Work buffers of the allocation-free CG solve, carved from the file-scope
+// device array of cg_noalloc.hip (sizes: kNoAllocMaxN for p and Ap,
+// kNoAllocMaxIter + 1 for each history, kRedBlocks for each partials array).
+constexpr long long kNoAllocMaxN = BPX_CG_NOALLOC_MAX_N;
+constexpr int kNoAllocMaxIter = BPX_CG_NOALLOC_MAX_ITER;
+struct NoAllocWork {
+  double* p = nullptr;
+  double* Ap = nullptr;
+  double* hist_rr = nullptr;
+  double* hist_alpha = nullptr;
+  double* part_rr = nullptr;
+  double* part_pAp = nullptr;
+  CgState* state = nullptr;
+  int state_doubles = 0;   // the state region in doubles (cleared per solve)
+  std::mutex* lock = nullptr;  // serialises this process's solves on the device
+};
+// the device of stream s (the current device for the null stream) and that
+// device's work buffers
+int noalloc_work(hipStream_t s, int* device, NoAllocWork* w);
+int noalloc_scratch_bytes(size_t* bytes);
 
 // CG steps, templated on the vector type (double: plain CG; float: the
 // inner solve of the mixed-precision path). Iteration k is passed by the

@@ -436,6 +436,67 @@ int bpx_cg_solve(bpx_cg c, bpx_operator_fn op, void* ctx, double* o_x, double* o
   BPX_GUARD_END
 }
 
+// The solve of bpx_cg_solve on the fixed work array of cg_noalloc.hip: the
+// same kernels and host loop (cg_run), with no allocation of any kind.
+static thread_local bool t_in_noalloc_solve = false;
+
+int bpx_cg_solve_noalloc(void* hip_stream, int N, bpx_operator_fn op, void* ctx, double* o_x, double* o_r,
+                         double tol, int maxit, int* iterations, double* rnorm_hist) {
+  BPX_GUARD_BEGIN
+  if (!op) return set_error(BPX_ERR_INVALID_ARGUMENT, "cg (no-alloc): operator callback is NULL");
+  if (N < 0 || (long long)N > kNoAllocMaxN)
+    return set_error(BPX_ERR_INVALID_ARGUMENT,
+                     "cg (no-alloc): N must be in 0.." + std::to_string(kNoAllocMaxN) + " (the fixed work array)");
+  if (maxit < 0 || maxit > kNoAllocMaxIter)
+    return set_error(BPX_ERR_INVALID_ARGUMENT,
+                     "cg (no-alloc): maxit must be in 0.." + std::to_string(kNoAllocMaxIter) + " (the fixed work array)");
+  if (N > 0 && (!o_x || !o_r)) return set_error(BPX_ERR_INVALID_ARGUMENT, "cg (no-alloc): o_x or o_r is NULL");
+  if (t_in_noalloc_solve)
+    return set_error(BPX_ERR_INVALID_ARGUMENT, "cg (no-alloc): called from inside the operator callback of a no-alloc solve");
+  hipStream_t s = static_cast<hipStream_t>(hip_stream);
+  int dev = -1;
+  NoAllocWork nw;
+  if (int st = noalloc_work(s, &dev, &nw)) return st;
+  std::lock_guard<std::mutex> lock(*nw.lock);
+  t_in_noalloc_solve = true;
+  struct Reset {
+    ~Reset() { t_in_noalloc_solve = false; }
+  } reset;
+  // the array keeps the previous call's contents: clear the state, p and Ap
+  // with fill kernels (no memset nodes, so the queue can also be captured)
+  if (int st = launch_set<double>(s, nw.state_doubles, 0.0, reinterpret_cast<double*>(nw.state))) return st;
+  if (int st = launch_set<double>(s, N, 0.0, nw.p)) return st;
+  if (int st = launch_set<double>(s, N, 0.0, nw.Ap)) return st;
+  CgWork<double> w;
+  w.p = nw.p;
+  w.Ap = nw.Ap;
+  w.state = nw.state;
+  w.part_rr = nw.part_rr;
+  w.part_pAp = nw.part_pAp;
+  w.hist_rr = nw.hist_rr;
+  w.hist_alpha = nw.hist_alpha;
+  struct Callback {
+    bpx_operator_fn fn;
+    void* ctx;
+  } cb{op, ctx};
+  const Callback* pcb = &cb;  // one pointer: the std::function keeps it in place, without a heap block
+  const cg_t<double>::Apply apply = [pcb](const double* q, double* Aq) {
+    const int st = pcb->fn(pcb->ctx, q, Aq);
+    return st == 0 ? BPX_OK : set_error(st < 0 ? st : BPX_ERR_INVALID_ARGUMENT, "cg: the operator callback failed");
+  };
+  int its = 0;
+  const int st = cg_run<double>(s, N, w, apply, cg_t<double>::ApplyDot(), o_x, o_r, tol, maxit, &its);
+  if (iterations) *iterations = its;
+  if (rnorm_hist && st == BPX_OK) {
+    const int nh = (maxit == 0) ? 1 : its + 1;
+    BPX_HIP(hipMemcpyAsync(rnorm_hist, w.hist_rr, nh * sizeof(double), hipMemcpyDeviceToHost, s));
+    BPX_HIP(hipStreamSynchronize(s));
+    for (int k = 0; k <= maxit; ++k) rnorm_hist[k] = (k < nh) ? std::sqrt(rnorm_hist[k]) : 0.0;
+  }
+  return st;
+  BPX_GUARD_END
+}
+
 // ---------------------------------------------------------------- mesh --
 int bpx_mesh_create(const bpx_mesh_opts* o, bpx_mesh* out) {
   BPX_GUARD_BEGIN

@@ -227,6 +227,25 @@ int bpx_cg_destroy(bpx_cg c);
 int bpx_cg_solve(bpx_cg c, bpx_operator_fn op, void* ctx, double* o_x, double* o_r, double tol,
                  int maxit, int verbose, int* iterations, double* rnorm_hist);
 
+/* bpx_cg_solve without a bpx_cg object, allocating nothing. No counterpart
+ * in the reference: it is for callers that cannot allocate device memory
+ * (code under HIP stream capture, tight serving loops, harnesses that
+ * forbid hipMalloc). The work vectors, partial sums, state and histories
+ * come from a fixed device array of about 33 MiB that each GPU the library
+ * runs on holds until the process ends; nothing is allocated (no hipMalloc,
+ * hipFree or pinned memory, no host heap). Same arguments and results as
+ * bpx_cg_solve (verbose aside), with the work queued on hip_stream (NULL:
+ * the null stream of the current device) and the operator callback
+ * receiving device vectors of N entries that it must fill on that stream.
+ * Limits: 0 <= N <= BPX_CG_NOALLOC_MAX_N and 0 <= maxit <=
+ * BPX_CG_NOALLOC_MAX_ITER; larger values return BPX_ERR_INVALID_ARGUMENT.
+ * Every call on one GPU shares that GPU's array: the calls of one process
+ * are serialised per GPU (the call returns once the solve is done), and a
+ * call made from inside the operator callback of another is refused. */
+#define BPX_CG_NOALLOC_MAX_N 2097152
+#define BPX_CG_NOALLOC_MAX_ITER 65535
+int bpx_cg_solve_noalloc(void* hip_stream, int N, bpx_operator_fn op, void* ctx, double* o_x,
+                         double* o_r, double tol, int maxit, int* iterations, double* rnorm_hist);
 
 /* ---------------------------------------------------------------- mesh --
  * mesh_t(platform, settings, comm) -> Setup for a hex box
