@@ -234,11 +234,34 @@ def bench_batch(capi, plat):
     seq()
     torch.cuda.synchronize()
     ts = time.perf_counter() - t0
+    # FP64 batch and FP32+refinement batch at 1e-10 (interval 32), and the
+    # single-problem mixed solve one problem after another
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, out
+    t64, (its64, _) = timed(lambda: B.solve(x, rhs, tol=1e-10, maxit=20000))
+    tmx, (itsm, upsm, relm) = timed(lambda: B.solve_mixed(x, rhs, tol=1e-10, interval=32, maxit=20000))
+    def seq_mixed():
+        for i, (m, bp) in enumerate(bps):
+            a, b = int(B.offsets[i]), int(B.offsets[i + 1])
+            xs = torch.zeros(b - a, dtype=torch.float64, device="cuda")
+            bp.solve_mixed(xs, rhs[a:b].contiguous(), tol=1e-10, interval=32)
+    tsm, _ = timed(seq_mixed)
+    mixed = {"workload": "the same 64 problems to 1e-10 true relative residual",
+             "fp64_batch_ms": t64 * 1e3, "fp64_iterations_max": int(its64.max()),
+             "ms_per_solve": tmx * 1e3, "fp32_iterations_min": int(itsm.min()),
+             "fp32_iterations_max": int(itsm.max()), "corrections_max": int(upsm.max()),
+             "true_rel_res_max": float(relm.max()), "speedup_vs_fp64_batch": t64 / tmx,
+             "sequential_mixed_ms": tsm * 1e3, "speedup_vs_sequential_mixed": tsm / tmx}
     return {"workload": "64 BP5 problems, nx/ny/nz in [2,8], p in 3..8, N(0,1) rhs, CG to 1e-8",
             "Ntot": B.Ntot, "iterations_min": int(its.min()), "iterations_max": int(its.max()),
             "ms_per_solve": tb * 1e3, "gdof_it_per_s": work / tb / 1e9,
             "sequential_ms": ts * 1e3, "speedup_vs_sequential": ts / tb, "tuning": B.tuning(),
-            "graph_replays": B.info()["replays"]}
+            "graph_replays": B.info()["replays"], "fp32_refinement": mixed}
 
 
 def bench_p15(capi, plat):
@@ -511,9 +534,15 @@ def main():
     ap.add_argument("--interval", type=int, default=64,
                     help="FP32 iterations between FP64 defect corrections")
     ap.add_argument("--ax-launches", type=int, default=50)
-    ap.add_argument("--cg-fold", type=int, default=1, choices=(0, 1),
-                    help="1: the fused CG operator also does the interior rows' direction update "
-                         "(the default); 0: one update kernel over all rows (same bits)")
+    ap.add_argument("--cg-fold", type=int, default=1, choices=(0, 1, 2, 3),
+                    help="where the fused CG operator also does the interior rows' direction update: "
+                         "bit 0 = FP64 solves (default 1), bit 1 = FP32 iterations (same bits)")
+    ap.add_argument("--setup", choices=("device", "host"), default="device",
+                    help="where the mesh, ogs maps and solve-space maps are built (same arrays either way)")
+    ap.add_argument("--no-setup-compare", action="store_true",
+                    help="skip timing the other setup of the same problem")
+    ap.add_argument("--setup-only", action="store_true",
+                    help="only time the device and the host setup of the problem (mesh, then BP5 problem)")
     ap.add_argument("--cpu-its", type=int, default=1000)
     ap.add_argument("--cpu-serial-its", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -554,12 +583,36 @@ def main():
             torch.cuda.empty_cache()
         return
 
-    t0 = time.perf_counter()
-    mesh = capi.Mesh(N=args.N, nx=args.nx, perturb=0.1, seed=1)
-    bp = capi.BP(plat, mesh, 5, lam=1.0)
+    def build(where):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        m = capi.Mesh(N=args.N, nx=args.nx, perturb=0.1, seed=1, platform=plat if where == "device" else None)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        b = capi.BP(plat, m, 5, lam=1.0)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        log(f"{where} setup: mesh {t1 - t0:.2f}s, BP5 problem {t2 - t1:.2f}s (of which the launch-choice "
+            f"timing is the same either way)")
+        return m, b, {"mesh_s": t1 - t0, "problem_s": t2 - t1, "total_s": t2 - t0}
+
+    setup = {"workload": f"BP5 hex p={args.N} {args.nx}^3 perturbed box: mesh (nodes, ids, geometric "
+                         f"factors), then the problem (masked ids, ogs T maps, GlobalToLocal, fused "
+                         f"operator's solve-space maps, operator data, launch-choice timing)"}
+    mesh, bp, setup[args.setup] = build(args.setup)
+    other = "host" if args.setup == "device" else "device"
+    if args.setup_only or not args.no_setup_compare:
+        mo, bo, setup[other] = build(other)
+        del mo, bo
+        torch.cuda.empty_cache()
+        setup["host_over_device"] = setup["host"]["total_s"] / setup["device"]["total_s"]
+        setup["host_over_device_without_problem"] = setup["host"]["mesh_s"] / setup["device"]["mesh_s"]
+    if args.setup_only:
+        print(json.dumps({"setup": setup}), flush=True)
+        return
     bp.set_cg_fold(args.cg_fold)
     Ndofs, Nel, Np = bp.Ngather, mesh.Nelements, mesh.Np
-    log(f"setup: {time.perf_counter() - t0:.1f}s  Ndofs={Ndofs}  Nel={Nel}  p={args.N}")
+    log(f"setup ({args.setup}): {setup[args.setup]['total_s']:.1f}s  Ndofs={Ndofs}  Nel={Nel}  p={args.N}")
 
     if args.ax_only:
         # counter calibration: one device-to-device copy of exactly 512 MiB
@@ -758,6 +811,7 @@ def main():
                      "gdof_per_s": Ndofs / t_ax / 1e9,
                      "gflops": bp5_ax_flops(args.N + 1, Nel) / t_ax / 1e9},
         "tuning": bp.tuning(),
+        "setup": setup,
         "cpu_baseline": cpu,
         "extensions": ext,
     }

@@ -54,10 +54,12 @@ _SIGS = {
     "bpx_linalg_norm2": (_I, [_P, _I, _P, C.POINTER(_D)]),
     "bpx_cg_update": (_I, [_P, _I, _P, _P, _D, _P, _P, C.POINTER(_D)]),
     "bpx_mesh_create": (_I, [_P, C.POINTER(_P)]),
+    "bpx_mesh_create_dev": (_I, [_P, _P, C.POINTER(_P)]),
     "bpx_mesh_destroy": (_I, [_P]),
     "bpx_mesh_info": (_I, [_P, _I64P, _I]),
     "bpx_mesh_array": (_I, [_P, C.c_char_p, _P, _I64P, C.POINTER(_I)]),
     "bpx_ogs_setup": (_I, [_I, _P, C.POINTER(_P)]),
+    "bpx_ogs_setup_dev": (_I, [_P, _I, _P, C.POINTER(_P)]),
     "bpx_ogs_setup_masked": (_I, [_P, _I, C.POINTER(_P)]),
     "bpx_ogs_destroy": (_I, [_P]),
     "bpx_ogs_info": (_I, [_P, _I64P, _I]),
@@ -77,6 +79,7 @@ _SIGS = {
     "bpx_bp_operator_f32": (_I, [_P, _P, _P]),
     "bpx_bp_operator_krhs": (_I, [_P, _I, _I, _P, _P]),
     "bpx_bp_tuning": (_I, [_P, _P, _I]),
+    "bpx_bp_array": (_I, [_P, C.c_char_p, _P, _I64P, C.POINTER(_I)]),
     "bpx_bp_operator_dot": (_I, [_P, _P, _P, C.POINTER(_D)]),
     "bpx_bp_set_tuning": (_I, [_P, _I, _I, _I]),
     "bpx_bp_time_operator": (_I, [_P, _I, _I, _P]),
@@ -91,6 +94,7 @@ _SIGS = {
     "bpx_batch_offsets": (_I, [_P, _I64P]),
     "bpx_batch_tuning": (_I, [_P, _P, _I]),
     "bpx_batch_solve": (_I, [_P, _P, _P, _D, _I, _P, _P]),
+    "bpx_batch_solve_mixed": (_I, [_P, _P, _P, _D, _I, _I, _P, _P, _P]),
     "bpx_bp_kappa_gradient": (_I, [_P, _P, _P, _P]),
     "bpx_bp_rhs_forcing": (_I, [_P, _P]),
     "bpx_mesh_plot_fields": (_I, [_P, _I, _P, C.c_char_p]),
@@ -201,10 +205,11 @@ def _read_array(fn, h, name):
 
 
 class Mesh:
-    """mesh_t for the hex box (host data)."""
+    """mesh_t for the hex box: set up on the host, or on the device of
+    `platform` when one is given (the same arrays, copied down on request)."""
 
     def __init__(self, N, nx, ny=None, nz=None, affine=False, cubature=False, lo=(-1.0, -1.0, -1.0),
-                 dim=(2.0, 2.0, 2.0), perturb=0.0, seed=0):
+                 dim=(2.0, 2.0, 2.0), perturb=0.0, seed=0, platform=None):
         o = _MeshOpts()
         o.N, o.nx = N, nx
         o.ny = nx if ny is None else ny
@@ -214,8 +219,12 @@ class Mesh:
             o.lo[c], o.dim[c] = lo[c], dim[c]
         o.perturb, o.seed = perturb, seed
         h = _P()
-        check(lib().bpx_mesh_create(C.byref(o), C.byref(h)))
+        if platform is not None:
+            check(lib().bpx_mesh_create_dev(platform.h, C.byref(o), C.byref(h)))
+        else:
+            check(lib().bpx_mesh_create(C.byref(o), C.byref(h)))
         self.h = h
+        self.platform = platform
         buf = (C.c_int64 * 8)()
         n = lib().bpx_mesh_info(self.h, buf, 8)
         keys = ["N", "Nq", "Np", "cubNq", "cubNp", "Nelements"]
@@ -238,12 +247,16 @@ class Mesh:
 
 
 class Ogs:
-    """ogs::ogs_t T maps built on the host."""
+    """ogs::ogs_t T maps: built on the host from host ids, on the device from
+    device ids (`platform` + a device tensor `ids`), or for a mesh where the
+    mesh was set up."""
 
-    def __init__(self, ids=None, mesh=None, nfields=1):
+    def __init__(self, ids=None, mesh=None, nfields=1, platform=None):
         h = _P()
         if mesh is not None:
             check(lib().bpx_ogs_setup_masked(mesh.h, nfields, C.byref(h)))
+        elif platform is not None:
+            check(lib().bpx_ogs_setup_dev(platform.h, int(ids.numel()), ptr(ids), C.byref(h)))
         else:
             ids = np.ascontiguousarray(ids, dtype=np.int64)
             check(lib().bpx_ogs_setup(len(ids), ids.ctypes.data, C.byref(h)))
@@ -328,10 +341,16 @@ class BP:
         return {"ax_us": float(out[0]), "gather_us": float(out[1]), "launches": int(out[2]),
                 "fused": bool(out[3])}
 
-    def set_cg_fold(self, on):
-        """1 (default): the fused CG operator also does the interior rows'
-        direction update; 0: one update kernel over all rows (same bits)."""
-        check(lib().bpx_bp_set_cg_fold(self.h, int(on)))
+    def array(self, name):
+        """A device array of the problem copied to the host (bpx_bp_array)."""
+        return _read_array(lib().bpx_bp_array, self.h, name)
+
+    def set_cg_fold(self, mask):
+        """Where the fused CG operator also does the interior rows' direction
+        update: mask bit 0 = FP64 solves (the default, mask 1), bit 1 = the
+        FP32 iterations of the mixed solve; elsewhere one update kernel covers
+        all rows. Same bits either way."""
+        check(lib().bpx_bp_set_cg_fold(self.h, int(mask)))
 
     def set_ax_engine(self, engine):
         """BP5 at N = 15, FP64: 0 = auto (matrix cores), 1 = vector-ALU Ax,
@@ -435,6 +454,16 @@ class Batch:
         rn = np.zeros(self.nprob)
         check(lib().bpx_batch_solve(self.h, ptr(x), ptr(b), tol, maxit, its.ctypes.data_as(_P), ptr(rn)))
         return its, rn
+
+    def solve_mixed(self, x, b, tol=1e-10, interval=32, maxit=100000):
+        """FP32 CG with FP64 corrections for every problem; returns per problem
+        (FP32 iterations, corrections, final ||b - A x|| / ||b||)."""
+        its = np.zeros(self.nprob, dtype=np.int32)
+        ups = np.zeros(self.nprob, dtype=np.int32)
+        rel = np.zeros(self.nprob)
+        check(lib().bpx_batch_solve_mixed(self.h, ptr(x), ptr(b), tol, interval, maxit, its.ctypes.data_as(_P),
+                                          ups.ctypes.data_as(_P), ptr(rel)))
+        return its, ups, rel
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:

@@ -180,11 +180,13 @@ __global__ __launch_bounds__(NQ* NQ* EPB) __attribute__((amdgpu_waves_per_eu(WAV
   // the LDS reads become the bound: D rows/columns in registers and the
   // broadcast D[k][m] from scalar (constant address space) loads
   // (measured Ax 44.1 -> 38.2 -> 36.5 us, the mixed solve 111.1 -> 101.6 ms).
-  // Both only at Nq = 8 and 16, where the FP32 operator was measured
-  // against the FP64 one at FP32 rounding. With D in registers the FP32
-  // operator came out 17-47% off at Nq = 5, 6, 7, 9, 10 and 12 (right at
-  // 2, 3, 4, 8, 16); the cause is not established, so those Nq keep D in
-  // LDS, the form that is right at every Nq.
+  // Both only at Nq = 8 and 16. With D in registers and D[k][m] from LDS
+  // the FP32 operator came out 20-42% off at Nq = 5..15 except 8 (right at
+  // 2, 3, 4): the compiler's IR load/store vectorizer merges the load of
+  // s_D[i][1] with the lane's column load s_D[0][i] as if the two were
+  // adjacent (they are only for i = 0), so lanes i > 0 use a wrong D entry;
+  // the same code built with that pass off is right at every Nq. Those Nq
+  // keep D in LDS, which the pass leaves intact.
   constexpr bool DREG = (TRI && NQ % 8 == 0) || bp5_dreg<T, NQ, NF, VAR>();
   constexpr bool SDK = TRI && NQ % 8 == 0;
   typedef const __attribute__((address_space(4))) T* cptr;
@@ -1009,6 +1011,25 @@ int launch_bp5_batch_nq(hipStream_t s, int variant, int Nelements, const int* el
   return set_error(BPX_ERR_INVALID_ARGUMENT, "batch Ax: variant not available for this degree");
 }
 
+// FP32 batch Ax (the mixed batch's inner operator): the default shape only
+template <int NQ>
+int launch_bp5_batch_f32_nq(hipStream_t s, int Nelements, const int* elementList, const int* G2L, const float* wJ,
+                            const float* ggeo, const float* D, float lambda, const float* q, float* Aq,
+                            const int* elemPid, const BatchState* done) {
+  constexpr int EPB = bp5_epb<NQ>();
+  constexpr int WV = bp5_waves<NQ, 1, false>();
+  const int blocks = (Nelements + EPB - 1) / EPB;
+  if (blocks == 0) return BPX_OK;
+  hipLaunchKernelGGL((k_bp5_ax<float, NQ, 1, EPB, false, WV, VAR_BATCH>), dim3(blocks), dim3(NQ * NQ * EPB), 0, s,
+                     Nelements, 1, elementList, G2L, wJ, ggeo, (const float*)nullptr, D, lambda, q, Aq, 0LL, 0LL,
+                     elemPid, done, (float*)nullptr, (double*)nullptr, CgFold<float>());
+  BPX_HIP(hipGetLastError());
+  return BPX_OK;
+}
+
+template int launch_bp5_batch_f32_nq<BPX_NQ>(hipStream_t, int, const int*, const int*, const float*, const float*,
+                                             const float*, float, const float*, float*, const int*,
+                                             const BatchState*);
 template int launch_bp5_batch_nq<BPX_NQ>(hipStream_t, int, int, const int*, const int*, const double*,
                                          const double*, const double*, double, const double*, double*, const int*,
                                          const BatchState*);

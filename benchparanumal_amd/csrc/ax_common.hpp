@@ -15,6 +15,10 @@ int launch_bp5_batch_nq(hipStream_t s, int variant, int Nelements, const int* el
                         const double* wJ, const double* ggeo, const double* D, double lambda, const double* q,
                         double* Aq, const int* elemPid, const BatchState* done);
 template <int NQ>
+int launch_bp5_batch_f32_nq(hipStream_t s, int Nelements, const int* elementList, const int* G2L, const float* wJ,
+                            const float* ggeo, const float* D, float lambda, const float* q, float* Aq,
+                            const int* elemPid, const BatchState* done);
+template <int NQ>
 int launch_bp5_kappa_grad_nq(hipStream_t s, int Nelements, int affine, const int* G2L, const double* ggeo,
                              const double* gllw, const double* D, const double* u, const double* lam, double* gL);
 template <typename T, int NQ, int NF> int launch_bp1_nq(hipStream_t s, const AxArgs<T>& a);

@@ -308,10 +308,32 @@ __global__ __launch_bounds__(MQ* MQ* EPB) __attribute__((amdgpu_waves_per_eu(WAV
 // are requested at the start. The tiles take 9.8 KB at NQ = 8 (16
 // wavefronts per CU). This replaces the sheet kernel, whose FP64 form
 // spilled at 2 waves/SIMD and held 3 elements per CU at 1.
-template <typename T, int NQ, int NF>
-__global__ __launch_bounds__(64) void k_bp1_wave(const int* __restrict__ elementList, const int* __restrict__ G2L,
-                                                 const T* __restrict__ W, const T* __restrict__ I,
-                                                 const T* __restrict__ q, T* __restrict__ Aq) {
+// FPAR (NF > 1): one workgroup per field instead of the fields one after
+// another; the NF workgroups of an element are consecutive on one XCD
+// (wave_element), so the second and third read its factors from that L2.
+// The same operations per field either way.
+template <int NF, bool FPAR>
+__device__ inline int wave_element(int Nelements, int& f0, int& f1) {
+  f0 = 0;
+  f1 = NF;
+  if (!FPAR) return blockIdx.x;
+  const int b = blockIdx.x, r = b >> 3;
+  f0 = r % NF;
+  f1 = f0 + 1;
+  const int eo = (r / NF) * 8 + (b & 7);
+  return eo < Nelements ? eo : -1;
+}
+
+template <int NF, bool FPAR>
+inline unsigned wave_blocks(int Nelements) {
+  return FPAR ? (unsigned)(8 * NF * ((Nelements + 7) / 8)) : (unsigned)Nelements;
+}
+
+template <typename T, int NQ, int NF, bool FPAR>
+__global__ __launch_bounds__(64) void k_bp1_wave(int Nelements, const int* __restrict__ elementList,
+                                                 const int* __restrict__ G2L, const T* __restrict__ W,
+                                                 const T* __restrict__ I, const T* __restrict__ q,
+                                                 T* __restrict__ Aq) {
   constexpr int MQ = NQ + 1;
   constexpr int NQ2 = NQ * NQ;
   constexpr int NP = NQ2 * NQ;
@@ -325,7 +347,10 @@ __global__ __launch_bounds__(64) void k_bp1_wave(const int* __restrict__ element
   typedef const __attribute__((address_space(4))) T* cptr;
   const cptr Ic = (cptr)I;  // I[g][c] = Ic[g * NQ + c]
   const int l = threadIdx.x;
-  const int e = elementList[blockIdx.x];
+  int f0, f1;
+  const int eo = wave_element<NF, FPAR>(Nelements, f0, f1);
+  if (eo < 0) return;  // uniform over the workgroup
+  const int e = elementList[eo];
 
   // cubwJ of the lane's cubature columns (b, a) = col, all g
   T wv[C3][MQ];
@@ -336,7 +361,7 @@ __global__ __launch_bounds__(64) void k_bp1_wave(const int* __restrict__ element
     for (int g = 0; g < MQ; ++g) wv[c][g] = W[(size_t)e * MP + g * MQ2 + col];
   }
 
-  for (int f = 0; f < NF; ++f) {
+  for (int f = f0; f < f1; ++f) {
     const size_t lb = (size_t)e * NP * NF + (size_t)f * NP;
     // rows (k, j) = row: gather q, interpolate along i
     if (l < NQ2) {
@@ -456,11 +481,12 @@ __global__ __launch_bounds__(64) void k_bp1_wave(const int* __restrict__ element
 //   rows (k,j):      back along i                         -> Aq
 // I and the t-direction D rows are uniform over the wavefront (scalar
 // loads); the lane's own D rows and columns come from global memory.
-template <typename T, int NQ, int NF>
-__global__ __launch_bounds__(128) void k_bp3_wave(const int* __restrict__ elementList, const int* __restrict__ G2L,
-                                                 const T* __restrict__ W, const T* __restrict__ G,
-                                                 const T* __restrict__ Dq, const T* __restrict__ I, T lambda,
-                                                 const T* __restrict__ q, T* __restrict__ Aq) {
+template <typename T, int NQ, int NF, bool FPAR>
+__global__ __launch_bounds__(128) void k_bp3_wave(int Nelements, const int* __restrict__ elementList,
+                                                 const int* __restrict__ G2L, const T* __restrict__ W,
+                                                 const T* __restrict__ G, const T* __restrict__ Dq,
+                                                 const T* __restrict__ I, T lambda, const T* __restrict__ q,
+                                                 T* __restrict__ Aq) {
   constexpr int MQ = NQ + 1;
   constexpr int NQ2 = NQ * NQ;
   constexpr int NP = NQ2 * NQ;
@@ -474,9 +500,12 @@ __global__ __launch_bounds__(128) void k_bp3_wave(const int* __restrict__ elemen
   const cptr Ic = (cptr)I;   // I[g][c] = Ic[g * NQ + c]
   const cptr Dc = (cptr)Dq;  // D[r][c] = Dc[r * MQ + c]
   const int l = threadIdx.x;
-  const int e = elementList[blockIdx.x];
+  int f0, f1;
+  const int eo = wave_element<NF, FPAR>(Nelements, f0, f1);
+  if (eo < 0) return;  // uniform over the workgroup
+  const int e = elementList[eo];
 
-  for (int f = 0; f < NF; ++f) {
+  for (int f = f0; f < f1; ++f) {
     const size_t lb = (size_t)e * NP * NF + (size_t)f * NP;
     // rows (k, j): gather q, along i -> x in tr
     if (l < NQ2) {
@@ -657,6 +686,12 @@ __global__ __launch_bounds__(128) void k_bp3_wave(const int* __restrict__ elemen
   }
 }
 
+// the 3-field wavefront forms split their fields over workgroups (set to 0
+// to run them one after another in one workgroup; same results)
+#ifndef BPX_FIELD_SPLIT
+#define BPX_FIELD_SPLIT 1
+#endif
+
 template <typename T, int NQ, int MQ, int NF, int MODE>
 int launch_mode(hipStream_t s, const AxArgs<T>& x, const T* W, const T* G, const T* Dq, const T* I) {
   constexpr int WV = cub_waves<T, MQ, NF, MODE>();
@@ -674,8 +709,9 @@ template <typename T, int NQ, int NF>
 int launch_bp1_nq(hipStream_t s, const AxArgs<T>& a) {
   if (a.affine) return launch_mode<T, NQ, NQ, NF, MODE_BP1A>(s, a, a.wJ, nullptr, nullptr, a.MM);
   if constexpr (NQ <= 8) {
-    launch_stamped(k_bp1_wave<T, NQ, NF>, dim3(a.Nelements), dim3(64), s, a.evStart, a.evStop, a.elementList,
-                   a.GlobalToLocal, a.wJ, a.I, a.q, a.Aq);
+    constexpr bool FP = NF > 1 && BPX_FIELD_SPLIT;
+    launch_stamped(k_bp1_wave<T, NQ, NF, FP>, dim3(wave_blocks<NF, FP>(a.Nelements)), dim3(64), s, a.evStart,
+                   a.evStop, a.Nelements, a.elementList, a.GlobalToLocal, a.wJ, a.I, a.q, a.Aq);
     BPX_HIP(hipGetLastError());
     return BPX_OK;
   }
@@ -686,8 +722,10 @@ template <typename T, int NQ, int NF>
 int launch_bp3_nq(hipStream_t s, const AxArgs<T>& a) {
   if (a.affine) return launch_mode<T, NQ, NQ, NF, MODE_BP3A>(s, a, a.wJ, a.ggeo, a.D, a.invV);
   if constexpr (NQ <= 8) {
-    launch_stamped(k_bp3_wave<T, NQ, NF>, dim3(a.Nelements), dim3(128), s, a.evStart, a.evStop, a.elementList,
-                   a.GlobalToLocal, a.wJ, a.ggeo, a.D, a.I, a.lambda, a.q, a.Aq);
+    constexpr bool FP = NF > 1 && BPX_FIELD_SPLIT;
+    launch_stamped(k_bp3_wave<T, NQ, NF, FP>, dim3(wave_blocks<NF, FP>(a.Nelements)), dim3(128), s, a.evStart,
+                   a.evStop, a.Nelements, a.elementList, a.GlobalToLocal, a.wJ, a.ggeo, a.D, a.I, a.lambda, a.q,
+                   a.Aq);
     BPX_HIP(hipGetLastError());
     return BPX_OK;
   }

@@ -150,6 +150,23 @@ int launch_bp5_batch(hipStream_t s, int N, int variant, int Nelements, const int
   }
 }
 
+#define BPX_BATCH32_CASE(NQ) \
+  case NQ:                    \
+    return launch_bp5_batch_f32_nq<NQ>(s, Nelements, elementList, G2L, wJ, ggeo, D, lambda, q, Aq, elemPid, done);
+
+int launch_bp5_batch_f32(hipStream_t s, int N, int Nelements, const int* elementList, const int* G2L,
+                         const float* wJ, const float* ggeo, const float* D, float lambda, const float* q,
+                         float* Aq, const int* elemPid, const BatchState* done) {
+  if (Nelements <= 0) return BPX_OK;
+  switch (N + 1) {
+    BPX_BATCH32_CASE(2) BPX_BATCH32_CASE(3) BPX_BATCH32_CASE(4) BPX_BATCH32_CASE(5) BPX_BATCH32_CASE(6)
+    BPX_BATCH32_CASE(7) BPX_BATCH32_CASE(8) BPX_BATCH32_CASE(9) BPX_BATCH32_CASE(10) BPX_BATCH32_CASE(11)
+    BPX_BATCH32_CASE(12) BPX_BATCH32_CASE(13) BPX_BATCH32_CASE(14) BPX_BATCH32_CASE(15) BPX_BATCH32_CASE(16)
+    default:
+      return set_error(BPX_ERR_UNSUPPORTED, "batch Ax: degree N must be in 1..15");
+  }
+}
+
 int bp5_batch_epb(int Nq, int variant) {
   const int epb = Nq >= 8 ? 1 : 256 / (Nq * Nq);
   if (variant == 0) return epb;

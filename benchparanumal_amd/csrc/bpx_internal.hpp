@@ -199,6 +199,7 @@ struct BatchState {
   double tol2;
   double rnorm;    // ||r|| when the problem stopped
   int iter, done, breakdown, maxit;
+  int updates, pad;  // mixed solve: FP64 corrections made
 };
 // BP5 Ax over the elements of one degree group; variant = workgroup shape
 // (bp5_batch_epb: elements per workgroup, 0 if the variant does not exist)
@@ -217,6 +218,32 @@ int launch_batch_gather(hipStream_t s, int nsupers, const BatchSuper* supers, co
                         const int* colIds, const double* q, double* gq, const double* p, double* part,
                         const BatchState* st);
 int launch_batch_count_done(hipStream_t s, int nprob, const BatchState* st, int* count);
+// FP32 Ax of the mixed batch (factors rounded to FP32), the one workgroup shape
+int launch_bp5_batch_f32(hipStream_t s, int N, int Nelements, const int* elementList, const int* G2L,
+                         const float* wJ, const float* ggeo, const float* D, float lambda, const float* q,
+                         float* Aq, const int* elemPid, const BatchState* done);
+// Mixed batch: FP32 CG per problem (inner state) with FP64 corrections
+// (outer state); see batch.hip
+int launch_to_f32(hipStream_t s, size_t n, const double* a, float* out);
+int launch_bm_init(hipStream_t s, int nchunks, const BatchChunk* chunks, const double* b, double* x, float* r,
+                   float* c, double* part_true, double* part_rr, BatchState* inner, BatchState* outer, int maxit);
+int launch_bm_decide(hipStream_t s, int nprob, const int2* rrRange, const double* part_true, BatchState* inner,
+                     BatchState* outer, double tol, int init);
+int launch_bm_update_p(hipStream_t s, int nchunks, const BatchChunk* chunks, const int2* rrRange, const float* r,
+                       float* p, const double* part_rr, BatchState* inner, const BatchState* outer);
+int launch_bm_update_xr(hipStream_t s, int nchunks, const BatchChunk* chunks, const int2* pApRange, const float* p,
+                        const float* Ap, float* c, float* r, const double* part_pAp, double* part_rr,
+                        BatchState* inner, BatchState* outer);
+int launch_bm_flush(hipStream_t s, int nchunks, const BatchChunk* chunks, float* c, double* x,
+                    const BatchState* outer);
+int launch_bm_true_residual(hipStream_t s, int nchunks, const BatchChunk* chunks, const double* b, const double* Ax,
+                            float* r, double* part_true, double* part_rr, const BatchState* outer);
+int launch_batch_gather_f32(hipStream_t s, int nsupers, const BatchSuper* supers, const int* rowStarts,
+                            const int* colIds, const float* q, float* gq, const float* p, double* part,
+                            const BatchState* st);
+// the gather alone (no p.Ap partials)
+int launch_batch_gather_only(hipStream_t s, int nsupers, const BatchSuper* supers, const int* rowStarts,
+                             const int* colIds, const double* q, double* gq, const BatchState* st);
 
 // per-workgroup partials (max |diff|, sum wJ diff^2, sum wJ (c u)^2) of a
 // one-field solution u (read through G2L) against c sin(pi x) sin(pi y)

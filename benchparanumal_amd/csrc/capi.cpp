@@ -20,6 +20,8 @@ struct bpx_ogs_s {
   ogs::ogs_t o;
   std::vector<dlong> G2L;
   std::vector<hlong> ids;
+  platform_t* plat = nullptr;    // device setup: the platform it ran on
+  deviceBuffer<dlong> o_G2L;     // device setup: GlobalToLocal
 };
 struct bpx_bp_s {
   bpx_platform_s* plat = nullptr;
@@ -87,6 +89,20 @@ static int copy_out(const std::vector<T>& v, void* dst, int64_t* count, int* dty
   if (count) *count = (int64_t)v.size();
   if (dtype) *dtype = code;
   if (dst && !v.empty()) std::memcpy(dst, v.data(), v.size() * sizeof(T));
+  return BPX_OK;
+}
+
+// a device buffer copied to host memory dst (synchronously on stream s)
+template <typename T>
+static int copy_out_dev(const deviceBuffer<T>& v, size_t n, void* dst, int64_t* count, int* dtype, int code,
+                        hipStream_t s) {
+  n = std::min(n, v.length());
+  if (count) *count = (int64_t)n;
+  if (dtype) *dtype = code;
+  if (dst && n > 0) {
+    BPX_HIP(hipMemcpyAsync(dst, v.ptr(), n * sizeof(T), hipMemcpyDeviceToHost, s));
+    BPX_HIP(hipStreamSynchronize(s));
+  }
   return BPX_OK;
 }
 
@@ -514,6 +530,23 @@ int bpx_mesh_create(const bpx_mesh_opts* o, bpx_mesh* out) {
   BPX_GUARD_END
 }
 
+int bpx_mesh_create_dev(bpx_platform p, const bpx_mesh_opts* o, bpx_mesh* out) {
+  BPX_GUARD_BEGIN
+  if (int st = need(p, "platform")) return st;
+  if (int st = need(o, "opts")) return st;
+  if (int st = need(out, "out")) return st;
+  MeshOpts mo;
+  to_mesh_opts(*o, mo);
+  bpx_mesh_s* m = new bpx_mesh_s;
+  if (int st = m->m.SetupDevice(*p->p, mo)) {
+    delete m;
+    return st;
+  }
+  *out = m;
+  return BPX_OK;
+  BPX_GUARD_END
+}
+
 int bpx_mesh_destroy(bpx_mesh m) {
   delete m;
   return BPX_OK;
@@ -531,6 +564,7 @@ int bpx_mesh_array(bpx_mesh m, const char* name, void* dst, int64_t* count, int*
   BPX_GUARD_BEGIN
   if (int st = need(m, "mesh")) return st;
   if (int st = need(name, "name")) return st;
+  if (int st = m->m.EnsureHost()) return st;
   const mesh_t& M = m->m;
   const std::string s(name);
 #define F64(f) if (s == #f) return copy_out(M.f, dst, count, dtype, 0);
@@ -561,11 +595,50 @@ int bpx_ogs_setup(int N, const int64_t* ids, bpx_ogs* out) {
   BPX_GUARD_END
 }
 
+int bpx_ogs_setup_dev(bpx_platform p, int N, const int64_t* o_ids, bpx_ogs* out) {
+  BPX_GUARD_BEGIN
+  if (int st = need(p, "platform")) return st;
+  if (int st = need(out, "out")) return st;
+  if (N > 0)
+    if (int st = need(o_ids, "o_ids")) return st;
+  bpx_ogs_s* o = new bpx_ogs_s;
+  o->plat = p->p;
+  if (int st = o->o_G2L.alloc(std::max(N, 0))) {
+    delete o;
+    return st;
+  }
+  if (int st = o->o.SetupDevice(*p->p, N, (const hlong*)o_ids, o->o_G2L.ptr())) {
+    delete o;
+    return st;
+  }
+  *out = o;
+  return BPX_OK;
+  BPX_GUARD_END
+}
+
 int bpx_ogs_setup_masked(bpx_mesh m, int Nfields, bpx_ogs* out) {
   BPX_GUARD_BEGIN
   if (int st = need(m, "mesh")) return st;
   if (int st = need(out, "out")) return st;
   if (Nfields < 1) return set_error(BPX_ERR_INVALID_ARGUMENT, "ogs: Nfields < 1");
+  if (m->m.onDevice) {  // a device mesh: ids and maps formed on its device
+    platform_t& P = *m->m.devPlatform;
+    const size_t NL = (size_t)m->m.Nelements * m->m.Np * Nfields;
+    deviceBuffer<hlong> ids;
+    if (int st = ids.alloc(NL)) return st;
+    if (int st = m->m.MaskedGlobalIdsDevice(Nfields, ids.ptr())) return st;
+    bpx_ogs_s* o = new bpx_ogs_s;
+    o->plat = &P;
+    int st = o->o_G2L.alloc(NL);
+    if (!st) st = o->o.SetupDevice(P, (dlong)NL, ids.ptr(), o->o_G2L.ptr());
+    if (!st) st = ids.download(o->ids, P.stream);
+    if (st) {
+      delete o;
+      return st;
+    }
+    *out = o;
+    return BPX_OK;
+  }
   bpx_ogs_s* o = new bpx_ogs_s;
   m->m.MaskedGlobalIds(Nfields, o->ids);
   if (int st = o->o.Setup((dlong)o->ids.size(), o->ids.data())) {
@@ -597,6 +670,11 @@ int bpx_ogs_array(bpx_ogs o, const char* name, void* dst, int64_t* count, int* d
   if (int st = need(o, "ogs")) return st;
   if (int st = need(name, "name")) return st;
   const std::string s(name);
+  if (o->o.onDevice) {  // maps formed on the device: copied down on first request
+    if (int st = o->o.EnsureHost(*o->plat)) return st;
+    if (o->G2L.empty() && o->o_G2L.length() > 0)
+      if (int st = o->o_G2L.download(o->G2L, o->plat->stream)) return st;
+  }
   if (s == "rowStartsT") return copy_out(o->o.rowStartsT, dst, count, dtype, 1);
   if (s == "colIdsT") return copy_out(o->o.colIdsT, dst, count, dtype, 1);
   if (s == "blockRowStartsT") return copy_out(o->o.blockRowStartsT, dst, count, dtype, 1);
@@ -667,6 +745,32 @@ int bpx_bp_info(bpx_bp b, int64_t* out, int capacity) {
   return n;
 }
 
+int bpx_bp_array(bpx_bp b, const char* name, void* dst, int64_t* count, int* dtype) {
+  BPX_GUARD_BEGIN
+  if (int st = need(b, "bp")) return st;
+  if (int st = need(name, "name")) return st;
+  bp_t& B = b->bp;
+  hipStream_t st = B.platform->stream;
+  const std::string s(name);
+  const size_t big = (size_t)-1;
+  if (s == "rowStartsT") return copy_out_dev(B.ogs.o_rowStartsT, big, dst, count, dtype, 1, st);
+  if (s == "colIdsT") return copy_out_dev(B.ogs.o_colIdsT, big, dst, count, dtype, 1, st);
+  if (s == "blockRowStartsT") return copy_out_dev(B.ogs.o_blockRowStartsT, big, dst, count, dtype, 1, st);
+  if (s == "GlobalToLocal") return copy_out_dev(B.o_GlobalToLocal, big, dst, count, dtype, 1, st);
+  if (s == "wJ") return copy_out_dev(B.o_wJ, big, dst, count, dtype, 0, st);
+  if (s == "ggeo") return copy_out_dev(B.o_ggeo, big, dst, count, dtype, 0, st);
+  if (s == "solvePerm") return copy_out_dev(B.o_sperm, big, dst, count, dtype, 1, st);
+  if (s == "solveGlobalToLocal") return copy_out_dev(B.o_G2Ls, big, dst, count, dtype, 1, st);
+  if (s == "solveRowStarts") return copy_out_dev(B.solveOgs.o_rowStartsT, big, dst, count, dtype, 1, st);
+  if (s == "solveColIds") return copy_out_dev(B.solveOgs.o_colIdsT, big, dst, count, dtype, 1, st);
+  if (s == "solveBlocks") return copy_out_dev(B.solveOgs.o_blockRowStartsT, big, dst, count, dtype, 1, st);
+  if (s == "sharedRowStarts") return copy_out_dev(B.sharedOgs.o_rowStartsT, big, dst, count, dtype, 1, st);
+  if (s == "sharedColIds") return copy_out_dev(B.sharedOgs.o_colIdsT, big, dst, count, dtype, 1, st);
+  if (s == "sharedBlocks") return copy_out_dev(B.sharedOgs.o_blockRowStartsT, big, dst, count, dtype, 1, st);
+  return set_error(BPX_ERR_INVALID_ARGUMENT, "bp: unknown array " + s);
+  BPX_GUARD_END
+}
+
 int bpx_bp_tuning(bpx_bp b, double* out, int capacity) {
   if (int st = need(b, "bp")) return st;
   const auto& t = b->bp.tuning;
@@ -710,10 +814,10 @@ int bpx_bp_set_fp32_geometry(bpx_bp b, int mode) {
   BPX_GUARD_END
 }
 
-int bpx_bp_set_cg_fold(bpx_bp b, int on) {
+int bpx_bp_set_cg_fold(bpx_bp b, int mask) {
   if (int st = need(b, "bp")) return st;
-  if (on != 0 && on != 1) return set_error(BPX_ERR_INVALID_ARGUMENT, "bp: cg fold must be 0 or 1");
-  b->bp.cgFold = on;
+  if (mask < 0 || mask > 3) return set_error(BPX_ERR_INVALID_ARGUMENT, "bp: cg fold mask must be 0..3");
+  b->bp.cgFold = mask;
   return BPX_OK;
 }
 
@@ -792,6 +896,7 @@ int bpx_mesh_plot_fields(bpx_mesh m, int Nfields, const double* QL, const char* 
   BPX_GUARD_BEGIN
   if (int st = need(m, "mesh")) return st;
   if (int st = need(filename, "filename")) return st;
+  if (int st = m->m.EnsureHost()) return st;
   return m->m.PlotFields(QL, Nfields, filename);
   BPX_GUARD_END
 }
@@ -912,6 +1017,14 @@ int bpx_batch_solve(bpx_batch b, double* o_x, const double* o_b, double tol, int
   BPX_GUARD_BEGIN
   if (int st = need(b, "batch")) return st;
   return b->b.Solve(o_x, o_b, tol, maxit, iterations, rnorm, nullptr);
+  BPX_GUARD_END
+}
+
+int bpx_batch_solve_mixed(bpx_batch b, double* o_x, const double* o_b, double tol, int interval, int maxit,
+                          int* iterations, int* updates, double* relres) {
+  BPX_GUARD_BEGIN
+  if (int st = need(b, "batch")) return st;
+  return b->b.SolveMixed(o_x, o_b, tol, interval, maxit, iterations, updates, relres, nullptr);
   BPX_GUARD_END
 }
 

@@ -10,6 +10,7 @@
 // concatenated with offsets, so one gather launch serves the batch and each
 // row still sums its entries in the reference's ascending order.
 #include <algorithm>
+#include <cmath>
 #include <map>
 
 #include "bpx.hpp"
@@ -24,6 +25,7 @@ constexpr int kGraphIters = 8;      // CG iterations per captured graph
 
 bp_batch_t::~bp_batch_t() {
   if (graphExec) (void)hipGraphExecDestroy(graphExec);
+  if (graphMixed) (void)hipGraphExecDestroy(graphMixed);
   if (h_count) (void)hipHostFree(h_count);
   if (ev0) (void)hipEventDestroy(ev0);
   if (ev1) (void)hipEventDestroy(ev1);
@@ -285,6 +287,149 @@ int bp_batch_t::Solve(double* o_x, const double* o_b, double tol, int maxit, int
   BPX_HIP(hipEventRecord(ev1, stream));
   BPX_HIP(hipStreamWaitEvent(platform->stream, ev1, 0));
   if (anyBreak) return set_error(BPX_ERR_BREAKDOWN, "batch: CG breakdown in at least one problem");
+  return BPX_OK;
+}
+
+// ---- FP32 CG with FP64 corrections, every problem at once -------------
+// The recurrence of bp_t::SolveMixed per problem: FP32 vectors and
+// operator (factors rounded to FP32), FP64 dot products, and every
+// `interval` iterations x += c and r = float(b - A x) with the FP64
+// operator, where the stopping test is made. Problems advance in lockstep
+// segments (interval FP32 iterations, then the correction), captured once
+// as a hipGraph; a finished problem's elements, rows and chunks return at
+// once in every kernel of later segments.
+
+int bp_batch_t::Prepare32() {
+  if (have32) return BPX_OK;
+  hipStream_t s = stream;
+  for (auto& G : groups) {
+    int st;
+    if ((st = G.wJ32.alloc(G.wJ.length())) || (st = G.ggeo32.alloc(G.ggeo.length())) ||
+        (st = G.D32.alloc(G.D.length())))
+      return st;
+    if ((st = launch_to_f32(s, G.wJ.length(), G.wJ.ptr(), G.wJ32.ptr())) ||
+        (st = launch_to_f32(s, G.ggeo.length(), G.ggeo.ptr(), G.ggeo32.ptr())) ||
+        (st = launch_to_f32(s, G.D.length(), G.D.ptr(), G.D32.ptr())))
+      return st;
+  }
+  const size_t n = std::max<dlong>(Ntot, 1), nl = std::max<dlong>(NLocalTot, 1);
+  int st;
+  if ((st = o_r32.alloc(n)) || (st = o_p32.alloc(n)) || (st = o_Ap32.alloc(n)) || (st = o_c32.alloc(n)) ||
+      (st = o_AqL32.alloc(nl)) || (st = o_Ax64.alloc(n)) || (st = o_part_true.alloc(std::max(nchunks, 1))) ||
+      (st = o_inner.alloc(nprob)))
+    return st;
+  BPX_HIP(hipStreamSynchronize(s));
+  have32 = true;
+  return BPX_OK;
+}
+
+// interval FP32 iterations, then the FP64 correction and the stopping test
+int bp_batch_t::Segment(double* o_x, const double* o_b, double tol, int interval) {
+  BatchState* in = o_inner.ptr();
+  BatchState* out = o_state.ptr();
+  const float lam32 = (float)lambda;
+  for (int j = 0; j < interval; ++j) {
+    if (int st = launch_bm_update_p(stream, nchunks, o_chunks.ptr(), o_rrRange.ptr(), o_r32.ptr(), o_p32.ptr(),
+                                    o_part_rr.ptr(), in, out))
+      return st;
+    for (const auto& G : groups)
+      if (int st = launch_bp5_batch_f32(stream, G.N, (int)G.Nel, G.elementList.ptr(), G.G2L.ptr(), G.wJ32.ptr(),
+                                        G.ggeo32.ptr(), G.D32.ptr(), lam32, o_p32.ptr(), o_AqL32.ptr() + G.loff,
+                                        G.elemPid.ptr(), in))
+        return st;
+    if (int st = launch_batch_gather_f32(stream, nsupers, o_supers.ptr(), o_rowStarts.ptr(), o_colIds.ptr(),
+                                         o_AqL32.ptr(), o_Ap32.ptr(), o_p32.ptr(), o_part_pAp.ptr(), in))
+      return st;
+    if (int st = launch_bm_update_xr(stream, nchunks, o_chunks.ptr(), o_pApRange.ptr(), o_p32.ptr(), o_Ap32.ptr(),
+                                     o_c32.ptr(), o_r32.ptr(), o_part_pAp.ptr(), o_part_rr.ptr(), in, out))
+      return st;
+  }
+  if (int st = launch_bm_flush(stream, nchunks, o_chunks.ptr(), o_c32.ptr(), o_x, out)) return st;
+  for (const auto& G : groups)
+    if (int st = launch_bp5_batch(stream, G.N, G.variant, (int)G.Nel, G.elementList.ptr(), G.G2L.ptr(), G.wJ.ptr(),
+                                  G.ggeo.ptr(), G.D.ptr(), lambda, o_x, o_AqL.ptr() + G.loff, G.elemPid.ptr(), out))
+      return st;
+  if (int st = launch_batch_gather_only(stream, nsupers, o_supers.ptr(), o_rowStarts.ptr(), o_colIds.ptr(),
+                                        o_AqL.ptr(), o_Ax64.ptr(), out))
+    return st;
+  if (int st = launch_bm_true_residual(stream, nchunks, o_chunks.ptr(), o_b, o_Ax64.ptr(), o_r32.ptr(),
+                                       o_part_true.ptr(), o_part_rr.ptr(), out))
+    return st;
+  if (int st = launch_bm_decide(stream, nprob, o_rrRange.ptr(), o_part_true.ptr(), in, out, tol, 0)) return st;
+  return launch_batch_count_done(stream, nprob, out, o_count.ptr());
+}
+
+int bp_batch_t::CaptureMixed(double* o_x, const double* o_b, double tol, int interval) {
+  if (graphMixed && graphMixedX == o_x && graphMixedB == o_b && graphMixedTol == tol &&
+      graphMixedInterval == interval)
+    return BPX_OK;
+  if (graphMixed) {
+    (void)hipGraphExecDestroy(graphMixed);
+    graphMixed = nullptr;
+  }
+  BPX_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+  const int st = Segment(o_x, o_b, tol, interval);
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(stream, &graph);
+  if (st) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return st;
+  }
+  if (e != hipSuccess) return hip_error(e, "hipStreamEndCapture");
+  const hipError_t e2 = hipGraphInstantiate(&graphMixed, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e2 != hipSuccess) {
+    graphMixed = nullptr;
+    return hip_error(e2, "hipGraphInstantiate");
+  }
+  graphMixedX = o_x;
+  graphMixedB = o_b;
+  graphMixedTol = tol;
+  graphMixedInterval = interval;
+  return BPX_OK;
+}
+
+int bp_batch_t::SolveMixed(double* o_x, const double* o_b, double tol, int interval, int maxit, int* iterations,
+                           int* updates, double* relres, int* breakdown) {
+  if (!(tol > 0) || interval < 1 || maxit < 1)
+    return set_error(BPX_ERR_INVALID_ARGUMENT, "batch mixed solve: tol > 0, interval >= 1 and maxit >= 1 required");
+  if (Ntot > 0 && (!o_x || !o_b)) return set_error(BPX_ERR_INVALID_ARGUMENT, "batch: null vector");
+  if (int st = Prepare32()) return st;
+  BPX_HIP(hipEventRecord(ev0, platform->stream));
+  BPX_HIP(hipStreamWaitEvent(stream, ev0, 0));
+  BatchState* in = o_inner.ptr();
+  BatchState* out = o_state.ptr();
+  if (int st = launch_bm_init(stream, nchunks, o_chunks.ptr(), o_b, o_x, o_r32.ptr(), o_c32.ptr(),
+                              o_part_true.ptr(), o_part_rr.ptr(), in, out, maxit))
+    return st;
+  if (int st = launch_bm_decide(stream, nprob, o_rrRange.ptr(), o_part_true.ptr(), in, out, tol, 1)) return st;
+  if (int st = CaptureMixed(o_x, o_b, tol, interval)) return st;
+  // a segment adds up to `interval` FP32 iterations per running problem;
+  // once a problem reaches maxit its recurrence pauses and the next
+  // correction ends it
+  const long long maxReplays = (long long)maxit / interval + 2;
+  replays = 0;
+  for (long long rep = 0; rep < maxReplays; ++rep) {
+    BPX_HIP(hipGraphLaunch(graphMixed, stream));
+    BPX_HIP(hipMemcpyAsync(h_count, o_count.ptr(), sizeof(int), hipMemcpyDeviceToHost, stream));
+    BPX_HIP(hipStreamSynchronize(stream));
+    ++replays;
+    if (*h_count >= nprob) break;
+  }
+  std::vector<BatchState> ho(nprob), hi(nprob);
+  BPX_HIP(hipMemcpy(ho.data(), out, sizeof(BatchState) * nprob, hipMemcpyDeviceToHost));
+  BPX_HIP(hipMemcpy(hi.data(), in, sizeof(BatchState) * nprob, hipMemcpyDeviceToHost));
+  int anyBreak = 0;
+  for (int i = 0; i < nprob; ++i) {
+    if (iterations) iterations[i] = ho[i].iter;
+    if (updates) updates[i] = ho[i].updates;
+    if (relres) relres[i] = hi[i].tol2 > 0 ? ho[i].rnorm / std::sqrt(hi[i].tol2) : 0.0;
+    if (breakdown) breakdown[i] = ho[i].breakdown;
+    anyBreak |= ho[i].breakdown;
+  }
+  BPX_HIP(hipEventRecord(ev1, stream));
+  BPX_HIP(hipStreamWaitEvent(platform->stream, ev1, 0));
+  if (anyBreak) return set_error(BPX_ERR_BREAKDOWN, "batch mixed solve: non-finite residual in at least one problem");
   return BPX_OK;
 }
 

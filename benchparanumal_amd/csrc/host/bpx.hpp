@@ -30,6 +30,20 @@ class deviceBuffer {
   ~deviceBuffer() { release(); }
   deviceBuffer(const deviceBuffer&) = delete;
   deviceBuffer& operator=(const deviceBuffer&) = delete;
+  deviceBuffer(deviceBuffer&& o) noexcept : ptr_(o.ptr_), n_(o.n_) {
+    o.ptr_ = nullptr;
+    o.n_ = 0;
+  }
+  deviceBuffer& operator=(deviceBuffer&& o) noexcept {
+    if (this != &o) {
+      release();
+      ptr_ = o.ptr_;
+      n_ = o.n_;
+      o.ptr_ = nullptr;
+      o.n_ = 0;
+    }
+    return *this;
+  }
   int alloc(size_t n) {
     release();
     if (n == 0) return BPX_OK;
@@ -49,6 +63,21 @@ class deviceBuffer {
     return BPX_OK;
   }
   int upload(const std::vector<T>& v, hipStream_t s) { return upload(v.data(), v.size(), s); }
+  // device-to-device copy of n values, queued on s
+  int copy_device(const T* d, size_t n, hipStream_t s) {
+    if (int st = alloc(n)) return st;
+    if (n == 0) return BPX_OK;
+    BPX_HIP(hipMemcpyAsync(ptr_, d, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+    return BPX_OK;
+  }
+  // synchronous copy of the whole buffer into h (resized to length())
+  int download(std::vector<T>& h, hipStream_t s) const {
+    h.resize(n_);
+    if (n_ == 0) return BPX_OK;
+    BPX_HIP(hipMemcpyAsync(h.data(), ptr_, n_ * sizeof(T), hipMemcpyDeviceToHost, s));
+    BPX_HIP(hipStreamSynchronize(s));
+    return BPX_OK;
+  }
   void release() {
     if (ptr_) (void)hipFree(ptr_);
     ptr_ = nullptr;
@@ -105,6 +134,9 @@ struct MeshOpts {
 class mesh_t {
  public:
   int Setup(const MeshOpts& o);
+  // the part of Setup that stays on the host in both setups: checks, 1D
+  // operators, element vertices
+  int SetupVertices(const MeshOpts& o);
   // mesh_t::MaskedGatherScatterSetup ids (meshMaskedGatherScatterSetup.cpp:31-77)
   void MaskedGlobalIds(int Nfields, std::vector<hlong>& ids) const;
   // per element, for x, y and z in turn, the coefficients of r, s, t, rs,
@@ -119,6 +151,23 @@ class mesh_t {
   // the local nodes with Nfields values each (QL: host, [e][f][n]); each
   // element is drawn as N^3 sub-cubes of 6 tetrahedra
   int PlotFields(const double* QL, int Nfields, const std::string& fileName) const;
+
+  // GPU-side setup: the same mesh as Setup, with the nodes, global ids and
+  // geometric and cubature factors formed on the device (bit-identical to
+  // Setup's host arrays) into the o_ buffers below. The vertices and the 1D
+  // operators stay on the host; the per-node host arrays are left empty
+  // until EnsureHost copies them down.
+  int SetupDevice(platform_t& p, const MeshOpts& o);
+  int EnsureHost();
+  // whether every node is its element's trilinear map (NodesAreTrilinear),
+  // checked on the device for a device mesh
+  bool NodesTrilinear();
+  // masked ids (MaskedGlobalIds) formed on the device from o_globalIds
+  int MaskedGlobalIdsDevice(int Nfields, hlong* o_ids) const;
+  bool onDevice = false;
+  platform_t* devPlatform = nullptr;
+  deviceBuffer<double> o_x, o_y, o_z, o_wJ, o_ggeo, o_cubwJ, o_cubggeo;
+  deviceBuffer<hlong> o_globalIds;
 
   MeshOpts opts;
   int N = 0, Nq = 0, Np = 0, cubNq = 0, cubNp = 0;
@@ -140,6 +189,13 @@ class ogs_t {
   int SetupRows(dlong N, dlong Nrows, const dlong* rowOf);
   void SetupGlobalToLocalMapping(dlong* GlobalToLocal) const;
   int Upload(platform_t& p);  // device copies of the T maps
+  // Setup and SetupRows on the device (device ids / rowOf; the same maps,
+  // left in the o_ buffers; host vectors empty until EnsureHost). o_G2L
+  // (device, N entries, may be null) gets GlobalToLocal.
+  int SetupDevice(platform_t& p, dlong N, const hlong* o_ids, dlong* o_G2L);
+  int SetupRowsDevice(platform_t& p, dlong N, dlong Nrows, const dlong* o_rowOf);
+  int EnsureHost(platform_t& p);
+  bool onDevice = false;
   template <typename T>
   int Gather(platform_t& p, T* o_gv, const T* o_v, int K) const;  // Add, Trans
   template <typename T>
@@ -359,13 +415,16 @@ class bp_t : public solver_t {
   // fused = the maps are built; tuning.fused64/fused32 = it is used.
   bool fused = false;
   int PrepareFused();
+  int PrepareFusedDevice();  // PrepareFused from the device maps (device mesh)
   template <typename T>
   int OperatorDotFused(const T* o_q, T* o_Aq, double* part, int* npart, const CgFold<T>* fold = nullptr);
   // the rows a folded CG operator leaves to k_cg_update_p (the solve space's
   // rows before the first interior run), or 0 where no fold applies:
   // precision 64 or 32, the fused operator in use, and cgFold set
   dlong FoldRows(int precision) const;
-  int cgFold = 1;  // fold the CG direction update into the fused operator (same bits)
+  // fold the CG direction update into the fused operator (same bits): bit 0
+  // for the FP64 solves, bit 1 for the FP32 iterations of the mixed solve
+  int cgFold = 1;
   int OperatorDotUnfusedS(const dfloat* o_q, dfloat* o_Aq, double* part, int* npart);
   int OperatorDotUnfused32(const float* o_q, float* o_Aq, double* part, int* npart);
   // Ax into the local vector; G2L = the GlobalToLocal map to read q
@@ -373,6 +432,7 @@ class bp_t : public solver_t {
   int AxLocal(const dfloat* o_q, dfloat* o_AqL, const dlong* G2L = nullptr);
   int Operator32(const float* o_q, float* o_Aq);
   int Forcing(dfloat* o_b);
+  int NodeCoordinates();  // o_x, o_y, o_z, o_gllwRhs on first use
   // the solution (reference row order) scattered to the local nodes and
   // written as a VTU file (BP/BP5/src/bp5Run.cpp:220-238)
   int PlotSolution(const dfloat* o_u, const std::string& fileName);
@@ -478,6 +538,13 @@ class bp_batch_t {
   // final ||r||, breakdown flag (host arrays of nprob, may be null)
   int Solve(double* o_x, const double* o_b, double tol, int maxit, int* iterations, double* rnorm,
             int* breakdown);
+  // FP32 CG per problem with FP64 defect corrections every `interval`
+  // iterations (the recurrence of bp_t::SolveMixed, every problem at once,
+  // x starting from 0) until ||b - A x|| <= tol ||b||; per problem (host
+  // arrays, may be null): FP32 iterations, corrections, final
+  // ||b - A x|| / ||b||, breakdown flag
+  int SolveMixed(double* o_x, const double* o_b, double tol, int interval, int maxit, int* iterations,
+                 int* updates, double* relres, int* breakdown);
 
   struct Group {  // problems of one degree: one Ax launch over their elements
     int N = 0, Nq = 0;
@@ -485,6 +552,7 @@ class bp_batch_t {
     long long loff = 0;  // start in the concatenated local vector
     deviceBuffer<dlong> elementList, G2L, elemPid;
     deviceBuffer<double> wJ, ggeo, D;
+    deviceBuffer<float> wJ32, ggeo32, D32;  // rounded to FP32 for the mixed solve
     int variant = 0;             // workgroup shape chosen by the tuner
     double us[3] = {-1, -1, -1};  // its timings (us; -1 = shape not available)
   };
@@ -501,6 +569,18 @@ class bp_batch_t {
   int AxGroup(const Group& G, int variant, const double* q, double* AqL) const;
   int Iteration(double* o_x, double tol);
   int Capture(double* o_x, double tol);
+  int Prepare32();
+  int Segment(double* o_x, const double* o_b, double tol, int interval);
+  int CaptureMixed(double* o_x, const double* o_b, double tol, int interval);
+  bool have32 = false;
+  hipGraphExec_t graphMixed = nullptr;
+  double* graphMixedX = nullptr;
+  const double* graphMixedB = nullptr;
+  double graphMixedTol = -1;
+  int graphMixedInterval = 0;
+  deviceBuffer<float> o_r32, o_p32, o_Ap32, o_c32, o_AqL32;
+  deviceBuffer<double> o_Ax64, o_part_true;
+  deviceBuffer<BatchState> o_inner;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int* h_count = nullptr;

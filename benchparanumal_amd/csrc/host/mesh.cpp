@@ -22,6 +22,7 @@
 
 #include "basis.hpp"
 #include "bpx.hpp"
+#include "geometry.hpp"
 #include "rng.hpp"
 
 namespace bpx {
@@ -31,29 +32,6 @@ namespace {
 // reference coordinates of the 8 vertices of an element, in vertex order
 const double kVr[8] = {-1, 1, 1, -1, -1, 1, 1, -1}, kVs[8] = {-1, -1, 1, 1, -1, -1, 1, 1},
              kVt[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
-
-struct Geo9 {
-  double xr, xs, xt, yr, ys, yt, zr, zs, zt;
-};
-
-// J and the 6 symmetric entries of J (grad r)(grad r)^T
-inline bool metric(const Geo9& d, double& J, double g[6]) {
-  J = d.xr * (d.ys * d.zt - d.zs * d.yt) - d.yr * (d.xs * d.zt - d.zs * d.xt) +
-      d.zr * (d.xs * d.yt - d.ys * d.xt);
-  const double rx = (d.ys * d.zt - d.zs * d.yt) / J, ry = -(d.xs * d.zt - d.zs * d.xt) / J,
-               rz = (d.xs * d.yt - d.ys * d.xt) / J;
-  const double sx = -(d.yr * d.zt - d.zr * d.yt) / J, sy = (d.xr * d.zt - d.zr * d.xt) / J,
-               sz = -(d.xr * d.yt - d.yr * d.xt) / J;
-  const double tx = (d.yr * d.zs - d.zr * d.ys) / J, ty = -(d.xr * d.zs - d.zr * d.xs) / J,
-               tz = (d.xr * d.ys - d.yr * d.xs) / J;
-  g[0] = J * (rx * rx + ry * ry + rz * rz);
-  g[1] = J * (rx * sx + ry * sy + rz * sz);
-  g[2] = J * (rx * tx + ry * ty + rz * tz);
-  g[3] = J * (sx * sx + sy * sy + sz * sz);
-  g[4] = J * (sx * tx + sy * ty + sz * tz);
-  g[5] = J * (tx * tx + ty * ty + tz * tz);
-  return true;
-}
 
 }  // namespace
 
@@ -106,7 +84,7 @@ bool mesh_t::NodesAreTrilinear() const {
   return ok;
 }
 
-int mesh_t::Setup(const MeshOpts& o) {
+int mesh_t::SetupVertices(const MeshOpts& o) {
   if (o.N < 1 || o.N > 15) return set_error(BPX_ERR_INVALID_ARGUMENT, "mesh: degree N must be in 1..15");
   if (o.nx < 1 || o.ny < 1 || o.nz < 1) return set_error(BPX_ERR_INVALID_ARGUMENT, "mesh: nx, ny, nz must be >= 1");
   if (!(o.dim[0] > 0 && o.dim[1] > 0 && o.dim[2] > 0))
@@ -123,9 +101,26 @@ int mesh_t::Setup(const MeshOpts& o) {
   Np = Nq * Nq * Nq;
   Nelements = (dlong)nel;
 
+  onDevice = false;
   basis::gll(N, gllz, gllw);
   basis::dmatrix_gll(N, gllz, D);
   basis::mass_gll(N, gllz, MM);
+  // 1D operators of the cubature (BP1-BP4)
+  cubNq = Nq + 1;
+  cubNp = cubNq * cubNq * cubNq;
+  cubr.clear(); cubw.clear(); cubInterp.clear(); cubD.clear(); invV.clear();
+  if (o.cubature) {
+    basis::gauss(cubNq, cubr, cubw);
+    basis::interp(gllz, cubr, cubInterp);
+    if (o.affine) {
+      basis::inv_vandermonde(N, gllz, invV);
+      basis::modal_dmatrix(N, cubD);
+    } else {
+      basis::dmatrix(cubr, cubD);
+    }
+  }
+  x.clear(); y.clear(); z.clear(); globalIds.clear();
+  wJ.clear(); ggeo.clear(); cubwJ.clear(); cubggeo.clear();
 
   // ---- vertices (meshSetupBoxHex3D.cpp:88-126) + seeded perturbation
   const int nx = o.nx, ny = o.ny, nz = o.nz;
@@ -157,20 +152,25 @@ int mesh_t::Setup(const MeshOpts& o) {
           EZ[e * 8 + v] = Z;
         }
       }
+  return BPX_OK;
+}
+
+int mesh_t::Setup(const MeshOpts& o) {
+  if (int st = SetupVertices(o)) return st;
+  const int nx = o.nx, ny = o.ny;
 
   // ---- nodal coordinates (trilinear map)
   const size_t NL = (size_t)Nelements * Np;
   x.assign(NL, 0.0);
   y.assign(NL, 0.0);
   z.assign(NL, 0.0);
-  const double *sr = kVr, *ss = kVs, *st = kVt;
 #pragma omp parallel for
   for (dlong e = 0; e < Nelements; ++e)
     for (int n = 0; n < Np; ++n) {
       const double r = gllz[n % Nq], s = gllz[(n / Nq) % Nq], t = gllz[n / (Nq * Nq)];
       double X = 0, Y = 0, Z = 0;
       for (int v = 0; v < 8; ++v) {
-        const double c = 0.125 * (1 + sr[v] * r) * (1 + ss[v] * s) * (1 + st[v] * t);
+        const double c = vertex_weight(v, r, s, t);
         X += c * EX[(size_t)e * 8 + v];
         Y += c * EY[(size_t)e * 8 + v];
         Z += c * EZ[(size_t)e * 8 + v];
@@ -239,19 +239,9 @@ int mesh_t::Setup(const MeshOpts& o) {
   }
   if (bad) return set_error(BPX_ERR_NEGATIVE_JACOBIAN, "mesh: negative or vanishing Jacobian found");
 
-  // ---- cubature (BP1-BP4)
-  cubNq = Nq + 1;
-  cubNp = cubNq * cubNq * cubNq;
-  cubr.clear(); cubw.clear(); cubInterp.clear(); cubD.clear(); invV.clear();
-  cubwJ.clear(); cubggeo.clear();
-  if (o.cubature) {
-    basis::gauss(cubNq, cubr, cubw);
-    basis::interp(gllz, cubr, cubInterp);
-    if (o.affine) {
-      basis::inv_vandermonde(N, gllz, invV);
-      basis::modal_dmatrix(N, cubD);
-    } else {
-      basis::dmatrix(cubr, cubD);
+  // ---- cubature factors (BP1-BP4, non-affine)
+  if (o.cubature && !o.affine) {
+    {
       const int cq = cubNq;
       cubwJ.assign((size_t)Nelements * cubNp, 0.0);
       cubggeo.assign((size_t)Nelements * cubNp * 6, 0.0);

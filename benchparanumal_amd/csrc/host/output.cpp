@@ -97,6 +97,7 @@ int mesh_t::PlotFields(const double* QL, int Nfields, const std::string& fileNam
 int bp_t::PlotSolution(const dfloat* o_u, const std::string& fileName) {
   // ogs.Scatter(o_xL, o_x) then mesh.PlotFields (BP/BP5/src/bp5Run.cpp:224-237);
   // Dirichlet nodes (no row) are written as 0
+  if (int st = mesh->EnsureHost()) return st;  // node coordinates on the host
   hipStream_t s = platform->stream;
   if (int st = launch_set<double>(s, NLocal, 0.0, o_AqL.ptr())) return st;
   if (int st = ogs.Scatter<double>(*platform, o_AqL.ptr(), o_u, 1)) return st;
@@ -108,17 +109,13 @@ int bp_t::PlotSolution(const dfloat* o_u, const std::string& fileName) {
 
 int bp_t::ErrorNorms(const dfloat* o_u, double* out) {
   if (Nfields != 1) return set_error(BPX_ERR_UNSUPPORTED, "error norms: one-field problems (BP1, BP3, BP5)");
+  if (int st = mesh->EnsureHost()) return st;  // wJ on the host
   const mesh_t& m = *mesh;
   hipStream_t s = platform->stream;
   // the forcing's solution: u = sin sin sin for the stiffness problems; the
   // mass problems (BP1) are forced with lambda = 0, so M u = M 3 pi^2 u0
   const double c = (bp <= 2) ? 3 * M_PI * M_PI : 1.0;
-  if (!o_x.ptr()) {
-    if (int st = o_x.upload(m.x, s)) return st;
-    if (int st = o_y.upload(m.y, s)) return st;
-    if (int st = o_z.upload(m.z, s)) return st;
-    if (int st = o_gllwRhs.upload(m.gllw, s)) return st;
-  }
+  if (int st = NodeCoordinates()) return st;
   // GLL weight times Jacobian per local node (affine meshes store J per element)
   std::vector<double> W((size_t)NLocal);
   const int Nq = m.Nq, Np = m.Np;

@@ -283,41 +283,55 @@ int bp_t::Setup(platform_t& p, mesh_t& m, int bp_, dfloat lambda_) {
   BPX_HIP(hipSetDevice(p.device));
   hipStream_t s = p.stream;
 
-  // masked ogs + GlobalToLocal (BP/BP5/src/bp5Setup.cpp:43-52)
-  std::vector<hlong> ids;
-  m.MaskedGlobalIds(Nfields, ids);
-  NLocal = (dlong)ids.size();
-  if (int st = ogs.Setup(NLocal, ids.data())) return st;
-  GlobalToLocal.resize(NLocal);
-  ogs.SetupGlobalToLocalMapping(GlobalToLocal.data());
-  if (int st = ogs.Upload(p)) return st;
+  // masked ogs + GlobalToLocal (BP/BP5/src/bp5Setup.cpp:43-52): on the
+  // device for a mesh set up there, else on the host
+  NLocal = (dlong)((size_t)m.Nelements * m.Np * Nfields);
+  GlobalToLocal.clear();
+  if (m.onDevice) {
+    deviceBuffer<hlong> ids;
+    if (int st = ids.alloc(NLocal)) return st;
+    if (int st = m.MaskedGlobalIdsDevice(Nfields, ids.ptr())) return st;
+    if (int st = o_GlobalToLocal.alloc(NLocal)) return st;
+    if (int st = ogs.SetupDevice(p, NLocal, ids.ptr(), o_GlobalToLocal.ptr())) return st;
+  } else {
+    std::vector<hlong> ids;
+    m.MaskedGlobalIds(Nfields, ids);
+    if (int st = ogs.Setup(NLocal, ids.data())) return st;
+    GlobalToLocal.resize(NLocal);
+    ogs.SetupGlobalToLocalMapping(GlobalToLocal.data());
+    if (int st = ogs.Upload(p)) return st;
+    if (int st = o_GlobalToLocal.upload(GlobalToLocal, s)) return st;
+  }
   if (int st = gHalo.SetupFromGather(ogs)) return st;
-  if (int st = o_GlobalToLocal.upload(GlobalToLocal, s)) return st;
   std::vector<dlong> list(m.Nelements);
   for (dlong e = 0; e < m.Nelements; ++e) list[e] = e;
   if (int st = o_elementList.upload(list, s)) return st;
   if (int st = o_AqL.alloc(NLocal)) return st;
 
-  // operator data (BP/BPn/src/bpnOperator.cpp argument lists)
+  // operator data (BP/BPn/src/bpnOperator.cpp argument lists); the factors
+  // of a device mesh are copied on the device
   const int affine = m.opts.affine;
+  auto factors = [&](deviceBuffer<double>& dst, const std::vector<double>& h, const deviceBuffer<double>& d) {
+    return m.onDevice ? dst.copy_device(d.ptr(), d.length(), s) : dst.upload(h, s);
+  };
   if (bp == 5 || bp == 6) {
-    if (int st = o_wJ.upload(m.wJ, s)) return st;
-    if (int st = o_ggeo.upload(m.ggeo, s)) return st;
+    if (int st = factors(o_wJ, m.wJ, m.o_wJ)) return st;
+    if (int st = factors(o_ggeo, m.ggeo, m.o_ggeo)) return st;
     if (int st = o_gllw.upload(m.gllw, s)) return st;
     if (int st = o_D.upload(m.D, s)) return st;
   } else if (bp == 1 || bp == 2) {
-    if (int st = o_wJ.upload(affine ? m.wJ : m.cubwJ, s)) return st;
+    if (int st = affine ? factors(o_wJ, m.wJ, m.o_wJ) : factors(o_wJ, m.cubwJ, m.o_cubwJ)) return st;
     if (int st = o_I.upload(m.cubInterp, s)) return st;
     if (int st = o_MM.upload(m.MM, s)) return st;
   } else {
-    if (int st = o_wJ.upload(affine ? m.wJ : m.cubwJ, s)) return st;
-    if (int st = o_ggeo.upload(affine ? m.ggeo : m.cubggeo, s)) return st;
+    if (int st = affine ? factors(o_wJ, m.wJ, m.o_wJ) : factors(o_wJ, m.cubwJ, m.o_cubwJ)) return st;
+    if (int st = affine ? factors(o_ggeo, m.ggeo, m.o_ggeo) : factors(o_ggeo, m.cubggeo, m.o_cubggeo)) return st;
     if (int st = o_D.upload(m.cubD, s)) return st;
     if (int st = o_I.upload(m.cubInterp, s)) return st;
     if (affine)
       if (int st = o_invV.upload(m.invV, s)) return st;
   }
-  if (int st = PrepareFused()) return st;
+  if (int st = m.onDevice ? PrepareFusedDevice() : PrepareFused()) return st;
   if (int st = Tune()) return st;
   return p.finish();
 }
@@ -376,6 +390,7 @@ int bp_t::SetKappa(const dfloat* o_kappa) {
   if (Nnodes >= ((hlong)1 << 31)) return set_error(BPX_ERR_UNSUPPORTED, "bp: lattice above 2^31 nodes");
   if (!o_lattice.ptr()) {
     // lattice point of each local node: the mesh's global ids are 1 + lattice index
+    if (int st = mesh->EnsureHost()) return st;
     std::vector<dlong> lat(NLocal);
     for (dlong l = 0; l < NLocal; ++l) lat[l] = (dlong)(m.globalIds[l] - 1);
     if (int st = o_lattice.upload(lat, s)) return st;
@@ -544,7 +559,7 @@ int bp_t::PrepareFused() {
 }
 
 dlong bp_t::FoldRows(int precision) const {
-  if (!cgFold || !fused || !permuted) return 0;
+  if (!(cgFold & (precision == 64 ? 1 : 2)) || !fused || !permuted) return 0;
   if (precision == 64 ? !tuning.fused64 : !tuning.fused32) return 0;
   // the FP64 operator at N = 15 runs on the matrix-core kernel, which keeps
   // the separate update kernel
@@ -891,7 +906,7 @@ int bp_t::Prepare32() {
   const int affine = m.opts.affine;
   int st = BPX_OK;
   tri32 = 0;
-  if (bp == 5 && !affine && fp32Geometry == 0 && m.NodesAreTrilinear()) {
+  if (bp == 5 && !affine && fp32Geometry == 0 && mesh->NodesTrilinear()) {
     // the FP32 operator forms its geometric factors from the elements'
     // trilinear maps: 96 B per element instead of 28 B per node
     std::vector<double> tri;
@@ -900,6 +915,8 @@ int bp_t::Prepare32() {
         (st = cvt(o_D32, m.D)))
       return st;
     tri32 = 1;
+  } else if ((st = mesh->EnsureHost())) {
+    return st;
   } else if (bp == 5 || bp == 6) {
     if ((st = cvt(o_wJ32, m.wJ)) || (st = cvt(o_ggeo32, m.ggeo)) || (st = cvt(o_gllw32, m.gllw)) ||
         (st = cvt(o_D32, m.D)))
@@ -941,19 +958,31 @@ int bp_t::Operator32(const float* o_q, float* o_Aq) {
   return ogs.Gather<float>(*platform, o_Aq, o_AqL32.ptr(), 1);
 }
 
+// device copies of the nodal coordinates and GLL weights (forcing, error norms)
+int bp_t::NodeCoordinates() {
+  if (o_x.ptr()) return BPX_OK;
+  hipStream_t s = platform->stream;
+  const mesh_t& m = *mesh;
+  if (m.onDevice) {
+    if (int st = o_x.copy_device(m.o_x.ptr(), m.o_x.length(), s)) return st;
+    if (int st = o_y.copy_device(m.o_y.ptr(), m.o_y.length(), s)) return st;
+    if (int st = o_z.copy_device(m.o_z.ptr(), m.o_z.length(), s)) return st;
+  } else {
+    if (int st = o_x.upload(m.x, s)) return st;
+    if (int st = o_y.upload(m.y, s)) return st;
+    if (int st = o_z.upload(m.z, s)) return st;
+  }
+  return o_gllwRhs.upload(m.gllw, s);
+}
+
 // b = gather(forcing) (BP/BP5/src/bp5Run.cpp:51-64); BP1/BP2 force with lambda = 0.
 int bp_t::Forcing(dfloat* o_b) {
   hipStream_t s = platform->stream;
   const mesh_t& m = *mesh;
-  if (!o_x.ptr()) {
-    if (int st = o_x.upload(m.x, s)) return st;
-    if (int st = o_y.upload(m.y, s)) return st;
-    if (int st = o_z.upload(m.z, s)) return st;
-    if (int st = o_gllwRhs.upload(m.gllw, s)) return st;
-  }
+  if (int st = NodeCoordinates()) return st;
   // the forcing uses the GLL-weighted wJ of the mesh (mesh.o_wJ)
   deviceBuffer<double> wJ;
-  if (int st = wJ.upload(m.wJ, s)) return st;
+  if (int st = m.onDevice ? wJ.copy_device(m.o_wJ.ptr(), m.o_wJ.length(), s) : wJ.upload(m.wJ, s)) return st;
   const double lam = (bp <= 2) ? 0.0 : lambda;
   if (int st = launch_rhs(s, m.N, Nfields, m.opts.affine, m.Nelements, wJ.ptr(), o_gllwRhs.ptr(), o_x.ptr(),
                           o_y.ptr(), o_z.ptr(), lam, o_AqL.ptr()))

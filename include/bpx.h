@@ -265,6 +265,16 @@ typedef struct {
 } bpx_mesh_opts;
 
 int bpx_mesh_create(const bpx_mesh_opts* opts, bpx_mesh* out);
+/* The same mesh set up on the device (the setup path of
+ * libs/mesh/meshPhysicalNodesHex3D.cpp:31-116,
+ * meshGeometricFactorsHex3D.cpp:31-153 and meshCubatureSetupHex3D.cpp:97-257
+ * run as GPU kernels): the vertices and 1D operators are formed on the host,
+ * the nodes, global ids and geometric and cubature factors on the device, with
+ * the same bits as bpx_mesh_create's host arrays. bpx_ogs_setup_masked and
+ * bpx_bp_create on such a mesh build their maps on the device too. The
+ * per-node host arrays are copied down on first host use (bpx_mesh_array,
+ * bpx_mesh_plot_fields). The mesh keeps using p's device and stream. */
+int bpx_mesh_create_dev(bpx_platform p, const bpx_mesh_opts* opts, bpx_mesh* out);
 int bpx_mesh_destroy(bpx_mesh m);
 /* counts: N Nq Np cubNq cubNp Nelements (int64 each) */
 int bpx_mesh_info(bpx_mesh m, int64_t* out, int capacity);
@@ -279,6 +289,11 @@ int bpx_mesh_array(bpx_mesh m, const char* name, void* dst, int64_t* count, int*
  * on one rank, Signed, unique: ids[N] (0 = masked), then the T maps and
  * GlobalToLocal (ogsSetup.cpp:833-849). Host-side; no device needed. */
 int bpx_ogs_setup(int N, const int64_t* ids, bpx_ogs* out);
+/* ogs_t::Setup on the device (GPU sort, scans and bisection in place of
+ * libs/ogs/ogsSetup.cpp:87-290 and ogsOperator.cpp:474-546 on the host):
+ * o_ids is a device array of N ids; the maps equal bpx_ogs_setup's. Host
+ * copies of the maps are made on first bpx_ogs_array request. */
+int bpx_ogs_setup_dev(bpx_platform p, int N, const int64_t* o_ids, bpx_ogs* out);
 /* mesh_t::MaskedGatherScatterSetup(Nfields) (meshMaskedGatherScatterSetup.
  * cpp:31-77): masked ids of the box, then ogs_t::Setup. */
 int bpx_ogs_setup_masked(bpx_mesh m, int Nfields, bpx_ogs* out);
@@ -315,6 +330,14 @@ int bpx_bp_info(bpx_bp b, int64_t* out, int capacity);
  * from the elements' trilinear maps, 0 if it reads stored factors rounded
  * to FP32, -1 if it has not been built yet. Returns the number of values. */
 int bpx_bp_tuning(bpx_bp b, double* out, int capacity);
+/* No reference counterpart: copy one of the problem's device arrays to host
+ * memory dst (NULL: query *count). Names: rowStartsT colIdsT blockRowStartsT
+ * GlobalToLocal (int32, the masked ogs), wJ ggeo (f64, operator factors),
+ * solvePerm solveGlobalToLocal solveRowStarts solveColIds solveBlocks
+ * sharedRowStarts sharedColIds sharedBlocks (int32, the fused BP5 operator's
+ * solve-space maps; empty when it has none). dtype as bpx_mesh_array, with
+ * 1 = int32. */
+int bpx_bp_array(bpx_bp b, const char* name, void* dst, int64_t* count, int* dtype);
 /* No reference counterpart: pin launch choices instead of the timed ones
  * (for reproducible timings; results do not depend on them): gather order
  * 0/1, FP64 operator fused 0/1, FP32 operator fused 0/1; -1 keeps a
@@ -335,12 +358,15 @@ int bpx_bp_set_fp32_geometry(bpx_bp b, int mode);
  * Covers the plain and the fused (CG) form; the two engines agree to
  * rounding, not bit for bit. Other problems run on the vector ALU. */
 int bpx_bp_set_ax_engine(bpx_bp b, int engine);
-/* No reference counterpart (launch choice, same results): with on = 1
- * (the default) the CG solves that use the fused BP5 operator do the
- * direction update of the elements' interior rows (x += alpha p, p = r +
- * beta p) inside that operator's Ax, and the separate update kernel covers
- * the shared rows only; on = 0 keeps one update kernel over all rows. */
-int bpx_bp_set_cg_fold(bpx_bp b, int on);
+/* No reference counterpart (launch choice, same results): mask = 1 (FP64
+ * solves), 2 (the FP32 iterations of the mixed solve) or 3 (both): those CG
+ * solves that use the fused BP5 operator do the direction update of the
+ * elements' interior rows (x += alpha p, p = r + beta p) inside that
+ * operator's Ax, and the separate update kernel covers the shared rows
+ * only; elsewhere (mask 0: everywhere) one update kernel covers all rows.
+ * Default 1: at 24^3, p = 7 the fold saves 2% of an FP64 iteration and
+ * costs 2% of an FP32 one, whose Ax is bound by instruction issue. */
+int bpx_bp_set_cg_fold(bpx_bp b, int mask);
 /* No reference counterpart (measurement): the CG operator as the solvers
  * apply it (their row order, the tuned fused or unfused form) in FP64
  * (precision 64) or FP32 (32), `launches` times back to back after 3
@@ -449,6 +475,17 @@ int bpx_batch_tuning(bpx_batch b, double* out, int capacity);
  * NULL). BPX_ERR_BREAKDOWN if any problem broke down. */
 int bpx_batch_solve(bpx_batch b, double* o_x, const double* o_b, double tol, int maxit,
                     int* iterations, double* rnorm);
+/* No reference counterpart (the reference solves in FP64 only): every
+ * problem solved by FP32 CG with FP64 defect corrections, the recurrence
+ * of bpx_bp_solve_mixed (FP32 vectors and operator with the geometric
+ * factors rounded to FP32, FP64 dot products; every `interval` FP32
+ * iterations x += c and r = b - A x in FP64, where ||b - A x|| <= tol ||b||
+ * is tested), x from 0, problems in lockstep segments replayed from one
+ * hipGraph. Host arrays of nprob (may be NULL): FP32 iterations, FP64
+ * corrections, final ||b - A x|| / ||b||. tol > 0, interval >= 1, maxit >= 1;
+ * BPX_ERR_BREAKDOWN if a residual became non-finite. */
+int bpx_batch_solve_mixed(bpx_batch b, double* o_x, const double* o_b, double tol, int interval,
+                          int maxit, int* iterations, int* updates, double* relres);
 
 #ifdef __cplusplus
 }

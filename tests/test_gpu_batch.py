@@ -108,3 +108,70 @@ def test_batch_single_problem_and_maxit(plat):
     it1, hist = bp.solve(xs, r, tol=0.0, maxit=37)
     assert it1 == 37
     assert abs(rn[0] - hist[-1]) <= 1e-9 * hist[0]
+
+
+@pytest.fixture(scope="module")
+def solved_mixed(plat, solved):
+    probs, B, rhs, _, _, _ = solved
+    x = torch.zeros(B.Ntot, dtype=torch.float64, device="cuda")
+    its, ups, rel = B.solve_mixed(x, rhs, tol=1e-10, interval=32, maxit=20000)
+    torch.cuda.synchronize()
+    return probs, B, rhs, x.cpu().numpy(), its, ups, rel
+
+
+def test_batch_mixed_true_residual_per_problem(plat, solved_mixed):
+    """FP32 CG with FP64 corrections, all 64 problems at once (the
+    batched form of the FP32+refinement solve): each problem's true FP64
+    residual, recomputed with its standalone FP64 operator, is below 1e-10
+    of ||b|| as reported, and the FP32 iteration counts are ragged."""
+    probs, B, rhs, x, its, ups, rel = solved_mixed
+    rhs_h = rhs.cpu().numpy()
+    assert len(set(its.tolist())) > 10
+    for i, pr in enumerate(probs):
+        a, b = int(B.offsets[i]), int(B.offsets[i + 1])
+        m = capi.Mesh(N=pr["N"], nx=pr["nx"], ny=pr["ny"], nz=pr["nz"], perturb=0.1, seed=pr["seed"])
+        bp = capi.BP(plat, m, 5, lam=1.0)
+        Ax = torch.zeros(b - a, dtype=torch.float64, device="cuda")
+        bp.operator(torch.from_numpy(x[a:b].copy()).cuda(), Ax)
+        nb = np.linalg.norm(rhs_h[a:b])
+        true_rel = np.linalg.norm(rhs_h[a:b] - Ax.cpu().numpy()) / nb
+        assert true_rel <= 1e-10, (i, true_rel)
+        assert abs(true_rel - rel[i]) <= 1e-3 * rel[i] + 1e-16, (i, true_rel, rel[i])
+        assert ups[i] >= 1 and its[i] >= 1
+
+
+def test_batch_mixed_matches_oracle_and_fp64_batch(plat, solved, solved_mixed):
+    """Against the oracle's FP64 CG solved to 1e-13 and the FP64 batch: the
+    mixed solutions are within 1e-8 of the converged solution (the FP64
+    batch at 1e-8 is within 1e-5), and the FP32 iteration counts stay near
+    the FP64 ones at the same tolerance."""
+    probs, B, rhs, x, its, ups, rel = solved_mixed
+    rhs_h = rhs.cpu().numpy()
+    oracle_lib.set_threads(16)
+    for i in range(0, len(probs), 4):
+        pr = probs[i]
+        a, b = int(B.offsets[i]), int(B.offsets[i + 1])
+        P = oracle_lib.Problem(bp=5, N=pr["N"], nx=pr["nx"], ny=pr["ny"], nz=pr["nz"], lam=1.0, perturb=0.1,
+                               seed=pr["seed"])
+        xt, it_t, _, _ = P.cg(rhs_h[a:b].copy(), 20000, tol=1e-13)
+        assert np.linalg.norm(x[a:b] - xt) <= 1e-8 * np.linalg.norm(xt), i
+        _, it10, _, _ = P.cg(rhs_h[a:b].copy(), 20000, tol=1e-10)
+        assert its[i] <= 1.5 * it10 + 64, (i, int(its[i]), it10)
+
+
+def test_batch_mixed_deterministic_and_limits(plat, solved_mixed):
+    probs, B, rhs, x, its, ups, rel = solved_mixed
+    x2 = torch.zeros(B.Ntot, dtype=torch.float64, device="cuda")
+    its2, ups2, rel2 = B.solve_mixed(x2, rhs, tol=1e-10, interval=32, maxit=20000)
+    np.testing.assert_array_equal(its2, its)
+    np.testing.assert_array_equal(ups2, ups)
+    np.testing.assert_array_equal(x2.cpu().numpy(), x)
+    # an iteration cap: every problem stops at the correction after it
+    x3 = torch.zeros_like(x2)
+    its3, ups3, rel3 = B.solve_mixed(x3, rhs, tol=1e-14, interval=16, maxit=40)
+    assert (its3 <= 40).all()
+    capped = rel3 > 1e-14
+    assert capped.sum() > 32
+    assert (its3[capped] == 40).all() and (ups3[capped] == 3).all()
+    with pytest.raises(capi.BpxError):
+        B.solve_mixed(x3, rhs, tol=0.0, interval=16, maxit=40)

@@ -322,8 +322,8 @@ def test_fp32_fused_operator_solves_identically(plat, N):
 @pytest.mark.parametrize("N,nx", [(2, 4), (3, 3), (7, 3), (8, 2), (15, 2)])
 def test_cg_fold_bit_identical(plat, N, nx):
     """The CG solves with the interior rows' direction update folded into
-    the fused operator (the default) and with one update kernel over all
-    rows: the same iterations, residual histories and solutions, bit for
+    the fused operator (mask 3: FP64 and FP32) and with one update kernel
+    over all rows: the same iterations, residual histories and solutions, bit for
     bit, for the FP64 CG stopped by its tolerance and for the mixed solve.
     The solution of a solve stopped by its tolerance carries every row's
     last update: its true residual is the recurrence's last one."""
@@ -332,7 +332,7 @@ def test_cg_fold_bit_identical(plat, N, nx):
     b.set_tuning(fused=1, fused32=1)
     rhs = capi.random_normal(12, b.Ngather)
     out = {}
-    for fold in (1, 0):
+    for fold in (3, 0):
         b.set_cg_fold(fold)
         x = torch.zeros(b.Ngather, dtype=torch.float64, device="cuda")
         its, hist = b.solve(x, dev(rhs), tol=1e-8, maxit=5000)
@@ -345,10 +345,10 @@ def test_cg_fold_bit_identical(plat, N, nx):
         st = b.solve_mixed(xm, dev(rhs), tol=1e-10, interval=16)
         out[fold] = (its, hist, host(x), st["iterations"], st["updates"], host(xm))
     b.set_cg_fold(1)
-    assert out[1][0] == out[0][0] and out[1][3] == out[0][3] and out[1][4] == out[0][4]
-    np.testing.assert_array_equal(out[1][1], out[0][1])
-    np.testing.assert_array_equal(out[1][2], out[0][2])
-    np.testing.assert_array_equal(out[1][5], out[0][5])
+    assert out[3][0] == out[0][0] and out[3][3] == out[0][3] and out[3][4] == out[0][4]
+    np.testing.assert_array_equal(out[3][1], out[0][1])
+    np.testing.assert_array_equal(out[3][2], out[0][2])
+    np.testing.assert_array_equal(out[3][5], out[0][5])
 
 
 def test_time_operator(plat):
