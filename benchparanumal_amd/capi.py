@@ -118,7 +118,10 @@ _SIGS["bpx_bp3_ax_hex_f32"] = (_I, [_P, _I, _I, _P, _P] + [_P] * 7 + [_F, _P, _P
 
 class _MeshOpts(C.Structure):
     _fields_ = [("N", _I), ("nx", _I), ("ny", _I), ("nz", _I), ("affine", _I), ("cubature", _I),
-                ("lo", _D * 3), ("dim", _D * 3), ("perturb", _D), ("seed", C.c_uint64)]
+                ("lo", _D * 3), ("dim", _D * 3), ("perturb", _D), ("seed", C.c_uint64), ("elementType", _I)]
+
+# element families (the reference's codes)
+TRI, QUAD, TET, HEX = 3, 4, 6, 12
 
 
 def lib():
@@ -209,7 +212,7 @@ class Mesh:
     `platform` when one is given (the same arrays, copied down on request)."""
 
     def __init__(self, N, nx, ny=None, nz=None, affine=False, cubature=False, lo=(-1.0, -1.0, -1.0),
-                 dim=(2.0, 2.0, 2.0), perturb=0.0, seed=0, platform=None):
+                 dim=(2.0, 2.0, 2.0), perturb=0.0, seed=0, platform=None, element=HEX):
         o = _MeshOpts()
         o.N, o.nx = N, nx
         o.ny = nx if ny is None else ny
@@ -218,6 +221,7 @@ class Mesh:
         for c in range(3):
             o.lo[c], o.dim[c] = lo[c], dim[c]
         o.perturb, o.seed = perturb, seed
+        o.elementType = element
         h = _P()
         if platform is not None:
             check(lib().bpx_mesh_create_dev(platform.h, C.byref(o), C.byref(h)))
@@ -225,9 +229,9 @@ class Mesh:
             check(lib().bpx_mesh_create(C.byref(o), C.byref(h)))
         self.h = h
         self.platform = platform
-        buf = (C.c_int64 * 8)()
-        n = lib().bpx_mesh_info(self.h, buf, 8)
-        keys = ["N", "Nq", "Np", "cubNq", "cubNp", "Nelements"]
+        buf = (C.c_int64 * 9)()
+        n = lib().bpx_mesh_info(self.h, buf, 9)
+        keys = ["N", "Nq", "Np", "cubNq", "cubNp", "Nelements", "elementType", "dim", "Nverts"]
         self.info = {k: int(buf[i]) for i, k in enumerate(keys[:n])}
         self.__dict__.update(self.info)
         self.nx, self.ny, self.nz = o.nx, o.ny, o.nz

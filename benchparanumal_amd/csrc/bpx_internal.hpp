@@ -218,6 +218,20 @@ int launch_batch_gather(hipStream_t s, int nsupers, const BatchSuper* supers, co
                         const int* colIds, const double* q, double* gq, const double* p, double* part,
                         const BatchState* st);
 int launch_batch_count_done(hipStream_t s, int nprob, const BatchState* st, int* count);
+// quads, triangles and tetrahedra (ax_other.hip, FP64): bp 1..6 on the
+// [element][field][node] local vectors. Quads: W = wJ (BP5/6, per node) or
+// cubwJ (BP1-4, per Gauss point), G = ggeo or cubggeo (3 per point), Dm = D
+// (Nq x Nq) or cubD ((Nq+1)^2), I = the Gauss interpolation. Simplices: wJ
+// and ggeo per element, S the packed stiffness, MM the mass matrix.
+int launch_quad_ax(hipStream_t s, int bp, int N, int Nfields, int Nelements, const int* elementList, const int* G2L,
+                   const double* W, const double* G, const double* Dm, const double* I, double lambda,
+                   const double* q, double* Aq);
+int launch_simplex_ax(hipStream_t s, int bp, int dim, int Np, int Nfields, int Nelements, const int* elementList,
+                      const int* G2L, const double* wJ, const double* ggeo, const double* S, const double* MM,
+                      double lambda, const double* q, double* Aq);
+int launch_rhs_other(hipStream_t s, int elementType, int N, int Np, int Nfields, int Nelements, const double* wJ,
+                     const double* MM, const double* x, const double* y, const double* z, double lambda,
+                     double* rhs);
 // FP32 Ax of the mixed batch (factors rounded to FP32), the one workgroup shape
 int launch_bp5_batch_f32(hipStream_t s, int N, int Nelements, const int* elementList, const int* G2L,
                          const float* wJ, const float* ggeo, const float* D, float lambda, const float* q,

@@ -62,6 +62,7 @@ static int to_mesh_opts(const bpx_mesh_opts& o, MeshOpts& mo) {
   }
   mo.perturb = o.perturb;
   mo.seed = o.seed;
+  mo.elementType = o.elementType == 0 ? ELEM_HEX : o.elementType;
   return BPX_OK;
 }
 
@@ -554,7 +555,8 @@ int bpx_mesh_destroy(bpx_mesh m) {
 
 int bpx_mesh_info(bpx_mesh m, int64_t* out, int capacity) {
   if (int st = need(m, "mesh")) return st;
-  const int64_t v[] = {m->m.N, m->m.Nq, m->m.Np, m->m.cubNq, m->m.cubNp, m->m.Nelements};
+  const int64_t v[] = {m->m.N,         m->m.Nq,   m->m.Np,     m->m.cubNq, m->m.cubNp,
+                       m->m.Nelements, m->m.elementType, m->m.dim, m->m.Nverts};
   const int n = (int)(sizeof(v) / sizeof(v[0]));
   for (int i = 0; i < n && i < capacity; ++i) out[i] = v[i];
   return n;
@@ -570,6 +572,7 @@ int bpx_mesh_array(bpx_mesh m, const char* name, void* dst, int64_t* count, int*
 #define F64(f) if (s == #f) return copy_out(M.f, dst, count, dtype, 0);
   F64(gllz) F64(gllw) F64(D) F64(MM) F64(cubr) F64(cubw) F64(cubInterp) F64(cubD) F64(invV)
   F64(x) F64(y) F64(z) F64(wJ) F64(ggeo) F64(cubwJ) F64(cubggeo) F64(EX) F64(EY) F64(EZ)
+  F64(rN) F64(sN) F64(tN) F64(Dsim) F64(Ssim)
 #undef F64
   if (s == "globalIds") return copy_out(M.globalIds, dst, count, dtype, 2);
   return set_error(BPX_ERR_INVALID_ARGUMENT, "mesh: unknown array " + s);

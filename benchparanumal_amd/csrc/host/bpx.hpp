@@ -122,12 +122,16 @@ class linAlg_t {
 };
 
 // -------------------------------------------------------------- mesh ----
+// element families, by the reference's codes (libs/mesh/meshSetElementType.cpp)
+enum { ELEM_TRI = 3, ELEM_QUAD = 4, ELEM_TET = 6, ELEM_HEX = 12 };
+
 struct MeshOpts {
   int N = 1, nx = 1, ny = 1, nz = 1;
   int affine = 0, cubature = 0;
   double lo[3] = {-1, -1, -1}, dim[3] = {2, 2, 2};
   double perturb = 0.0;
   uint64_t seed = 0;
+  int elementType = ELEM_HEX;  // 0 is taken as ELEM_HEX
 };
 
 // mesh_t for the hex box (libs/mesh/meshSetup.cpp:31-85), host data.
@@ -168,6 +172,17 @@ class mesh_t {
   platform_t* devPlatform = nullptr;
   deviceBuffer<double> o_x, o_y, o_z, o_wJ, o_ggeo, o_cubwJ, o_cubggeo;
   deviceBuffer<hlong> o_globalIds;
+
+  // quad, triangle and tetrahedral boxes (host/mesh_other.cpp)
+  int SetupOther(const MeshOpts& o);
+  int elementType = ELEM_HEX, dim = 3, Nverts = 8, Nggeo = 6;
+  // triangles and tetrahedra: reference nodes, their lattice indices, the
+  // packed D (dim x Np x Np) and stiffness matrices (3 or 6 x Np x Np); MM
+  // holds their Np x Np mass matrix (1D mass for hex and quad)
+  std::vector<double> rN, sN, tN, Dsim, Ssim;
+  std::vector<int> nodeIdx;
+  // non-hex meshes: 1 for a local node on the box boundary
+  std::vector<char> boundary;
 
   MeshOpts opts;
   int N = 0, Nq = 0, Np = 0, cubNq = 0, cubNp = 0;
@@ -474,6 +489,7 @@ class bp_t : public solver_t {
   std::vector<dlong> GlobalToLocal;
   deviceBuffer<dlong> o_GlobalToLocal, o_elementList;
   deviceBuffer<double> o_wJ, o_ggeo, o_gllw, o_D, o_I, o_MM, o_invV;
+  deviceBuffer<double> o_S;  // triangles and tetrahedra: the packed stiffness matrices
   deviceBuffer<double> o_AqL;
   deviceBuffer<double> o_AqLk;  // K local vectors for OperatorKrhs, grown on demand
   // variable coefficient: lattice point of each local node, the lattice

@@ -85,6 +85,13 @@ bool mesh_t::NodesAreTrilinear() const {
 }
 
 int mesh_t::SetupVertices(const MeshOpts& o) {
+  if (o.elementType != 0 && o.elementType != ELEM_HEX)
+    return set_error(BPX_ERR_INVALID_ARGUMENT, "mesh: the vertex setup shared with the device covers hexahedra");
+  elementType = ELEM_HEX;
+  dim = 3;
+  Nverts = 8;
+  Nggeo = 6;
+  boundary.clear();
   if (o.N < 1 || o.N > 15) return set_error(BPX_ERR_INVALID_ARGUMENT, "mesh: degree N must be in 1..15");
   if (o.nx < 1 || o.ny < 1 || o.nz < 1) return set_error(BPX_ERR_INVALID_ARGUMENT, "mesh: nx, ny, nz must be >= 1");
   if (!(o.dim[0] > 0 && o.dim[1] > 0 && o.dim[2] > 0))
@@ -156,6 +163,7 @@ int mesh_t::SetupVertices(const MeshOpts& o) {
 }
 
 int mesh_t::Setup(const MeshOpts& o) {
+  if (o.elementType != 0 && o.elementType != ELEM_HEX) return SetupOther(o);
   if (int st = SetupVertices(o)) return st;
   const int nx = o.nx, ny = o.ny;
 
@@ -314,6 +322,16 @@ int mesh_t::Setup(const MeshOpts& o) {
 void mesh_t::MaskedGlobalIds(int Nfields, std::vector<hlong>& ids) const {
   const size_t NL = (size_t)Nelements * Np;
   ids.assign(NL * Nfields, 0);
+  if (elementType != ELEM_HEX) {  // boundary flags per node
+#pragma omp parallel for
+    for (dlong e = 0; e < Nelements; ++e)
+      for (int n = 0; n < Np; ++n) {
+        const size_t l = (size_t)e * Np + n;
+        for (int f = 0; f < Nfields; ++f)
+          ids[(size_t)e * Np * Nfields + (size_t)f * Np + n] = boundary[l] ? 0 : globalIds[l] * Nfields + f;
+      }
+    return;
+  }
   const long long LX = (long long)opts.nx * N + 1, LY = (long long)opts.ny * N + 1,
                   LZ = (long long)opts.nz * N + 1;
 #pragma omp parallel for

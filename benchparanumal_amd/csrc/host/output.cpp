@@ -39,6 +39,7 @@ static void cube_tets(int tets[6][4]) {
 
 int mesh_t::PlotFields(const double* QL, int Nfields, const std::string& fileName) const {
   if (!QL || Nfields < 1) return set_error(BPX_ERR_INVALID_ARGUMENT, "plot: no field data");
+  if (elementType != ELEM_HEX) return set_error(BPX_ERR_UNSUPPORTED, "plot: the VTU writer covers hexahedra");
   std::unique_ptr<FILE, int (*)(FILE*)> fp(std::fopen(fileName.c_str(), "w"), &std::fclose);
   if (!fp) return set_error(BPX_ERR_INVALID_ARGUMENT, "plot: cannot open " + fileName);
   FILE* f = fp.get();
@@ -98,6 +99,7 @@ int bp_t::PlotSolution(const dfloat* o_u, const std::string& fileName) {
   // ogs.Scatter(o_xL, o_x) then mesh.PlotFields (BP/BP5/src/bp5Run.cpp:224-237);
   // Dirichlet nodes (no row) are written as 0
   if (int st = mesh->EnsureHost()) return st;  // node coordinates on the host
+  if (mesh->elementType != ELEM_HEX) return set_error(BPX_ERR_UNSUPPORTED, "plot: the VTU writer covers hexahedra");
   hipStream_t s = platform->stream;
   if (int st = launch_set<double>(s, NLocal, 0.0, o_AqL.ptr())) return st;
   if (int st = ogs.Scatter<double>(*platform, o_AqL.ptr(), o_u, 1)) return st;
@@ -109,6 +111,7 @@ int bp_t::PlotSolution(const dfloat* o_u, const std::string& fileName) {
 
 int bp_t::ErrorNorms(const dfloat* o_u, double* out) {
   if (Nfields != 1) return set_error(BPX_ERR_UNSUPPORTED, "error norms: one-field problems (BP1, BP3, BP5)");
+  if (mesh->elementType != ELEM_HEX) return set_error(BPX_ERR_UNSUPPORTED, "error norms: hexahedral meshes");
   if (int st = mesh->EnsureHost()) return st;  // wJ on the host
   const mesh_t& m = *mesh;
   hipStream_t s = platform->stream;

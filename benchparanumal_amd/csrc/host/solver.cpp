@@ -273,13 +273,16 @@ int cg::Solve(solver_t& solver, dfloat* o_x, dfloat* o_r, dfloat tol, int MAXIT,
 int bp_t::Setup(platform_t& p, mesh_t& m, int bp_, dfloat lambda_) {
   if (bp_ < 1 || bp_ > 6) return set_error(BPX_ERR_INVALID_ARGUMENT, "bp: bp must be in 1..6");
   const bool needCub = bp_ <= 4;
-  if (needCub && m.cubD.empty())
+  const bool simplexMesh = m.elementType == ELEM_TRI || m.elementType == ELEM_TET;
+  if (needCub && !simplexMesh && m.cubD.empty())
     return set_error(BPX_ERR_INVALID_ARGUMENT, "bp: BP1-BP4 need a mesh built with cubature=1");
   platform = &p;
   mesh = &m;
   bp = bp_;
   lambda = lambda_;
-  Nfields = (bp == 2 || bp == 4 || bp == 6) ? 3 : 1;
+  Nfields = (bp == 2 || bp == 4 || bp == 6) ? m.dim : 1;
+  if (m.elementType != ELEM_HEX && m.onDevice)
+    return set_error(BPX_ERR_UNSUPPORTED, "bp: the device setup covers hexahedra");
   BPX_HIP(hipSetDevice(p.device));
   hipStream_t s = p.stream;
 
@@ -314,7 +317,23 @@ int bp_t::Setup(platform_t& p, mesh_t& m, int bp_, dfloat lambda_) {
   auto factors = [&](deviceBuffer<double>& dst, const std::vector<double>& h, const deviceBuffer<double>& d) {
     return m.onDevice ? dst.copy_device(d.ptr(), d.length(), s) : dst.upload(h, s);
   };
-  if (bp == 5 || bp == 6) {
+  if (simplexMesh) {  // per-element J and G, the reference matrices
+    if (int st = o_wJ.upload(m.wJ, s)) return st;
+    if (int st = o_ggeo.upload(m.ggeo, s)) return st;
+    if (int st = o_MM.upload(m.MM, s)) return st;
+    if (int st = o_S.upload(m.Ssim, s)) return st;
+  } else if (m.elementType == ELEM_QUAD) {  // per-node factors; the Gauss forms for BP1-BP4
+    if (bp == 5 || bp == 6) {
+      if (int st = o_wJ.upload(m.wJ, s)) return st;
+      if (int st = o_ggeo.upload(m.ggeo, s)) return st;
+      if (int st = o_D.upload(m.D, s)) return st;
+    } else {
+      if (int st = o_wJ.upload(m.cubwJ, s)) return st;
+      if (int st = o_ggeo.upload(m.cubggeo, s)) return st;
+      if (int st = o_D.upload(m.cubD, s)) return st;
+      if (int st = o_I.upload(m.cubInterp, s)) return st;
+    }
+  } else if (bp == 5 || bp == 6) {
     if (int st = factors(o_wJ, m.wJ, m.o_wJ)) return st;
     if (int st = factors(o_ggeo, m.ggeo, m.o_ggeo)) return st;
     if (int st = o_gllw.upload(m.gllw, s)) return st;
@@ -360,6 +379,17 @@ int bp_t::ax(const T* q, T* AqL, const T* wJ, const T* ggeo, const T* gllw, cons
   if constexpr (std::is_same<T, float>::value) a.tri = tri32;
   a.evStart = axEvStart;
   a.evStop = axEvStop;
+  if (mesh->elementType != ELEM_HEX) {
+    if constexpr (std::is_same<T, double>::value) {
+      if (mesh->elementType == ELEM_QUAD)
+        return launch_quad_ax(platform->stream, bp, mesh->N, Nfields, mesh->Nelements, a.elementList, a.GlobalToLocal,
+                              wJ, ggeo, D, I, lambda, q, AqL);
+      return launch_simplex_ax(platform->stream, bp, mesh->dim, mesh->Np, Nfields, mesh->Nelements, a.elementList,
+                               a.GlobalToLocal, wJ, ggeo, o_S.ptr(), MM, lambda, q, AqL);
+    } else {
+      return set_error(BPX_ERR_UNSUPPORTED, "bp: the FP32 operators cover hexahedra");
+    }
+  }
   if (bp == 5 || bp == 6) return launch_bp5_ax<T>(platform->stream, a);
   if (bp == 1 || bp == 2) return launch_bp1_ax<T>(platform->stream, a);
   return launch_bp3_ax<T>(platform->stream, a);
@@ -383,7 +413,8 @@ int bp_t::SetKappa(const dfloat* o_kappa) {
     haveKappa = false;
     return BPX_OK;
   }
-  if (bp != 5) return set_error(BPX_ERR_UNSUPPORTED, "bp: a variable coefficient is implemented for BP5");
+  if (bp != 5 || mesh->elementType != ELEM_HEX)
+    return set_error(BPX_ERR_UNSUPPORTED, "bp: a variable coefficient is implemented for BP5 on hexahedra");
   hipStream_t s = platform->stream;
   const mesh_t& m = *mesh;
   const hlong Nnodes = LatticeNodes();
@@ -501,7 +532,7 @@ int bp_t::PrepareFused() {
   fused = false;
   permuted = false;
   Ns = ogs.Ngather;
-  if (bp != 5 || Nfields != 1) return BPX_OK;
+  if (bp != 5 || Nfields != 1 || mesh->elementType != ELEM_HEX) return BPX_OK;
   const dlong Ng = ogs.Ngather, Nel = mesh->Nelements;
   const int Nq = mesh->Nq, Np = mesh->Np;
   const long long NIP = fused_interior_pad(Nq), NSP = fused_surface_pad(Nq);
@@ -618,7 +649,8 @@ template int bp_t::OperatorDotFused<double>(const double*, double*, double*, int
 template int bp_t::OperatorDotFused<float>(const float*, float*, double*, int*, const CgFold<float>*);
 
 int bp_t::AxLocalKrhs(int K, const dfloat* o_q, dfloat* o_AqL, int chunk) {
-  if (bp != 5) return set_error(BPX_ERR_UNSUPPORTED, "bp: k right-hand sides are implemented for BP5");
+  if (bp != 5 || mesh->elementType != ELEM_HEX)
+    return set_error(BPX_ERR_UNSUPPORTED, "bp: k right-hand sides are implemented for BP5 on hexahedra");
   if (haveKappa) return set_error(BPX_ERR_UNSUPPORTED, "bp: k right-hand sides use the constant coefficient");
   AxArgs<double> a;
   a.N = mesh->N;
@@ -897,6 +929,7 @@ int bp_t::TimeOperator(int precision, int launches, double* out) {
 
 int bp_t::Prepare32() {
   if (have32) return BPX_OK;
+  if (mesh->elementType != ELEM_HEX) return set_error(BPX_ERR_UNSUPPORTED, "bp: the FP32 operators cover hexahedra");
   hipStream_t s = platform->stream;
   auto cvt = [&](deviceBuffer<float>& dst, const std::vector<double>& src) {
     std::vector<float> f(src.begin(), src.end());
@@ -970,7 +1003,8 @@ int bp_t::NodeCoordinates() {
   } else {
     if (int st = o_x.upload(m.x, s)) return st;
     if (int st = o_y.upload(m.y, s)) return st;
-    if (int st = o_z.upload(m.z, s)) return st;
+    if (m.dim == 3)
+      if (int st = o_z.upload(m.z, s)) return st;
   }
   return o_gllwRhs.upload(m.gllw, s);
 }
@@ -980,6 +1014,17 @@ int bp_t::Forcing(dfloat* o_b) {
   hipStream_t s = platform->stream;
   const mesh_t& m = *mesh;
   if (int st = NodeCoordinates()) return st;
+  if (m.elementType != ELEM_HEX) {
+    deviceBuffer<double> wJ, MMd;
+    if (int st = wJ.upload(m.wJ, s)) return st;
+    if (int st = MMd.upload(m.MM, s)) return st;
+    const double lam = (bp <= 2) ? 0.0 : lambda;
+    if (int st = launch_rhs_other(s, m.elementType, m.N, m.Np, Nfields, m.Nelements, wJ.ptr(), MMd.ptr(), o_x.ptr(),
+                                  o_y.ptr(), m.dim == 3 ? o_z.ptr() : o_x.ptr(), lam, o_AqL.ptr()))
+      return st;
+    if (int st = ogs.Gather<double>(*platform, o_b, o_AqL.ptr(), 1)) return st;
+    return platform->finish();
+  }
   // the forcing uses the GLL-weighted wJ of the mesh (mesh.o_wJ)
   deviceBuffer<double> wJ;
   if (int st = m.onDevice ? wJ.copy_device(m.o_wJ.ptr(), m.o_wJ.length(), s) : wJ.upload(m.wJ, s)) return st;

@@ -254,7 +254,12 @@ int bpx_cg_solve_noalloc(void* hip_stream, int N, bpx_operator_fn op, void* ctx,
  * interior-vertex displacement as a fraction of h (0 = the reference's
  * undistorted box). cubature: also build the Gauss (p+2) factors
  * (meshCubatureSetupHex3D.cpp). Host-side setup; copies to the device on
- * first use. */
+ * first use. elementType: the reference's codes (meshSetElementType.cpp):
+ * 12 (or 0) hexahedra, 4 quadrilaterals (meshSetupBoxQuad2D.cpp), 3
+ * triangles (meshSetupBoxTri2D.cpp, two per square), 6 tetrahedra
+ * (meshSetupBoxTet3D.cpp, six per cube); the 2D boxes ignore nz, lo[2] and
+ * dim[2]. Triangles and tetrahedra are straight-sided (affine per element
+ * whatever `affine` says); cubature factors exist for hex and quad. */
 typedef struct {
   int N, nx, ny, nz;
   int affine;
@@ -262,6 +267,7 @@ typedef struct {
   double lo[3], dim[3];
   double perturb;
   uint64_t seed;
+  int elementType;
 } bpx_mesh_opts;
 
 int bpx_mesh_create(const bpx_mesh_opts* opts, bpx_mesh* out);
@@ -276,10 +282,12 @@ int bpx_mesh_create(const bpx_mesh_opts* opts, bpx_mesh* out);
  * bpx_mesh_plot_fields). The mesh keeps using p's device and stream. */
 int bpx_mesh_create_dev(bpx_platform p, const bpx_mesh_opts* opts, bpx_mesh* out);
 int bpx_mesh_destroy(bpx_mesh m);
-/* counts: N Nq Np cubNq cubNp Nelements (int64 each) */
+/* counts: N Nq Np cubNq cubNp Nelements elementType dim Nverts (int64 each) */
 int bpx_mesh_info(bpx_mesh m, int64_t* out, int capacity);
 /* Copy a host array of the mesh (names: gllz gllw D MM cubr cubw cubInterp
- * cubD invV x y z wJ ggeo cubwJ cubggeo EX EY EZ globalIds) into dst.
+ * cubD invV x y z wJ ggeo cubwJ cubggeo EX EY EZ globalIds; triangles and
+ * tetrahedra also rN sN tN (reference nodes), Dsim (the dim packed
+ * derivative matrices) and Ssim (the packed stiffness matrices)) into dst.
  * *count gets the element count; dtype 0 = f64, 2 = i64. dst may be NULL
  * to query the size. */
 int bpx_mesh_array(bpx_mesh m, const char* name, void* dst, int64_t* count, int* dtype);

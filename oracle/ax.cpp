@@ -242,6 +242,7 @@ void bp3_element(const Problem& P, dlong e, int f, const double* q, double* Aq) 
 // BPn split the same elements into two launches (BP/BP5/src/bp5Operator.cpp:
 // 29-85), which writes the same AqL.
 void ax_local(const Problem& P, const double* q, double* AqL) {
+  if (P.elementType != 12) return ax_local_other(P, q, AqL);
   const dlong Nel = P.Nelements;
 #pragma omp parallel for schedule(static)
   for (dlong ee = 0; ee < Nel; ++ee) {
@@ -276,6 +277,7 @@ void ogs_scatter(const Problem& P, const double* v, double* vL) {
 
 // okl/rhsHex3D.okl:28-128 (forcing times wJ at the local nodes)
 void rhs_forcing_local(const Problem& P, double* rhsL) {
+  if (P.elementType != 12) return rhs_forcing_local_other(P, rhsL);
   const dlong Nel = P.Nelements;
   const int Nq = P.Nq, Np = P.Np, Nf = P.Nfields;
   const double PI = M_PI, lambda = P.lambda;

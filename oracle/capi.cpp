@@ -76,6 +76,32 @@ void* orc_problem_create(int bp, int N, int nx, int ny, int nz, int affine, doub
   return P;
 }
 
+// a quad (4), triangle (3) or tetrahedron (6) box problem
+void* orc_problem_create_family(int elementType, int bp, int N, int nx, int ny, int nz, double lambda,
+                                const double* lo, const double* dim, double perturb, unsigned long long seed) {
+  Problem* P = new Problem;
+  P->elementType = elementType;
+  P->bp = bp;
+  P->N = N;
+  P->nx = nx;
+  P->ny = ny;
+  P->nz = nz;
+  P->lambda = lambda;
+  for (int c = 0; c < 3; ++c) {
+    if (lo) P->lo[c] = lo[c];
+    if (dim) P->dim[c] = dim[c];
+  }
+  P->perturb = perturb;
+  P->seed = seed;
+  const int st = problem_setup(*P);
+  if (st) {
+    g_err = P->error.empty() ? "setup failed" : P->error;
+    delete P;
+    return nullptr;
+  }
+  return P;
+}
+
 void orc_problem_destroy(void* h) { delete static_cast<Problem*>(h); }
 
 // ints: bp N Nq Np cubNq cubNp Nelements Nfields affine NLocal Ngather nnzT NrowBlocksT
@@ -111,7 +137,7 @@ const void* orc_problem_array(void* h, const char* name, long long* n, int* dtyp
   }
   F64(r) F64(s) F64(t) F64(gllz) F64(gllw) F64(D) F64(MM) F64(V) F64(cubr) F64(cubw)
   F64(cubInterp) F64(cubProject) F64(cubD) F64(invV) F64(EX) F64(EY) F64(EZ) F64(x) F64(y)
-  F64(z) F64(wJ) F64(ggeo) F64(cubwJ) F64(cubggeo)
+  F64(z) F64(wJ) F64(ggeo) F64(cubwJ) F64(cubggeo) F64(Dsim) F64(Ssim)
   I32(faceNodes) I32(EToE) I32(EToF) I32(vmapM) I32(vmapP) I32(elementList)
   I32(rowStartsT) I32(colIdsT) I32(blockRowStartsT) I32(GlobalToLocal)
   I64(EToV) I64(globalIds) I64(maskedIds)

@@ -588,7 +588,11 @@ static int OgsSetup(Problem& P) {
   return 0;
 }
 
+int ogs_setup_maps(Problem& P) { return OgsSetup(P); }
+
 int problem_setup(Problem& P) {
+  if (P.elementType != 12) return problem_setup_other(P);
+  P.dimension = 3;
   if (P.N < 1 || P.N > 15 || P.nx < 1 || P.ny < 1 || P.nz < 1) {
     P.error = "invalid problem size";
     return -1;

@@ -23,6 +23,11 @@
 //             multiplicities, hand-derived blockRows answers.
 //   CG      : README.md:109-113 residual norms (soft, 1e-6), textbook
 //             properties (A-orthogonality, residual recurrence).
+//   quad/tri/tet (other.cpp): node sets (edge nodes = GLL, symmetry),
+//             MM and D exact on polynomials (numpy monomial integrals and
+//             derivatives), S = D^T MM D, areas/volumes, node grouping =
+//             coincident coordinates, Ax against numpy-assembled element
+//             matrices.
 // ============================================================================
 #pragma once
 #include <cstdint>
@@ -68,6 +73,12 @@ struct Problem {
   uint64_t seed = 0;
   int mask = 1;  // 0: keep boundary nodes (no Dirichlet mask)
   int Nverts = 8, Nfaces = 6;
+  // element family (libs/mesh/meshSetElementType.cpp codes): 12 hex,
+  // 4 quad, 3 triangle, 6 tetrahedron (other.cpp); dimension 2 or 3
+  int elementType = 12, dimension = 3;
+  // triangles / tetrahedra: packed D (dimension x Np x Np) and stiffness
+  // (3 or 6 x Np x Np); MM then holds the Np x Np mass matrix
+  std::vector<double> Dsim, Ssim;
 
   // reference element
   std::vector<double> r, s, t;
@@ -98,6 +109,8 @@ struct Problem {
 };
 
 int problem_setup(Problem& P);
+int problem_setup_other(Problem& P);  // quad / tri / tet (other.cpp)
+int ogs_setup_maps(Problem& P);       // masked ids -> T maps (mesh.cpp)
 int blockRows(dlong Nrows, const std::vector<dlong>& rowStarts, dlong& Nblocks,
               std::vector<dlong>& blockStarts);
 
@@ -106,6 +119,8 @@ void ax_local(const Problem& P, const double* q, double* AqL);
 void ogs_gather(const Problem& P, const double* vL, double* v);
 void ogs_scatter(const Problem& P, const double* v, double* vL);
 void rhs_forcing_local(const Problem& P, double* rhsL);
+void ax_local_other(const Problem& P, const double* q, double* AqL);
+void rhs_forcing_local_other(const Problem& P, double* rhsL);
 
 // ------------------------------------- variable coefficient and adjoint --
 hlong lattice_index(const Problem& P, dlong e, int n);

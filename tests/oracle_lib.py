@@ -42,6 +42,8 @@ def lib():
     L.orc_rng_normal.restype = C.c_double
     L.orc_problem_create.argtypes = [C.c_int] * 6 + [C.c_double, _D, _D, C.c_double, C.c_ulonglong]
     L.orc_problem_create.restype = C.c_void_p
+    L.orc_problem_create_family.argtypes = [C.c_int] * 6 + [C.c_double, _D, _D, C.c_double, C.c_ulonglong]
+    L.orc_problem_create_family.restype = C.c_void_p
     L.orc_problem_destroy.argtypes = [C.c_void_p]
     L.orc_problem_info.argtypes = [C.c_void_p, C.POINTER(C.c_longlong)]
     L.orc_problem_array.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_longlong), _I]
@@ -124,13 +126,17 @@ class Problem:
     """One BP problem on a box mesh, built by the oracle's setup restatement."""
 
     def __init__(self, bp=5, N=2, nx=2, ny=None, nz=None, affine=False, lam=1.0,
-                 lo=(-1.0, -1.0, -1.0), dim=(2.0, 2.0, 2.0), perturb=0.0, seed=0):
+                 lo=(-1.0, -1.0, -1.0), dim=(2.0, 2.0, 2.0), perturb=0.0, seed=0, element=12):
         ny = nx if ny is None else ny
         nz = nx if nz is None else nz
         lo_a = np.ascontiguousarray(lo, dtype=np.float64)
         dim_a = np.ascontiguousarray(dim, dtype=np.float64)
-        h = lib().orc_problem_create(bp, N, nx, ny, nz, int(affine), lam, dptr(lo_a),
-                                     dptr(dim_a), perturb, seed)
+        if element == 12:
+            h = lib().orc_problem_create(bp, N, nx, ny, nz, int(affine), lam, dptr(lo_a),
+                                         dptr(dim_a), perturb, seed)
+        else:  # quad (4), triangle (3), tetrahedron (6)
+            h = lib().orc_problem_create_family(element, bp, N, nx, ny, nz, lam, dptr(lo_a),
+                                                dptr(dim_a), perturb, seed)
         if not h:
             raise RuntimeError(lib().orc_last_error().decode())
         self.h = C.c_void_p(h)
