@@ -110,6 +110,15 @@ constexpr bool bp5_dreg() {
 // cache line of Ap and Aq is written whole by one workgroup.
 enum { VAR_PLAIN = 0, VAR_KRHS = 1, VAR_BATCH = 2, VAR_FUSED = 3 };
 
+// k-RHS workgroup kernel: geometry loads non-temporal (1) or cached (0);
+// elements dealt to the XCDs round-robin (0) or in contiguous eighths (1)
+#ifndef BPX_KRHS_GEO_NT
+#define BPX_KRHS_GEO_NT 1
+#endif
+#ifndef BPX_KRHS_XCD_BLOCK
+#define BPX_KRHS_XCD_BLOCK 0
+#endif
+
 // Geometry loads of the fused (CG) operator are non-temporal: an FP32 CG
 // iteration cycles ~340 MB through the 256 MB Infinity Cache, ~200 MB of it
 // the geometric factors, read once per iteration. Streaming them past the
@@ -580,7 +589,12 @@ __global__ __launch_bounds__(NQ* NQ* W) __attribute__((amdgpu_waves_per_eu(DR ? 
   const int xcd = blockIdx.x & 7;
   const int m = blockIdx.x >> 3;
   const int pass = m % npass;
+#if BPX_KRHS_XCD_BLOCK
+  // each XCD a contiguous eighth of the elements (neighbours share q lines in its L2)
+  const int eo = xcd * (int)((Nelements + 7) / 8) + m / npass;
+#else
   const int eo = (m / npass) * 8 + xcd;
+#endif
   if (eo >= Nelements) return;  // uniform over the workgroup, before any barrier
   const int e = elementList[eo];
 
@@ -612,7 +626,11 @@ __global__ __launch_bounds__(NQ* NQ* W) __attribute__((amdgpu_waves_per_eu(DR ? 
 #pragma unroll
     for (int u = 0; u < GMAX; ++u) {
       const int f = t + u * NT;
+#if BPX_KRHS_GEO_NT
       gv[u] = (f < 6 * NQ2) ? __builtin_nontemporal_load(gs + f) : (f < NG ? __builtin_nontemporal_load(ws + (f - 6 * NQ2)) : T(0));
+#else
+      gv[u] = (f < 6 * NQ2) ? gs[f] : (f < NG ? ws[f - 6 * NQ2] : T(0));
+#endif
     }
   };
   auto store_slab = [&](int buf) {

@@ -44,18 +44,24 @@ void close(const char* what, const std::vector<double>& a, const std::vector<dou
 struct Config {
   int bp, N, nx, ny, nz, affine;
   double perturb;
+  int elementType;  // 12 hex, 4 quad, 3 triangle, 6 tetrahedron
 };
 
 }  // namespace
 
 int main() {
-  const Config configs[] = {{5, 1, 1, 1, 1, 0, 0.0}, {5, 2, 2, 3, 2, 0, 0.1}, {3, 3, 2, 2, 1, 0, 0.1},
-                            {1, 4, 3, 2, 2, 1, 0.0}, {6, 5, 2, 1, 2, 0, 0.2}, {4, 7, 2, 2, 2, 0, 0.1},
-                            {5, 8, 1, 2, 1, 1, 0.0}, {2, 11, 1, 1, 2, 0, 0.1}, {5, 15, 1, 1, 1, 0, 0.1}};
+  const Config configs[] = {{5, 1, 1, 1, 1, 0, 0.0, 12}, {5, 2, 2, 3, 2, 0, 0.1, 12}, {3, 3, 2, 2, 1, 0, 0.1, 12},
+                            {1, 4, 3, 2, 2, 1, 0.0, 12}, {6, 5, 2, 1, 2, 0, 0.2, 12}, {4, 7, 2, 2, 2, 0, 0.1, 12},
+                            {5, 8, 1, 2, 1, 1, 0.0, 12}, {2, 11, 1, 1, 2, 0, 0.1, 12}, {5, 15, 1, 1, 1, 0, 0.1, 12},
+                            {5, 4, 3, 2, 1, 0, 0.1, 4},  {3, 3, 2, 3, 1, 0, 0.1, 4},   {2, 2, 2, 2, 1, 0, 0.0, 4},
+                            {5, 5, 3, 2, 1, 0, 0.1, 3},  {6, 3, 2, 2, 1, 0, 0.2, 3},   {5, 3, 2, 2, 2, 0, 0.1, 6},
+                            {4, 2, 1, 2, 1, 0, 0.1, 6}};
   for (const Config& c : configs) {
-    std::printf("BP%d p=%d %dx%dx%d affine=%d perturb=%.1f\n", c.bp, c.N, c.nx, c.ny, c.nz, c.affine, c.perturb);
-    const bool cub = c.bp <= 4;
-    const int Nfields = (c.bp % 2 == 0) ? 3 : 1;
+    std::printf("BP%d p=%d %dx%dx%d affine=%d perturb=%.1f element %d\n", c.bp, c.N, c.nx, c.ny, c.nz, c.affine,
+                c.perturb, c.elementType);
+    const bool hex = c.elementType == 12;
+    const bool cub = c.bp <= 4 && (hex || c.elementType == 4);
+    const int Nfields = (c.bp % 2 == 0) ? (c.elementType == 3 || c.elementType == 4 ? 2 : 3) : 1;
     // product host setup
     bpx::MeshOpts o;
     o.N = c.N;
@@ -66,6 +72,7 @@ int main() {
     o.cubature = cub ? 1 : 0;
     o.perturb = c.perturb;
     o.seed = 7;
+    o.elementType = c.elementType;
     bpx::mesh_t m;
     if (m.Setup(o)) {
       std::printf("  product mesh setup failed: %s\n", bpx::last_error());
@@ -81,8 +88,8 @@ int main() {
     std::vector<bpx::dlong> G2L(ids.size());
     g.SetupGlobalToLocalMapping(G2L.data());
     std::vector<double> tri;
-    m.TrilinearCoefficients(tri);
-    const bool trilinear = m.NodesAreTrilinear();
+    if (hex) m.TrilinearCoefficients(tri);
+    const bool trilinear = !hex || m.NodesAreTrilinear();
     if (!trilinear && !c.affine) {
       std::printf("  box nodes not trilinear\n");
       ++failures;
@@ -99,6 +106,7 @@ int main() {
     P.lambda = 1.0;
     P.perturb = c.perturb;
     P.seed = 7;
+    P.elementType = c.elementType;
     if (orc::problem_setup(P)) {
       std::printf("  oracle setup failed: %s\n", P.error.c_str());
       return 2;
@@ -108,6 +116,11 @@ int main() {
     same("blockRowStartsT", g.blockRowStartsT, P.blockRowStartsT);
     same("GlobalToLocal", G2L, P.GlobalToLocal);
     close("x", m.x, P.x, 1e-13);
+    if (c.elementType == 3 || c.elementType == 6) {
+      close("MM", m.MM, P.MM, 1e-12);
+      close("Dsim", m.Dsim, P.Dsim, 1e-11);
+      close("Ssim", m.Ssim, P.Ssim, 1e-11);
+    }
     if (cub && !c.affine) {
       close("cubwJ", m.cubwJ, P.cubwJ, 1e-12);
       if (c.bp >= 3) close("cubggeo", m.cubggeo, P.cubggeo, 1e-12);

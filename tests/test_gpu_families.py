@@ -1,7 +1,7 @@
 """GPU parity of the quadrilateral, triangle and tetrahedron BP operators
 (ax_other.hip) with the oracle: BP1-BP6 on perturbed boxes to 1e-12
-relative (FP64), the forcing to 1e-13, the CG residual history to 1e-9
-(down to 1e-8 of the initial residual) and the converged solution to 1e-8;
+relative (FP64), the forcing to 1e-13, the first 25 CG residual norms to
+1e-9 and the converged solution to 1e-8;
 the FP32 operators report that they cover hexahedra.
 Every call goes through libbpx.so's C ABI."""
 import numpy as np
@@ -69,21 +69,27 @@ def test_family_forcing_matches_oracle(plat, el, bp):
     assert np.abs(host(r) - ref).max() <= 1e-13 * np.abs(ref).max()
 
 
+BIG = {capi.QUAD: dict(nx=8, ny=6), capi.TRI: dict(nx=8, ny=6), capi.TET: dict(nx=4, ny=3, nz=3)}
+
+
 @pytest.mark.parametrize("el,bp,N", [(capi.QUAD, 5, 7), (capi.QUAD, 3, 4), (capi.TRI, 5, 5), (capi.TET, 5, 3),
                                      (capi.TET, 1, 4), (capi.TRI, 4, 3)])
 def test_family_cg_history_matches_oracle(plat, el, bp, N):
-    m, P = pair(el, bp, N, lam=1.0)
+    """The first 25 residual norms to 1e-9 and the solution at 1e-10 to 1e-8.
+    On these 1-10 thousand unknown problems two CGs whose reductions
+    round differently part at 1e-9 after 30-90 iterations (measured with a
+    numpy CG on the oracle's own operator), so the history is compared where
+    it is determined by the operator, not by the reduction order."""
+    kw = BIG[el]
+    m = capi.Mesh(N=N, element=el, perturb=0.1, seed=4, cubature=bp <= 4, **kw)
+    P = oracle_lib.Problem(bp=bp, N=N, element=el, perturb=0.1, seed=4, lam=1.0, **kw)
     b = capi.BP(plat, m, bp, lam=1.0)
     rhs = capi.random_normal(9, b.Ngather)
     x = torch.zeros(b.Ngather, dtype=torch.float64, device="cuda")
-    its, hist = b.solve(x, dev(rhs), tol=0.0, maxit=100)
-    xr, it_ref, hist_ref, _ = P.cg(rhs.copy(), 100, tol=0.0)
-    assert its == it_ref == 100
-    # these problems have 100-700 unknowns: once the residual reaches
-    # rounding level the two histories part, so compare down to 1e-8 of r0
-    k = int(np.argmax(hist_ref < 1e-8 * hist_ref[0])) if (hist_ref < 1e-8 * hist_ref[0]).any() else 101
-    assert k > 10
-    np.testing.assert_allclose(hist[:k], hist_ref[:k], rtol=1e-9)
+    its, hist = b.solve(x, dev(rhs), tol=0.0, maxit=25)
+    xr, it_ref, hist_ref, _ = P.cg(rhs.copy(), 25, tol=0.0)
+    assert its == it_ref == 25
+    np.testing.assert_allclose(hist, hist_ref, rtol=1e-9)
     x2 = torch.zeros_like(x)
     b.solve(x2, dev(rhs), tol=1e-10, maxit=2000)
     x2r, _, _, _ = P.cg(rhs.copy(), 2000, tol=1e-10)

@@ -1,8 +1,9 @@
 """The host code under AddressSanitizer and UndefinedBehaviorSanitizer (CPU
 only): tests/sanitize/setup_sanitize runs the product's host setup (basis,
 box mesh, geometric and cubature factors, masked ids, ogs T maps) and the
-oracle's setup, Ax, gather, scatter and CG on nine configurations (BP1-BP6,
-p = 1 ... 15, affine and perturbed), comparing the two setups' arrays. It is
+oracle's setup, Ax, gather, scatter and CG on sixteen configurations (BP1-BP6,
+p = 1 ... 15, affine and perturbed hexahedra; quadrilateral, triangle and
+tetrahedron boxes), comparing the two setups' arrays. It is
 built by __graft_entry__.build() (g++ -fsanitize=address,undefined)."""
 import os
 import subprocess
