@@ -150,46 +150,40 @@ def event_time(fn, reps, warm=2):
 
 
 def bench_krhs(capi, plat):
-    """k-RHS BP5 Ax at 20^3, p=7 (kernel entry on the mesh's own arrays)."""
+    """k-RHS BP5 Ax at 20^3, p=7 on the problem's arrays: each form (chunk 0 =
+    the default, the pencil form here; -1 the sheet workgroup form; 1, 2, 4
+    the sweep form) against the single-vector Ax, both timed by HIP events
+    their own dispatches stamp (kernel time, 10 launches after 2 warm-ups),
+    on random right-hand sides."""
     import numpy as np
-    import torch
 
     m = capi.Mesh(N=7, nx=20, perturb=0.1, seed=1)
-    o = capi.Ogs(mesh=m)
     bp = capi.BP(plat, m, 5, lam=1.0)
-    Ndofs, Nel, Np, NL = bp.Ngather, m.Nelements, m.Np, bp.NLocal
-    arr = {k: torch.from_numpy(np.ascontiguousarray(m.array(k))).cuda() for k in ("wJ", "ggeo", "gllw", "D")}
-    G2L = torch.from_numpy(np.ascontiguousarray(o.array("GlobalToLocal"))).cuda()
-    elist = torch.arange(Nel, dtype=torch.int32, device="cuda")
-    L = capi.lib()
+    Ndofs, Nel, Np = bp.Ngather, m.Nelements, m.Np
     out = {"workload": "BP5 Ax, 20^3 hex p=7 perturbed, K right-hand sides stored one after another",
-           "Ndofs": Ndofs}
-    q1 = torch.from_numpy(capi.random_normal(3, Ndofs)).cuda()
-    A1 = torch.empty(NL, dtype=torch.float64, device="cuda")
-    t1 = event_time(lambda: bp.ax_local(q1, A1), 20)
-    out["single_us"] = t1 * 1e6
+           "Ndofs": Ndofs, "timing": "dispatch-stamped HIP events, mean of 10 launches"}
+    import torch
+
     rows = []
+    singles = []
     for K in (4, 8, 16, 32):
         q = torch.from_numpy(capi.random_normal(4, Ndofs * K)).cuda()
-        A = torch.empty(NL * K, dtype=torch.float64, device="cuda")
-        best = None
         per_chunk = {}
-        for chunk in (0, 1, 2, 4):
-            def run():
-                capi.check(L.bpx_bp5_ax_hex_krhs(plat.h, 7, K, chunk, Nel, capi.ptr(elist), capi.ptr(G2L),
-                                                 capi.ptr(arr["wJ"]), None, capi.ptr(arr["ggeo"]),
-                                                 capi.ptr(arr["gllw"]), capi.ptr(arr["D"]), None, None, 1.0,
-                                                 capi.ptr(q), Ndofs, capi.ptr(A), NL))
-            t = event_time(run, 10)
-            per_chunk[chunk] = t * 1e6
-            if best is None or t < best[1]:
-                best = (chunk, t)
-        chunk, t = best
+        for chunk in (0, -1, 1, 2, 4):
+            if chunk > 0 and K % chunk:
+                continue
+            tk, t1 = bp.time_krhs(K, chunk, 10, q)
+            per_chunk[chunk] = tk
+            singles.append(t1)
+        chunk = min(per_chunk, key=per_chunk.get)
+        t = per_chunk[chunk] * 1e-6
+        t1 = float(np.median(singles)) * 1e-6
         nb = krhs_bytes(K, Ndofs, Nel, Np)
         rows.append({"K": K, "chunk": chunk, "us": t * 1e6, "gdof_per_s": K * Ndofs / t / 1e9,
                      "gbs": nb / t / 1e9, "algorithmic_bytes": nb, "speedup_vs_K_single": K * t1 / t,
                      "us_by_chunk": per_chunk})
-        del q, A
+        del q
+    out["single_us"] = float(np.median(singles))
     out["rows"] = rows
     return out
 
@@ -553,6 +547,10 @@ def main():
     ap.add_argument("--counter-kernels", action="store_true",
                     help="only 3 launches each of the FP64 Ax of BK5 p=15 (vector-ALU and matrix-core "
                          "kernels), BP3 p=7 and BK5 p=7 on the 24^3 perturbed box (for counter runs)")
+    ap.add_argument("--krhs-only", action="store_true",
+                    help="only 3 single-vector and 3 k-RHS (K = 32, --krhs-chunk) FP64 BP5 Ax launches "
+                         "on the 20^3 p=7 perturbed box, random right-hand sides (for counter runs)")
+    ap.add_argument("--krhs-chunk", type=int, default=0)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -567,6 +565,22 @@ def main():
 
     torch.cuda.set_device(0)
     plat = capi.Platform(0)
+    if args.krhs_only:
+        K = 32
+        mk = capi.Mesh(N=7, nx=20, perturb=0.1, seed=1)
+        bk = capi.BP(plat, mk, 5, lam=1.0)
+        q1 = torch.from_numpy(capi.random_normal(3, bk.Ngather)).cuda()
+        A1 = torch.empty(bk.NLocal, dtype=torch.float64, device="cuda")
+        qk = torch.from_numpy(capi.random_normal(4, bk.Ngather * K)).cuda()
+        Ak = torch.empty(bk.Ngather * K, dtype=torch.float64, device="cuda")
+        for _ in range(3):
+            bk.ax_local(q1, A1)
+        for _ in range(3):
+            bk.operator_krhs(qk, Ak, K, args.krhs_chunk)  # the k-RHS Ax, then a gather per right-hand side
+        torch.cuda.synchronize()
+        log(f"krhs-only: 3 single-vector Ax and 3 k-RHS operator launches (K = {K}, chunk "
+            f"{args.krhs_chunk}), 20^3 p=7, Ndofs {bk.Ngather}")
+        return
     if args.counter_kernels:
         for bp, N, cub, engines in ((5, 15, False, (1, 2)), (3, 7, True, (0,)), (5, 7, False, (0,))):
             mk = capi.Mesh(N=N, nx=args.nx, cubature=cub, perturb=0.1, seed=1)

@@ -83,6 +83,7 @@ _SIGS = {
     "bpx_bp_operator_dot": (_I, [_P, _P, _P, C.POINTER(_D)]),
     "bpx_bp_set_tuning": (_I, [_P, _I, _I, _I]),
     "bpx_bp_time_operator": (_I, [_P, _I, _I, _P]),
+    "bpx_bp_time_krhs": (_I, [_P, _I, _I, _I, _P, _P]),
     "bpx_bp_set_ax_engine": (_I, [_P, _I]),
     "bpx_bp_set_cg_fold": (_I, [_P, _I]),
     "bpx_bp_set_fp32_geometry": (_I, [_P, _I]),
@@ -112,6 +113,8 @@ for _name in ("bpx_bp3_ax_hex", "bpx_bp3_ax_affine_hex", "bpx_bp4_ax_hex", "bpx_
     _SIGS[_name] = (_I, [_P, _I, _I, _P, _P] + [_P] * 7 + [_D, _P, _P])
 for _name in ("bpx_bp5_ax_hex_krhs", "bpx_bp5_ax_affine_hex_krhs"):
     _SIGS[_name] = (_I, [_P, _I, _I, _I, _I, _P, _P] + [_P] * 7 + [_D, _P, C.c_int64, _P, C.c_int64])
+for _name in ("bpx_bp5_ax_hex_krhs_f32", "bpx_bp5_ax_affine_hex_krhs_f32"):
+    _SIGS[_name] = (_I, [_P, _I, _I, _I, _I, _P, _P] + [_P] * 7 + [_F, _P, C.c_int64, _P, C.c_int64])
 _SIGS["bpx_bp5_ax_hex_f32"] = (_I, [_P, _I, _I, _P, _P] + [_P] * 7 + [_F, _P, _P])
 _SIGS["bpx_bp3_ax_hex_f32"] = (_I, [_P, _I, _I, _P, _P] + [_P] * 7 + [_F, _P, _P])
 
@@ -360,6 +363,13 @@ class BP:
         """BP5 at N = 15, FP64: 0 = auto (matrix cores), 1 = vector-ALU Ax,
         2 = matrix-core Ax (the two agree to rounding, not bit for bit)."""
         check(lib().bpx_bp_set_ax_engine(self.h, int(engine)))
+
+    def time_krhs(self, K, chunk=0, launches=10, q=None):
+        """(k-RHS Ax us, single-vector Ax us), dispatch-stamped means; q:
+        device tensor of K * Ngather values (None: zeros)."""
+        out = np.zeros(2)
+        check(lib().bpx_bp_time_krhs(self.h, K, chunk, launches, ptr(q), ptr(out)))
+        return float(out[0]), float(out[1])
 
     def time_operator(self, precision=32, launches=50):
         """The CG operator (solver row order, tuned form) timed per kernel:

@@ -737,6 +737,282 @@ __global__ __launch_bounds__(NQ* NQ* W) __attribute__((amdgpu_waves_per_eu(DR ? 
   }
 }
 
+// k right-hand sides, pencil form: one element per Nq^2 lanes (one
+// wavefront at Nq = 8; floor(64 / Nq^2) elements per workgroup below Nq = 6).
+// The element's geometric factors are loaded once, into registers (a lane
+// keeps the 7 factors of its pencil along k), and the workgroup applies the
+// operator to the K right-hand sides one after another, with q loaded two
+// right-hand sides ahead (gathered through bounds-checked buffer loads, so
+// a masked node costs no compare or select). Each
+// one-dimensional contraction runs along a pencil that a single lane holds,
+// with D[a][b] the same for every lane (scalar loads): the t derivative on
+// the lane's pencil along k, the r and s derivatives on pencils along i and
+// along j, to which q moves through LDS (lane (a, b) = (t % Nq, t / Nq) of
+// an element holds node (k, b, a) for k = 0..Nq-1, the pencil along i of
+// (j, k) = (a, b), or the pencil along j of (i, k) = (a, b)). A node's value
+// crosses LDS once per move: 16 LDS accesses per node and right-hand side
+// (q out twice and in twice, qr and qs back, Gr and Gs out, their divergence
+// parts back), against 42 in the sheet kernels, where every contraction
+// reads its Nq operands from LDS. Two LDS buffers, one per moving direction:
+// s_b (i pencils) holds node (k, j, i) at k*LK + j*Nq + (i + j) mod Nq,
+// s_c (j pencils) at k*LK + j*Nq + i; at Nq = 8 with LK = 72 each 16 lanes
+// of an access fall on distinct 8-byte banks in both layouts of each
+// buffer, and every offset but the i pencils' is a lane base plus a
+// constant. The divergence's r and s parts are summed apart (each along its
+// own pencil), so results agree with k_bp5_ax to rounding, not bit for bit.
+// raw buffer resources (stride 0): loads at offsets >= num_records return 0
+constexpr int kBufRange = 0x7FFFFFF8;          // num_records (bytes)
+constexpr int kBufMasked = (int)0x80000000u;   // an offset beyond it
+constexpr int kBufFlags = 0x00020000;          // dword 3: 32-bit data format (gfx9)
+
+template <typename T>
+__device__ inline T buffer_load(__amdgpu_buffer_rsrc_t r, int offset) {
+  if constexpr (sizeof(T) == 8)
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, offset, 0, 0));
+  else
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, offset, 0, 0));
+}
+
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+template <typename T>
+__device__ inline void buffer_store(T v, __amdgpu_buffer_rsrc_t r, int offset) {
+  if constexpr (sizeof(T) == 8)
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, offset, 0, 0);
+  else
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, offset, 0, 0);
+}
+
+template <int NQ>
+constexpr int krhs_pencil_epb() {
+  return NQ * NQ >= 32 ? 1 : 64 / (NQ * NQ);
+}
+
+template <typename T, int NQ, bool AFFINE>
+__global__ __launch_bounds__(NQ* NQ* krhs_pencil_epb<NQ>()) __attribute__((amdgpu_waves_per_eu(2))) void k_bp5_krhs_pencil(
+    int Nelements, int K, const int* __restrict__ elementList, const int* __restrict__ G2L,
+    const T* __restrict__ wJ, const T* __restrict__ ggeo, const T* __restrict__ gllw, const T* __restrict__ D,
+    T lambda, const T* __restrict__ q, T* __restrict__ Aq, long long ldq, long long ldA) {
+  constexpr int NQ2 = NQ * NQ;
+  constexpr int NP = NQ2 * NQ;
+  constexpr int EPB = krhs_pencil_epb<NQ>();
+  constexpr int LK = NQ2 + 8;  // slab stride in LDS
+  constexpr int LE = NQ * LK;  // element stride in LDS
+  __shared__ T s_b[EPB * LE];  // i pencils: q, qr, Gr, the divergence's r part
+  __shared__ T s_c[EPB * LE];  // j pencils: q, qs, Gs, the divergence's s part
+  typedef const __attribute__((address_space(4))) T* cptr;
+  cptr Dc = (cptr)D;
+  // D[a][b] is read with scalar loads where it is used: an opaque update of
+  // the pointer before each row keeps the compiler from holding all Nq^2
+  // entries in scalar registers (128 at Nq = 8, more than a wavefront has)
+#define BPX_DROW()                     \
+  do {                                 \
+    __builtin_amdgcn_sched_barrier(0); \
+    asm volatile("" : "+s"(Dc));       \
+  } while (0)
+  const int t = threadIdx.x;
+  const int es = t / NQ2;
+  const int tl = t - es * NQ2;
+  const int a = tl % NQ;
+  const int b = tl / NQ;
+  // EPB elements per workgroup; each XCD (block mod 8) takes a contiguous
+  // eighth of the element groups, so face neighbours share q lines in its L2
+  const int groups = (Nelements + EPB - 1) / EPB;
+  const int per_xcd = (groups + 7) / 8;
+  const int grp = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+  if (grp >= groups) return;  // uniform over the workgroup
+  const int eo = grp * EPB + es;
+  const bool active = eo < Nelements;  // the last group may be partly empty
+  const int e = elementList[active ? eo : 0];
+  const size_t base = (size_t)e * NP + tl;  // node (k, b, a) is base + k*NQ2
+
+  // LDS slots. Along k (node (k, b, a)): kb + k*LK in s_b, kc + k*LK in s_c;
+  // along i (node (b, a, m)): oi[m] in s_b; along j (node (b, m, a)):
+  // jc + m*NQ in s_c
+  const int kb = es * LE + b * NQ + (a + b) % NQ;
+  const int kc = es * LE + b * NQ + a;
+  const int jc = es * LE + b * LK + a;
+  int oi[NQ];
+#pragma unroll
+  for (int m = 0; m < NQ; ++m) oi[m] = es * LE + b * LK + a * NQ + (m + a) % NQ;
+
+  // q is read through a buffer resource of kBufRange bytes per right-hand
+  // side: a masked node (or an idle lane of a partly empty group, which
+  // reads element elementList[0] and stores nothing) gets an offset beyond
+  // that range, which the hardware answers with 0 without touching memory
+  // (no compare or select per load). Rows must lie below kBufRange / sizeof(T).
+  int qoff[NQ];
+#pragma unroll
+  for (int k = 0; k < NQ; ++k) {
+    const int v = G2L[base + k * NQ2];
+    qoff[k] = (active && v >= 0) ? v * (int)sizeof(T) : kBufMasked;
+  }
+  // byte offset of node (0, b, a) in a right-hand side of Aq (the same range)
+  const int aoff = active ? (int)(base * sizeof(T)) : kBufMasked;
+  T g[7][AFFINE ? 1 : NQ];
+  T ga[6] = {0, 0, 0, 0, 0, 0}, gw = 0, wab = 0;
+  if constexpr (AFFINE) {
+#pragma unroll
+    for (int f = 0; f < 6; ++f) ga[f] = ggeo[6 * (size_t)e + f];
+    gw = wJ[e];
+    wab = gllw[a] * gllw[b];
+  } else {
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) {
+      const T* gp = ggeo + 6 * (base + k * NQ2);
+#pragma unroll
+      for (int f = 0; f < 6; ++f) g[f][k] = __builtin_nontemporal_load(gp + f);
+      g[6][k] = __builtin_nontemporal_load(wJ + base + k * NQ2);
+    }
+  }
+
+  // rows of D through scalar registers, row n+1 loaded while row n is in
+  // use; body(n, row) for n = 0..Nq-1
+  auto for_rows = [&](auto&& body) {
+    T d[2][NQ];
+    BPX_DROW();
+#pragma unroll
+    for (int m = 0; m < NQ; ++m) d[0][m] = Dc[m];
+#pragma unroll
+    for (int n = 0; n < NQ; ++n) {
+      if (n + 1 < NQ) {
+        BPX_DROW();
+#pragma unroll
+        for (int m = 0; m < NQ; ++m) d[(n + 1) & 1][m] = Dc[(n + 1) * NQ + m];
+      }
+      body(n, d[n & 1]);
+    }
+  };
+
+  // q of right-hand side r along the lane's pencil
+  auto load_q = [&](T(&dst)[NQ], int r) {
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(q + (size_t)r * ldq), (short)0, kBufRange, kBufFlags);
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) dst[k] = buffer_load<T>(rs, qoff[k]);
+  };
+
+  // the operator on right-hand side r, q along the lane's pencil in rq
+  auto apply = [&](const T(&rq)[NQ], int r) {
+    // (each lane writes and, after the last barrier of the previous
+    // right-hand side, last read only its own slots along k)
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) {
+      s_b[k * LK + kb] = rq[k];
+      s_c[k * LK + kc] = rq[k];
+    }
+    __syncthreads();
+
+    // r derivative along i at (j, k) = (a, b); s derivative along j at
+    // (i, k) = (a, b); each lane overwrites only its own pencil
+    {
+      T v[NQ];
+#pragma unroll
+      for (int m = 0; m < NQ; ++m) v[m] = s_b[oi[m]];
+      for_rows([&](int n, const T* dr) {
+        T acc = 0;
+#pragma unroll
+        for (int m = 0; m < NQ; ++m) acc = fmad(dr[m], v[m], acc);
+        s_b[oi[n]] = acc;
+      });
+#pragma unroll
+      for (int m = 0; m < NQ; ++m) v[m] = s_c[jc + m * NQ];
+      for_rows([&](int n, const T* dr) {
+        T acc = 0;
+#pragma unroll
+        for (int m = 0; m < NQ; ++m) acc = fmad(dr[m], v[m], acc);
+        s_c[jc + n * NQ] = acc;
+      });
+    }
+    __syncthreads();
+
+    // along k: t derivative, G, the mass term and the divergence's t part
+    T rA[NQ];
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) rA[k] = T(0);
+    for_rows([&](int k, const T* dr) {
+      T qt = 0;
+#pragma unroll
+      for (int m = 0; m < NQ; ++m) qt = fmad(dr[m], rq[m], qt);
+      const T qr = s_b[k * LK + kb];
+      const T qs = s_c[k * LK + kc];
+      T G00, G01, G02, G11, G12, G22, W;
+      if constexpr (AFFINE) {
+        const T w = wab * gllw[k];
+        G00 = ga[0] * w; G01 = ga[1] * w; G02 = ga[2] * w; G11 = ga[3] * w; G12 = ga[4] * w; G22 = ga[5] * w;
+        W = gw * w;
+      } else {
+        G00 = g[0][k]; G01 = g[1][k]; G02 = g[2][k]; G11 = g[3][k]; G12 = g[4][k]; G22 = g[5][k]; W = g[6][k];
+      }
+      s_b[k * LK + kb] = fmad(G02, qt, fmad(G01, qs, G00 * qr));
+      s_c[k * LK + kc] = fmad(G12, qt, fmad(G11, qs, G01 * qr));
+      const T Gt = fmad(G22, qt, fmad(G12, qs, G02 * qr));
+      rA[k] = fmad(lambda * W, rq[k], rA[k]);
+#pragma unroll
+      for (int m = 0; m < NQ; ++m) rA[m] = fmad(dr[m], Gt, rA[m]);
+    });
+    __syncthreads();
+
+    // the divergence's r part along i, its s part along j (rows of D;
+    // each output summed over m in order)
+    {
+      T v[NQ], acc[NQ];
+#pragma unroll
+      for (int m = 0; m < NQ; ++m) v[m] = s_b[oi[m]];
+#pragma unroll
+      for (int n = 0; n < NQ; ++n) acc[n] = T(0);
+      for_rows([&](int m, const T* dr) {
+#pragma unroll
+        for (int n = 0; n < NQ; ++n) acc[n] = fmad(dr[n], v[m], acc[n]);
+      });
+#pragma unroll
+      for (int n = 0; n < NQ; ++n) s_b[oi[n]] = acc[n];
+#pragma unroll
+      for (int m = 0; m < NQ; ++m) v[m] = s_c[jc + m * NQ];
+#pragma unroll
+      for (int n = 0; n < NQ; ++n) acc[n] = T(0);
+      for_rows([&](int m, const T* dr) {
+#pragma unroll
+        for (int n = 0; n < NQ; ++n) acc[n] = fmad(dr[n], v[m], acc[n]);
+      });
+#pragma unroll
+      for (int n = 0; n < NQ; ++n) s_c[jc + n * NQ] = acc[n];
+    }
+    __syncthreads();
+
+    // buffer stores, no branch: an idle lane's offset lies beyond the range
+    // and its stores are dropped (every path then has the same stores in
+    // flight, so the next load's wait need not cover them)
+    const __amdgpu_buffer_rsrc_t ro =
+        __builtin_amdgcn_make_buffer_rsrc(Aq + (size_t)r * ldA, (short)0, kBufRange, kBufFlags);
+#pragma unroll
+    for (int k = 0; k < NQ; ++k)
+      buffer_store<T>(rA[k] + (s_b[k * LK + kb] + s_c[k * LK + kc]), ro, aoff + k * NQ2 * (int)sizeof(T));
+  };
+
+  // three register sets for q, each loaded two right-hand sides ahead: a
+  // load is waited for only where its values are first used, past the
+  // stores of the right-hand sides before (the first right-hand side is
+  // peeled so that the loop is entered with the same loads and stores in
+  // flight as its back edge carries; the early exits leave the loop)
+  T q0[NQ], q1[NQ], q2[NQ];
+  load_q(q0, 0);
+  load_q(q1, min(1, K - 1));
+  load_q(q2, min(2, K - 1));
+  apply(q0, 0);
+  for (int r = 1; r < K; r += 3) {
+    load_q(q0, min(r + 2, K - 1));
+    apply(q1, r);
+    if (r + 1 >= K) break;
+    load_q(q1, min(r + 3, K - 1));
+    apply(q2, r + 1);
+    if (r + 2 >= K) break;
+    load_q(q2, min(r + 4, K - 1));
+    apply(q0, r + 2);
+  }
+#undef BPX_DROW
+}
+
 // Adjoint gradient of the variable-coefficient BP5 (an extension):
 // gL[e*Np + n] = -(D lam)_n . G_n (D u)_n with the unscaled factors G_n,
 // u and lam global vectors read through GlobalToLocal (-1 -> 0). Same lane
@@ -895,9 +1171,9 @@ int launch_bp5_krhs_wg_w(hipStream_t s, const AxArgs<T>& a) {
   const long long groups = (a.Nelements + 7) / 8;
   if (groups * 8 * npass > 0x7fffffffLL) return set_error(BPX_ERR_UNSUPPORTED, "bp5 k-RHS Ax: grid too large");
   const size_t smem = (fixed + W * per_sheet) * sizeof(T);
-  hipLaunchKernelGGL((k_bp5_krhs_wg<T, NQ, NF, W, DR>), dim3((unsigned)(groups * 8 * npass)), dim3(W * NQ2), smem, s,
-                     a.Nelements, a.K, npass, a.elementList, a.GlobalToLocal, a.wJ, a.ggeo, a.D, a.lambda, a.q, a.Aq,
-                     (long long)a.ldq, (long long)a.ldAq);
+  launch_stamped_lds(k_bp5_krhs_wg<T, NQ, NF, W, DR>, dim3((unsigned)(groups * 8 * npass)), dim3(W * NQ2), smem, s,
+                     a.evStart, a.evStop, a.Nelements, a.K, npass, a.elementList, a.GlobalToLocal, a.wJ, a.ggeo, a.D,
+                     a.lambda, a.q, a.Aq, (long long)a.ldq, (long long)a.ldAq);
   BPX_HIP(hipGetLastError());
   return BPX_OK;
 }
@@ -953,14 +1229,54 @@ int launch_bp5_krhs_wg(hipStream_t s, const AxArgs<T>& a) {
   return set_error(BPX_ERR_INVALID_ARGUMENT, "bp5 k-RHS Ax: sheets per workgroup must be 1, 2, 4, 8 or 16");
 }
 
+// the pencil form keeps the 7 factors of a pencil along k in registers:
+// up to Nq = 10 (FP64: 140 of its 256 registers at two waves per SIMD)
+template <int NQ>
+constexpr bool krhs_pencil_fits() {
+  return NQ >= 2 && NQ <= 10;
+}
+
+template <typename T, int NQ>
+int launch_bp5_krhs_pencil(hipStream_t s, const AxArgs<T>& a) {
+  constexpr int EPB = krhs_pencil_epb<NQ>();
+  const long long groups = (a.Nelements + EPB - 1) / EPB;
+  const long long blocks = (groups + 7) / 8 * 8;
+  if (blocks > 0x7fffffffLL) return set_error(BPX_ERR_UNSUPPORTED, "bp5 k-RHS Ax: grid too large");
+  if (a.affine)
+    launch_stamped(k_bp5_krhs_pencil<T, NQ, true>, dim3((unsigned)blocks), dim3(NQ * NQ * EPB), s, a.evStart,
+                   a.evStop, a.Nelements, a.K, a.elementList, a.GlobalToLocal, a.wJ, a.ggeo, a.gllw, a.D, a.lambda,
+                   a.q, a.Aq, (long long)a.ldq, (long long)a.ldAq);
+  else
+    launch_stamped(k_bp5_krhs_pencil<T, NQ, false>, dim3((unsigned)blocks), dim3(NQ * NQ * EPB), s, a.evStart,
+                   a.evStop, a.Nelements, a.K, a.elementList, a.GlobalToLocal, a.wJ, a.ggeo, a.gllw, a.D, a.lambda,
+                   a.q, a.Aq, (long long)a.ldq, (long long)a.ldAq);
+  BPX_HIP(hipGetLastError());
+  return BPX_OK;
+}
+
+// chunk: 0 = the default form, -2 = the pencil form, -1 = the sheet
+// workgroup form (non-affine), 1, 2 or 4 = the sweep form with that many
+// right-hand sides per sweep
 template <typename T, int NQ>
 int launch_bp5_krhs_nq(hipStream_t s, const AxArgs<T>& a, int chunk) {
+  if (chunk == 0 || chunk == -2) {
+    // q's rows lie below the local node count (every gather row has a local copy)
+    const bool in_range = (double)a.Nelements * NQ * NQ * NQ * sizeof(T) < (double)kBufRange;
+    if constexpr (krhs_pencil_fits<NQ>()) {
+      if (in_range) return launch_bp5_krhs_pencil<T, NQ>(s, a);
+    }
+    if (chunk == -2)
+      return set_error(BPX_ERR_UNSUPPORTED,
+                       "bp5 k-RHS Ax: the pencil form covers N <= 9 and Nelements*Np*sizeof(dfloat) < 2^31");
+  }
   // non-affine: the factors are read once per workgroup (shared by its
   // sheets); affine factors are per element, so the sweep kernel serves
-  if (chunk == 0) {
+  if (chunk == 0 || chunk == -1) {
     if constexpr (krhs_wg_fits<T, NQ, 1>()) {
       if (!a.affine) return launch_bp5_krhs_wg<T, NQ, 1>(s, a);
     }
+    if (chunk == -1)
+      return set_error(BPX_ERR_UNSUPPORTED, "bp5 k-RHS Ax: the sheet workgroup form is for non-affine meshes");
     chunk = (a.K % 2 == 0) ? 2 : 1;  // the sweep form's default
   }
   switch (chunk) {
@@ -1064,5 +1380,6 @@ template int launch_bp5_nq<double, BPX_NQ, 1>(hipStream_t, const AxArgs<double>&
 template int launch_bp5_nq<double, BPX_NQ, 3>(hipStream_t, const AxArgs<double>&);
 template int launch_bp5_nq<float, BPX_NQ, 1>(hipStream_t, const AxArgs<float>&);
 template int launch_bp5_krhs_nq<double, BPX_NQ>(hipStream_t, const AxArgs<double>&, int);
+template int launch_bp5_krhs_nq<float, BPX_NQ>(hipStream_t, const AxArgs<float>&, int);
 
 }  // namespace bpx

@@ -81,8 +81,9 @@ int launch_bp5_ax_krhs(hipStream_t s, const AxArgs<T>& a, int chunk) {
   if (a.K > 1 && (a.ldq < 1 || a.ldAq < 1))
     return set_error(BPX_ERR_INVALID_ARGUMENT, "bp5 k-RHS Ax: leading dimensions must be positive");
   if (a.Nfields != 1) return set_error(BPX_ERR_UNSUPPORTED, "bp5 k-RHS Ax: one field per right-hand side");
-  if (chunk != 0 && ((chunk != 1 && chunk != 2 && chunk != 4) || a.K % chunk != 0))
-    return set_error(BPX_ERR_INVALID_ARGUMENT, "bp5 k-RHS Ax: chunk must be 0, or 1, 2 or 4 dividing K");
+  if (chunk > 0 && ((chunk != 1 && chunk != 2 && chunk != 4) || a.K % chunk != 0))
+    return set_error(BPX_ERR_INVALID_ARGUMENT, "bp5 k-RHS Ax: chunk must be 0, -1, -2, or 1, 2 or 4 dividing K");
+  if (chunk < -2) return set_error(BPX_ERR_INVALID_ARGUMENT, "bp5 k-RHS Ax: chunk must be 0, -1, -2, 1, 2 or 4");
   if (a.Nelements == 0) return BPX_OK;
   if (!a.wJ || !a.ggeo || !a.D || (a.affine && !a.gllw))
     return set_error(BPX_ERR_INVALID_ARGUMENT, "bp5 k-RHS Ax: null geometry/operator pointer");
@@ -96,6 +97,7 @@ int launch_bp5_ax_krhs(hipStream_t s, const AxArgs<T>& a, int chunk) {
 }
 
 template int launch_bp5_ax_krhs<double>(hipStream_t, const AxArgs<double>&, int);
+template int launch_bp5_ax_krhs<float>(hipStream_t, const AxArgs<float>&, int);
 
 #define BPX_FUSED_CASE(NQ) \
   case NQ:                  \

@@ -105,6 +105,15 @@ void launch_stamped(K kernel, dim3 grid, dim3 block, hipStream_t s, hipEvent_t e
   else
     hipLaunchKernelGGL(kernel, grid, block, 0, s, args...);
 }
+// the same with dynamic LDS
+template <typename K, typename... A>
+void launch_stamped_lds(K kernel, dim3 grid, dim3 block, size_t lds, hipStream_t s, hipEvent_t e0, hipEvent_t e1,
+                        A... args) {
+  if (e0)
+    hipExtLaunchKernelGGL(kernel, grid, block, lds, s, e0, e1, 0, args...);
+  else
+    hipLaunchKernelGGL(kernel, grid, block, lds, s, args...);
+}
 #endif
 // whether the BP5 Ax of these arguments runs on the matrix cores
 template <typename T>

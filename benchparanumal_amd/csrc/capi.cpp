@@ -197,12 +197,13 @@ int bpx_bp5_ax_affine_hex(bpx_platform p, int N, int Nelements, const int* eleme
   (void)vgeo; (void)S_; (void)MM;
   return bp5_entry(p, N, Nelements, elementList, GlobalToLocal, wJ, ggeo, gllw, D, lambda, q, Aq, 1, 1);
 }
+}  // extern "C"
+template <typename T>
 static int bp5_krhs_entry(bpx_platform p, int N, int K, int chunk, int Nelements, const int* elementList,
-                          const int* GlobalToLocal, const double* wJ, const double* ggeo, const double* gllw,
-                          const double* D, double lambda, const double* q, int64_t ldq, double* Aq, int64_t ldAq,
-                          int affine) {
+                          const int* GlobalToLocal, const T* wJ, const T* ggeo, const T* gllw, const T* D,
+                          T lambda, const T* q, int64_t ldq, T* Aq, int64_t ldAq, int affine) {
   if (int st = need(p, "platform")) return st;
-  AX_COMMON(double)
+  AX_COMMON(T)
   a.affine = affine;
   a.wJ = wJ;
   a.ggeo = ggeo;
@@ -212,24 +213,42 @@ static int bp5_krhs_entry(bpx_platform p, int N, int K, int chunk, int Nelements
   a.K = K;
   a.ldq = ldq;
   a.ldAq = ldAq;
-  return launch_bp5_ax_krhs<double>(S(p), a, chunk);
+  return launch_bp5_ax_krhs<T>(S(p), a, chunk);
 }
+extern "C" {
 
 int bpx_bp5_ax_hex_krhs(bpx_platform p, int N, int K, int chunk, int Nelements, const int* elementList,
                         const int* GlobalToLocal, const double* wJ, const double* vgeo, const double* ggeo,
                         const double* gllw, const double* D, const double* S_, const double* MM, double lambda,
                         const double* q, int64_t ldq, double* Aq, int64_t ldAq) {
   (void)vgeo; (void)S_; (void)MM;
-  return bp5_krhs_entry(p, N, K, chunk, Nelements, elementList, GlobalToLocal, wJ, ggeo, gllw, D, lambda, q, ldq,
-                        Aq, ldAq, 0);
+  return bp5_krhs_entry<double>(p, N, K, chunk, Nelements, elementList, GlobalToLocal, wJ, ggeo, gllw, D, lambda,
+                                q, ldq, Aq, ldAq, 0);
 }
 int bpx_bp5_ax_affine_hex_krhs(bpx_platform p, int N, int K, int chunk, int Nelements, const int* elementList,
                                const int* GlobalToLocal, const double* wJ, const double* vgeo, const double* ggeo,
                                const double* gllw, const double* D, const double* S_, const double* MM,
                                double lambda, const double* q, int64_t ldq, double* Aq, int64_t ldAq) {
   (void)vgeo; (void)S_; (void)MM;
-  return bp5_krhs_entry(p, N, K, chunk, Nelements, elementList, GlobalToLocal, wJ, ggeo, gllw, D, lambda, q, ldq,
-                        Aq, ldAq, 1);
+  return bp5_krhs_entry<double>(p, N, K, chunk, Nelements, elementList, GlobalToLocal, wJ, ggeo, gllw, D, lambda,
+                                q, ldq, Aq, ldAq, 1);
+}
+int bpx_bp5_ax_hex_krhs_f32(bpx_platform p, int N, int K, int chunk, int Nelements, const int* elementList,
+                            const int* GlobalToLocal, const float* wJ, const float* vgeo, const float* ggeo,
+                            const float* gllw, const float* D, const float* S_, const float* MM, float lambda,
+                            const float* q, int64_t ldq, float* Aq, int64_t ldAq) {
+  (void)vgeo; (void)S_; (void)MM;
+  return bp5_krhs_entry<float>(p, N, K, chunk, Nelements, elementList, GlobalToLocal, wJ, ggeo, gllw, D, lambda,
+                               q, ldq, Aq, ldAq, 0);
+}
+int bpx_bp5_ax_affine_hex_krhs_f32(bpx_platform p, int N, int K, int chunk, int Nelements,
+                                   const int* elementList, const int* GlobalToLocal, const float* wJ,
+                                   const float* vgeo, const float* ggeo, const float* gllw, const float* D,
+                                   const float* S_, const float* MM, float lambda, const float* q, int64_t ldq,
+                                   float* Aq, int64_t ldAq) {
+  (void)vgeo; (void)S_; (void)MM;
+  return bp5_krhs_entry<float>(p, N, K, chunk, Nelements, elementList, GlobalToLocal, wJ, ggeo, gllw, D, lambda,
+                               q, ldq, Aq, ldAq, 1);
 }
 int bpx_bp6_ax_hex(bpx_platform p, int N, int Nelements, const int* elementList, const int* GlobalToLocal,
                    const double* wJ, const double* vgeo, const double* ggeo, const double* gllw, const double* D,
@@ -842,6 +861,14 @@ int bpx_bp_time_operator(bpx_bp b, int precision, int launches, double* out) {
   if (int st = need(b, "bp")) return st;
   if (int st = need(out, "out")) return st;
   return b->bp.TimeOperator(precision, launches, out);
+  BPX_GUARD_END
+}
+
+int bpx_bp_time_krhs(bpx_bp b, int K, int chunk, int launches, const double* o_q, double* out) {
+  BPX_GUARD_BEGIN
+  if (int st = need(b, "bp")) return st;
+  if (int st = need(out, "out")) return st;
+  return b->bp.TimeKrhs(K, chunk, launches, o_q, out);
   BPX_GUARD_END
 }
 

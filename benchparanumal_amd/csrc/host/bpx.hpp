@@ -458,6 +458,11 @@ class bp_t : public solver_t {
   // o_Aq hold K vectors of Ngather, o_AqL K of NLocal; chunk as for
   // launch_bp5_ax_krhs (0 = default)
   int AxLocalKrhs(int K, const dfloat* o_q, dfloat* o_AqL, int chunk);
+  // the k-RHS Ax (K, chunk) on o_q (K vectors of Ngather; null: zeros)
+  // and the single-vector Ax on its first vector, each launched `launches`
+  // times after 2 warm-ups and timed by HIP events its own dispatch
+  // stamps: out = {mean k-RHS launch us, mean single launch us}
+  int TimeKrhs(int K, int chunk, int launches, const dfloat* o_q, double* out);
   int OperatorKrhs(int K, const dfloat* o_q, dfloat* o_Aq, int chunk);
   // Variable coefficient (BP5): kappa on the node lattice (device, Nnodes
   // values); nullptr restores the constant coefficient. The operator then

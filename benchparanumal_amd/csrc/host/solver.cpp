@@ -668,7 +668,60 @@ int bp_t::AxLocalKrhs(int K, const dfloat* o_q, dfloat* o_AqL, int chunk) {
   a.K = K;
   a.ldq = ogs.Ngather;
   a.ldAq = NLocal;
+  a.evStart = axEvStart;
+  a.evStop = axEvStop;
   return launch_bp5_ax_krhs<double>(platform->stream, a, chunk);
+}
+
+int bp_t::TimeKrhs(int K, int chunk, int launches, const dfloat* o_q, double* out) {
+  if (K < 1 || launches < 1) return set_error(BPX_ERR_INVALID_ARGUMENT, "time k-RHS: K and launches must be >= 1");
+  hipStream_t s = platform->stream;
+  const size_t ng = (size_t)std::max<dlong>(ogs.Ngather, 1);
+  deviceBuffer<double> qz, A;
+  if (!o_q) {  // zeros (a kernel's power, and so its clock, depends on the data: pass q)
+    if (int st = qz.alloc(ng * K)) return st;
+    BPX_HIP(hipMemsetAsync(qz.ptr(), 0, ng * K * sizeof(double), s));
+    o_q = qz.ptr();
+  }
+  if (int st = A.alloc((size_t)NLocal * K)) return st;
+  // K right-hand sides at once, then one right-hand side per launch
+  auto run = [&](int form) { return form == 0 ? AxLocalKrhs(K, o_q, A.ptr(), chunk) : AxLocal(o_q, A.ptr()); };
+  std::vector<hipEvent_t> ev(4 * (size_t)launches, nullptr);
+  int st = BPX_OK;
+  for (auto& e : ev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      st = set_error(BPX_ERR_HIP, "time k-RHS: event create failed");
+      break;
+    }
+  for (int form = 0; form < 2 && st == BPX_OK; ++form) {
+    for (int w = 0; w < 2 && st == BPX_OK; ++w) st = run(form);
+    for (int i = 0; i < launches && st == BPX_OK; ++i) {
+      struct Clear {  // the hooks must not outlive their events, whatever run() does
+        hipEvent_t &a, &b;
+        ~Clear() { a = b = nullptr; }
+      } clear{axEvStart, axEvStop};
+      axEvStart = ev[4 * i + 2 * form];  // stamped by the dispatch itself
+      axEvStop = ev[4 * i + 2 * form + 1];
+      st = run(form);
+    }
+  }
+  if (st == BPX_OK && hipStreamSynchronize(s) != hipSuccess) st = set_error(BPX_ERR_HIP, "time k-RHS: sync");
+  double t[2] = {0, 0};
+  for (int i = 0; i < launches && st == BPX_OK; ++i)
+    for (int form = 0; form < 2 && st == BPX_OK; ++form) {
+      float ms = 0;
+      if (hipEventElapsedTime(&ms, ev[4 * i + 2 * form], ev[4 * i + 2 * form + 1]) != hipSuccess) {
+        (void)hipGetLastError();  // do not leave the failure for the next launch check
+        st = set_error(BPX_ERR_HIP, "time k-RHS: elapsed time");
+      }
+      t[form] += ms;
+    }
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
+  if (st) return st;
+  out[0] = t[0] * 1e3 / launches;
+  out[1] = t[1] * 1e3 / launches;
+  return BPX_OK;
 }
 
 int bp_t::OperatorKrhs(int K, const dfloat* o_q, dfloat* o_Aq, int chunk) {

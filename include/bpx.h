@@ -92,13 +92,22 @@ int bpx_bp5_ax_affine_hex(bpx_platform p, int N, int Nelements, const int* eleme
  * launch; okl/bp5AxHex3D.okl:415-427 is the single-vector form): BP5 Ax on K
  * right-hand sides at once. The right-hand sides are stored one after
  * another: q + r*ldq (global, indexed through GlobalToLocal) and Aq +
- * r*ldAq (local, e*Np + n), r = 0..K-1. chunk = 0 (default): one workgroup
- * per element and group of right-hand sides, whose sheets share each slab
- * of geometric factors through LDS (non-affine; affine meshes use the
- * sweep form with its default chunk); chunk = 1, 2 or 4 (dividing K): each
- * element swept K/chunk times with chunk right-hand sides in registers.
- * Each right-hand side equals the single-vector bpx_bp5_ax_hex bit for bit
- * (to rounding at N = 15, where that one runs on the matrix cores). */
+ * r*ldAq (local, e*Np + n), r = 0..K-1. The form:
+ *  - chunk = 0 (default): the pencil form up to N = 9 (and
+ *    Nelements*Np*8 < 2^31), else the sheet form (non-affine) or the sweep
+ *    form with its default chunk;
+ *  - chunk = -2, the pencil form: each element's geometric factors loaded
+ *    once into registers, the K right-hand sides applied one after another,
+ *    every contraction along a pencil one lane holds. Agrees with
+ *    bpx_bp5_ax_hex to rounding (the weak divergence's r and s parts are
+ *    summed apart), not bit for bit;
+ *  - chunk = -1, the sheet form (non-affine): one workgroup per element and
+ *    group of right-hand sides, whose sheets share each slab of geometric
+ *    factors through LDS;
+ *  - chunk = 1, 2 or 4 (dividing K), the sweep form: each element swept
+ *    K/chunk times with chunk right-hand sides in registers.
+ * The sheet and sweep forms equal bpx_bp5_ax_hex bit for bit (to rounding
+ * at N = 15, where that one runs on the matrix cores). */
 int bpx_bp5_ax_hex_krhs(bpx_platform p, int N, int K, int chunk, int Nelements,
                         const int* elementList, const int* GlobalToLocal, const double* wJ,
                         const double* vgeo, const double* ggeo, const double* gllw,
@@ -110,6 +119,22 @@ int bpx_bp5_ax_affine_hex_krhs(bpx_platform p, int N, int K, int chunk, int Nele
                                const double* gllw, const double* D, const double* S,
                                const double* MM, double lambda, const double* q, int64_t ldq,
                                double* Aq, int64_t ldAq);
+/* No reference counterpart (single-vector FP32 form: okl/bp5AxHex3D.okl:415-427
+ * with dfloat = float): the k-RHS Ax above in FP32 (the same forms and
+ * chunk values), stored factors rounded to float. Each right-hand side
+ * agrees with bpx_bp5_ax_hex_f32 on it (bit for bit in the sheet and sweep
+ * forms, to rounding in the pencil form). */
+int bpx_bp5_ax_hex_krhs_f32(bpx_platform p, int N, int K, int chunk, int Nelements,
+                            const int* elementList, const int* GlobalToLocal, const float* wJ,
+                            const float* vgeo, const float* ggeo, const float* gllw,
+                            const float* D, const float* S, const float* MM, float lambda,
+                            const float* q, int64_t ldq, float* Aq, int64_t ldAq);
+int bpx_bp5_ax_affine_hex_krhs_f32(bpx_platform p, int N, int K, int chunk, int Nelements,
+                                   const int* elementList, const int* GlobalToLocal,
+                                   const float* wJ, const float* vgeo, const float* ggeo,
+                                   const float* gllw, const float* D, const float* S,
+                                   const float* MM, float lambda, const float* q, int64_t ldq,
+                                   float* Aq, int64_t ldAq);
 /* okl/bp6AxHex3D.okl:555-567 (3 fields, local layout [e][f][n]) */
 int bpx_bp6_ax_hex(bpx_platform p, int N, int Nelements, const int* elementList,
                    const int* GlobalToLocal, const double* wJ, const double* vgeo,
@@ -383,6 +408,15 @@ int bpx_bp_set_cg_fold(bpx_bp b, int mask);
  * out[1] = mean gather launch in us, out[2] = 1 if the fused form ran.
  * Synchronises. */
 int bpx_bp_time_operator(bpx_bp b, int precision, int launches, double* out);
+/* No reference counterpart (measurement): the BP5 k-RHS Ax (K right-hand
+ * sides, chunk as bpx_bp5_ax_hex_krhs) on o_q (device, K vectors of Ngather
+ * one after another; NULL: zeros, which draw less power and so run at a
+ * higher clock than data do) and the single-vector Ax on o_q's first
+ * vector, on the problem's arrays, each launched `launches` times back to
+ * back after 2 warm-ups and timed by HIP events its own dispatch stamps.
+ * out[0] = mean k-RHS launch in us, out[1] = mean single-vector launch in
+ * us. Synchronises. */
+int bpx_bp_time_krhs(bpx_bp b, int K, int chunk, int launches, const double* o_q, double* out);
 /* bpN_t::Operator(o_q, o_Aq) (BP/BP5/src/bp5Operator.cpp:29-85): Ax over the
  * local elements, then the T gather. Device vectors of length Ngather. */
 int bpx_bp_operator(bpx_bp b, const double* o_q, double* o_Aq);
