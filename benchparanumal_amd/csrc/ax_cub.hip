@@ -346,6 +346,12 @@ __global__ __launch_bounds__(64) void k_bp1_wave(int Nelements, const int* __res
   __shared__ T s_y[NQ * MQ * MQ];
   typedef const __attribute__((address_space(4))) T* cptr;
   const cptr Ic = (cptr)I;  // I[g][c] = Ic[g * NQ + c]
+  // entries of I are read with scalar loads where they are used: an opaque
+  // update of the pointer before each output keeps the compiler from holding
+  // all MQ * NQ of them in scalar registers (144 at NQ = 8), which spilled
+#define BPX_IROW() \
+  cptr Ir = Ic;    \
+  asm volatile("" : "+s"(Ir))
   const int l = threadIdx.x;
   int f0, f1;
   const int eo = wave_element<NF, FPAR>(Nelements, f0, f1);
@@ -376,9 +382,10 @@ __global__ __launch_bounds__(64) void k_bp1_wave(int Nelements, const int* __res
       }
 #pragma unroll
       for (int a = 0; a < MQ; ++a) {
+        BPX_IROW();
         T acc = 0;
 #pragma unroll
-        for (int i = 0; i < NQ; ++i) acc += Ic[a * NQ + i] * v[i];
+        for (int i = 0; i < NQ; ++i) acc += Ir[a * NQ + i] * v[i];
         s_x[l * LX + a] = acc;
       }
     }
@@ -394,9 +401,10 @@ __global__ __launch_bounds__(64) void k_bp1_wave(int Nelements, const int* __res
         for (int j = 0; j < NQ; ++j) v[j] = s_x[(k * NQ + j) * LX + a];
 #pragma unroll
         for (int b = 0; b < MQ; ++b) {
+          BPX_IROW();
           T acc = 0;
 #pragma unroll
-          for (int j = 0; j < NQ; ++j) acc += Ic[b * NQ + j] * v[j];
+          for (int j = 0; j < NQ; ++j) acc += Ir[b * NQ + j] * v[j];
           s_y[(k * MQ + b) * MQ + a] = acc;
         }
       }
@@ -413,16 +421,18 @@ __global__ __launch_bounds__(64) void k_bp1_wave(int Nelements, const int* __res
         T z[MQ];
 #pragma unroll
         for (int g = 0; g < MQ; ++g) {
+          BPX_IROW();
           T acc = 0;
 #pragma unroll
-          for (int k = 0; k < NQ; ++k) acc += Ic[g * NQ + k] * v[k];
+          for (int k = 0; k < NQ; ++k) acc += Ir[g * NQ + k] * v[k];
           z[g] = acc * wv[c][g];
         }
 #pragma unroll
         for (int k = 0; k < NQ; ++k) {
+          BPX_IROW();
           T acc = 0;
 #pragma unroll
-          for (int g = 0; g < MQ; ++g) acc += Ic[g * NQ + k] * z[g];
+          for (int g = 0; g < MQ; ++g) acc += Ir[g * NQ + k] * z[g];
           s_y[k * MQ2 + col] = acc;
         }
       }
@@ -439,9 +449,10 @@ __global__ __launch_bounds__(64) void k_bp1_wave(int Nelements, const int* __res
         for (int b = 0; b < MQ; ++b) v[b] = s_y[(k * MQ + b) * MQ + a];
 #pragma unroll
         for (int j = 0; j < NQ; ++j) {
+          BPX_IROW();
           T acc = 0;
 #pragma unroll
-          for (int b = 0; b < MQ; ++b) acc += Ic[b * NQ + j] * v[b];
+          for (int b = 0; b < MQ; ++b) acc += Ir[b * NQ + j] * v[b];
           s_x[(k * NQ + j) * LX + a] = acc;
         }
       }
@@ -454,14 +465,16 @@ __global__ __launch_bounds__(64) void k_bp1_wave(int Nelements, const int* __res
       for (int a = 0; a < MQ; ++a) v[a] = s_x[l * LX + a];
 #pragma unroll
       for (int i = 0; i < NQ; ++i) {
+        BPX_IROW();
         T acc = 0;
 #pragma unroll
-        for (int a = 0; a < MQ; ++a) acc += Ic[a * NQ + i] * v[a];
+        for (int a = 0; a < MQ; ++a) acc += Ir[a * NQ + i] * v[a];
         Aq[lb + l * NQ + i] = acc;
       }
     }
     __syncthreads();  // the next field rewrites the tiles
   }
+#undef BPX_IROW
 }
 
 // BP3/BP4 non-affine (Aq = I^T (cubD^T G cubD + lambda cubwJ) I q,
