@@ -85,6 +85,7 @@ _SIGS = {
     "bpx_bp_time_operator": (_I, [_P, _I, _I, _P]),
     "bpx_bp_time_krhs": (_I, [_P, _I, _I, _I, _P, _P]),
     "bpx_bp_set_ax_engine": (_I, [_P, _I]),
+    "bpx_bp_set_cub_form": (_I, [_P, _I]),
     "bpx_bp_set_cg_fold": (_I, [_P, _I]),
     "bpx_bp_set_fp32_geometry": (_I, [_P, _I]),
     "bpx_bp_time_operator_in_solve": (_I, [_P, _P, _P, _D, _I, _I, _P]),
@@ -363,6 +364,11 @@ class BP:
         """BP5 at N = 15, FP64: 0 = auto (matrix cores), 1 = vector-ALU Ax,
         2 = matrix-core Ax (the two agree to rounding, not bit for bit)."""
         check(lib().bpx_bp_set_ax_engine(self.h, int(engine)))
+
+    def set_cub_form(self, form):
+        """BP1-BP4 non-affine at N <= 7: 0 = auto (pencil kernel), 1 =
+        pencil, 2 = one element per wavefront (the two agree to rounding)."""
+        check(lib().bpx_bp_set_cub_form(self.h, int(form)))
 
     def time_krhs(self, K, chunk=0, launches=10, q=None):
         """(k-RHS Ax us, single-vector Ax us), dispatch-stamped means; q:

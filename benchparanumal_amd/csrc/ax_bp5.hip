@@ -30,6 +30,7 @@
 #include <type_traits>
 
 #include "ax_common.hpp"
+#include "buffer_rsrc.hpp"
 
 #ifndef BPX_NQ
 #error "compile with -DBPX_NQ=<Nq>"
@@ -760,29 +761,6 @@ __global__ __launch_bounds__(NQ* NQ* W) __attribute__((amdgpu_waves_per_eu(DR ? 
 // buffer, and every offset but the i pencils' is a lane base plus a
 // constant. The divergence's r and s parts are summed apart (each along its
 // own pencil), so results agree with k_bp5_ax to rounding, not bit for bit.
-// raw buffer resources (stride 0): loads at offsets >= num_records return 0
-constexpr int kBufRange = 0x7FFFFFF8;          // num_records (bytes)
-constexpr int kBufMasked = (int)0x80000000u;   // an offset beyond it
-constexpr int kBufFlags = 0x00020000;          // dword 3: 32-bit data format (gfx9)
-
-template <typename T>
-__device__ inline T buffer_load(__amdgpu_buffer_rsrc_t r, int offset) {
-  if constexpr (sizeof(T) == 8)
-    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, offset, 0, 0));
-  else
-    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, offset, 0, 0));
-}
-
-typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
-
-template <typename T>
-__device__ inline void buffer_store(T v, __amdgpu_buffer_rsrc_t r, int offset) {
-  if constexpr (sizeof(T) == 8)
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, offset, 0, 0);
-  else
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, offset, 0, 0);
-}
-
 template <int NQ>
 constexpr int krhs_pencil_epb() {
   return NQ * NQ >= 32 ? 1 : 64 / (NQ * NQ);

@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include "ax_common.hpp"
+#include "buffer_rsrc.hpp"
 
 #ifndef BPX_NQ
 #error "compile with -DBPX_NQ=<Nq>"
@@ -699,6 +700,393 @@ __global__ __launch_bounds__(128) void k_bp3_wave(int Nelements, const int* __re
   }
 }
 
+// BP1-BP4 non-affine, pencil form (NQ <= 8; the default there). A slot is
+// one field of one element; a 256-lane workgroup holds S = 256 / MQ^2 slots
+// (three at p = 7: the three fields of a BP2/BP4 element, or three BP1/BP3
+// elements), so the 64-, 72- and 81-line passes of three slots fill 192,
+// 216 and 243 lanes instead of taking two passes of one wavefront each.
+// Every one-dimensional contraction runs along a line (pencil) that one
+// lane holds in registers, with the 1D matrix uniform over the workgroup
+// (scalar loads, one row in use and the next in flight); values move
+// between line directions through LDS once per move. Two LDS tiles per
+// slot, A and B, with line stride PR = MQ | 1 (odd strides keep the
+// strided accesses free of bank conflicts) and slab stride PS = MQ * PR:
+//   i-lines (k,j):  gather q, x = I q along i                 -> A
+//   j-lines (k,a):  y = I x along j                           -> B
+//   t-lines (b,a):  u = I y along k, in registers
+//     BP1/BP2:      times cubwJ, back along k (in place)      -> B
+//     BP3/BP4:      u                                         -> A
+//   r-lines (g,b):  ur = D u along a                          -> B
+//   t-lines (b,a):  ur into registers
+//   s-lines (g,a):  us = D u along b                          -> B
+//   t-lines (b,a):  ut = D u along g (registers); G: Gr -> A, Gs -> B;
+//                   res = lambda W u + D^T Gt (registers)
+//   r-lines, s-lines: D^T Gr (A), D^T Gs (B), in place
+//   t-lines (b,a):  res += both; back along g                 -> A
+//   j-lines (k,a):  back along j: z (A; B for BP1/BP2)        -> x (B; A)
+//   i-lines (k,j):  back along i                              -> Aq
+// A node crosses LDS 4 (BP1) or 16 (BP3) times, against 9 values read per
+// output of a gradient or divergence in the wavefront forms. The geometric
+// factors of the first PF cubature slabs are requested when u is formed,
+// so they are in flight through the r and s derivatives. The r, s and t
+// parts of the divergence are summed apart, so BP3/BP4 results agree with
+// the other forms to rounding, not bit for bit (BP1/BP2 add in the same
+// order as the wavefront form). q is gathered through a bounds-checked
+// buffer resource: a masked node (G2L = -1) or an idle slot reads 0 without
+// a compare or select per load.
+template <typename T>
+__device__ inline T cfma(T a, T b, T c) {
+  if constexpr (sizeof(T) == 8)
+    return __builtin_fma(a, b, c);
+  else
+    return __builtin_fmaf(a, b, c);
+}
+
+// rows of a uniform R x C row-major matrix through scalar registers, row
+// n + 1 requested while row n is in use: body(n, row) for n = 0..R-1. The
+// opaque update of the pointer before each row keeps the compiler from
+// holding the whole matrix in scalar registers.
+template <int R, int C, typename T, typename F>
+__device__ inline void uniform_rows(const T* M, F&& body) {
+  typedef const __attribute__((address_space(4))) T* cptr;
+  cptr Mc = (cptr)M;
+  T d[2][C];
+  asm volatile("" : "+s"(Mc));
+#pragma unroll
+  for (int m = 0; m < C; ++m) d[0][m] = Mc[m];
+#pragma unroll
+  for (int n = 0; n < R; ++n) {
+    if (n + 1 < R) {
+      __builtin_amdgcn_sched_barrier(0);
+      asm volatile("" : "+s"(Mc));
+#pragma unroll
+      for (int m = 0; m < C; ++m) d[(n + 1) & 1][m] = Mc[(n + 1) * C + m];
+    }
+    body(n, d[n & 1]);
+  }
+}
+
+template <int MQ>
+constexpr int pencil_slots() {
+  return MQ * MQ >= 256 ? 1 : 256 / (MQ * MQ);
+}
+
+// LDS tiles per slot (values): A holds x (NQ^2 lines of PR) and, for
+// BP3/BP4, u, Gr and z (MQ slabs); B holds y and z (NQ slabs) or, for
+// BP3/BP4, ur, us and Gs (MQ slabs)
+template <int MQ, bool STIFF>
+constexpr int pencil_tile_a() {
+  return STIFF ? MQ * MQ * (MQ | 1) : (MQ - 1) * (MQ - 1) * (MQ | 1);
+}
+template <int MQ, bool STIFF>
+constexpr int pencil_tile_b() {
+  return (STIFF ? MQ : MQ - 1) * MQ * (MQ | 1);
+}
+
+template <typename T, int MQ, bool STIFF>
+constexpr int pencil_lds_bytes() {
+  return pencil_slots<MQ>() * (pencil_tile_a<MQ, STIFF>() + pencil_tile_b<MQ, STIFF>()) * (int)sizeof(T);
+}
+
+// waves per SIMD that the 160 KB of LDS per CU allows (a workgroup is one
+// wavefront on each of the 4 SIMDs), at most 8; BP3/BP4 at most 4 (FP64)
+// or 5 (FP32), above which they spill
+template <typename T, int MQ, bool STIFF>
+constexpr int pencil_waves() {
+  constexpr int w = (160 * 1024) / pencil_lds_bytes<T, MQ, STIFF>();
+  constexpr int cap = STIFF ? (sizeof(T) == 8 ? 4 : 5) : 8;
+  return w < 1 ? 1 : (w > cap ? cap : w);
+}
+
+// cubature slabs whose geometric factors are requested ahead (BP3/BP4)
+template <typename T, int MQ>
+constexpr int pencil_prefetch() {
+  return sizeof(T) == 8 ? (MQ < 4 ? MQ : 4) : (MQ < 6 ? MQ : 6);
+}
+
+template <typename T, int NQ, int NF, bool STIFF>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(pencil_waves<T, NQ + 1, STIFF>()))) void
+k_cub_pencil(int Nelements, const int* __restrict__ elementList, const int* __restrict__ G2L,
+             const T* __restrict__ W, const T* __restrict__ G, const T* __restrict__ Dq,
+             const T* __restrict__ I, T lambda, const T* __restrict__ q, T* __restrict__ Aq) {
+  constexpr int MQ = NQ + 1;
+  constexpr int NQ2 = NQ * NQ;
+  constexpr int NP = NQ2 * NQ;
+  constexpr int MQ2 = MQ * MQ;
+  constexpr int MP = MQ2 * MQ;
+  constexpr int PR = MQ | 1;  // line stride in LDS
+  constexpr int PS = MQ * PR;  // slab stride in LDS
+  constexpr int SA = pencil_tile_a<MQ, STIFF>();
+  constexpr int SB = pencil_tile_b<MQ, STIFF>();
+  constexpr int S = pencil_slots<MQ>();
+  constexpr int PF = STIFF ? pencil_prefetch<T, MQ>() : 1;
+  constexpr bool SG = sizeof(T) == 4;  // FP32: geometry loads non-temporal
+  static_assert(S * MQ2 <= 256, "one line per lane and pass");
+  static_assert(NQ * NQ * PR <= SA && NQ * PS <= SB, "x and y fit their tiles");
+  __shared__ T s_a[S * SA];
+  __shared__ T s_b[S * SB];
+
+  const int pairs = Nelements * NF;
+  const int groups = (pairs + S - 1) / S;
+  const int per_xcd = (groups + 7) / 8;
+  // each XCD (block mod 8) takes a contiguous run of slot groups, so face
+  // neighbours share q lines in its L2
+  const int grp = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+  if (grp >= groups) return;  // uniform over the workgroup
+  const int t = threadIdx.x;
+
+  // the element and field of slot s (an idle slot reads element 0's data
+  // and stores nothing)
+  auto slot = [&](int s, int& e, int& f) -> bool {
+    const int p = grp * S + s;
+    const bool act = s < S && p < pairs;
+    const int eo = act ? p / NF : 0;
+    f = act ? p - eo * NF : 0;
+    e = elementList[eo];
+    return act;
+  };
+
+  // i-line lanes (passes 1 and 9): line (k, j) = li of slot si
+  const int si = t / NQ2, li = t - si * NQ2;
+  const bool oni = si < S;
+  int ei, fi;
+  const bool acti = slot(si, ei, fi);
+  const size_t lbi = ((size_t)ei * NF + fi) * NP + (size_t)li * NQ;
+  // j-line lanes (passes 2 and 8): line (k, a)
+  const int sj = t / (NQ * MQ), lj = t - sj * (NQ * MQ);
+  const bool onj = sj < S;
+  const int jk = lj / MQ, ja = lj - jk * MQ;
+  // t-line lanes: line (b, a) = lt of slot st; the r- and s-line passes
+  // use the same split, line (g, c) = lt
+  const int st = t / MQ2, lt = t - st * MQ2;
+  const bool ont = st < S;
+  const int tb = lt / MQ, ta = lt - tb * MQ;
+  const int tp = tb * PR + ta;  // LDS offset of point (., b, a) in a slab
+  int et, ft;
+  slot(st, et, ft);
+  T* const At = s_a + st * SA;
+  T* const Bt = s_b + st * SB;
+
+  T wv[STIFF ? 1 : MQ];
+  if constexpr (!STIFF) {
+    // cubwJ of the t-line, requested first
+    if (ont) {
+#pragma unroll
+      for (int g = 0; g < MQ; ++g) wv[g] = W[(size_t)et * MP + g * MQ2 + lt];
+    }
+  }
+
+  // pass 1: i-lines, gather q, interpolate along i -> A
+  if (oni) {
+    const __amdgpu_buffer_rsrc_t rq = buffer_rsrc(q);
+    T v[NQ];
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+      const int id = G2L[lbi + i];
+      v[i] = buffer_load<T>(rq, (acti && id >= 0) ? id * (int)sizeof(T) : kBufMasked);
+    }
+    T* const x = s_a + si * SA + li * PR;
+    uniform_rows<MQ, NQ>(I, [&](int a, const T* r) {
+      T acc = 0;
+#pragma unroll
+      for (int i = 0; i < NQ; ++i) acc = cfma(r[i], v[i], acc);
+      x[a] = acc;
+    });
+  }
+  __syncthreads();
+  // pass 2: j-lines, along j -> B (y[k][b][a] at k*PS + b*PR + a)
+  if (onj) {
+    const T* x = s_a + sj * SA + jk * NQ * PR + ja;
+    T* const y = s_b + sj * SB + jk * PS + ja;
+    T v[NQ];
+#pragma unroll
+    for (int j = 0; j < NQ; ++j) v[j] = x[j * PR];
+    uniform_rows<MQ, NQ>(I, [&](int b, const T* r) {
+      T acc = 0;
+#pragma unroll
+      for (int j = 0; j < NQ; ++j) acc = cfma(r[j], v[j], acc);
+      y[b * PR] = acc;
+    });
+  }
+  __syncthreads();
+  // pass 3: t-lines, along k
+  T u[MQ];
+  T gpre[PF][7];
+  if (ont) {
+    T v[NQ];
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) v[k] = Bt[tp + k * PS];
+    if constexpr (STIFF) {
+      // the first PF slabs' geometric factors, in flight from here
+#pragma unroll
+      for (int g = 0; g < PF; ++g) {
+        const size_t cb = (size_t)et * MP + g * MQ2 + lt;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) gpre[g][c] = geo_load<SG>(G + 6 * cb + c);
+        gpre[g][6] = geo_load<SG>(W + cb);
+      }
+    }
+    uniform_rows<MQ, NQ>(I, [&](int g, const T* r) {
+      T acc = 0;
+#pragma unroll
+      for (int k = 0; k < NQ; ++k) acc = cfma(r[k], v[k], acc);
+      u[g] = acc;
+    });
+    if constexpr (!STIFF) {
+      // times cubwJ, back along k, in place
+      T z[NQ];
+#pragma unroll
+      for (int k = 0; k < NQ; ++k) z[k] = T(0);
+      uniform_rows<MQ, NQ>(I, [&](int g, const T* r) {
+        const T zg = u[g] * wv[g];
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) z[k] = cfma(r[k], zg, z[k]);
+      });
+#pragma unroll
+      for (int k = 0; k < NQ; ++k) Bt[tp + k * PS] = z[k];
+    } else {
+#pragma unroll
+      for (int g = 0; g < MQ; ++g) At[tp + g * PS] = u[g];
+    }
+  }
+  __syncthreads();
+
+  if constexpr (STIFF) {
+    // contraction of the line at p (stride ls) with D (rows n) or D^T
+    // (rows m), written back to the same points
+    auto line_d = [&](const T* src, T* dst, int ls) {
+      T v[MQ];
+#pragma unroll
+      for (int m = 0; m < MQ; ++m) v[m] = src[m * ls];
+      uniform_rows<MQ, MQ>(Dq, [&](int n, const T* r) {
+        T acc = 0;
+#pragma unroll
+        for (int m = 0; m < MQ; ++m) acc = cfma(r[m], v[m], acc);
+        dst[n * ls] = acc;
+      });
+    };
+    auto line_dt = [&](T* p, int ls) {
+      T v[MQ], o[MQ];
+#pragma unroll
+      for (int m = 0; m < MQ; ++m) {
+        v[m] = p[m * ls];
+        o[m] = T(0);
+      }
+      uniform_rows<MQ, MQ>(Dq, [&](int m, const T* r) {
+#pragma unroll
+        for (int n = 0; n < MQ; ++n) o[n] = cfma(r[n], v[m], o[n]);
+      });
+#pragma unroll
+      for (int n = 0; n < MQ; ++n) p[n * ls] = o[n];
+    };
+    // the r-line (g, b) = (tb, ta) runs along a at tb*PS + ta*PR; the
+    // s-line (g, a) = (tb, ta) along b at tb*PS + ta
+    const int ro = tb * PS + ta * PR, so = tb * PS + ta;
+    // pass 4: r-lines, ur along a -> B
+    if (ont) line_d(At + ro, Bt + ro, 1);
+    __syncthreads();
+    // pass 5: t-lines take their ur
+    T ur[MQ];
+    if (ont) {
+#pragma unroll
+      for (int g = 0; g < MQ; ++g) ur[g] = Bt[tp + g * PS];
+    }
+    __syncthreads();
+    // pass 6: s-lines, us along b -> B
+    if (ont) line_d(At + so, Bt + so, PR);
+    __syncthreads();
+    // pass 7: t-lines: ut, G (Gr -> A, Gs -> B), the mass term and the
+    // divergence's t part
+    T res[MQ];
+    if (ont) {
+#pragma unroll
+      for (int m = 0; m < MQ; ++m) res[m] = T(0);
+      uniform_rows<MQ, MQ>(Dq, [&](int g, const T* r) {
+        T ut = 0;
+#pragma unroll
+        for (int m = 0; m < MQ; ++m) ut = cfma(r[m], u[m], ut);
+        const T us = Bt[tp + g * PS];
+        T gg[7];
+        if (g < PF) {
+#pragma unroll
+          for (int c = 0; c < 7; ++c) gg[c] = gpre[g < PF ? g : 0][c];
+        } else {
+          const size_t cb = (size_t)et * MP + g * MQ2 + lt;
+#pragma unroll
+          for (int c = 0; c < 6; ++c) gg[c] = geo_load<SG>(G + 6 * cb + c);
+          gg[6] = geo_load<SG>(W + cb);
+        }
+        const T G00 = gg[0], G01 = gg[1], G02 = gg[2], G11 = gg[3], G12 = gg[4], G22 = gg[5], WW = gg[6];
+        At[tp + g * PS] = cfma(G02, ut, cfma(G01, us, G00 * ur[g]));
+        Bt[tp + g * PS] = cfma(G12, ut, cfma(G11, us, G01 * ur[g]));
+        const T Gt = cfma(G22, ut, cfma(G12, us, G02 * ur[g]));
+        res[g] = cfma(lambda * WW, u[g], res[g]);
+#pragma unroll
+        for (int m = 0; m < MQ; ++m) res[m] = cfma(r[m], Gt, res[m]);
+      });
+    }
+    __syncthreads();
+    // pass 8: r-lines D^T Gr (A), s-lines D^T Gs (B), in place
+    if (ont) {
+      line_dt(At + ro, 1);
+      line_dt(Bt + so, PR);
+    }
+    __syncthreads();
+    // pass 9: t-lines: add the r and s parts, back along g -> A
+    if (ont) {
+#pragma unroll
+      for (int g = 0; g < MQ; ++g) res[g] += At[tp + g * PS] + Bt[tp + g * PS];
+      T z[NQ];
+#pragma unroll
+      for (int k = 0; k < NQ; ++k) z[k] = T(0);
+      uniform_rows<MQ, NQ>(I, [&](int g, const T* r) {
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) z[k] = cfma(r[k], res[g], z[k]);
+      });
+#pragma unroll
+      for (int k = 0; k < NQ; ++k) At[tp + k * PS] = z[k];
+    }
+    __syncthreads();
+  }
+
+  // j-lines, back along j: z (B; A for BP3/BP4) -> x (A; B)
+  {
+    T* const Z = STIFF ? s_a + sj * SA : s_b + sj * SB;
+    T* const X = STIFF ? s_b + sj * SB : s_a + sj * SA;
+    if (onj) {
+      const T* z = Z + jk * PS + ja;
+      T v[MQ], o[NQ];
+#pragma unroll
+      for (int b = 0; b < MQ; ++b) v[b] = z[b * PR];
+#pragma unroll
+      for (int j = 0; j < NQ; ++j) o[j] = T(0);
+      uniform_rows<MQ, NQ>(I, [&](int b, const T* r) {
+#pragma unroll
+        for (int j = 0; j < NQ; ++j) o[j] = cfma(r[j], v[b], o[j]);
+      });
+      T* const x = X + jk * NQ * PR + ja;
+#pragma unroll
+      for (int j = 0; j < NQ; ++j) x[j * PR] = o[j];
+    }
+  }
+  __syncthreads();
+  // i-lines, back along i -> Aq
+  if (acti) {
+    const T* x = (STIFF ? s_b + si * SB : s_a + si * SA) + li * PR;
+    T v[MQ], o[NQ];
+#pragma unroll
+    for (int a = 0; a < MQ; ++a) v[a] = x[a];
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) o[i] = T(0);
+    uniform_rows<MQ, NQ>(I, [&](int a, const T* r) {
+#pragma unroll
+      for (int i = 0; i < NQ; ++i) o[i] = cfma(r[i], v[a], o[i]);
+    });
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) Aq[lbi + i] = o[i];
+  }
+}
+
 // the 3-field wavefront forms split their fields over workgroups (set to 0
 // to run them one after another in one workgroup; same results)
 #ifndef BPX_FIELD_SPLIT
@@ -716,12 +1104,32 @@ int launch_mode(hipStream_t s, const AxArgs<T>& x, const T* W, const T* G, const
   return BPX_OK;
 }
 
+// the pencil form gathers q at byte offsets below kBufRange: every
+// gathered index lies below the number of local nodes
+template <typename T, int NQ, int NF>
+bool pencil_in_range(const AxArgs<T>& a) {
+  return (double)a.Nelements * NF * NQ * NQ * NQ * sizeof(T) < (double)kBufRange;
+}
+
+template <typename T, int NQ, int NF, bool STIFF>
+int launch_cub_pencil(hipStream_t s, const AxArgs<T>& a, const T* G, const T* Dq) {
+  constexpr int S = pencil_slots<NQ + 1>();
+  const long long groups = ((long long)a.Nelements * NF + S - 1) / S;
+  const unsigned blocks = 8u * (unsigned)((groups + 7) / 8);
+  launch_stamped(k_cub_pencil<T, NQ, NF, STIFF>, dim3(blocks), dim3(256), s, a.evStart, a.evStop, a.Nelements,
+                 a.elementList, a.GlobalToLocal, a.wJ, G, Dq, a.I, a.lambda, a.q, a.Aq);
+  BPX_HIP(hipGetLastError());
+  return BPX_OK;
+}
+
 }  // namespace
 
 template <typename T, int NQ, int NF>
 int launch_bp1_nq(hipStream_t s, const AxArgs<T>& a) {
   if (a.affine) return launch_mode<T, NQ, NQ, NF, MODE_BP1A>(s, a, a.wJ, nullptr, nullptr, a.MM);
   if constexpr (NQ <= 8) {
+    if (a.cubForm != CUB_FORM_WAVE && pencil_in_range<T, NQ, NF>(a))
+      return launch_cub_pencil<T, NQ, NF, false>(s, a, nullptr, nullptr);
     constexpr bool FP = NF > 1 && BPX_FIELD_SPLIT;
     launch_stamped(k_bp1_wave<T, NQ, NF, FP>, dim3(wave_blocks<NF, FP>(a.Nelements)), dim3(64), s, a.evStart,
                    a.evStop, a.Nelements, a.elementList, a.GlobalToLocal, a.wJ, a.I, a.q, a.Aq);
@@ -735,6 +1143,8 @@ template <typename T, int NQ, int NF>
 int launch_bp3_nq(hipStream_t s, const AxArgs<T>& a) {
   if (a.affine) return launch_mode<T, NQ, NQ, NF, MODE_BP3A>(s, a, a.wJ, a.ggeo, a.D, a.invV);
   if constexpr (NQ <= 8) {
+    if (a.cubForm != CUB_FORM_WAVE && pencil_in_range<T, NQ, NF>(a))
+      return launch_cub_pencil<T, NQ, NF, true>(s, a, a.ggeo, a.D);
     constexpr bool FP = NF > 1 && BPX_FIELD_SPLIT;
     launch_stamped(k_bp3_wave<T, NQ, NF, FP>, dim3(wave_blocks<NF, FP>(a.Nelements)), dim3(128), s, a.evStart,
                    a.evStop, a.Nelements, a.elementList, a.GlobalToLocal, a.wJ, a.ggeo, a.D, a.I, a.lambda, a.q,

@@ -87,11 +87,16 @@ struct AxArgs {
   // trilinear map (24 per element) and wJ the GLL nodes; the kernel forms
   // the geometric factors itself
   int tri = 0;
+  // BP1-BP4 non-affine at Nq <= 8: the pencil kernel (CUB_FORM_AUTO or
+  // CUB_FORM_PENCIL) or the wavefront-per-element kernels (CUB_FORM_WAVE);
+  // the two agree to rounding, not bit for bit
+  int cubForm = 0;
   // when set (BP1-BP6 Ax), the kernel's dispatch stamps its start and end
   // into these events (hipExtLaunchKernelGGL: no barrier packets around it)
   hipEvent_t evStart = nullptr, evStop = nullptr;
 };
 enum { AX_ENGINE_AUTO = 0, AX_ENGINE_VECTOR = 1, AX_ENGINE_MATRIX = 2 };
+enum { CUB_FORM_AUTO = 0, CUB_FORM_PENCIL = 1, CUB_FORM_WAVE = 2 };
 #if defined(__HIPCC__)
 }  // namespace bpx
 #include <hip/hip_ext.h>

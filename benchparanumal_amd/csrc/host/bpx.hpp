@@ -407,6 +407,8 @@ class bp_t : public solver_t {
   // BP5, one field, N = 15, FP64: where the r and s contractions run
   // (AX_ENGINE_AUTO: the matrix cores; see AxArgs::engine)
   int axEngine = AX_ENGINE_AUTO;
+  // BP1-BP4 non-affine at Nq <= 8: pencil or wavefront-per-element kernels
+  int cubForm = CUB_FORM_AUTO;
   // the CG operator launched back to back and timed kernel by kernel: the
   // Ax by its own dispatch stamps, the gather from the Ax's end to an event
   // after it

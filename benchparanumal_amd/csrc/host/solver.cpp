@@ -376,6 +376,7 @@ int bp_t::ax(const T* q, T* AqL, const T* wJ, const T* ggeo, const T* gllw, cons
   a.q = q;
   a.Aq = AqL;
   if constexpr (std::is_same<T, double>::value) a.engine = axEngine;
+  a.cubForm = cubForm;
   if constexpr (std::is_same<T, float>::value) a.tri = tri32;
   a.evStart = axEvStart;
   a.evStop = axEvStop;
