@@ -771,16 +771,18 @@ constexpr int pencil_slots() {
   return MQ * MQ >= 256 ? 1 : 256 / (MQ * MQ);
 }
 
-// LDS tiles per slot (values): A holds x (NQ^2 lines of PR) and, for
-// BP3/BP4, u, Gr and z (MQ slabs); B holds y and z (NQ slabs) or, for
-// BP3/BP4, ur, us and Gs (MQ slabs)
+// LDS tiles per slot (values). BP3/BP4: A holds x (NQ^2 lines of PR), u,
+// Gr and z (MQ slabs), B holds y, ur, us and Gs. BP1/BP2: one tile holds
+// x, y, z and x in turn (NQ^2 lines or NQ slabs), the values of a pass
+// that rewrites it staged in registers across a barrier.
 template <int MQ, bool STIFF>
 constexpr int pencil_tile_a() {
-  return STIFF ? MQ * MQ * (MQ | 1) : (MQ - 1) * (MQ - 1) * (MQ | 1);
+  constexpr int x = (MQ - 1) * (MQ - 1) * (MQ | 1), y = (MQ - 1) * MQ * (MQ | 1);
+  return STIFF ? MQ * MQ * (MQ | 1) : (x > y ? x : y);
 }
 template <int MQ, bool STIFF>
 constexpr int pencil_tile_b() {
-  return (STIFF ? MQ : MQ - 1) * MQ * (MQ | 1);
+  return STIFF ? MQ * MQ * (MQ | 1) : 0;
 }
 
 template <typename T, int MQ, bool STIFF>
@@ -817,14 +819,15 @@ k_cub_pencil(int Nelements, const int* __restrict__ elementList, const int* __re
   constexpr int PR = MQ | 1;  // line stride in LDS
   constexpr int PS = MQ * PR;  // slab stride in LDS
   constexpr int SA = pencil_tile_a<MQ, STIFF>();
-  constexpr int SB = pencil_tile_b<MQ, STIFF>();
+  constexpr int SB = STIFF ? pencil_tile_b<MQ, STIFF>() : SA;
   constexpr int S = pencil_slots<MQ>();
   constexpr int PF = STIFF ? pencil_prefetch<T, MQ>() : 1;
   constexpr bool SG = sizeof(T) == 4;  // FP32: geometry loads non-temporal
   static_assert(S * MQ2 <= 256, "one line per lane and pass");
   static_assert(NQ * NQ * PR <= SA && NQ * PS <= SB, "x and y fit their tiles");
   __shared__ T s_a[S * SA];
-  __shared__ T s_b[S * SB];
+  __shared__ T s_bt[STIFF ? S * SB : 1];
+  T* const s_b = STIFF ? s_bt : s_a;  // BP1/BP2: one tile
 
   const int pairs = Nelements * NF;
   const int groups = (pairs + S - 1) / S;
@@ -895,18 +898,25 @@ k_cub_pencil(int Nelements, const int* __restrict__ elementList, const int* __re
   }
   __syncthreads();
   // pass 2: j-lines, along j -> B (y[k][b][a] at k*PS + b*PR + a)
-  if (onj) {
-    const T* x = s_a + sj * SA + jk * NQ * PR + ja;
-    T* const y = s_b + sj * SB + jk * PS + ja;
-    T v[NQ];
+  {
+    T v[NQ], yv[MQ];
+    if (onj) {
+      const T* x = s_a + sj * SA + jk * NQ * PR + ja;
 #pragma unroll
-    for (int j = 0; j < NQ; ++j) v[j] = x[j * PR];
-    uniform_rows<MQ, NQ>(I, [&](int b, const T* r) {
-      T acc = 0;
+      for (int j = 0; j < NQ; ++j) v[j] = x[j * PR];
+      uniform_rows<MQ, NQ>(I, [&](int b, const T* r) {
+        T acc = 0;
 #pragma unroll
-      for (int j = 0; j < NQ; ++j) acc = cfma(r[j], v[j], acc);
-      y[b * PR] = acc;
-    });
+        for (int j = 0; j < NQ; ++j) acc = cfma(r[j], v[j], acc);
+        yv[b] = acc;
+      });
+    }
+    if constexpr (!STIFF) __syncthreads();  // y overwrites x in the one tile
+    if (onj) {
+      T* const y = s_b + sj * SB + jk * PS + ja;
+#pragma unroll
+      for (int b = 0; b < MQ; ++b) y[b * PR] = yv[b];
+    }
   }
   __syncthreads();
   // pass 3: t-lines, along k
@@ -1053,9 +1063,9 @@ k_cub_pencil(int Nelements, const int* __restrict__ elementList, const int* __re
   {
     T* const Z = STIFF ? s_a + sj * SA : s_b + sj * SB;
     T* const X = STIFF ? s_b + sj * SB : s_a + sj * SA;
+    T v[MQ], o[NQ];
     if (onj) {
       const T* z = Z + jk * PS + ja;
-      T v[MQ], o[NQ];
 #pragma unroll
       for (int b = 0; b < MQ; ++b) v[b] = z[b * PR];
 #pragma unroll
@@ -1064,6 +1074,9 @@ k_cub_pencil(int Nelements, const int* __restrict__ elementList, const int* __re
 #pragma unroll
         for (int j = 0; j < NQ; ++j) o[j] = cfma(r[j], v[b], o[j]);
       });
+    }
+    if constexpr (!STIFF) __syncthreads();  // x overwrites z in the one tile
+    if (onj) {
       T* const x = X + jk * NQ * PR + ja;
 #pragma unroll
       for (int j = 0; j < NQ; ++j) x[j * PR] = o[j];
