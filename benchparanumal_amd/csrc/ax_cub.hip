@@ -766,6 +766,61 @@ __device__ inline void uniform_rows(const T* M, F&& body) {
   }
 }
 
+// Even-odd form of y = A x for a 1D operator A (N x M) with A[N-1-n][M-1-m]
+// = SIGN * A[n][m] (the interpolation matrix between symmetric point sets
+// and its transpose: SIGN = 1; the derivative on symmetric points and its
+// transpose: SIGN = -1). With xe[m] = x[m] + x[M-1-m], xo[m] = x[m] -
+// x[M-1-m] (m < M/2) and Ae = (A[n][m] + A[n][M-1-m]) / 2, Ao = (A[n][m] -
+// A[n][M-1-m]) / 2: ye[n] = Ae xe (+ A[n][M/2] x[M/2] for odd M), yo[n] =
+// Ao xo, y[n] = ye + yo and y[N-1-n] = SIGN (ye - yo), which halves the
+// multiply-adds (Kopriva's even-odd decomposition). Table row n (n <=
+// (N-1)/2): Ae[n][0..M/2-1], A[n][M/2] (odd M), Ao[n][0..M/2-1]; in the
+// middle row of an odd N only the part that does not vanish is used.
+template <int N, int M>
+struct EoShape {
+  static constexpr int hM = M / 2, oM = M & 1, LE = hM + oM, RL = LE + hM;
+  static constexpr int hN = N / 2, oN = N & 1, rows = hN + oN, size = rows * RL;
+};
+
+template <int N, int M, int SIGN, typename T, typename P>
+__device__ inline void eo_apply(const T* tab, const T (&x)[M], P&& put) {  // put(n, y[n])
+  using E = EoShape<N, M>;
+  T xe[E::hM > 0 ? E::hM : 1], xo[E::hM > 0 ? E::hM : 1];
+#pragma unroll
+  for (int m = 0; m < E::hM; ++m) {
+    xe[m] = x[m] + x[M - 1 - m];
+    xo[m] = x[m] - x[M - 1 - m];
+  }
+  uniform_rows<E::rows, E::RL>(tab, [&](int n, const T* r) {
+    const bool mid = E::oN && n == E::hN;
+    T ye = 0, yo = 0;
+    if (!mid || SIGN > 0) {
+#pragma unroll
+      for (int m = 0; m < E::hM; ++m) ye = cfma(r[m], xe[m], ye);
+      if constexpr (E::oM) ye = cfma(r[E::hM], x[E::hM], ye);
+    }
+    if (!mid || SIGN < 0) {
+#pragma unroll
+      for (int m = 0; m < E::hM; ++m) yo = cfma(r[E::LE + m], xo[m], yo);
+    }
+    if (mid) {
+      put(n, SIGN > 0 ? ye : yo);
+    } else {
+      put(n, ye + yo);
+      put(N - 1 - n, SIGN > 0 ? ye - yo : yo - ye);
+    }
+  });
+}
+
+// offsets of the four tables in the kernel's even-odd array: I (MQ x NQ),
+// I^T, D (MQ x MQ, the cubature derivative) and D^T
+template <int NQ>
+struct EoTables {
+  static constexpr int MQ = NQ + 1;
+  static constexpr int I = 0, IT = I + EoShape<MQ, NQ>::size, D = IT + EoShape<NQ, MQ>::size,
+                       DT = D + EoShape<MQ, MQ>::size, size = DT + EoShape<MQ, MQ>::size;
+};
+
 template <int MQ>
 constexpr int pencil_slots() {
   return MQ * MQ >= 256 ? 1 : 256 / (MQ * MQ);
@@ -806,11 +861,12 @@ constexpr int pencil_prefetch() {
   return sizeof(T) == 8 ? (MQ < 4 ? MQ : 4) : (MQ < 6 ? MQ : 6);
 }
 
-template <typename T, int NQ, int NF, bool STIFF>
+template <typename T, int NQ, int NF, bool STIFF, bool EO>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(pencil_waves<T, NQ + 1, STIFF>()))) void
 k_cub_pencil(int Nelements, const int* __restrict__ elementList, const int* __restrict__ G2L,
              const T* __restrict__ W, const T* __restrict__ G, const T* __restrict__ Dq,
-             const T* __restrict__ I, T lambda, const T* __restrict__ q, T* __restrict__ Aq) {
+             const T* __restrict__ I, const T* __restrict__ eo, T lambda, const T* __restrict__ q,
+             T* __restrict__ Aq) {
   constexpr int MQ = NQ + 1;
   constexpr int NQ2 = NQ * NQ;
   constexpr int NP = NQ2 * NQ;
@@ -821,7 +877,8 @@ k_cub_pencil(int Nelements, const int* __restrict__ elementList, const int* __re
   constexpr int SA = pencil_tile_a<MQ, STIFF>();
   constexpr int SB = STIFF ? pencil_tile_b<MQ, STIFF>() : SA;
   constexpr int S = pencil_slots<MQ>();
-  constexpr int PF = STIFF ? pencil_prefetch<T, MQ>() : 1;
+  // (the FP64 even-odd form holds one slab less: with four it spills)
+  constexpr int PF = STIFF ? (EO && sizeof(T) == 8 ? (MQ < 3 ? MQ : 3) : pencil_prefetch<T, MQ>()) : 1;
   constexpr bool SG = sizeof(T) == 4;  // FP32: geometry loads non-temporal
   static_assert(S * MQ2 <= 256, "one line per lane and pass");
   static_assert(NQ * NQ * PR <= SA && NQ * PS <= SB, "x and y fit their tiles");
@@ -840,18 +897,44 @@ k_cub_pencil(int Nelements, const int* __restrict__ elementList, const int* __re
 
   // the element and field of slot s (an idle slot reads element 0's data
   // and stores nothing)
+  // The group's slots cover at most NE consecutive entries of elementList
+  // from eo0; where NE is small they are read with scalar loads (uniform
+  // over the workgroup, through the scalar cache), which takes a vector
+  // memory round trip off the chain element -> G2L -> q.
+  constexpr int NE = (S + NF - 1) / NF + 1;
+  const int eo0 = (grp * S) / NF;
+  int E[NE <= 4 ? NE : 1];
+  if constexpr (NE <= 4) {
+    typedef const __attribute__((address_space(4))) int* ciptr;
+    const ciptr el = (ciptr)elementList;
+#pragma unroll
+    for (int j = 0; j < NE; ++j) E[j] = el[min(eo0 + j, Nelements - 1)];
+  }
   auto slot = [&](int s, int& e, int& f) -> bool {
     const int p = grp * S + s;
     const bool act = s < S && p < pairs;
-    const int eo = act ? p / NF : 0;
+    const int eo = act ? p / NF : eo0;
     f = act ? p - eo * NF : 0;
-    e = elementList[eo];
+    if constexpr (NE <= 4) {
+      const int rel = eo - eo0;  // in [0, NE)
+      e = E[0];
+#pragma unroll
+      for (int j = 1; j < NE; ++j) e = rel == j ? E[j] : e;
+    } else {
+      e = elementList[eo];
+    }
     return act;
   };
 
   // i-line lanes (passes 1 and 9): line (k, j) = li of slot si
   const int si = t / NQ2, li = t - si * NQ2;
   const bool oni = si < S;
+  // BP2 (three fields of a mass operator, the most gather-bound form)
+  // gathers q and stores Aq in node order through LDS; the others gather
+  // and store along the i-lines (faster for them: BP1 already streams at
+  // the device-copy rate, and the extra pass costs BP3/BP4 more than the
+  // gather gains)
+  constexpr bool NODE_ORDER = NF > 1 && !STIFF;
   int ei, fi;
   const bool acti = slot(si, ei, fi);
   const size_t lbi = ((size_t)ei * NF + fi) * NP + (size_t)li * NQ;
@@ -870,6 +953,34 @@ k_cub_pencil(int Nelements, const int* __restrict__ elementList, const int* __re
   T* const At = s_a + st * SA;
   T* const Bt = s_b + st * SB;
 
+  // the 1D contractions: row by row from I and D, or (EO) in even-odd form
+  // from the tables in eo
+  using ET = EoTables<NQ>;
+  auto interp = [&](const T(&x)[NQ], T(&y)[MQ]) {  // y = I x
+    if constexpr (EO) {
+      eo_apply<MQ, NQ, 1>(eo + ET::I, x, [&](int n, T v) { y[n] = v; });
+    } else {
+      uniform_rows<MQ, NQ>(I, [&](int n, const T* r) {
+        T acc = 0;
+#pragma unroll
+        for (int m = 0; m < NQ; ++m) acc = cfma(r[m], x[m], acc);
+        y[n] = acc;
+      });
+    }
+  };
+  auto project = [&](const T(&x)[MQ], T(&y)[NQ]) {  // y = I^T x
+    if constexpr (EO) {
+      eo_apply<NQ, MQ, 1>(eo + ET::IT, x, [&](int n, T v) { y[n] = v; });
+    } else {
+#pragma unroll
+      for (int n = 0; n < NQ; ++n) y[n] = T(0);
+      uniform_rows<MQ, NQ>(I, [&](int m, const T* r) {
+#pragma unroll
+        for (int n = 0; n < NQ; ++n) y[n] = cfma(r[n], x[m], y[n]);
+      });
+    }
+  };
+
   T wv[STIFF ? 1 : MQ];
   if constexpr (!STIFF) {
     // cubwJ of the t-line, requested first
@@ -879,22 +990,64 @@ k_cub_pencil(int Nelements, const int* __restrict__ elementList, const int* __re
     }
   }
 
-  // pass 1: i-lines, gather q, interpolate along i -> A
-  if (oni) {
+  // The slots' nodes in node order: node c of slot s is lane t of round r,
+  // c + s * NP = r * 256 + t, so consecutive lanes read consecutive G2L
+  // entries and write consecutive Aq entries, and a q gather touches a few
+  // lattice rows per instruction instead of one cache line per lane. Row
+  // (k, j) of a slot sits in LDS at (k * NQ + j) * PR (the i-lines' rows).
+  // Bounds-checked buffer accesses: an idle slot or lane gets an offset
+  // beyond the range and reads 0 or writes nothing.
+  constexpr int R = NODE_ORDER ? (S * NP + 255) / 256 : 1;
+  static_assert(STIFF ? pencil_tile_b<MQ, STIFF>() == SA : true, "the output rows use the same offsets");
+  // round r: the local node index (kBufMasked: none) and its LDS offset
+  // (-1: none); recomputed for the stores rather than held in registers
+  auto node = [&](int r, int lane, int& nlds) -> int {
+    const int n = r * 256 + lane;
+    const int s = n / NP, c = n - s * NP;
+    int e, f;
+    const bool act = slot(s, e, f);
+    nlds = s < S ? s * SA + (c / NQ) * PR + c % NQ : -1;
+    return act ? (int)(((size_t)e * NF + f) * NP + c) : kBufMasked;
+  };
+  if constexpr (NODE_ORDER) {
+    int noff[R], nlds[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) noff[r] = node(r, t, nlds[r]);
+    const __amdgpu_buffer_rsrc_t rg = buffer_rsrc(G2L);
     const __amdgpu_buffer_rsrc_t rq = buffer_rsrc(q);
-    T v[NQ];
+    int id[R];
 #pragma unroll
-    for (int i = 0; i < NQ; ++i) {
-      const int id = G2L[lbi + i];
-      v[i] = buffer_load<T>(rq, (acti && id >= 0) ? id * (int)sizeof(T) : kBufMasked);
-    }
+    for (int r = 0; r < R; ++r) id[r] = buffer_load<int>(rg, noff[r] == kBufMasked ? kBufMasked : noff[r] * 4);
+    T qv[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      qv[r] = buffer_load<T>(rq, (noff[r] != kBufMasked && id[r] >= 0) ? id[r] * (int)sizeof(T) : kBufMasked);
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      if (nlds[r] >= 0) s_a[nlds[r]] = qv[r];
+    __syncthreads();
+  }
+
+  // pass 1: i-lines, interpolate q along i -> A (node order: in place,
+  // each lane reads its row before rewriting it; else gathered here)
+  if (oni) {
     T* const x = s_a + si * SA + li * PR;
-    uniform_rows<MQ, NQ>(I, [&](int a, const T* r) {
-      T acc = 0;
+    T v[NQ];
+    if constexpr (NODE_ORDER) {
 #pragma unroll
-      for (int i = 0; i < NQ; ++i) acc = cfma(r[i], v[i], acc);
-      x[a] = acc;
-    });
+      for (int i = 0; i < NQ; ++i) v[i] = x[i];
+    } else {
+      const __amdgpu_buffer_rsrc_t rq = buffer_rsrc(q);
+#pragma unroll
+      for (int i = 0; i < NQ; ++i) {
+        const int id = G2L[lbi + i];
+        v[i] = buffer_load<T>(rq, (acti && id >= 0) ? id * (int)sizeof(T) : kBufMasked);
+      }
+    }
+    T xv[MQ];
+    interp(v, xv);
+#pragma unroll
+    for (int a = 0; a < MQ; ++a) x[a] = xv[a];
   }
   __syncthreads();
   // pass 2: j-lines, along j -> B (y[k][b][a] at k*PS + b*PR + a)
@@ -904,12 +1057,7 @@ k_cub_pencil(int Nelements, const int* __restrict__ elementList, const int* __re
       const T* x = s_a + sj * SA + jk * NQ * PR + ja;
 #pragma unroll
       for (int j = 0; j < NQ; ++j) v[j] = x[j * PR];
-      uniform_rows<MQ, NQ>(I, [&](int b, const T* r) {
-        T acc = 0;
-#pragma unroll
-        for (int j = 0; j < NQ; ++j) acc = cfma(r[j], v[j], acc);
-        yv[b] = acc;
-      });
+      interp(v, yv);
     }
     if constexpr (!STIFF) __syncthreads();  // y overwrites x in the one tile
     if (onj) {
@@ -936,22 +1084,13 @@ k_cub_pencil(int Nelements, const int* __restrict__ elementList, const int* __re
         gpre[g][6] = geo_load<SG>(W + cb);
       }
     }
-    uniform_rows<MQ, NQ>(I, [&](int g, const T* r) {
-      T acc = 0;
-#pragma unroll
-      for (int k = 0; k < NQ; ++k) acc = cfma(r[k], v[k], acc);
-      u[g] = acc;
-    });
+    interp(v, u);
     if constexpr (!STIFF) {
       // times cubwJ, back along k, in place
-      T z[NQ];
+      T zg[MQ], z[NQ];
 #pragma unroll
-      for (int k = 0; k < NQ; ++k) z[k] = T(0);
-      uniform_rows<MQ, NQ>(I, [&](int g, const T* r) {
-        const T zg = u[g] * wv[g];
-#pragma unroll
-        for (int k = 0; k < NQ; ++k) z[k] = cfma(r[k], zg, z[k]);
-      });
+      for (int g = 0; g < MQ; ++g) zg[g] = u[g] * wv[g];
+      project(zg, z);
 #pragma unroll
       for (int k = 0; k < NQ; ++k) Bt[tp + k * PS] = z[k];
     } else {
@@ -968,24 +1107,31 @@ k_cub_pencil(int Nelements, const int* __restrict__ elementList, const int* __re
       T v[MQ];
 #pragma unroll
       for (int m = 0; m < MQ; ++m) v[m] = src[m * ls];
-      uniform_rows<MQ, MQ>(Dq, [&](int n, const T* r) {
-        T acc = 0;
+      if constexpr (EO) {
+        eo_apply<MQ, MQ, -1>(eo + ET::D, v, [&](int n, T y) { dst[n * ls] = y; });
+      } else {
+        uniform_rows<MQ, MQ>(Dq, [&](int n, const T* r) {
+          T acc = 0;
 #pragma unroll
-        for (int m = 0; m < MQ; ++m) acc = cfma(r[m], v[m], acc);
-        dst[n * ls] = acc;
-      });
+          for (int m = 0; m < MQ; ++m) acc = cfma(r[m], v[m], acc);
+          dst[n * ls] = acc;
+        });
+      }
     };
     auto line_dt = [&](T* p, int ls) {
       T v[MQ], o[MQ];
 #pragma unroll
-      for (int m = 0; m < MQ; ++m) {
-        v[m] = p[m * ls];
-        o[m] = T(0);
-      }
-      uniform_rows<MQ, MQ>(Dq, [&](int m, const T* r) {
+      for (int m = 0; m < MQ; ++m) v[m] = p[m * ls];
+      if constexpr (EO) {
+        eo_apply<MQ, MQ, -1>(eo + ET::DT, v, [&](int n, T y) { o[n] = y; });
+      } else {
 #pragma unroll
-        for (int n = 0; n < MQ; ++n) o[n] = cfma(r[n], v[m], o[n]);
-      });
+        for (int n = 0; n < MQ; ++n) o[n] = T(0);
+        uniform_rows<MQ, MQ>(Dq, [&](int m, const T* r) {
+#pragma unroll
+          for (int n = 0; n < MQ; ++n) o[n] = cfma(r[n], v[m], o[n]);
+        });
+      }
 #pragma unroll
       for (int n = 0; n < MQ; ++n) p[n * ls] = o[n];
     };
@@ -1047,12 +1193,7 @@ k_cub_pencil(int Nelements, const int* __restrict__ elementList, const int* __re
 #pragma unroll
       for (int g = 0; g < MQ; ++g) res[g] += At[tp + g * PS] + Bt[tp + g * PS];
       T z[NQ];
-#pragma unroll
-      for (int k = 0; k < NQ; ++k) z[k] = T(0);
-      uniform_rows<MQ, NQ>(I, [&](int g, const T* r) {
-#pragma unroll
-        for (int k = 0; k < NQ; ++k) z[k] = cfma(r[k], res[g], z[k]);
-      });
+      project(res, z);
 #pragma unroll
       for (int k = 0; k < NQ; ++k) At[tp + k * PS] = z[k];
     }
@@ -1068,12 +1209,7 @@ k_cub_pencil(int Nelements, const int* __restrict__ elementList, const int* __re
       const T* z = Z + jk * PS + ja;
 #pragma unroll
       for (int b = 0; b < MQ; ++b) v[b] = z[b * PR];
-#pragma unroll
-      for (int j = 0; j < NQ; ++j) o[j] = T(0);
-      uniform_rows<MQ, NQ>(I, [&](int b, const T* r) {
-#pragma unroll
-        for (int j = 0; j < NQ; ++j) o[j] = cfma(r[j], v[b], o[j]);
-      });
+      project(v, o);
     }
     if constexpr (!STIFF) __syncthreads();  // x overwrites z in the one tile
     if (onj) {
@@ -1083,20 +1219,36 @@ k_cub_pencil(int Nelements, const int* __restrict__ elementList, const int* __re
     }
   }
   __syncthreads();
-  // i-lines, back along i -> Aq
-  if (acti) {
-    const T* x = (STIFF ? s_b + si * SB : s_a + si * SA) + li * PR;
+  // i-lines, back along i: to Aq, or (node order) in place
+  T* const Xo = STIFF ? s_b : s_a;
+  if (NODE_ORDER ? oni : acti) {
+    T* const x = Xo + si * (STIFF ? SB : SA) + li * PR;
     T v[MQ], o[NQ];
 #pragma unroll
     for (int a = 0; a < MQ; ++a) v[a] = x[a];
+    project(v, o);
 #pragma unroll
-    for (int i = 0; i < NQ; ++i) o[i] = T(0);
-    uniform_rows<MQ, NQ>(I, [&](int a, const T* r) {
+    for (int i = 0; i < NQ; ++i) {
+      if constexpr (NODE_ORDER)
+        x[i] = o[i];
+      else
+        Aq[lbi + i] = o[i];
+    }
+  }
+  // Aq in node order (the lane index made opaque, so the compiler
+  // recomputes the offsets instead of keeping the gather's live)
+  if constexpr (NODE_ORDER) {
+    __syncthreads();
+    const __amdgpu_buffer_rsrc_t ra = buffer_rsrc(Aq);
+    int lane = t;
+    asm volatile("" : "+v"(lane));
 #pragma unroll
-      for (int i = 0; i < NQ; ++i) o[i] = cfma(r[i], v[a], o[i]);
-    });
-#pragma unroll
-    for (int i = 0; i < NQ; ++i) Aq[lbi + i] = o[i];
+    for (int r = 0; r < R; ++r) {
+      int nl;
+      const int no = node(r, lane, nl);
+      const T v = nl >= 0 ? Xo[nl] : T(0);
+      buffer_store<T>(v, ra, no == kBufMasked ? kBufMasked : no * (int)sizeof(T));
+    }
   }
 }
 
@@ -1124,13 +1276,20 @@ bool pencil_in_range(const AxArgs<T>& a) {
   return (double)a.Nelements * NF * NQ * NQ * NQ * sizeof(T) < (double)kBufRange;
 }
 
+// even-odd tables (a.eo, formed by the host from I and D) where given: the
+// problem objects' operators; the kernel-level entry points take I and D
+// only and run the row-by-row form
 template <typename T, int NQ, int NF, bool STIFF>
 int launch_cub_pencil(hipStream_t s, const AxArgs<T>& a, const T* G, const T* Dq) {
   constexpr int S = pencil_slots<NQ + 1>();
   const long long groups = ((long long)a.Nelements * NF + S - 1) / S;
   const unsigned blocks = 8u * (unsigned)((groups + 7) / 8);
-  launch_stamped(k_cub_pencil<T, NQ, NF, STIFF>, dim3(blocks), dim3(256), s, a.evStart, a.evStop, a.Nelements,
-                 a.elementList, a.GlobalToLocal, a.wJ, G, Dq, a.I, a.lambda, a.q, a.Aq);
+  if (a.eo && a.cubForm != CUB_FORM_PENCIL_ROWS)
+    launch_stamped(k_cub_pencil<T, NQ, NF, STIFF, true>, dim3(blocks), dim3(256), s, a.evStart, a.evStop,
+                   a.Nelements, a.elementList, a.GlobalToLocal, a.wJ, G, Dq, a.I, a.eo, a.lambda, a.q, a.Aq);
+  else
+    launch_stamped(k_cub_pencil<T, NQ, NF, STIFF, false>, dim3(blocks), dim3(256), s, a.evStart, a.evStop,
+                   a.Nelements, a.elementList, a.GlobalToLocal, a.wJ, G, Dq, a.I, a.eo, a.lambda, a.q, a.Aq);
   BPX_HIP(hipGetLastError());
   return BPX_OK;
 }

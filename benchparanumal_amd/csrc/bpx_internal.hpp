@@ -91,12 +91,15 @@ struct AxArgs {
   // CUB_FORM_PENCIL) or the wavefront-per-element kernels (CUB_FORM_WAVE);
   // the two agree to rounding, not bit for bit
   int cubForm = 0;
+  // BP1-BP4 non-affine pencil kernel: the even-odd tables of I, I^T, D and
+  // D^T (cub_even_odd_tables), or null for the row-by-row form
+  const T* eo = nullptr;
   // when set (BP1-BP6 Ax), the kernel's dispatch stamps its start and end
   // into these events (hipExtLaunchKernelGGL: no barrier packets around it)
   hipEvent_t evStart = nullptr, evStop = nullptr;
 };
 enum { AX_ENGINE_AUTO = 0, AX_ENGINE_VECTOR = 1, AX_ENGINE_MATRIX = 2 };
-enum { CUB_FORM_AUTO = 0, CUB_FORM_PENCIL = 1, CUB_FORM_WAVE = 2 };
+enum { CUB_FORM_AUTO = 0, CUB_FORM_PENCIL = 1, CUB_FORM_WAVE = 2, CUB_FORM_PENCIL_ROWS = 3 };
 #if defined(__HIPCC__)
 }  // namespace bpx
 #include <hip/hip_ext.h>

@@ -859,8 +859,10 @@ int bpx_bp_set_ax_engine(bpx_bp b, int engine) {
 int bpx_bp_set_cub_form(bpx_bp b, int form) {
   BPX_GUARD_BEGIN
   if (int st = need(b, "bp")) return st;
-  if (form != CUB_FORM_AUTO && form != CUB_FORM_PENCIL && form != CUB_FORM_WAVE)
-    return set_error(BPX_ERR_INVALID_ARGUMENT, "bp: cubature kernel form must be 0 (auto), 1 (pencil) or 2 (wavefront)");
+  if (form < CUB_FORM_AUTO || form > CUB_FORM_PENCIL_ROWS)
+    return set_error(BPX_ERR_INVALID_ARGUMENT,
+                     "bp: cubature kernel form must be 0 (auto), 1 (pencil), 2 (wavefront) or 3 (pencil, row-by-row "
+                     "contractions)");
   b->bp.cubForm = form;
   return BPX_OK;
   BPX_GUARD_END

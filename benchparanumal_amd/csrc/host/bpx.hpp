@@ -496,6 +496,7 @@ class bp_t : public solver_t {
   std::vector<dlong> GlobalToLocal;
   deviceBuffer<dlong> o_GlobalToLocal, o_elementList;
   deviceBuffer<double> o_wJ, o_ggeo, o_gllw, o_D, o_I, o_MM, o_invV;
+  deviceBuffer<double> o_eo;  // even-odd tables of the BP1-BP4 pencil kernel (N <= 7, non-affine)
   deviceBuffer<double> o_S;  // triangles and tetrahedra: the packed stiffness matrices
   deviceBuffer<double> o_AqL;
   deviceBuffer<double> o_AqLk;  // K local vectors for OperatorKrhs, grown on demand
@@ -526,6 +527,7 @@ class bp_t : public solver_t {
   int fp32Geometry = 0;
   int SetFp32Geometry(int mode);
   deviceBuffer<float> o_wJ32, o_ggeo32, o_gllw32, o_D32, o_I32, o_MM32, o_invV32, o_AqL32;
+  deviceBuffer<float> o_eo32;
   // nodal coordinates for the forcing (created on first use)
   deviceBuffer<double> o_x, o_y, o_z, o_gllwRhs;
 

@@ -11,6 +11,45 @@
 
 namespace bpx {
 
+// Even-odd tables of a 1D operator A (N x M, row-major) with
+// A[N-1-n][M-1-m] = sign * A[n][m]: row n (n <= (N-1)/2) holds
+// (A[n][m] + A[n][M-1-m]) / 2 for m < M/2, A[n][M/2] for odd M, then
+// (A[n][m] - A[n][M-1-m]) / 2 for m < M/2 (the layout of eo_apply in
+// ax_cub.hip). False if A is not (anti)symmetric to rounding.
+static bool even_odd_rows(const double* A, int N, int M, int sign, std::vector<double>& out) {
+  double amax = 0;
+  for (int i = 0; i < N * M; ++i) amax = std::max(amax, std::fabs(A[i]));
+  for (int n = 0; n < N; ++n)
+    for (int m = 0; m < M; ++m)
+      if (std::fabs(A[(N - 1 - n) * M + (M - 1 - m)] - sign * A[n * M + m]) > 1e-12 * amax) return false;
+  const int hM = M / 2, hN = N / 2;
+  for (int n = 0; n < hN + (N & 1); ++n) {
+    for (int m = 0; m < hM; ++m) out.push_back(0.5 * (A[n * M + m] + A[n * M + M - 1 - m]));
+    if (M & 1) out.push_back(A[n * M + hM]);
+    for (int m = 0; m < hM; ++m) out.push_back(0.5 * (A[n * M + m] - A[n * M + M - 1 - m]));
+  }
+  return true;
+}
+
+// the pencil kernel's tables (BP1-BP4, non-affine hex, N <= 7): I (cubNq x
+// Nq), I^T, D and D^T of the cubature grid, one after another; empty where
+// they do not apply
+static std::vector<double> cub_even_odd_tables(const mesh_t& m) {
+  std::vector<double> t;
+  const int nq = m.Nq, mq = m.cubNq;
+  if (m.elementType != ELEM_HEX || m.opts.affine || nq > 8 || mq != nq + 1) return t;
+  std::vector<double> IT((size_t)nq * mq), DT((size_t)mq * mq);
+  for (int a = 0; a < mq; ++a)
+    for (int i = 0; i < nq; ++i) IT[(size_t)i * mq + a] = m.cubInterp[(size_t)a * nq + i];
+  for (int a = 0; a < mq; ++a)
+    for (int b = 0; b < mq; ++b) DT[(size_t)b * mq + a] = m.cubD[(size_t)a * mq + b];
+  if (!even_odd_rows(m.cubInterp.data(), mq, nq, 1, t) || !even_odd_rows(IT.data(), nq, mq, 1, t) ||
+      !even_odd_rows(m.cubD.data(), mq, mq, -1, t) || !even_odd_rows(DT.data(), mq, mq, -1, t))
+    t.clear();
+  return t;
+}
+
+
 // ------------------------------------------------------------- errors ---
 static thread_local std::string g_last_error;
 
@@ -342,6 +381,9 @@ int bp_t::Setup(platform_t& p, mesh_t& m, int bp_, dfloat lambda_) {
     if (int st = affine ? factors(o_wJ, m.wJ, m.o_wJ) : factors(o_wJ, m.cubwJ, m.o_cubwJ)) return st;
     if (int st = o_I.upload(m.cubInterp, s)) return st;
     if (int st = o_MM.upload(m.MM, s)) return st;
+    const std::vector<double> eo = cub_even_odd_tables(m);
+    if (!eo.empty())
+      if (int st = o_eo.upload(eo, s)) return st;
   } else {
     if (int st = affine ? factors(o_wJ, m.wJ, m.o_wJ) : factors(o_wJ, m.cubwJ, m.o_cubwJ)) return st;
     if (int st = affine ? factors(o_ggeo, m.ggeo, m.o_ggeo) : factors(o_ggeo, m.cubggeo, m.o_cubggeo)) return st;
@@ -349,6 +391,9 @@ int bp_t::Setup(platform_t& p, mesh_t& m, int bp_, dfloat lambda_) {
     if (int st = o_I.upload(m.cubInterp, s)) return st;
     if (affine)
       if (int st = o_invV.upload(m.invV, s)) return st;
+    const std::vector<double> eo = cub_even_odd_tables(m);
+    if (!eo.empty())
+      if (int st = o_eo.upload(eo, s)) return st;
   }
   if (int st = m.onDevice ? PrepareFusedDevice() : PrepareFused()) return st;
   if (int st = Tune()) return st;
@@ -377,6 +422,12 @@ int bp_t::ax(const T* q, T* AqL, const T* wJ, const T* ggeo, const T* gllw, cons
   a.Aq = AqL;
   if constexpr (std::is_same<T, double>::value) a.engine = axEngine;
   a.cubForm = cubForm;
+  if (bp <= 4 && !haveKappa) {
+    if constexpr (std::is_same<T, double>::value)
+      a.eo = o_eo.length() ? o_eo.ptr() : nullptr;
+    else
+      a.eo = o_eo32.length() ? o_eo32.ptr() : nullptr;
+  }
   if constexpr (std::is_same<T, float>::value) a.tri = tri32;
   a.evStart = axEvStart;
   a.evStop = axEvStop;
@@ -1016,6 +1067,10 @@ int bp_t::Prepare32() {
         (st = cvt(o_D32, m.cubD)) || (st = cvt(o_I32, m.cubInterp)))
       return st;
     if (affine && (st = cvt(o_invV32, m.invV))) return st;
+  }
+  if (bp <= 4) {
+    const std::vector<double> eo = cub_even_odd_tables(m);
+    if (!eo.empty() && (st = cvt(o_eo32, eo))) return st;
   }
   if ((st = o_AqL32.alloc(NLocal))) return st;
   if (fused && (st = o_slots32.alloc(o_slots.length()))) return st;
