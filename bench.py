@@ -551,6 +551,9 @@ def main():
                     help="only 3 single-vector and 3 k-RHS (K = 32, --krhs-chunk) FP64 BP5 Ax launches "
                          "on the 20^3 p=7 perturbed box, random right-hand sides (for counter runs)")
     ap.add_argument("--krhs-chunk", type=int, default=0)
+    ap.add_argument("--cub-only", action="store_true",
+                    help="only 10 launches each of the FP64 BP1-BP4 Ax (default kernel form) on the 24^3 p=7 "
+                         "perturbed box and one 512 MiB device copy (for counter runs)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -580,6 +583,23 @@ def main():
         torch.cuda.synchronize()
         log(f"krhs-only: 3 single-vector Ax and 3 k-RHS operator launches (K = {K}, chunk "
             f"{args.krhs_chunk}), 20^3 p=7, Ndofs {bk.Ngather}")
+        return
+    if args.cub_only:
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        del src, dst
+        for bpn in (1, 2, 3, 4):
+            mk = capi.Mesh(N=7, nx=args.nx, cubature=True, perturb=0.1, seed=1)
+            bk = capi.BP(plat, mk, bpn, lam=1.0)
+            qk = torch.from_numpy(capi.random_normal(2, bk.Ngather)).cuda()
+            AqLk = torch.empty(bk.NLocal, dtype=torch.float64, device="cuda")
+            for _ in range(10):
+                bk.ax_local(qk, AqLk)
+            torch.cuda.synchronize()
+            log(f"cub-only: BP{bpn} p=7 {args.nx}^3, 10 Ax launches")
+            del bk, mk, qk, AqLk
+            torch.cuda.empty_cache()
         return
     if args.counter_kernels:
         for bp, N, cub, engines in ((5, 15, False, (1, 2)), (3, 7, True, (0,)), (5, 7, False, (0,))):

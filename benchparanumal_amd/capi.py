@@ -367,7 +367,8 @@ class BP:
 
     def set_cub_form(self, form):
         """BP1-BP4 non-affine at N <= 7: 0 = auto (pencil kernel), 1 =
-        pencil, 2 = one element per wavefront (the two agree to rounding)."""
+        pencil (even-odd contractions), 2 = one element per wavefront, 3 =
+        pencil with row-by-row contractions (the forms agree to rounding)."""
         check(lib().bpx_bp_set_cub_form(self.h, int(form)))
 
     def time_krhs(self, K, chunk=0, launches=10, q=None):

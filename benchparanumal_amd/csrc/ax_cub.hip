@@ -1001,13 +1001,40 @@ k_cub_pencil(int Nelements, const int* __restrict__ elementList, const int* __re
   static_assert(STIFF ? pencil_tile_b<MQ, STIFF>() == SA : true, "the output rows use the same offsets");
   // round r: the local node index (kBufMasked: none) and its LDS offset
   // (-1: none); recomputed for the stores rather than held in registers
+  // (offsets fit in 32 bits: pencil_in_range bounds the local vector).
+  // Up to 4 slots, each slot's local base is uniform, formed once and
+  // selected per round; beyond that each round looks its slot up.
+  constexpr bool FEW = S <= 4;
+  int sbase[FEW ? S : 1];
+  if constexpr (NODE_ORDER && FEW) {
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      int e, f;
+      sbase[s] = slot(s, e, f) ? (e * NF + f) * NP : -1;
+    }
+  }
   auto node = [&](int r, int lane, int& nlds) -> int {
+    if constexpr (FEW && NP % 256 == 0) {
+      // a round lies inside one slot (Nq = 8): the slot and its base are
+      // fixed per round
+      const int s = (r * 256) / NP, c = (r * 256) % NP + lane;
+      nlds = s < S ? s * SA + (c / NQ) * PR + c % NQ : -1;
+      const int b = s < S ? sbase[s < S ? s : 0] : -1;
+      return b >= 0 ? b + c : kBufMasked;
+    }
     const int n = r * 256 + lane;
     const int s = n / NP, c = n - s * NP;
-    int e, f;
-    const bool act = slot(s, e, f);
     nlds = s < S ? s * SA + (c / NQ) * PR + c % NQ : -1;
-    return act ? (int)(((size_t)e * NF + f) * NP + c) : kBufMasked;
+    if constexpr (FEW) {
+      int b = -1;
+#pragma unroll
+      for (int j = 0; j < S; ++j) b = s == j ? sbase[j] : b;
+      return b >= 0 ? b + c : kBufMasked;
+    } else {
+      int e, f;
+      const bool act = slot(s, e, f);
+      return act ? (e * NF + f) * NP + c : kBufMasked;
+    }
   };
   if constexpr (NODE_ORDER) {
     int noff[R], nlds[R];
@@ -1108,7 +1135,12 @@ k_cub_pencil(int Nelements, const int* __restrict__ elementList, const int* __re
 #pragma unroll
       for (int m = 0; m < MQ; ++m) v[m] = src[m * ls];
       if constexpr (EO) {
-        eo_apply<MQ, MQ, -1>(eo + ET::D, v, [&](int n, T y) { dst[n * ls] = y; });
+        // outputs held until the rows are done: an LDS store inside the row
+        // loop would join each row's wait for its scalar load
+        T o[MQ];
+        eo_apply<MQ, MQ, -1>(eo + ET::D, v, [&](int n, T y) { o[n] = y; });
+#pragma unroll
+        for (int n = 0; n < MQ; ++n) dst[n * ls] = o[n];
       } else {
         uniform_rows<MQ, MQ>(Dq, [&](int n, const T* r) {
           T acc = 0;

@@ -392,10 +392,12 @@ int bpx_bp_set_fp32_geometry(bpx_bp b, int mode);
  * rounding, not bit for bit. Other problems run on the vector ALU. */
 int bpx_bp_set_ax_engine(bpx_bp b, int engine);
 /* No reference counterpart (launch choice): which kernel runs the BP1-BP4
- * Ax on non-affine meshes at N <= 7: 0 = auto (the pencil kernel; default),
- * 1 = pencil (several fields or elements per workgroup, every contraction
- * along a line one lane holds), 2 = one element per wavefront. The two agree
- * to rounding, not bit for bit. Other problems have one kernel. */
+ * Ax of a problem object on non-affine meshes at N <= 7: 0 = auto (the
+ * pencil kernel; default), 1 = pencil (several fields or elements per
+ * workgroup, every contraction along a line one lane holds, in even-odd
+ * form), 2 = one element per wavefront, 3 = pencil with row-by-row
+ * contractions (the form the kernel-level entry points run). The forms
+ * agree to rounding, not bit for bit. Other problems have one kernel. */
 int bpx_bp_set_cub_form(bpx_bp b, int form);
 /* No reference counterpart (launch choice, same results): mask = 1 (FP64
  * solves), 2 (the FP32 iterations of the mixed solve) or 3 (both): those CG

@@ -60,6 +60,43 @@ def test_bp_operator_matches_oracle(plat, bp, N, affine):
     assert rel_err(got, ref) <= AX_RTOL
 
 
+@pytest.mark.parametrize("form", [1, 2, 3])
+@pytest.mark.parametrize("bp,N", [(bp, N) for bp in (1, 2, 3, 4) for N in range(1, 8)])
+def test_cub_kernel_forms_match_oracle(plat, bp, N, form):
+    """Every form of the BP1-BP4 non-affine Ax at N <= 7 (1: pencil kernel
+    with even-odd contractions, the default; 3: pencil kernel, row-by-row
+    contractions, as the kernel-level entry points run it; 2: one element
+    per wavefront) against the oracle, on meshes whose element count leaves
+    the last workgroup's slots partly empty."""
+    m, P = make_pair(bp, N, 3, 2, 2) if N == 1 else make_pair(bp, N, 5, 1, 1)
+    b = capi.BP(plat, m, bp, lam=1.3)
+    b.set_cub_form(form)
+    q = np.random.default_rng(bp * 100 + N).standard_normal(P.Ngather)
+    Aq = torch.zeros(P.Ngather, dtype=torch.float64, device="cuda")
+    b.operator(dev(q), Aq)
+    assert rel_err(host(Aq), P.operator(q)) <= AX_RTOL
+
+
+@pytest.mark.parametrize("form", [1, 2, 3])
+@pytest.mark.parametrize("bp,N", [(1, 4), (1, 7), (3, 2), (3, 5), (3, 7)])
+def test_cub_kernel_forms_fp32_close_to_oracle(plat, bp, N, form):
+    """The FP32 BP1/BP3 operator in each kernel form against the FP64 oracle."""
+    m, P = make_pair(bp, N, 3, 2, 2, lam=1.0)
+    b = capi.BP(plat, m, bp, lam=1.0)
+    b.set_cub_form(form)
+    q = np.random.default_rng(7 + N).standard_normal(b.Ngather)
+    A32 = torch.zeros(b.Ngather, dtype=torch.float32, device="cuda")
+    b.operator_f32(dev(q, torch.float32), A32)
+    assert rel_err(host(A32).astype(np.float64), P.operator(q)) <= 2e-5
+
+
+def test_cub_form_rejects_unknown(plat):
+    m = capi.Mesh(N=3, nx=2, cubature=True)
+    b = capi.BP(plat, m, 3, lam=1.0)
+    with pytest.raises(capi.BpxError):
+        b.set_cub_form(4)
+
+
 def test_bp5_kernel_entry_full_size_24cube_p7(plat):
     """BK5 at the headline size (24^3, p=7, perturbed): the kernel-level entry
     bpx_bp5_ax_hex on the oracle's own arrays, against the oracle's Ax."""
