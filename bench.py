@@ -799,9 +799,12 @@ def main():
 
     ext = None
     if not args.no_extensions:
-        ext = {"krhs_ax": bench_krhs(capi, plat), "ragged_batch_cg": bench_batch(capi, plat),
-               "adjoint": bench_adjoint(capi, plat), "bp3": bench_bp3(capi, plat, args),
-               "bk5_p15": bench_p15(capi, plat), "other_bps": bench_other_bps(capi, plat)}
+        # the BP Ax rows first: run after the others (the p = 15 problem takes
+        # several GB), the same kernels read up to 18% slower (BP2 112.8 against
+        # 95.3 us at 24^3, p = 7)
+        ext = {"other_bps": bench_other_bps(capi, plat), "bp3": bench_bp3(capi, plat, args),
+               "krhs_ax": bench_krhs(capi, plat), "ragged_batch_cg": bench_batch(capi, plat),
+               "adjoint": bench_adjoint(capi, plat), "bk5_p15": bench_p15(capi, plat)}
         log("extensions: " + json.dumps({k: {kk: vv for kk, vv in v.items() if kk in
                                                  ("ms_per_solve", "speedup_vs_sequential",
                                                   "ms_forward_adjoint_gradient", "single_us",
