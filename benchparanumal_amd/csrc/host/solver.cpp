@@ -37,7 +37,9 @@ static bool even_odd_rows(const double* A, int N, int M, int sign, std::vector<d
 static std::vector<double> cub_even_odd_tables(const mesh_t& m) {
   std::vector<double> t;
   const int nq = m.Nq, mq = m.cubNq;
-  if (m.elementType != ELEM_HEX || m.opts.affine || nq > 8 || mq != nq + 1) return t;
+  if (m.elementType != ELEM_HEX || m.opts.affine || nq > 8 || mq != nq + 1 ||
+      m.cubInterp.size() != (size_t)mq * nq || m.cubD.size() != (size_t)mq * mq)
+    return t;
   std::vector<double> IT((size_t)nq * mq), DT((size_t)mq * mq);
   for (int a = 0; a < mq; ++a)
     for (int i = 0; i < nq; ++i) IT[(size_t)i * mq + a] = m.cubInterp[(size_t)a * nq + i];
