@@ -169,7 +169,8 @@ __device__ inline T geo_load(const T* p) {
 // solve gives the bits it gives with the update in its own kernel. q is
 // then the direction p itself (fold.p, updated in place: no other lane
 // reads an interior row).
-template <typename T, int NQ, int NF, int EPB, bool AFFINE, int WAVES, int VAR, bool TRI = false, bool FOLD = false>
+template <typename T, int NQ, int NF, int EPB, bool AFFINE, int WAVES, int VAR, bool TRI = false, bool FOLD = false,
+          int FS = 1>
 __global__ __launch_bounds__(NQ* NQ* EPB) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_bp5_ax(
     int Nelements, int K, const int* __restrict__ elementList, const int* __restrict__ G2L,
     const T* __restrict__ wJ, const T* __restrict__ ggeo, const T* __restrict__ gllw,
@@ -183,6 +184,8 @@ __global__ __launch_bounds__(NQ* NQ* EPB) __attribute__((amdgpu_waves_per_eu(WAV
   static_assert(!FOLD || FUSED, "the folded direction update is a form of the fused operator");
   static_assert(!TRI || (sizeof(T) == 4 && NF == 1 && !AFFINE && (VAR == VAR_PLAIN || FUSED)),
                 "TRI: FP32, one field, non-affine");
+  static_assert(FS == 1 || (NF == 1 && EPB == 1 && VAR == VAR_PLAIN && !TRI),
+                "field split: the plain one-field sweep, one element per workgroup");
   constexpr int NQ2 = NQ * NQ;
   constexpr int NP = NQ2 * NQ;
   constexpr int LD = NQ + 1;  // padded LDS row
@@ -212,7 +215,18 @@ __global__ __launch_bounds__(NQ* NQ* EPB) __attribute__((amdgpu_waves_per_eu(WAV
   const int ij = t - es * NQ2;
   const int i = ij % NQ;
   const int j = ij / NQ;
-  const int eo = blockIdx.x * EPB + es;
+  // FS > 1: each workgroup applies the operator to one field of an
+  // FS-field layout ([element][field][node]). Block b takes field
+  // (b >> 3) % FS of element ((b >> 3) / FS) * 8 + b % 8, so the FS
+  // workgroups of an element run one after another on one XCD (block mod
+  // 8), and the second and third read its geometric factors from that L2.
+  int fsel = 0;
+  int eo = blockIdx.x * EPB + es;
+  if constexpr (FS > 1) {
+    const int r = (int)(blockIdx.x >> 3);
+    fsel = r % FS;
+    eo = (r / FS) * 8 + (int)(blockIdx.x & 7);
+  }
   bool active = eo < Nelements;
   const int e = active ? elementList[eo] : 0;
   if (BATCH) {
@@ -261,7 +275,7 @@ __global__ __launch_bounds__(NQ* NQ* EPB) __attribute__((amdgpu_waves_per_eu(WAV
       }
     }
   } else {
-    const size_t lbase = (size_t)e * NP * NF + ij;  // e = 0 on an idle sheet: in bounds
+    const size_t lbase = ((size_t)e * FS + fsel) * NP * NF + ij;  // e = 0 on an idle sheet: in bounds
     int id[NF][NQ];
 #pragma unroll
     for (int f = 0; f < NF; ++f)
@@ -546,7 +560,7 @@ __global__ __launch_bounds__(NQ* NQ* EPB) __attribute__((amdgpu_waves_per_eu(WAV
         for (int f = 0; f < NF; ++f) out[(size_t)f * ldA] = rA[f][k];
       }
     } else {
-      T* out = Aq + (size_t)e * NP * NF + ij;
+      T* out = Aq + ((size_t)e * FS + fsel) * NP * NF + ij;
 #pragma unroll
       for (int f = 0; f < NF; ++f)
 #pragma unroll
@@ -1079,6 +1093,20 @@ int launch_bp5_v(hipStream_t s, const AxArgs<T>& a, T* Ap, double* part) {
     if (a.tri && !a.affine) {
       constexpr int WT = bp5_waves_tri<NQ, VAR>();
       launch_stamped(k_bp5_ax<T, NQ, 1, EPB, false, WT, VAR, true, FOLD>, dim3(blocks), dim3(NQ * NQ * EPB), s,
+                     a.evStart, a.evStop, a.Nelements, a.K, a.elementList, a.GlobalToLocal, a.wJ, a.ggeo, a.gllw,
+                     a.D, a.lambda, a.q, a.Aq, a.ldq, a.ldAq, noPid, noState, Ap, part, a.fold);
+      BPX_HIP(hipGetLastError());
+      return BPX_OK;
+    }
+  }
+  if constexpr (NF == 3 && VAR == VAR_PLAIN && EPB == 1) {
+    // three fields, non-affine: one workgroup per field running the
+    // one-field sweep (3 waves per SIMD in FP64 at Nq = 8, where the
+    // three-field sweep needs 256 VGPRs and spills at 2)
+    if (!a.affine) {
+      constexpr int W1 = bp5_waves_v<T, NQ, 1, VAR>();
+      const unsigned fblocks = 8u * 3u * (unsigned)((a.Nelements + 7) / 8);
+      launch_stamped(k_bp5_ax<T, NQ, 1, 1, false, W1, VAR, false, false, 3>, dim3(fblocks), dim3(NQ * NQ), s,
                      a.evStart, a.evStop, a.Nelements, a.K, a.elementList, a.GlobalToLocal, a.wJ, a.ggeo, a.gllw,
                      a.D, a.lambda, a.q, a.Aq, a.ldq, a.ldAq, noPid, noState, Ap, part, a.fold);
       BPX_HIP(hipGetLastError());
