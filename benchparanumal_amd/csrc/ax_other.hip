@@ -23,6 +23,9 @@
 // products from the node values in LDS, the reference matrices (shared by
 // every element) coming through the caches.
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+
+#include <algorithm>
 
 #include "bpx_internal.hpp"
 
@@ -76,6 +79,50 @@ __global__ __launch_bounds__(1024) void k_simplex_ax(int Np, int NF, const int* 
       out = J * aM;
     }
     Aq[((size_t)e * NF + f) * Np + n] = out;
+  }
+}
+
+// The simplex operator as one library GEMM: the element values gathered
+// into Q (Np x Nel*NF, column e*NF + f), Z = Sall Q with Sall the reference
+// matrices stacked by rows (the NG stiffness blocks A_c[n][k] = S_c[k][n],
+// then MM; MM alone for BP1/BP2), then each element's factors applied to
+// its column of Z. Each dense matrix entry is read once per GEMM tile of
+// elements instead of once per element.
+__global__ __launch_bounds__(256) void k_simplex_gather(long long n, int Np, int NF, const int* __restrict__ elementList,
+                                                        const int* __restrict__ G2L, const double* __restrict__ q,
+                                                        double* __restrict__ Q) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const long long col = i / Np;
+    const int node = (int)(i - col * Np);
+    const int eo = (int)(col / NF), f = (int)(col - (long long)eo * NF);
+    const int id = G2L[((size_t)elementList[eo] * NF + f) * Np + node];
+    Q[i] = id >= 0 ? q[id] : 0.0;
+  }
+}
+
+template <int NG, bool STIFF>
+__global__ __launch_bounds__(256) void k_simplex_combine(long long n, int Np, int NF, const int* __restrict__ elementList,
+                                                         const double* __restrict__ wJ,
+                                                         const double* __restrict__ ggeo, double lambda,
+                                                         const double* __restrict__ Z, double* __restrict__ Aq) {
+  const long long ldz = (long long)(STIFF ? NG + 1 : 1) * Np;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const long long col = i / Np;
+    const int node = (int)(i - col * Np);
+    const int eo = (int)(col / NF), f = (int)(col - (long long)eo * NF);
+    const int e = elementList[eo];
+    const double* z = Z + col * ldz + node;
+    const double J = wJ[e];
+    double out;
+    if (STIFF) {
+      out = 0;
+#pragma unroll
+      for (int c = 0; c < NG; ++c) out += ggeo[(size_t)e * NG + c] * z[(size_t)c * Np];
+      out += J * lambda * z[(size_t)NG * Np];
+    } else {
+      out = J * z[0];
+    }
+    Aq[((size_t)e * NF + f) * Np + node] = out;
   }
 }
 
@@ -258,6 +305,57 @@ __global__ __launch_bounds__(1024) void k_rhs_simplex(int Np, int dim, int NF, c
 inline int round64(int n) { return (n + 63) / 64 * 64; }
 
 }  // namespace
+
+int simplex_blas_create(hipStream_t s, void** handle) {
+  rocblas_handle h = nullptr;
+  if (rocblas_create_handle(&h) != rocblas_status_success)
+    return set_error(BPX_ERR_HIP, "simplex Ax: rocblas_create_handle failed");
+  if (rocblas_set_stream(h, s) != rocblas_status_success) {
+    rocblas_destroy_handle(h);
+    return set_error(BPX_ERR_HIP, "simplex Ax: rocblas_set_stream failed");
+  }
+  *handle = h;
+  return BPX_OK;
+}
+
+void simplex_blas_destroy(void* handle) {
+  if (handle) rocblas_destroy_handle((rocblas_handle)handle);
+}
+
+int launch_simplex_ax_gemm(hipStream_t s, void* blas, int bp, int dim, int Np, int Nfields, int Nelements,
+                           const int* elementList, const int* G2L, const double* wJ, const double* ggeo,
+                           const double* Sall, double* Q, double* Z, double lambda, const double* q, double* Aq) {
+  if (Nelements <= 0) return BPX_OK;
+  const bool stiff = bp >= 3;
+  const int NG = dim == 2 ? 3 : 6;
+  const int M = (stiff ? NG + 1 : 1) * Np;
+  const long long cols = (long long)Nelements * Nfields, n = cols * Np;
+  const unsigned grid = (unsigned)std::min<long long>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_simplex_gather, dim3(grid), dim3(256), 0, s, n, Np, Nfields, elementList, G2L, q, Q);
+  BPX_HIP(hipGetLastError());
+  const double one = 1.0, zero = 0.0;
+  if (rocblas_set_stream((rocblas_handle)blas, s) != rocblas_status_success ||
+      rocblas_dgemm((rocblas_handle)blas, rocblas_operation_none, rocblas_operation_none, M, (rocblas_int)cols, Np,
+                    &one, Sall, M, Q, Np, &zero, Z, M) != rocblas_status_success)
+    return set_error(BPX_ERR_HIP, "simplex Ax: rocblas_dgemm failed");
+  if (dim == 2) {
+    if (stiff)
+      hipLaunchKernelGGL((k_simplex_combine<3, true>), dim3(grid), dim3(256), 0, s, n, Np, Nfields, elementList, wJ,
+                         ggeo, lambda, Z, Aq);
+    else
+      hipLaunchKernelGGL((k_simplex_combine<3, false>), dim3(grid), dim3(256), 0, s, n, Np, Nfields, elementList, wJ,
+                         ggeo, lambda, Z, Aq);
+  } else {
+    if (stiff)
+      hipLaunchKernelGGL((k_simplex_combine<6, true>), dim3(grid), dim3(256), 0, s, n, Np, Nfields, elementList, wJ,
+                         ggeo, lambda, Z, Aq);
+    else
+      hipLaunchKernelGGL((k_simplex_combine<6, false>), dim3(grid), dim3(256), 0, s, n, Np, Nfields, elementList, wJ,
+                         ggeo, lambda, Z, Aq);
+  }
+  BPX_HIP(hipGetLastError());
+  return BPX_OK;
+}
 
 int launch_simplex_ax(hipStream_t s, int bp, int dim, int Np, int Nfields, int Nelements, const int* elementList,
                       const int* G2L, const double* wJ, const double* ggeo, const double* S, const double* MM,

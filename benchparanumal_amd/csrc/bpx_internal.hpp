@@ -243,6 +243,14 @@ int launch_batch_count_done(hipStream_t s, int nprob, const BatchState* st, int*
 int launch_quad_ax(hipStream_t s, int bp, int N, int Nfields, int Nelements, const int* elementList, const int* G2L,
                    const double* W, const double* G, const double* Dm, const double* I, double lambda,
                    const double* q, double* Aq);
+// the simplex Ax as gather -> one rocBLAS DGEMM (Z = Sall Q) -> combine;
+// Sall: (NG+1)*Np x Np (MM alone, Np x Np, for BP1/BP2), column-major;
+// Q: Np x Nelements*Nfields; Z: rows(Sall) x Nelements*Nfields
+int simplex_blas_create(hipStream_t s, void** handle);
+void simplex_blas_destroy(void* handle);
+int launch_simplex_ax_gemm(hipStream_t s, void* blas, int bp, int dim, int Np, int Nfields, int Nelements,
+                           const int* elementList, const int* G2L, const double* wJ, const double* ggeo,
+                           const double* Sall, double* Q, double* Z, double lambda, const double* q, double* Aq);
 int launch_simplex_ax(hipStream_t s, int bp, int dim, int Np, int Nfields, int Nelements, const int* elementList,
                       const int* G2L, const double* wJ, const double* ggeo, const double* S, const double* MM,
                       double lambda, const double* q, double* Aq);

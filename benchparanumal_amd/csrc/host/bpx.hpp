@@ -498,6 +498,11 @@ class bp_t : public solver_t {
   deviceBuffer<double> o_wJ, o_ggeo, o_gllw, o_D, o_I, o_MM, o_invV;
   deviceBuffer<double> o_eo;  // even-odd tables of the BP1-BP4 pencil kernel (N <= 7, non-affine)
   deviceBuffer<double> o_S;  // triangles and tetrahedra: the packed stiffness matrices
+  // triangles and tetrahedra: the Ax as one GEMM (launch_simplex_ax_gemm):
+  // the stacked reference matrices, the gathered element values, the
+  // product, and the rocBLAS handle
+  deviceBuffer<double> o_Sall, o_Qs, o_Zs;
+  std::shared_ptr<void> blas;
   deviceBuffer<double> o_AqL;
   deviceBuffer<double> o_AqLk;  // K local vectors for OperatorKrhs, grown on demand
   // variable coefficient: lattice point of each local node, the lattice

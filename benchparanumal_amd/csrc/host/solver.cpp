@@ -363,6 +363,23 @@ int bp_t::Setup(platform_t& p, mesh_t& m, int bp_, dfloat lambda_) {
     if (int st = o_ggeo.upload(m.ggeo, s)) return st;
     if (int st = o_MM.upload(m.MM, s)) return st;
     if (int st = o_S.upload(m.Ssim, s)) return st;
+    // the GEMM form: Sall column-major with the NG stiffness blocks
+    // A_c[n][k] = S_c[k][n] stacked over MM (MM alone for BP1/BP2)
+    const int np = m.Np, ng = m.dim == 2 ? 3 : 6, blocks = bp >= 3 ? ng + 1 : 1;
+    const size_t M = (size_t)blocks * np, NN = (size_t)np * np;
+    std::vector<double> sall(M * np);
+    for (int k = 0; k < np; ++k)
+      for (int n = 0; n < np; ++n) {
+        for (int c = 0; c + 1 < blocks; ++c) sall[(size_t)c * np + n + k * M] = m.Ssim[c * NN + (size_t)k * np + n];
+        sall[(size_t)(blocks - 1) * np + n + k * M] = m.MM[(size_t)k * np + n];
+      }
+    const size_t cols = (size_t)m.Nelements * Nfields;
+    void* h = nullptr;
+    if (int st = simplex_blas_create(s, &h)) return st;
+    blas = std::shared_ptr<void>(h, simplex_blas_destroy);
+    if (int st = o_Sall.upload(sall, s)) return st;
+    if (int st = o_Qs.alloc(std::max<size_t>(1, cols * np))) return st;
+    if (int st = o_Zs.alloc(std::max<size_t>(1, cols * M))) return st;
   } else if (m.elementType == ELEM_QUAD) {  // per-node factors; the Gauss forms for BP1-BP4
     if (bp == 5 || bp == 6) {
       if (int st = o_wJ.upload(m.wJ, s)) return st;
@@ -438,6 +455,10 @@ int bp_t::ax(const T* q, T* AqL, const T* wJ, const T* ggeo, const T* gllw, cons
       if (mesh->elementType == ELEM_QUAD)
         return launch_quad_ax(platform->stream, bp, mesh->N, Nfields, mesh->Nelements, a.elementList, a.GlobalToLocal,
                               wJ, ggeo, D, I, lambda, q, AqL);
+      if (blas)
+        return launch_simplex_ax_gemm(platform->stream, blas.get(), bp, mesh->dim, mesh->Np, Nfields,
+                                      mesh->Nelements, a.elementList, a.GlobalToLocal, wJ, ggeo, o_Sall.ptr(),
+                                      o_Qs.ptr(), o_Zs.ptr(), lambda, q, AqL);
       return launch_simplex_ax(platform->stream, bp, mesh->dim, mesh->Np, Nfields, mesh->Nelements, a.elementList,
                                a.GlobalToLocal, wJ, ggeo, o_S.ptr(), MM, lambda, q, AqL);
     } else {
