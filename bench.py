@@ -313,6 +313,39 @@ def bench_other_bps(capi, plat):
     return {"workload": "BP1/BP2/BP4/BP6 Ax, 24^3 hex p=7 perturbed, FP64", "rows": rows}
 
 
+def bench_families(capi, plat):
+    """BP5 Ax (FP64) on the quadrilateral, triangle and tetrahedron boxes at
+    p = 7 (perturbed vertices), against the reference's byte model
+    (BP/BP5/src/bp5Run.cpp:106-133: quads 4 factors per node, simplices 4 or
+    7 per element as its affine meshes). The simplex operator is a GEMM with
+    the dense reference matrices, so its rate is also given in FP64 TFLOP/s
+    (2 (NG+1) Np^2 flops per element, NG = 3 or 6) against 78.6."""
+    import torch
+
+    rows = []
+    for el, name, nx, ny, nz in ((capi.QUAD, "quadrilateral", 256, 256, 1), (capi.TRI, "triangle", 192, 192, 1),
+                                 (capi.TET, "tetrahedron", 20, 20, 20)):
+        m = capi.Mesh(N=7, nx=nx, ny=ny, nz=nz, element=el, perturb=0.1, seed=1)
+        bp = capi.BP(plat, m, 5, lam=1.0)
+        Nd, Nel, Np = bp.Ngather, m.Nelements, m.Np
+        q = torch.from_numpy(capi.random_normal(2, Nd)).cuda()
+        AqL = torch.empty(bp.NLocal, dtype=torch.float64, device="cuda")
+        t = event_time(lambda: bp.ax_local(q, AqL), 20, warm=20)
+        geo = Np * 4 * 8 if el == capi.QUAD else (7 if el == capi.TET else 4) * 8
+        nb = Nd * 8 + Nel * (geo + 4 + Np * 12)
+        row = {"element": name, "N": 7, "Nelements": Nel, "Np": Np, "Ndofs": Nd, "us": t * 1e6,
+               "gdof_per_s": Nd / t / 1e9, "algorithmic_bytes": nb, "gbs": nb / t / 1e9,
+               "frac": nb / t / 1e9 / HBM_PEAK_GBS}
+        if el != capi.QUAD:
+            ng = 6 if el == capi.TET else 3
+            fl = 2.0 * (ng + 1) * Np * Np * Nel
+            row.update({"gemm_flops": fl, "tflops": fl / t / 1e12, "tflops_frac_fp64": fl / t / 1e12 / 78.6})
+        rows.append(row)
+        del bp, m, q, AqL
+        torch.cuda.empty_cache()
+    return {"workload": "BP5 Ax FP64, p=7: 256^2 quads, 192^2 x 2 triangles, 20^3 x 6 tetrahedra", "rows": rows}
+
+
 def bench_bp3(capi, plat, args):
     """BP3 (over-integrated stiffness, cubNq = p+2) at 24^3, p = 7: Ax, and
     the FP64 CG against the FP32 CG with FP64 defect correction, both to tol."""
@@ -803,6 +836,7 @@ def main():
         # several GB), the same kernels read up to 18% slower (BP2 112.8 against
         # 95.3 us at 24^3, p = 7)
         ext = {"other_bps": bench_other_bps(capi, plat), "bp3": bench_bp3(capi, plat, args),
+               "families": bench_families(capi, plat),
                "krhs_ax": bench_krhs(capi, plat), "ragged_batch_cg": bench_batch(capi, plat),
                "adjoint": bench_adjoint(capi, plat), "bk5_p15": bench_p15(capi, plat)}
         log("extensions: " + json.dumps({k: {kk: vv for kk, vv in v.items() if kk in
