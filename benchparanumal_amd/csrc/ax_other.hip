@@ -17,11 +17,10 @@
 //    sin(pi y) in 2D (two fields: (3 pi^2 + lambda) {sin(2 pi x) sin(pi y),
 //    sin(pi x) sin(2 pi y)}) and the hex box's forcing in 3D.
 //
-// Kernels: one workgroup per element. A quad's element runs on one lane
-// per point of its GLL or Gauss grid with the grid values in LDS; a simplex
-// element on one lane per node, each lane forming its row of the dense
-// products from the node values in LDS, the reference matrices (shared by
-// every element) coming through the caches.
+// Kernels: a quad's element runs in one workgroup, one lane per point of
+// its GLL or Gauss grid with the grid values in LDS. The simplex operator is
+// one rocBLAS DGEMM over all elements between a gather and a combine kernel
+// (launch_simplex_ax_gemm).
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
 
@@ -35,52 +34,6 @@ namespace {
 constexpr double kPi = 3.14159265358979323846;
 
 // ---------------------------------------------------------- simplices ---
-
-// BP1/BP2 (STIFF = false) and BP3-BP6 (STIFF = true); NG = 3 (2D) or 6 (3D)
-template <int NG, bool STIFF>
-__global__ __launch_bounds__(1024) void k_simplex_ax(int Np, int NF, const int* __restrict__ elementList,
-                                                     const int* __restrict__ G2L, const double* __restrict__ wJ,
-                                                     const double* __restrict__ ggeo, const double* __restrict__ S,
-                                                     const double* __restrict__ MM, double lambda,
-                                                     const double* __restrict__ q, double* __restrict__ Aq) {
-  extern __shared__ double s_q[];  // NF x Np
-  const int e = elementList[blockIdx.x];
-  const int n = threadIdx.x;
-  for (int f = 0; f < NF; ++f)
-    if (n < Np) {
-      const int id = G2L[((size_t)e * NF + f) * Np + n];
-      s_q[f * Np + n] = id >= 0 ? q[id] : 0.0;
-    }
-  __syncthreads();
-  if (n >= Np) return;
-  const double J = wJ[e];
-  double G[NG];
-#pragma unroll
-  for (int c = 0; c < NG; ++c) G[c] = STIFF ? ggeo[(size_t)e * NG + c] : 0.0;
-  const size_t NN = (size_t)Np * Np;
-  for (int f = 0; f < NF; ++f) {
-    double aM = 0, aS[NG];
-#pragma unroll
-    for (int c = 0; c < NG; ++c) aS[c] = 0;
-    for (int k = 0; k < Np; ++k) {
-      const double qk = s_q[f * Np + k];
-      aM += MM[(size_t)k * Np + n] * qk;
-      if (STIFF)
-#pragma unroll
-        for (int c = 0; c < NG; ++c) aS[c] += S[c * NN + (size_t)k * Np + n] * qk;
-    }
-    double out;
-    if (STIFF) {
-      out = 0;
-#pragma unroll
-      for (int c = 0; c < NG; ++c) out += G[c] * aS[c];
-      out += J * lambda * aM;
-    } else {
-      out = J * aM;
-    }
-    Aq[((size_t)e * NF + f) * Np + n] = out;
-  }
-}
 
 // The simplex operator as one library GEMM: the element values gathered
 // into Q (Np x Nel*NF, column e*NF + f), Z = Sall Q with Sall the reference
@@ -357,33 +310,6 @@ int launch_simplex_ax_gemm(hipStream_t s, void* blas, int bp, int dim, int Np, i
   return BPX_OK;
 }
 
-int launch_simplex_ax(hipStream_t s, int bp, int dim, int Np, int Nfields, int Nelements, const int* elementList,
-                      const int* G2L, const double* wJ, const double* ggeo, const double* S, const double* MM,
-                      double lambda, const double* q, double* Aq) {
-  if (Nelements <= 0) return BPX_OK;
-  if (Np > 1024 || (size_t)Nfields * Np * sizeof(double) > 64 * 1024)
-    return set_error(BPX_ERR_UNSUPPORTED, "simplex Ax: more than 1024 nodes per element");
-  const bool stiff = bp >= 3;
-  const dim3 grid(Nelements), block(round64(Np));
-  const size_t shm = (size_t)Nfields * Np * sizeof(double);
-  if (dim == 2) {
-    if (stiff)
-      hipLaunchKernelGGL((k_simplex_ax<3, true>), grid, block, shm, s, Np, Nfields, elementList, G2L, wJ, ggeo, S, MM,
-                         lambda, q, Aq);
-    else
-      hipLaunchKernelGGL((k_simplex_ax<3, false>), grid, block, shm, s, Np, Nfields, elementList, G2L, wJ, ggeo, S,
-                         MM, lambda, q, Aq);
-  } else {
-    if (stiff)
-      hipLaunchKernelGGL((k_simplex_ax<6, true>), grid, block, shm, s, Np, Nfields, elementList, G2L, wJ, ggeo, S, MM,
-                         lambda, q, Aq);
-    else
-      hipLaunchKernelGGL((k_simplex_ax<6, false>), grid, block, shm, s, Np, Nfields, elementList, G2L, wJ, ggeo, S,
-                         MM, lambda, q, Aq);
-  }
-  BPX_HIP(hipGetLastError());
-  return BPX_OK;
-}
 
 int launch_quad_ax(hipStream_t s, int bp, int N, int Nfields, int Nelements, const int* elementList, const int* G2L,
                    const double* W, const double* G, const double* Dm, const double* I, double lambda,

@@ -251,9 +251,6 @@ void simplex_blas_destroy(void* handle);
 int launch_simplex_ax_gemm(hipStream_t s, void* blas, int bp, int dim, int Np, int Nfields, int Nelements,
                            const int* elementList, const int* G2L, const double* wJ, const double* ggeo,
                            const double* Sall, double* Q, double* Z, double lambda, const double* q, double* Aq);
-int launch_simplex_ax(hipStream_t s, int bp, int dim, int Np, int Nfields, int Nelements, const int* elementList,
-                      const int* G2L, const double* wJ, const double* ggeo, const double* S, const double* MM,
-                      double lambda, const double* q, double* Aq);
 int launch_rhs_other(hipStream_t s, int elementType, int N, int Np, int Nfields, int Nelements, const double* wJ,
                      const double* MM, const double* x, const double* y, const double* z, double lambda,
                      double* rhs);

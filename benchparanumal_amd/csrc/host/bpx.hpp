@@ -497,7 +497,6 @@ class bp_t : public solver_t {
   deviceBuffer<dlong> o_GlobalToLocal, o_elementList;
   deviceBuffer<double> o_wJ, o_ggeo, o_gllw, o_D, o_I, o_MM, o_invV;
   deviceBuffer<double> o_eo;  // even-odd tables of the BP1-BP4 pencil kernel (N <= 7, non-affine)
-  deviceBuffer<double> o_S;  // triangles and tetrahedra: the packed stiffness matrices
   // triangles and tetrahedra: the Ax as one GEMM (launch_simplex_ax_gemm):
   // the stacked reference matrices, the gathered element values, the
   // product, and the rocBLAS handle

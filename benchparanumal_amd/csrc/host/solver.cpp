@@ -362,7 +362,6 @@ int bp_t::Setup(platform_t& p, mesh_t& m, int bp_, dfloat lambda_) {
     if (int st = o_wJ.upload(m.wJ, s)) return st;
     if (int st = o_ggeo.upload(m.ggeo, s)) return st;
     if (int st = o_MM.upload(m.MM, s)) return st;
-    if (int st = o_S.upload(m.Ssim, s)) return st;
     // the GEMM form: Sall column-major with the NG stiffness blocks
     // A_c[n][k] = S_c[k][n] stacked over MM (MM alone for BP1/BP2)
     const int np = m.Np, ng = m.dim == 2 ? 3 : 6, blocks = bp >= 3 ? ng + 1 : 1;
@@ -455,12 +454,9 @@ int bp_t::ax(const T* q, T* AqL, const T* wJ, const T* ggeo, const T* gllw, cons
       if (mesh->elementType == ELEM_QUAD)
         return launch_quad_ax(platform->stream, bp, mesh->N, Nfields, mesh->Nelements, a.elementList, a.GlobalToLocal,
                               wJ, ggeo, D, I, lambda, q, AqL);
-      if (blas)
-        return launch_simplex_ax_gemm(platform->stream, blas.get(), bp, mesh->dim, mesh->Np, Nfields,
-                                      mesh->Nelements, a.elementList, a.GlobalToLocal, wJ, ggeo, o_Sall.ptr(),
-                                      o_Qs.ptr(), o_Zs.ptr(), lambda, q, AqL);
-      return launch_simplex_ax(platform->stream, bp, mesh->dim, mesh->Np, Nfields, mesh->Nelements, a.elementList,
-                               a.GlobalToLocal, wJ, ggeo, o_S.ptr(), MM, lambda, q, AqL);
+      return launch_simplex_ax_gemm(platform->stream, blas.get(), bp, mesh->dim, mesh->Np, Nfields, mesh->Nelements,
+                                    a.elementList, a.GlobalToLocal, wJ, ggeo, o_Sall.ptr(), o_Qs.ptr(),
+                                    o_Zs.ptr(), lambda, q, AqL);
     } else {
       return set_error(BPX_ERR_UNSUPPORTED, "bp: the FP32 operators cover hexahedra");
     }
