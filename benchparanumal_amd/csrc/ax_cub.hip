@@ -708,9 +708,11 @@ __global__ __launch_bounds__(128) void k_bp3_wave(int Nelements, const int* __re
 // Every one-dimensional contraction runs along a line (pencil) that one
 // lane holds in registers, with the 1D matrix uniform over the workgroup
 // (scalar loads, one row in use and the next in flight); values move
-// between line directions through LDS once per move. Two LDS tiles per
-// slot, A and B, with line stride PR = MQ | 1 (odd strides keep the
-// strided accesses free of bank conflicts) and slab stride PS = MQ * PR:
+// between line directions through LDS once per move. LDS tiles per slot
+// with line stride PR = MQ | 1 (odd strides keep the strided accesses free
+// of bank conflicts) and slab stride PS = MQ * PR: A and B for BP3/BP4; one
+// tile for BP1/BP2 (A and B below name the same one; a pass that rewrites
+// it holds its outputs in registers across a barrier):
 //   i-lines (k,j):  gather q, x = I q along i                 -> A
 //   j-lines (k,a):  y = I x along j                           -> B
 //   t-lines (b,a):  u = I y along k, in registers
@@ -728,12 +730,13 @@ __global__ __launch_bounds__(128) void k_bp3_wave(int Nelements, const int* __re
 // A node crosses LDS 4 (BP1) or 16 (BP3) times, against 9 values read per
 // output of a gradient or divergence in the wavefront forms. The geometric
 // factors of the first PF cubature slabs are requested when u is formed,
-// so they are in flight through the r and s derivatives. The r, s and t
-// parts of the divergence are summed apart, so BP3/BP4 results agree with
-// the other forms to rounding, not bit for bit (BP1/BP2 add in the same
-// order as the wavefront form). q is gathered through a bounds-checked
-// buffer resource: a masked node (G2L = -1) or an idle slot reads 0 without
-// a compare or select per load.
+// so they are in flight through the r and s derivatives. With the host's
+// even-odd tables (EO) each contraction runs in even-odd form; the r, s and
+// t parts of the divergence are summed apart. Results agree with the
+// wavefront forms to rounding, not bit for bit (the row-by-row BP1/BP2
+// form adds in their order). q is gathered through a bounds-checked buffer
+// resource: a masked node (G2L = -1) or an idle slot reads 0 without a
+// compare or select per load; BP2 gathers and stores in node order.
 template <typename T>
 __device__ inline T cfma(T a, T b, T c) {
   if constexpr (sizeof(T) == 8)
