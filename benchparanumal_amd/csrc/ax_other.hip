@@ -92,10 +92,17 @@ __global__ __launch_bounds__(320) void k_quad_ax(int Nq, int NF, const int* __re
                                                  const double* __restrict__ I, double lambda,
                                                  const double* __restrict__ q, double* __restrict__ Aq) {
   __shared__ double s_q[16 * 16], s_a[17 * 17], s_b[17 * 17], s_c[17 * 17];
+  // the 1D matrices, staged once: the inner loops then read LDS instead of
+  // issuing one global (cache) load per multiply-add
+  __shared__ double s_Dm[17 * 17], s_I[17 * 16];
   const int MQ = MODE == QUAD_BP5 ? Nq : Nq + 1;
   const int Np = Nq * Nq, MP = MQ * MQ;
   const int e = elementList[blockIdx.x];
   const int t = threadIdx.x;
+  for (int n = t; n < MQ * MQ; n += blockDim.x) s_Dm[n] = Dm[n];
+  if (MODE != QUAD_BP5)
+    for (int n = t; n < MQ * Nq; n += blockDim.x) s_I[n] = I[n];
+  __syncthreads();
   const int a = t % MQ, b = t / MQ;  // the lane's grid point (a along r, b along s)
   const bool on = t < MP;
   for (int f = 0; f < NF; ++f) {
@@ -111,8 +118,8 @@ __global__ __launch_bounds__(320) void k_quad_ax(int Nq, int NF, const int* __re
       if (on) {
         double qr = 0, qs = 0;
         for (int m = 0; m < Nq; ++m) {
-          qr += Dm[a * Nq + m] * s_q[b * Nq + m];
-          qs += Dm[b * Nq + m] * s_q[m * Nq + a];
+          qr += s_Dm[a * Nq + m] * s_q[b * Nq + m];
+          qs += s_Dm[b * Nq + m] * s_q[m * Nq + a];
         }
         const size_t gb = (size_t)e * Np + t;
         const double G00 = G[3 * gb + 0], G01 = G[3 * gb + 1], G11 = G[3 * gb + 2];
@@ -126,8 +133,8 @@ __global__ __launch_bounds__(320) void k_quad_ax(int Nq, int NF, const int* __re
       if (on) {
         double acc = lambda * qn;
         for (int m = 0; m < Nq; ++m) {
-          acc += Dm[m * Nq + a] * s_a[b * Nq + m];
-          acc += Dm[m * Nq + b] * s_b[m * Nq + a];
+          acc += s_Dm[m * Nq + a] * s_a[b * Nq + m];
+          acc += s_Dm[m * Nq + b] * s_b[m * Nq + a];
         }
         Aq[lb + t] = acc;
       }
@@ -136,13 +143,13 @@ __global__ __launch_bounds__(320) void k_quad_ax(int Nq, int NF, const int* __re
       if (t < Nq * MQ) {
         const int j = t / MQ, aa = t % MQ;
         double acc = 0;
-        for (int i = 0; i < Nq; ++i) acc += I[aa * Nq + i] * s_q[j * Nq + i];
+        for (int i = 0; i < Nq; ++i) acc += s_I[aa * Nq + i] * s_q[j * Nq + i];
         s_a[j * MQ + aa] = acc;
       }
       __syncthreads();
       double u = 0;
       if (on) {
-        for (int j = 0; j < Nq; ++j) u += I[b * Nq + j] * s_a[j * MQ + a];
+        for (int j = 0; j < Nq; ++j) u += s_I[b * Nq + j] * s_a[j * MQ + a];
         s_b[t] = u;
       }
       __syncthreads();
@@ -156,8 +163,8 @@ __global__ __launch_bounds__(320) void k_quad_ax(int Nq, int NF, const int* __re
         if (on) {
           double ur = 0, us = 0;
           for (int m = 0; m < MQ; ++m) {
-            ur += Dm[a * MQ + m] * s_b[b * MQ + m];
-            us += Dm[b * MQ + m] * s_b[m * MQ + a];
+            ur += s_Dm[a * MQ + m] * s_b[b * MQ + m];
+            us += s_Dm[b * MQ + m] * s_b[m * MQ + a];
           }
           const double G00 = G[3 * cb + 0], G01 = G[3 * cb + 1], G11 = G[3 * cb + 2];
           fr = G00 * ur + G01 * us;
@@ -172,8 +179,8 @@ __global__ __launch_bounds__(320) void k_quad_ax(int Nq, int NF, const int* __re
         if (on) {
           res = lambda * W[cb] * u;
           for (int m = 0; m < MQ; ++m) {
-            res += Dm[m * MQ + a] * s_c[b * MQ + m];
-            res += Dm[m * MQ + b] * s_a[m * MQ + a];
+            res += s_Dm[m * MQ + a] * s_c[b * MQ + m];
+            res += s_Dm[m * MQ + b] * s_a[m * MQ + a];
           }
         }
       }
@@ -184,7 +191,7 @@ __global__ __launch_bounds__(320) void k_quad_ax(int Nq, int NF, const int* __re
       if (t < Nq * MQ) {
         const int j = t / MQ, aa = t % MQ;
         double acc = 0;
-        for (int bb = 0; bb < MQ; ++bb) acc += I[bb * Nq + j] * s_b[bb * MQ + aa];
+        for (int bb = 0; bb < MQ; ++bb) acc += s_I[bb * Nq + j] * s_b[bb * MQ + aa];
         s_a[j * MQ + aa] = acc;
       }
       __syncthreads();
@@ -192,7 +199,7 @@ __global__ __launch_bounds__(320) void k_quad_ax(int Nq, int NF, const int* __re
       if (t < Np) {
         const int j = t / Nq, i = t % Nq;
         double acc = 0;
-        for (int aa = 0; aa < MQ; ++aa) acc += I[aa * Nq + i] * s_a[j * MQ + aa];
+        for (int aa = 0; aa < MQ; ++aa) acc += s_I[aa * Nq + i] * s_a[j * MQ + aa];
         Aq[lb + t] = acc;
       }
     }
