@@ -292,7 +292,7 @@ def bench_other_bps(capi, plat):
     BP/BP1/src/bp1Run.cpp:107-120 (cubature weights), BP4
     BP/BP4/src/bp4Run.cpp:106-119 (cubature factors), BP6
     BP/BP6/src/bp6Run.cpp:106-134 (node factors). Each is timed over 20
-    launches after 50 untimed ones: the first few milliseconds of launches
+    launches after 200 untimed ones (8-48 ms): launches in the first ~20 ms
     after a problem's setup run up to 15% slower (BP4 268 against 235 us)."""
     import torch
 
@@ -303,7 +303,7 @@ def bench_other_bps(capi, plat):
         Nd, Nel, Np, nf = bp.Ngather, m.Nelements, m.Np, bp.Nfields
         q = torch.from_numpy(capi.random_normal(2, Nd)).cuda()
         AqL = torch.empty(bp.NLocal, dtype=torch.float64, device="cuda")
-        t = event_time(lambda: bp.ax_local(q, AqL), 20, warm=50)
+        t = event_time(lambda: bp.ax_local(q, AqL), 20, warm=200)
         geo = m.cubNp * 8 if bpn <= 2 else (m.cubNp if bpn == 4 else Np) * 7 * 8
         nb = Nd * 8 + Nel * (geo + 4 + Np * nf * (4 + 8))
         rows.append({"bp": bpn, "fields": nf, "Ndofs": Nd, "us": t * 1e6, "gdof_per_s": Nd / t / 1e9,
@@ -357,7 +357,7 @@ def bench_bp3(capi, plat, args):
     Nd, Nel, Np, cubNp = bp.Ngather, m.Nelements, m.Np, m.cubNp
     q = torch.from_numpy(capi.random_normal(2, Nd)).cuda()
     AqL = torch.empty(bp.NLocal, dtype=torch.float64, device="cuda")
-    t_ax = event_time(lambda: bp.ax_local(q, AqL), 20, warm=50)
+    t_ax = event_time(lambda: bp.ax_local(q, AqL), 20, warm=200)
     nb = Nd * 8 + Nel * (cubNp * 7 * 8 + 4 + Np * 12)  # BP/BP3/src/bp3Run.cpp:114-118
     b = torch.from_numpy(capi.random_normal(7, Nd)).cuda()
     x = torch.zeros_like(b)
